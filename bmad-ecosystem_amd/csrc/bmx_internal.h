@@ -1,0 +1,66 @@
+// bmx_internal.h -- device program format shared by the lattice compiler
+// (bmx_lattice.cpp) and the tracking kernels (bmx_track.hip).
+//
+// A lattice is compiled once into a flat program of "ops".  Each Bmad element
+// becomes a short op sequence that reproduces track1 -> track1_bmad ->
+// track_a_* for that element (bmad/code/track1.f90:43-342,
+// bmad/low_level/track1_bmad.f90:22-283), with every particle-independent
+// quantity (multipole_ele_to_ab results, tilt/pitch rotations, edge-kick
+// coefficients, RF constants) folded into the op's constants.  The kernels
+// walk the op list with a wave-uniform index, so the ops are read through
+// the scalar cache and the dispatch is a scalar branch.
+#pragma once
+#include <stdint.h>
+
+namespace bmx {
+
+enum OpType : int32_t {
+  OP_DRIFT = 1,          // exact drift, c[0] = L
+  OP_KICK = 2,           // thin ab multipole kick, complex Horner over n = 0..nmax
+  OP_KICK_REF0 = 3,      // n = 0 kt kick of a multipole element with ref_orb_offset
+  OP_QUAD = 4,           // linear quad body step with quadratic z and low-energy z
+  OP_BEND = 5,           // exact sector-bend body step (g != 0, k1 = 0)
+  OP_BEND_K1 = 6,        // sbend body step with k1 (sbend_body_with_k1_map)
+  OP_BEND_KICK = 7,      // multipole kick inside a bend, scaled by (1 + g x)
+  OP_HWANG = 8,          // Hwang bend edge kick (basic_bend / hard_edge_only)
+  OP_LINEAR_EDGE = 9,    // linear bend edge kick
+  OP_HARD_MULT_EDGE = 10,// hard multipole edge kick of quadrupole / sextupole
+  OP_OFFSET_SET = 11,    // lab -> body frame (offsets, pitches, tilt)
+  OP_OFFSET_UNSET = 12,  // body -> lab frame
+  OP_TOO_LARGE = 13,     // orbit_too_large check inside an element (early return)
+  OP_RFCAVITY = 14,      // standing-wave cavity body (n_slice energy kicks and drifts)
+  OP_APERTURE_IN = 15,   // aperture check at the entrance edge
+  OP_ELE_END = 16,       // orbit_too_large + exit aperture + loss bookkeeping
+  OP_TILT = 17,          // tilt_coords rotation, c[0] = cos, c[1] = sin
+};
+
+// op flags
+enum : int32_t {
+  F_APERTURE_ELLIPTICAL = 1,
+  F_HWANG_ENTER = 2,      // OP_HWANG / OP_HARD_MULT_EDGE at the first track edge
+  F_OFFSET_PITCH = 4,     // OP_OFFSET_*: general rotation, else pure translation
+  F_HAS_APERTURE = 16,    // OP_ELE_END: check the exit aperture
+  F_OFFSET_MOVE = 32,     // OP_OFFSET_*: has a translation/pitch part
+};
+
+struct alignas(64) DevOp {
+  int32_t type;   // OpType
+  int32_t ie;     // element index in its variant (0-based)
+  int32_t n;      // aux integer (kick order, slice count)
+  int32_t flags;  // F_*
+  double c[14];   // constants, meaning per op type (see bmx_lattice.cpp)
+};
+static_assert(sizeof(DevOp) == 128, "DevOp must be 128 bytes");
+
+// Per-lattice constants shared by every op (one reference particle, constant p0c).
+struct LatticeConsts {
+  double p0c;        // reference momentum [eV]
+  double e_tot;      // reference energy [eV]
+  double mc2;        // particle mass [eV]
+  double beta0;      // p0c / e_tot
+  double max_aperture_limit;
+  int32_t aperture_limit_on;
+  int32_t species;
+};
+
+}  // namespace bmx

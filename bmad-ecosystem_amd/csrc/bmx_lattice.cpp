@@ -1,0 +1,500 @@
+// bmx_lattice.cpp -- compiles bmx_ele_rec element lists into device op
+// programs (bmx_internal.h).  This is the hoisting pass that replaces Bmad's
+// per-particle element bookkeeping: multipole_ele_to_ab
+// (bmad/code/multipole_ele_to_ab.f90:31-235), the tilt/pitch set-up of
+// offset_particle (bmad/code/offset_particle.f90:66-530), the constants of the
+// Hwang edge map (bmad/modules/fringe_mod.f90:240-418) and the RF set-up of
+// track_a_rfcavity (bmad/low_level/track_a_rfcavity.f90:17-80).
+//
+// Scope of the device path: forward tracking (direction = +1, time_dir = +1),
+// element orientation +1, magnetic multipoles, relative-time RF.  Anything
+// else is rejected with an error message naming the element; the CPU side
+// never silently substitutes another path.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bmx.h"
+#include "bmx_internal.h"
+
+namespace bmx {
+
+namespace {
+
+const double kPi = 3.141592653589793238462643383279;
+const double kCLight = 2.99792458e8;
+
+double species_mass(int s) {
+  switch (s) {
+    case BMX_ELECTRON: case BMX_POSITRON: return 0.51099895069e6;
+    case BMX_PROTON: case BMX_ANTIPROTON: return 0.93827208943e9;
+    case BMX_MUON: case BMX_ANTIMUON: return 105.6583755e6;
+  }
+  return 0;
+}
+
+int nint(double x) { return (int)(x < 0 ? -std::floor(-x + 0.5) : std::floor(x + 0.5)); }
+
+bool has_hkick_attr(int key) {
+  switch (key) {
+    case BMX_KEY_QUADRUPOLE: case BMX_KEY_SBEND: case BMX_KEY_SEXTUPOLE: case BMX_KEY_OCTUPOLE:
+    case BMX_KEY_THICK_MULTIPOLE: case BMX_KEY_RFCAVITY: case BMX_KEY_KICKER: case BMX_KEY_INSTRUMENT:
+    case BMX_KEY_MONITOR: case BMX_KEY_ECOLLIMATOR: case BMX_KEY_RCOLLIMATOR:
+      return true;
+  }
+  return false;
+}
+
+// Magnetic An/Bn of an element as multipole_ele_to_ab(ele, use_ele_tilt =
+// false, magnetic, include_kicks) returns them; *b1 receives the pulled
+// quadrupole term when b1 != nullptr.
+struct AB {
+  double a[BMX_N_POLE], b[BMX_N_POLE];
+  int ix_max;
+};
+
+int max_nonzero(const double* a, const double* b, int n_hi) {
+  int ix = -1;
+  for (int n = 0; n <= n_hi; n++)
+    if (a[n] != 0 || b[n] != 0) ix = n;
+  return ix;
+}
+
+bool magnetic_ab(const bmx_ele_rec& e, bool include_kicks, AB* out, double* b1, std::string* err) {
+  AB& r = *out;
+  std::memset(&r, 0, sizeof(r));
+  r.ix_max = -1;
+  if (b1) *b1 = 0;
+  if (!e.is_on || e.key == BMX_KEY_PIPE) return true;
+  static double nfact[BMX_N_POLE];
+  nfact[0] = 1;
+  for (int n = 1; n < BMX_N_POLE; n++) nfact[n] = n * nfact[n - 1];
+
+  if (e.key == BMX_KEY_MULTIPOLE) {
+    if (!e.has_mag_poles) return true;
+    for (int n = 0; n < BMX_N_POLE; n++) {  // multipole1_kt_to_ab
+      double knl = e.a[n], tn = e.b[n];
+      if (knl == 0) continue;
+      double ang = -tn * (n + 1);
+      r.b[n] = knl * std::cos(ang) / nfact[n];
+      r.a[n] = knl * std::sin(ang) / nfact[n];
+    }
+  } else {
+    if (e.has_mag_poles) {
+      if (e.field_master) { *err = "field_master multipoles are not supported"; return false; }
+      for (int n = 0; n < BMX_N_POLE; n++) { r.a[n] = e.a[n]; r.b[n] = e.b[n]; }
+      if (max_nonzero(r.a, r.b, BMX_N_POLE - 1) >= 0 && e.scale_multipoles) {
+        double cnst = 0;
+        int ref_exp = 0;
+        bool scale = true;
+        switch (e.key) {
+          case BMX_KEY_SBEND: cnst = e.l * (e.g + e.dg); ref_exp = 0; break;
+          case BMX_KEY_KICKER:
+            cnst = (e.hkick == 0) ? e.vkick : (e.vkick == 0 ? e.hkick : std::sqrt(e.hkick * e.hkick + e.vkick * e.vkick));
+            break;
+          case BMX_KEY_HKICKER: case BMX_KEY_VKICKER: cnst = e.kick; break;
+          case BMX_KEY_QUADRUPOLE: cnst = e.k1 * e.l; ref_exp = 1; break;
+          case BMX_KEY_SEXTUPOLE: cnst = e.k2 * e.l; ref_exp = 2; break;
+          case BMX_KEY_OCTUPOLE: cnst = e.k3 * e.l; ref_exp = 3; break;
+          case BMX_KEY_AB_MULTIPOLE: case BMX_KEY_THICK_MULTIPOLE: case BMX_KEY_RFCAVITY: scale = false; break;
+          default: *err = "multipoles on a non-scalable element"; return false;
+        }
+        if (scale) {
+          if (e.r0_mag == 0) {
+            for (int n = 0; n < BMX_N_POLE; n++) { r.a[n] *= cnst; r.b[n] *= cnst; }
+          } else {
+            double f = cnst * std::pow(e.r0_mag, ref_exp + 1);
+            for (int n = 0; n < BMX_N_POLE; n++) { f /= e.r0_mag; r.a[n] *= f; r.b[n] *= f; }
+          }
+        }
+      }
+    }
+    // this_ele_non_multipoles (multipole_ele_to_ab.f90:326-405)
+    double ak[4] = {0, 0, 0, 0}, bk[4] = {0, 0, 0, 0};
+    switch (e.key) {
+      case BMX_KEY_HKICKER: bk[0] = -e.kick; break;
+      case BMX_KEY_VKICKER: ak[0] = e.kick; break;
+      case BMX_KEY_KICKER: bk[0] = -e.hkick; ak[0] = e.vkick; break;
+      default:
+        if (has_hkick_attr(e.key) && (e.hkick != 0 || e.vkick != 0)) {
+          double tilt = (e.key == BMX_KEY_SBEND) ? e.ref_tilt : e.tilt;
+          if (tilt != 0) {
+            double c = std::cos(tilt), s = std::sin(tilt);
+            bk[0] = -e.hkick * c - e.vkick * s;
+            ak[0] = -e.hkick * s + e.vkick * c;
+          } else {
+            bk[0] = -e.hkick;
+            ak[0] = e.vkick;
+          }
+        }
+    }
+    switch (e.key) {
+      case BMX_KEY_QUADRUPOLE: bk[1] = e.k1 * e.l; break;
+      case BMX_KEY_SBEND: bk[0] += e.dg * e.l; bk[1] = e.k1 * e.l; bk[2] = e.k2 * e.l / 2.0; break;
+      case BMX_KEY_SEXTUPOLE: bk[2] = e.k2 * e.l / 2.0; break;
+      case BMX_KEY_OCTUPOLE: bk[3] = e.k3 * e.l / 6.0; break;
+    }
+    if (include_kicks)
+      for (int n = 0; n < 4; n++) { r.a[n] += ak[n]; r.b[n] += bk[n]; }
+  }
+  r.ix_max = max_nonzero(r.a, r.b, BMX_N_POLE - 1);
+  if (b1) {  // pull_this_b1
+    *b1 = r.b[1];
+    r.b[1] = 0;
+    if (r.ix_max <= 1) r.ix_max = max_nonzero(r.a, r.b, 1);
+  }
+  return true;
+}
+
+DevOp make_op(int type, int ie) {
+  DevOp op;
+  std::memset(&op, 0, sizeof(op));
+  op.type = type;
+  op.ie = ie;
+  return op;
+}
+
+// Thin kick ops for orders 0..ix_max scaled by `scale` (and by (1 + g x) in a
+// bend).  Each op carries up to 7 complex coefficients c_n = scale (b_n + i a_n)
+// for orders n_lo..n_lo+k; the kick is dpx = -Re K, dpy = Im K with
+// K = sum c_n (x + i y)^n  (ab_multipole_kick, bmad/modules/multipole_mod.f90:314-401).
+void emit_kicks(std::vector<DevOp>& prog, int ie, const AB& ab, double scale, int type, double g = 0) {
+  const int cap = (type == OP_BEND_KICK) ? 6 : 7;  // OP_BEND_KICK keeps g in c[13]
+  if (ab.ix_max < 0) return;
+  int lo = 0;
+  while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
+  while (lo <= ab.ix_max) {
+    int hi = std::min(ab.ix_max, lo + cap - 1);
+    DevOp op = make_op(type, ie);
+    op.n = lo;             // lowest order
+    op.flags = hi - lo;    // number of Horner steps
+    for (int n = lo; n <= hi; n++) {
+      op.c[2 * (n - lo)] = scale * ab.b[n];
+      op.c[2 * (n - lo) + 1] = scale * ab.a[n];
+    }
+    if (type == OP_BEND_KICK) op.c[13] = g;
+    prog.push_back(op);
+    lo = hi + 1;
+    while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
+  }
+}
+
+bool is_misaligned(const bmx_ele_rec& e) {
+  return e.x_offset != 0 || e.y_offset != 0 || e.z_offset != 0 || e.x_pitch != 0 || e.y_pitch != 0;
+}
+
+// offset_particle set/unset for a straight element (bmad/code/offset_particle.f90:150-330):
+// OP_OFFSET_* c[0..2] = x_off, y_off, z_off; c[3] = L/2; c[4..12] = W row-major
+// (W^-1 when setting); the element tilt follows (set) or precedes (unset) as OP_TILT.
+void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set, bool with_tilt) {
+  bool mis = is_misaligned(e);
+  bool tilt = with_tilt && e.tilt != 0;
+  DevOp t = make_op(OP_TILT, ie);
+  if (tilt) {
+    t.c[0] = std::cos(set ? e.tilt : -e.tilt);
+    t.c[1] = std::sin(set ? e.tilt : -e.tilt);
+  }
+  if (!set && tilt) prog.push_back(t);
+  if (mis) {
+    DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
+    op.flags |= F_OFFSET_MOVE;
+    if (e.x_pitch != 0 || e.y_pitch != 0) op.flags |= F_OFFSET_PITCH;
+    op.c[0] = e.x_offset;
+    op.c[1] = e.y_offset;
+    op.c[2] = e.z_offset;
+    op.c[3] = 0.5 * e.l;
+    // floor_angles_to_w_mat(theta = x_pitch, phi = y_pitch, psi = 0)
+    double st = std::sin(e.x_pitch), ct = std::cos(e.x_pitch), sp = std::sin(e.y_pitch), cp = std::cos(e.y_pitch);
+    double w[9] = {ct, -st * sp, st * cp, 0.0, cp, sp, -st, -ct * sp, ct * cp};
+    if (set) {  // inverse = transpose
+      double wi[9] = {w[0], w[3], w[6], w[1], w[4], w[7], w[2], w[5], w[8]};
+      std::memcpy(w, wi, sizeof(w));
+    }
+    for (int i = 0; i < 9; i++) op.c[4 + i] = w[i];
+    prog.push_back(op);
+  }
+  if (set && tilt) prog.push_back(t);
+}
+
+}  // namespace
+
+// Compile one variant.  Returns false and fills *err for unsupported input.
+bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, const bmx_config& cfg,
+                     std::vector<DevOp>& prog, std::vector<int32_t>& ele_op_begin, LatticeConsts* lc,
+                     std::string* err) {
+  (void)cfg;
+  char buf[256];
+  if (n_ele <= 0) { *err = "empty lattice"; return false; }
+  double p0c = eles[0].p0c, e_tot = eles[0].e_tot;
+  if (!(p0c > 0) || !(e_tot > 0)) { *err = "element 0 has no reference energy (p0c, e_tot)"; return false; }
+  double mc2 = species_mass(ref.species);
+  if (mc2 == 0) { *err = "unknown species"; return false; }
+  lc->p0c = p0c;
+  lc->e_tot = e_tot;
+  lc->mc2 = mc2;
+  lc->beta0 = p0c / e_tot;
+  lc->species = ref.species;
+
+  for (int ie = 0; ie < n_ele; ie++) {
+    const bmx_ele_rec& e = eles[ie];
+    auto fail = [&](const char* what) {
+      std::snprintf(buf, sizeof(buf), "element %d (key %d): %s", ie, e.key, what);
+      *err = buf;
+      return false;
+    };
+    ele_op_begin.push_back((int32_t)prog.size());
+    if (e.p0c != p0c || e.p0c_start != p0c || e.e_tot != e_tot)
+      return fail("reference energy changes along the lattice (only rfcavity-style lattices are supported)");
+    if (e.orientation != 1) return fail("orientation -1 is not supported on the device path");
+    if (e.has_elec_poles) {
+      bool any = false;
+      for (int n = 0; n < BMX_N_POLE; n++) any = any || e.a_elec[n] != 0 || e.b_elec[n] != 0;
+      if (any) return fail("electric multipoles are not supported on the device path");
+    }
+    if (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 || e.pz_aperture_width2 != 0)
+      return fail("collimator phase-space apertures are not supported on the device path");
+    if (e.exact_multipoles != 0 && e.exact_multipoles != 1) return fail("exact_multipoles must be off");
+    if (e.num_steps < 1 && e.l != 0) return fail("num_steps must be >= 1");
+
+    // entrance aperture (check_aperture_limit at the first track edge)
+    bool ap_set = (e.x1_limit != 0 || e.x2_limit != 0 || e.y1_limit != 0 || e.y2_limit != 0);
+    int ap_flags = (e.aperture_type == BMX_APERTURE_ELLIPTICAL) ? F_APERTURE_ELLIPTICAL : 0;
+    if (e.aperture_type != BMX_APERTURE_ELLIPTICAL && e.aperture_type != BMX_APERTURE_RECTANGULAR)
+      return fail("aperture_type must be rectangular or elliptical");
+    if (e.aperture_type == BMX_APERTURE_ELLIPTICAL && ap_set &&
+        ((e.x1_limit == 0) != (e.x2_limit == 0) || (e.y1_limit == 0) != (e.y2_limit == 0)))
+      return fail("elliptical aperture needs all four limits");
+    auto ap_consts = [&](DevOp& op) {
+      double x1 = -e.x1_limit, x2 = e.x2_limit, y1 = -e.y1_limit, y2 = e.y2_limit;
+      op.c[0] = x1; op.c[1] = x2; op.c[2] = y1; op.c[3] = y2;
+      if (e.aperture_type == BMX_APERTURE_ELLIPTICAL) {  // elliptical_params_setup
+        double wx = (x2 - x1) / 2, cx = (x2 + x1) / 2, wy = (y2 - y1) / 2, cy = (y2 + y1) / 2;
+        if (wx == 0) wx = cfg.max_aperture_limit;
+        if (wy == 0) wy = cfg.max_aperture_limit;
+        op.c[4] = std::max(wx, 0.0); op.c[5] = cx; op.c[6] = std::max(wy, 0.0); op.c[7] = cy;
+      }
+    };
+    if (cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_ENTRANCE_END || e.aperture_at == BMX_BOTH_ENDS)) {
+      DevOp op = make_op(OP_APERTURE_IN, ie);
+      op.flags = ap_flags;
+      ap_consts(op);
+      prog.push_back(op);
+    }
+
+    int key = e.key;
+    if (!e.is_on) {
+      if (key == BMX_KEY_MULTIPOLE || key == BMX_KEY_AB_MULTIPOLE) key = BMX_KEY_MARKER;
+      else if (key != BMX_KEY_SBEND) key = BMX_KEY_DRIFT;
+    }
+    std::string e2;
+    switch (key) {
+      case BMX_KEY_MARKER:
+        break;
+      case BMX_KEY_DRIFT: {
+        if (e.l != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = e.l; prog.push_back(op); }
+        break;
+      }
+      case BMX_KEY_MULTIPOLE: case BMX_KEY_AB_MULTIPOLE: {
+        // track1_bmad.f90:142-166: offsets without tilt, kt kicks (tilt folded into ab)
+        AB ab;
+        if (!magnetic_ab(e, false, &ab, nullptr, &e2)) return fail(e2.c_str());
+        if (key == BMX_KEY_AB_MULTIPOLE && e.tilt != 0) {  // use_ele_tilt = true
+          for (int n = 0; n <= ab.ix_max; n++) {
+            if (ab.a[n] == 0 && ab.b[n] == 0) continue;
+            double an = ab.a[n], bn = ab.b[n], c = std::cos((n + 1) * e.tilt), s = std::sin((n + 1) * e.tilt);
+            ab.b[n] = bn * c + an * s;
+            ab.a[n] = -bn * s + an * c;
+          }
+        }
+        if (key == BMX_KEY_MULTIPOLE && e.tilt != 0) {  // tilt_tot adds to every Tn
+          for (int n = 0; n < BMX_N_POLE; n++) {
+            double an = ab.a[n], bn = ab.b[n];
+            if (an == 0 && bn == 0) continue;
+            double c = std::cos((n + 1) * e.tilt), s = std::sin((n + 1) * e.tilt);
+            ab.b[n] = bn * c + an * s;
+            ab.a[n] = -bn * s + an * c;
+          }
+        }
+        bool mis = is_misaligned(e);
+        if (mis) emit_offset(prog, ie, e, true, false);
+        if (key == BMX_KEY_MULTIPOLE && e.has_mag_poles && e.a[0] != 0) {
+          DevOp op = make_op(OP_KICK_REF0, ie);
+          double t = e.b[0] + e.tilt;
+          op.c[0] = e.a[0] * std::cos(t);
+          op.c[1] = e.a[0] * std::sin(t);
+          prog.push_back(op);
+          ab.a[0] = ab.b[0] = 0;
+          ab.ix_max = max_nonzero(ab.a, ab.b, BMX_N_POLE - 1);
+        }
+        emit_kicks(prog, ie, ab, 1.0, OP_KICK);
+        if (mis) emit_offset(prog, ie, e, false, false);
+        break;
+      }
+      case BMX_KEY_QUADRUPOLE: {
+        // track_a_quadrupole.f90:17-148
+        if (e.fringe_type != BMX_FRINGE_NONE && e.fringe_at != BMX_NO_END)
+          return fail("quadrupole fringe kicks are not supported on the device path");
+        AB ab;
+        double b1;
+        if (!magnetic_ab(e, true, &ab, &b1, &e2)) return fail(e2.c_str());
+        int n_step = 1;
+        if (ab.ix_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
+        if (e.l == 0) n_step = 0;
+        double step_len = e.l / (n_step > 0 ? n_step : 1);
+        double r_step = 1.0 / (n_step > 0 ? n_step : 1);
+        emit_offset(prog, ie, e, true, true);
+        emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
+        for (int i = 1; i <= n_step; i++) {
+          DevOp op = make_op(OP_QUAD, ie);
+          op.c[0] = step_len;
+          op.c[1] = b1 / e.l;  // k1 * rel_p
+          prog.push_back(op);
+          emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
+        }
+        emit_offset(prog, ie, e, false, true);
+        break;
+      }
+      case BMX_KEY_SEXTUPOLE: case BMX_KEY_OCTUPOLE: case BMX_KEY_THICK_MULTIPOLE: case BMX_KEY_HKICKER:
+      case BMX_KEY_VKICKER: case BMX_KEY_KICKER: case BMX_KEY_INSTRUMENT: case BMX_KEY_MONITOR:
+      case BMX_KEY_ECOLLIMATOR: case BMX_KEY_RCOLLIMATOR: case BMX_KEY_PIPE: {
+        // track_a_thick_multipole.f90:18-133
+        if (key == BMX_KEY_SEXTUPOLE && e.fringe_type != BMX_FRINGE_NONE && e.fringe_at != BMX_NO_END &&
+            (e.fringe_type == BMX_FRINGE_FULL || e.fringe_type == BMX_FRINGE_HARD_EDGE_ONLY))
+          return fail("sextupole hard-edge fringe is not supported on the device path");
+        AB ab;
+        if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
+        int n_step = (ab.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
+        double step_len = e.l / n_step, r_step = 1.0 / n_step;
+        emit_offset(prog, ie, e, true, true);
+        emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
+        for (int i = 1; i <= n_step; i++) {
+          if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
+          emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
+        }
+        emit_offset(prog, ie, e, false, true);
+        break;
+      }
+      case BMX_KEY_SBEND: {
+        // track_a_bend.f90:18-333
+        if (is_misaligned(e) || e.ref_tilt != 0) return fail("sbend misalignment / ref_tilt is not supported on the device path");
+        if (e.fringe_type == BMX_FRINGE_FULL || e.fringe_type == BMX_FRINGE_SAD_FULL || e.fringe_type == BMX_FRINGE_SOFT_EDGE_ONLY)
+          return fail("sbend fringe_type full / sad_full / soft_edge_only is not supported on the device path");
+        auto emit_edge = [&](bool enter) {
+          bool here = e.fringe_type != BMX_FRINGE_NONE &&
+                      (enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
+                             : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS));
+          if (!here) return;
+          double g_tot = e.is_on ? (e.g + e.dg) : 0.0, k1 = e.is_on ? e.k1 : 0.0;
+          double ang = enter ? e.e1 : e.e2;
+          double fint_gap = enter ? e.fint * e.hgap : e.fintx * e.hgapx;
+          if (e.fringe_type == BMX_FRINGE_HARD_EDGE_ONLY) fint_gap = 0;
+          double ce = std::cos(ang), se = std::sin(ang), te = se / ce, sec = 1 / ce;
+          if (e.fringe_type == BMX_FRINGE_LINEAR_EDGE) {
+            double fint = enter ? e.fint : e.fintx, hgap = enter ? e.hgap : e.hgapx;
+            DevOp op = make_op(OP_LINEAR_EDGE, ie);
+            op.c[0] = g_tot * te;
+            op.c[1] = (fint == 0) ? -op.c[0] : -g_tot * std::tan(ang - 2 * fint * g_tot * hgap * (1 + se * se) / ce);
+            prog.push_back(op);
+            return;
+          }
+          DevOp op = make_op(OP_HWANG, ie);
+          if (enter) op.flags |= F_HWANG_ENTER;
+          op.c[0] = g_tot * te;                                   // gt
+          op.c[1] = g_tot * te * te;                              // gt2
+          op.c[2] = g_tot * sec * sec;                            // gs2
+          op.c[3] = k1 * te;                                      // k1_tane
+          op.c[4] = 2 * fint_gap * op.c[2] * g_tot * sec * (1 + se * se);  // fg_factor
+          op.c[5] = g_tot;
+          op.c[6] = te;
+          prog.push_back(op);
+        };
+        emit_edge(true);
+        prog.push_back(make_op(OP_TOO_LARGE, ie));
+        AB ab;
+        double b1;
+        if (!magnetic_ab(e, true, &ab, &b1, &e2)) return fail(e2.c_str());
+        if (std::fabs(b1) < 1e-10) { ab.b[1] = b1; b1 = 0; }
+        double g = e.g, dg;
+        if (e.l == 0) dg = 0;
+        else { dg = ab.b[0] / e.l; ab.b[0] = 0; }
+        double g_tot = g + dg;
+        if (!e.is_on) { dg = -g; g_tot = 0; b1 = 0; }
+        int ix_mag_max = ab.ix_max;
+        int n_step = 1;
+        if (ix_mag_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
+        double r_step = 1.0 / n_step, step_len = e.l / n_step, angle = g * step_len;
+        AB abk = ab;  // kicks run over orders 0..ix_mag_max as computed before dg/b1 were pulled out
+        abk.ix_max = ix_mag_max;
+        if (ix_mag_max > -1) emit_kicks(prog, ie, abk, 0.5 * r_step, OP_BEND_KICK, g);
+        for (int n = 1; n <= n_step; n++) {
+          if (b1 != 0) {
+            DevOp op = make_op(OP_BEND_K1, ie);
+            double k1 = b1 / e.l;
+            op.c[0] = step_len; op.c[1] = g; op.c[2] = g_tot; op.c[3] = k1;
+            prog.push_back(op);
+          } else if ((g == 0 && dg == 0) || step_len == 0) {
+            if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
+          } else {
+            DevOp op = make_op(OP_BEND, ie);
+            double sinc_a = (std::fabs(angle) < 1e-8) ? 1.0 : std::sin(angle) / angle;
+            double s2 = (std::fabs(angle / 2) < 1e-8) ? 1.0 : std::sin(angle / 2) / (angle / 2);
+            double cosc_a = (std::fabs(angle) < 1e-8) ? 0.5 : 0.5 * s2 * s2;
+            double x2 = angle * angle;
+            double sincc_a = (std::fabs(angle) < 0.1)
+                                 ? 1.0 / 6.0 + x2 * (-1.0 / 120.0 + x2 * (1.0 / 5040.0 + x2 * (-1.0 / 362880.0)))
+                                 : (angle - std::sin(angle)) / (angle * x2);
+            op.c[0] = step_len; op.c[1] = g; op.c[2] = g_tot; op.c[3] = angle;
+            op.c[4] = std::cos(angle); op.c[5] = std::sin(angle); op.c[6] = sinc_a; op.c[7] = cosc_a;
+            op.c[8] = sincc_a; op.c[9] = lc->beta0; op.c[10] = dg;
+            prog.push_back(op);
+          }
+          if (ix_mag_max > -1) emit_kicks(prog, ie, abk, (n == n_step ? 0.5 : 1.0) * r_step, OP_BEND_KICK, g);
+        }
+        prog.push_back(make_op(OP_TOO_LARGE, ie));
+        emit_edge(false);
+        break;
+      }
+      case BMX_KEY_RFCAVITY: {
+        // track_a_rfcavity.f90:17-146 (relative time, standing wave, no coupler kick)
+        if (e.coupler_strength != 0) return fail("rfcavity coupler kicks are not supported on the device path");
+        if (e.hkick != 0 || e.vkick != 0) return fail("rfcavity hkick/vkick are not supported on the device path");
+        if (e.rf_frequency == 0 && e.voltage != 0) return fail("rfcavity with zero rf_frequency");
+        AB ab;
+        if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
+        emit_offset(prog, ie, e, true, true);
+        emit_kicks(prog, ie, ab, 0.5, OP_KICK);
+        int n_slice = std::max(1, nint(std::fabs(e.l) / (e.l / (e.num_steps > 0 ? e.num_steps : 1))));
+        if (e.l == 0) n_slice = 1;
+        DevOp op = make_op(OP_RFCAVITY, ie);
+        op.n = n_slice;
+        op.c[0] = e.l;
+        op.c[1] = e.is_on ? e.voltage : 0.0;  // e_accel_field * charge_dir (charge_dir = 1)
+        op.c[2] = e.phi0 + e.phi0_multipass;
+        op.c[3] = e.rf_frequency;
+        op.c[4] = (e.rf_frequency != 0) ? 0.5 / e.rf_frequency : 0.0;
+        prog.push_back(op);
+        emit_kicks(prog, ie, ab, 0.5, OP_KICK);
+        emit_offset(prog, ie, e, false, true);
+        break;
+      }
+      default:
+        return fail("element key not supported");
+    }
+
+    DevOp end = make_op(OP_ELE_END, ie);
+    if (cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS)) {
+      end.flags = F_HAS_APERTURE | ap_flags;
+      ap_consts(end);
+    }
+    prog.push_back(end);
+  }
+  ele_op_begin.push_back((int32_t)prog.size());
+  (void)kCLight;
+  (void)kPi;
+  return true;
+}
+
+}  // namespace bmx

@@ -1,0 +1,853 @@
+// bmx_track.hip -- gfx950 tracking kernels and the libbmx C ABI.
+//
+// One particle per lane.  Each lane keeps its 6-D coordinates, beta and two
+// per-particle invariants of the drift map in registers for the whole launch
+// (all turns, all elements): the structure-of-arrays store in HBM is read once
+// and written once per launch.  The element program (bmx_internal.h) is walked
+// with a wave-uniform index, so op constants come through the scalar cache and
+// the op dispatch is a scalar branch.  Lost lanes leave the loop early and
+// record Bmad's loss state, element and turn.
+//
+// The arithmetic of every op restates the reference routine cited at it in
+// FP64, with particle-independent factors hoisted by the lattice compiler and
+// divisions shared where the algebra allows (1/(1+pz) and beta/beta0 - 1 are
+// per-particle constants between energy kicks).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bmx.h"
+#include "bmx_internal.h"
+
+namespace bmx {
+bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, const bmx_config& cfg,
+                     std::vector<DevOp>& prog, std::vector<int32_t>& ele_op_begin, LatticeConsts* lc,
+                     std::string* err);
+}
+
+using namespace bmx;
+
+#define D_C_LIGHT 2.99792458e8
+#define D_TWOPI 6.283185307179586476925286766559
+#define D_PI 3.141592653589793238462643383279
+
+// ---------------------------------------------------------------------------
+// per-lane particle state
+// ---------------------------------------------------------------------------
+struct Lane {
+  double x, px, y, py, z, pz, beta;
+  double irp;  // 1 / (1 + pz)
+  double bb;   // beta/beta0 - 1 as track_a_drift forms it (track_a_drift.f90:68-69)
+  int state;
+};
+
+// sqrt_one (sim_utils/special_functions/sqrt_one.f90:18-41)
+__device__ __forceinline__ double d_sqrt_one(double x) { return x / (sqrt(1.0 + x) + 1.0); }
+
+// beta/beta0 - 1 = sqrt_one(mc2^2 (2 pz + pz^2) / ((p0c (1+pz))^2 + mc2^2))
+__device__ __forceinline__ void refresh_invariants(Lane& L, const LatticeConsts& lc) {
+  double rel = 1.0 + L.pz;
+  L.irp = 1.0 / rel;
+  double pc = lc.p0c * rel;
+  double mc2sq = lc.mc2 * lc.mc2;
+  L.bb = d_sqrt_one((mc2sq * (2.0 * L.pz + L.pz * L.pz)) / (pc * pc + mc2sq));
+}
+
+// modulo2 (sim_utils/special_functions/modulo2_mod.f90:85-100)
+__device__ __forceinline__ double d_modulo2(double x, double amp) {
+  double p = 2.0 * amp;
+  double m = fmod(x, p);
+  if (m != 0.0 && m < 0.0) m += p;
+  if (m >= amp) m -= p;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// ops
+// ---------------------------------------------------------------------------
+
+// track_a_drift (bmad/low_level/track_a_drift.f90:25-116), forward, with reference motion.
+// One division: t = 1/(ps (ps+1)) gives 1/ps = (ps+1) t and sqrt_one(-pxy2)/ps = -pxy2 t.
+__device__ __forceinline__ void op_drift(Lane& L, double len) {
+  double pxr = L.px * L.irp, pyr = L.py * L.irp;
+  double pxy2 = pxr * pxr + pyr * pyr;
+  if (pxy2 >= 1.0) { L.state = BMX_LOST_PZ; return; }
+  double ps = sqrt(1.0 - pxy2);
+  double t = 1.0 / (ps * (ps + 1.0));
+  double lps = len * ((ps + 1.0) * t);
+  L.x += lps * pxr;
+  L.y += lps * pyr;
+  L.z += len * (L.bb - pxy2 * t);
+}
+
+// drift with include_ref_motion = false (offset_particle's drift to the edge)
+__device__ __forceinline__ void op_drift_noref(Lane& L, double len) {
+  double pxr = L.px * L.irp, pyr = L.py * L.irp;
+  double pxy2 = pxr * pxr + pyr * pyr;
+  if (pxy2 >= 1.0) { L.state = BMX_LOST_PZ; return; }
+  double ips = 1.0 / sqrt(1.0 - pxy2);
+  L.x += len * pxr * ips;
+  L.y += len * pyr * ips;
+  L.z += -len * ips;
+}
+
+// Thin multipole kick, ab form (bmad/modules/multipole_mod.f90:82-174, :314-401):
+// K = sum_n c_n (x + i y)^n, dpx = -Re K, dpy = Im K, Horner in complex FP64.
+__device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, double& kr, double& ki) {
+  int nh = op.flags;  // Horner steps
+  kr = op.c[2 * nh];
+  ki = op.c[2 * nh + 1];
+  for (int j = nh - 1; j >= 0; j--) {
+    double tr = kr * x - ki * y + op.c[2 * j];
+    double ti = kr * y + ki * x + op.c[2 * j + 1];
+    kr = tr;
+    ki = ti;
+  }
+  int lo = op.n;  // multiply by (x + i y)^lo
+  for (int j = 0; j < lo; j++) {
+    double tr = kr * x - ki * y;
+    double ti = kr * y + ki * x;
+    kr = tr;
+    ki = ti;
+  }
+}
+
+__device__ __forceinline__ void op_kick(Lane& L, const DevOp& op) {
+  double kr, ki;
+  kick_eval(op, L.x, L.y, kr, ki);
+  L.px -= kr;
+  L.py += ki;
+}
+
+// apply_multipole_kicks of track_a_bend (track_a_bend.f90:338-471): kick scaled by (1 + g x).
+__device__ __forceinline__ void op_bend_kick(Lane& L, const DevOp& op) {
+  double kr, ki;
+  kick_eval(op, L.x, L.y, kr, ki);
+  double f = 1.0 + op.c[13] * L.x;
+  L.px -= f * kr;
+  L.py += f * ki;
+}
+
+// n = 0 kt kick of a multipole element with ref_orb_offset (multipole_mod.f90:249-255)
+__device__ __forceinline__ void op_kick_ref0(Lane& L, const DevOp& op) {
+  double kc = op.c[0], ks = op.c[1];
+  L.px += kc * L.pz;
+  L.py += ks * L.pz;
+  L.z -= kc * L.x + ks * L.y;
+}
+
+// low_energy_z_correction (bmad/low_level/low_energy_z_correction.f90:22-58)
+__device__ __forceinline__ double low_energy_dz(const Lane& L, const LatticeConsts& lc, double ds) {
+  double pz = L.pz, b0 = lc.beta0;
+  if (lc.mc2 * (b0 * pz) * (b0 * pz) < 3e-7 * lc.e_tot) {
+    double me = lc.mc2 / lc.e_tot;
+    double f = b0 * b0 * (2.0 * b0 * b0 - me * me / 2.0);
+    return ds * pz * (1.0 - 3.0 * (pz * b0 * b0) / 2.0 + pz * pz * f) * (me * me);
+  }
+  return ds * (L.beta - b0) / b0;
+}
+
+// One quadrupole body step: quad_mat2_calc (bmad/low_level/quad_mat2_calc.f90:22-81)
+// for both planes plus the quadratic z terms (track_a_quadrupole.f90:82-112).
+__device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeConsts& lc) {
+  double len = op.c[0];
+  double rel_p = 1.0 + L.pz;
+  double k1 = op.c[1] * L.irp;  // charge_dir b1 / (L rel_p)
+  double kx = -k1, ky = k1;     // quad_mat2_calc(-k1, ...) for x, quad_mat2_calc(k1, ...) for y
+  double sqrt_k = sqrt(fabs(k1));
+  double skl = sqrt_k * len;
+  double cx, sx, cy, sy;
+  if (fabs(skl) < 1e-10) {
+    cx = 1.0 + kx * len * len / 2.0;
+    sx = (1.0 + kx * len * len / 6.0) * len;
+    cy = 1.0 + ky * len * len / 2.0;
+    sy = (1.0 + ky * len * len / 6.0) * len;
+  } else {
+    double sn, cs;
+    sincos(skl, &sn, &cs);
+    double em1 = expm1(skl);  // cosh, sinh without cancellation at small skl
+    double ch = 1.0 + em1 * em1 / (2.0 * (em1 + 1.0));
+    double sh = 0.5 * (em1 + em1 / (em1 + 1.0));
+    double isk = 1.0 / sqrt_k;
+    if (k1 > 0.0) { cx = cs; sx = sn * isk; cy = ch; sy = sh * isk; }  // x focusing
+    else          { cx = ch; sx = sh * isk; cy = cs; sy = sn * isk; }  // y focusing
+  }
+  double irp = L.irp, irp2 = irp * irp;
+  double zx0 = kx * (-cx * sx + len) / 4.0, zx1 = -kx * sx * sx * irp / 2.0, zx2 = -(cx * sx + len) * irp2 / 4.0;
+  double zy0 = ky * (-cy * sy + len) / 4.0, zy1 = -ky * sy * sy * irp / 2.0, zy2 = -(cy * sy + len) * irp2 / 4.0;
+  double x = L.x, px = L.px, y = L.y, py = L.py;
+  L.z += zx0 * x * x + zx1 * x * px + zx2 * px * px + zy0 * y * y + zy1 * y * py + zy2 * py * py;
+  L.x = cx * x + (sx * irp) * px;
+  L.px = (kx * sx * rel_p) * x + cx * px;
+  L.y = cy * y + (sy * irp) * py;
+  L.py = (ky * sy * rel_p) * y + cy * py;
+  L.z += low_energy_dz(L, lc, len);
+}
+
+// sinc (sim_utils/special_functions/sinc.f90)
+__device__ __forceinline__ double d_sinc(double x) { return (fabs(x) < 1e-8) ? 1.0 : sin(x) / x; }
+
+// Exact sector-bend body step (track_a_bend.f90:120-209).
+// c: 0 step_len, 1 g, 2 g_tot, 3 angle, 4 cos a, 5 sin a, 6 sinc a, 7 cosc a, 8 sincc a, 9 beta_ref, 10 dg
+__device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeConsts& lc) {
+  double sl = op.c[0], g = op.c[1], g_tot = op.c[2], angle = op.c[3];
+  double cos_a = op.c[4], sin_a = op.c[5], sinc_a = op.c[6], cosc_a = op.c[7];
+  double x = L.x, px = L.px, y = L.y, py = L.py, z = L.z, pz = L.pz;
+  double rel_p = 1.0 + pz;
+  if (op.c[10] == 0.0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 && fabs(pz) < 1e-9) {
+    double pc = rel_p * lc.p0c;
+    double gam2 = lc.mc2 * lc.mc2 / (pc * pc + lc.mc2 * lc.mc2);
+    double m56 = sl * (gam2 - (g * sl) * (g * sl) * op.c[8]);
+    L.x = cos_a * x + sl * sinc_a * px + g * sl * sl * cosc_a * pz;
+    L.px = -g * sin_a * x + cos_a * px + g * sl * sinc_a * pz;
+    L.y = y + sl * py;
+    L.z = -g * sl * sinc_a * x - g * sl * sl * cosc_a * px + z + m56 * pz;
+    return;
+  }
+  double pt = sqrt(rel_p * rel_p - py * py);
+  if (fabs(px) > pt) { L.state = BMX_LOST_PZ; return; }
+  double ipt = 1.0 / pt;
+  double g_p = g_tot * ipt;
+  double phi_1 = asin(px * ipt);
+  double sin_plus, cos_plus;
+  sincos(angle + phi_1, &sin_plus, &cos_plus);
+  double q = (1.0 + g * x) * sl * sinc_a;
+  double alpha = 2.0 * (1.0 + g * x) * sin_plus * sl * sinc_a - g_p * q * q;
+  double r = cos_plus * cos_plus + g_p * alpha;
+  if (r < 0.0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
+    L.state = BMX_LOST;
+    L.x = 2.0 * lc.max_aperture_limit;
+    return;
+  }
+  double rad = sqrt(r);
+  double xi;
+  if (cos_plus > 0.0) xi = alpha / (rad + cos_plus);
+  else xi = (rad - cos_plus) / g_p;
+  double xn = x * cos_a - sl * sl * g * cosc_a + xi;
+  L.x = xn;
+  if (fabs(xn) > lc.max_aperture_limit) { L.state = BMX_LOST; return; }
+  double L_u = xi;
+  double L_v = -(sl * sinc_a + x * sin_a);
+  double L_c = sqrt(L_v * L_v + L_u * L_u);
+  double angle_p = 2.0 * (angle + phi_1 - atan2(L_u, -L_v));
+  double L_p = L_c / d_sinc(angle_p / 2.0);
+  L.px = pt * sin(phi_1 + angle - angle_p);
+  L.y = y + py * L_p * ipt;
+  L.z = z + L.beta * sl / op.c[9] - rel_p * L_p * ipt;
+}
+
+// sbend_body_with_k1_map (track_a_bend.f90:500-659), forward, orientation +1.
+// c: 0 step_len, 1 g, 2 g_tot, 3 k1
+__device__ void op_bend_k1(Lane& L, const DevOp& op, const LatticeConsts& lc) {
+  double length = op.c[0], g = op.c[1], g_tot = op.c[2], k1 = op.c[3];
+  double rel_p = 1.0 + L.pz, rel_p2 = rel_p * rel_p;
+  double k_x = k1 + g * g_tot;
+  double x_c = (g * rel_p - g_tot) / k_x;
+  double om_x = sqrt(fabs(k_x) / rel_p), om_y = sqrt(fabs(k1) / rel_p);
+  double tau_x = (k_x >= 0.0) ? -1.0 : 1.0, tau_y = (k1 >= 0.0) ? 1.0 : -1.0;
+  double arg = om_x * length, s_x, c_x, z2, s_y, c_y;
+  if (arg < 1e-6) {
+    s_x = (1.0 + tau_x * arg * arg / 6.0) * length;
+    c_x = 1.0 + tau_x * arg * arg / 2.0;
+    z2 = g * length * length / (2.0 * rel_p);
+  } else if (k_x > 0.0) {
+    s_x = sin(arg) / om_x;
+    c_x = cos(arg);
+    z2 = tau_x * g * (1.0 - c_x) / (rel_p * om_x * om_x);
+  } else {
+    s_x = sinh(arg) / om_x;
+    c_x = cosh(arg);
+    z2 = tau_x * g * (1.0 - c_x) / (rel_p * om_x * om_x);
+  }
+  arg = om_y * length;
+  if (arg < 1e-6) {
+    s_y = (1.0 + tau_y * arg * arg / 6.0) * length;
+    c_y = 1.0 + tau_y * arg * arg / 2.0;
+  } else if (k1 < 0.0) {
+    s_y = sin(arg) / om_y;
+    c_y = cos(arg);
+  } else {
+    s_y = sinh(arg) / om_y;
+    c_y = cosh(arg);
+  }
+  double r0 = L.x - x_c, r1 = L.px, r2 = L.y, r3 = L.py, r4 = L.z;
+  double z0 = -g * x_c * length, z1 = -g * s_x;
+  double z11 = tau_x * om_x * om_x * (length - c_x * s_x) / 4.0;
+  double z12 = -tau_x * om_x * om_x * s_x * s_x / (2.0 * rel_p);
+  double z22 = -(length + c_x * s_x) / (4.0 * rel_p2);
+  double z33 = tau_y * om_y * om_y * (length - c_y * s_y) / 4.0;
+  double z34 = -tau_y * om_y * om_y * s_y * s_y / (2.0 * rel_p);
+  double z44 = -(length + c_y * s_y) / (4.0 * rel_p2);
+  L.x = c_x * r0 + s_x * r1 / rel_p + x_c;
+  L.px = tau_x * om_x * om_x * rel_p * s_x * r0 + c_x * r1;
+  L.y = c_y * r2 + s_y * r3 / rel_p;
+  L.py = tau_y * om_y * om_y * rel_p * s_y * r2 + c_y * r3;
+  L.z = r4 + z0 + z1 * r0 + z2 * r1 + z11 * r0 * r0 + z12 * r0 * r1 + z22 * r1 * r1 + z33 * r2 * r2 + z34 * r2 * r3 +
+        z44 * r3 * r3;
+  L.z += low_energy_dz(L, lc, length);
+}
+
+// hwang_bend_edge_kick (bmad/modules/fringe_mod.f90:240-418), time_dir = +1.
+// c: 0 gt, 1 gt2, 2 gs2, 3 k1_tane, 4 fg_factor, 5 g_tot, 6 tan_e
+__device__ __forceinline__ void op_hwang(Lane& L, const DevOp& op) {
+  double gt = op.c[0], gt2 = op.c[1], gs2 = op.c[2], k1t = op.c[3], fg = op.c[4], g_tot = op.c[5], te = op.c[6];
+  double ef = L.irp;
+  double v0 = L.x, v1 = L.px, v2 = L.y, v3 = L.py;
+  double dx, dpx, dy, dpy, dz;
+  if (op.flags & F_HWANG_ENTER) {
+    dx = (-gt2 * v0 * v0 + gs2 * v2 * v2) * ef / 2.0;
+    dpx = (gt * g_tot * (1.0 + 2.0 * te * te) * v2 * v2 / 2.0 + gt2 * (v0 * v1 - v2 * v3) + k1t * (v0 * v0 - v2 * v2)) * ef;
+    dy = gt2 * v0 * v2 * ef;
+    dpy = (fg * v2 - gt2 * v0 * v3 - (g_tot + gt2) * v1 * v2 - 2.0 * k1t * v0 * v2) * ef;
+    dz = ef * ef * 0.5 *
+         (v2 * v2 * fg + v0 * v0 * v0 * (4.0 * k1t - gt * gt2) / 6.0 + 0.5 * v0 * v2 * v2 * (-4.0 * k1t + gt * gs2) +
+          (v0 * v0 * v1 - 2.0 * v0 * v2 * v3) * gt2 - v1 * v2 * v2 * gs2);
+  } else {
+    dx = (gt2 * v0 * v0 - gs2 * v2 * v2) * ef / 2.0;
+    dpx = (gt2 * (v2 * v3 - v0 * v1) + k1t * (v0 * v0 - v2 * v2) - gt * gt2 * (v0 * v0 + v2 * v2) / 2.0) * ef;
+    dy = -gt2 * v0 * v2 * ef;
+    dpy = (fg * v2 + gt2 * v0 * v3 + (g_tot + gt2) * v1 * v2 + (gt * gs2 - 2.0 * k1t) * v0 * v2) * ef;
+    dz = ef * ef * 0.5 *
+         (v2 * v2 * fg + v0 * v0 * v0 * (4.0 * k1t - gt * gt2) / 6.0 + 0.5 * v0 * v2 * v2 * (-4.0 * k1t + gt * gs2) -
+          (v0 * v0 * v1 - 2.0 * v0 * v2 * v3) * gt2 + v1 * v2 * v2 * gs2);
+  }
+  L.x = v0 + dx;
+  L.px = v1 + (dpx + gt * v0);
+  L.y = v2 + dy;
+  L.py = v3 + (dpy - gt * v2);
+  L.z += dz;
+}
+
+// linear_bend_edge_kick (fringe_mod.f90:129-190): c0 = ht_x, c1 = ht_y
+__device__ __forceinline__ void op_linear_edge(Lane& L, const DevOp& op) {
+  L.px += op.c[0] * L.x;
+  L.py += op.c[1] * L.y;
+}
+
+// tilt_coords (bmad/code/tilt_coords.f90)
+__device__ __forceinline__ void op_tilt(Lane& L, const DevOp& op) {
+  double c = op.c[0], s = op.c[1];
+  double x = L.x, y = L.y, px = L.px, py = L.py;
+  L.x = c * x + s * y;
+  L.y = -s * x + c * y;
+  L.px = c * px + s * py;
+  L.py = -s * px + c * py;
+}
+
+// offset_particle for a straight element, set (offset_particle.f90:170-200, 255-282)
+// and unset (:410-470); c: 0..2 offsets, 3 L/2, 4..12 W (row-major).
+__device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set) {
+  double rel_p = 1.0 + L.pz;
+  double r0, r1, r2;
+  if (set) {
+    r0 = L.x - op.c[0];
+    r1 = L.y - op.c[1];
+    r2 = 0.0 - (op.c[2] + op.c[3]);
+  } else {
+    r0 = L.x;
+    r1 = L.y;
+    r2 = 0.0 + op.c[3];  // r(3) - ds_center with ds_center = -L/2
+  }
+  double s_pos;  // s_body (set) or s_lab (unset)
+  if (op.flags & F_OFFSET_PITCH) {
+    const double* w = &op.c[4];
+    double n0 = w[0] * r0 + w[1] * r1 + w[2] * r2;
+    double n1 = w[3] * r0 + w[4] * r1 + w[5] * r2;
+    double n2 = w[6] * r0 + w[7] * r1 + w[8] * r2;
+    if (set) { r0 = n0; r1 = n1; r2 = n2 + op.c[3]; }
+    else { r0 = n0 + op.c[0]; r1 = n1 + op.c[1]; r2 = n2 + (op.c[2] + op.c[3]); }
+    double pzz = rel_p * rel_p - L.px * L.px - L.py * L.py;
+    if (pzz <= 0.0) { L.state = BMX_LOST_PZ; return; }
+    double p2 = sqrt(pzz);
+    double q0 = w[0] * L.px + w[1] * L.py + w[2] * p2;
+    double q1 = w[3] * L.px + w[4] * L.py + w[5] * p2;
+    L.px = q0;
+    L.py = q1;
+  } else {
+    if (set) { r2 = r2 + op.c[3]; }
+    else { r0 = r0 + op.c[0]; r1 = r1 + op.c[1]; r2 = r2 + (op.c[2] + op.c[3]); }
+    double pzz = rel_p * rel_p - L.px * L.px - L.py * L.py;
+    if (pzz <= 0.0) { L.state = BMX_LOST_PZ; return; }
+  }
+  L.x = r0;
+  L.y = r1;
+  s_pos = r2;
+  double s_target = set ? 0.0 : 2.0 * op.c[3];
+  double ds = s_target - s_pos;
+  if (fabs(ds) > 1e-14) op_drift_noref(L, ds);
+}
+
+// apply_energy_kick (bmad/low_level/apply_energy_kick.f90:23-74)
+__device__ __forceinline__ void energy_kick(Lane& L, double dE, const LatticeConsts& lc) {
+  double pc = (1.0 + L.pz) * lc.p0c;
+  double beta_old = L.beta;
+  double E_old = pc / L.beta;
+  double E_new = E_old + dE;
+  if (E_new < lc.mc2) {
+    L.pz = -1.0;
+    L.beta = 0.0;
+    L.state = BMX_LOST_PZ;
+    return;
+  }
+  L.pz = L.pz + (1.0 + L.pz) * d_sqrt_one((2.0 * E_old * dE + dE * dE) / (pc * pc));
+  double pc_new = lc.p0c * (1.0 + L.pz);
+  double beta_new = pc_new / E_new;
+  L.z = L.z * beta_new / beta_old;
+  L.beta = beta_new;
+  refresh_invariants(L, lc);
+}
+
+// track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140), relative time:
+// particle_rf_time = -z/(beta c) (bmad/code/particle_rf_time.f90:89).
+// c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f); n = n_slice
+__device__ void op_rfcavity(Lane& L, const DevOp& op, const LatticeConsts& lc) {
+  double len = op.c[0], volt = op.c[1], freq = op.c[3], half_t = op.c[4];
+  int n_slice = op.n;
+  double time = -L.z / (L.beta * D_C_LIGHT);
+  if (freq != 0.0 && !(time >= -half_t && time < half_t)) time = d_modulo2(time, half_t);
+  double phase = d_modulo2(D_TWOPI * (op.c[2] - time * freq), D_PI);
+  double dl = len / n_slice;
+  for (int i = 0; i <= n_slice; i++) {
+    double dE = volt * sin(phase) / n_slice;
+    if (i == 0 || i == n_slice) dE = dE / 2.0;
+    energy_kick(L, dE, lc);
+    if (L.state != BMX_ALIVE) return;
+    if (L.pz == -1.0) { L.state = BMX_LOST_PZ; return; }
+    if (i != n_slice && dl != 0.0) {
+      double z0 = L.z;
+      op_drift(L, dl);
+      if (L.state != BMX_ALIVE) return;
+      phase = phase + D_TWOPI * freq * (L.z - z0) / (D_C_LIGHT * L.beta);
+    }
+  }
+}
+
+// orbit_too_large (bmad/low_level/orbit_too_large.f90:26-120)
+__device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
+  double mal = lc.max_aperture_limit;
+  if (L.x > mal) { L.state = BMX_LOST_POS_X; return true; }
+  if (-L.x > mal) { L.state = BMX_LOST_NEG_X; return true; }
+  if (L.y > mal) { L.state = BMX_LOST_POS_Y; return true; }
+  if (-L.y > mal) { L.state = BMX_LOST_NEG_Y; return true; }
+  double rel_p = 1.0 + L.pz;
+  if (rel_p < 0.0) { L.state = BMX_LOST_PZ; return true; }
+  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
+  if (f > 0.0) {
+    if (fabs(L.px) > fabs(L.py)) L.state = (L.px > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
+    else L.state = (L.py > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
+    return true;
+  }
+  return false;
+}
+
+// aperture part of check_aperture_limit (bmad/low_level/check_aperture_limit.f90:257-309)
+// c: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim, 4 wx, 5 cx, 6 wy, 7 cy
+__device__ __forceinline__ void aperture(Lane& L, const DevOp& op) {
+  if (op.flags & F_APERTURE_ELLIPTICAL) {
+    double xp = L.x - op.c[5], yp = L.y - op.c[7];
+    double ax = xp / op.c[4], ay = yp / op.c[6];
+    double r = ax * ax + ay * ay;
+    if (r < 1.0) return;
+    if (fabs(ax) > fabs(ay)) L.state = (xp > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
+    else L.state = (yp > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
+    return;
+  }
+  double x1 = op.c[0], x2 = op.c[1], y1 = op.c[2], y2 = op.c[3], uf = 0.0, f;
+  if (x1 != 0.0 && L.x < x1) { f = fabs((L.x - x1) / (x2 - x1)); if (f > uf) { L.state = BMX_LOST_NEG_X; uf = f; } }
+  if (x2 != 0.0 && L.x > x2) { f = fabs((L.x - x2) / (x2 - x1)); if (f > uf) { L.state = BMX_LOST_POS_X; uf = f; } }
+  if (y1 != 0.0 && L.y < y1) { f = fabs((L.y - y1) / (y2 - y1)); if (f > uf) { L.state = BMX_LOST_NEG_Y; uf = f; } }
+  if (y2 != 0.0 && L.y > y2) { f = fabs((L.y - y2) / (y2 - y1)); if (f > uf) { L.state = BMX_LOST_POS_Y; uf = f; } }
+}
+
+// ---------------------------------------------------------------------------
+// one element-program pass for one lane; returns false when the lane is lost
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool run_ops(Lane& L, const DevOp* __restrict__ ops, int lo, int hi,
+                                        const LatticeConsts& lc, int& ie_lost) {
+  for (int k = lo; k < hi; k++) {
+    const DevOp& op = ops[k];
+    switch (op.type) {
+      case OP_DRIFT: op_drift(L, op.c[0]); break;
+      case OP_KICK: op_kick(L, op); break;
+      case OP_KICK_REF0: op_kick_ref0(L, op); break;
+      case OP_QUAD: op_quad(L, op, lc); break;
+      case OP_BEND: op_bend(L, op, lc); break;
+      case OP_BEND_K1: op_bend_k1(L, op, lc); break;
+      case OP_BEND_KICK: op_bend_kick(L, op); break;
+      case OP_HWANG: op_hwang(L, op); break;
+      case OP_LINEAR_EDGE: op_linear_edge(L, op); break;
+      case OP_TILT: op_tilt(L, op); break;
+      case OP_OFFSET_SET: op_offset(L, op, true); break;
+      case OP_OFFSET_UNSET: op_offset(L, op, false); break;
+      case OP_TOO_LARGE: too_large(L, lc); break;
+      case OP_RFCAVITY: op_rfcavity(L, op, lc); break;
+      case OP_APERTURE_IN: aperture(L, op); break;
+      case OP_ELE_END:
+        if (L.state == BMX_ALIVE && !too_large(L, lc) && (op.flags & F_HAS_APERTURE)) aperture(L, op);
+        break;
+      default: break;
+    }
+    if (L.state != BMX_ALIVE) {
+      ie_lost = op.ie;
+      return false;
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct SoA {
+  double *x, *px, *y, *py, *z, *pz, *beta;
+  int32_t *state, *ix_lost, *turn_lost;
+};
+
+// Multi-turn bunch tracking: segments [lo1, hi1) then [lo2, hi2) per turn.
+__global__ __launch_bounds__(256) void bmx_track_bunch_kernel(const DevOp* __restrict__ ops, int lo1, int hi1,
+                                                              int lo2, int hi2, LatticeConsts lc, int n_turns,
+                                                              SoA p, int64_t n, unsigned long long* __restrict__ n_lost) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Lane L;
+  L.state = p.state[i];
+  if (L.state != BMX_ALIVE) return;
+  L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
+  refresh_invariants(L, lc);
+  int ie_lost = -1, turn = 0;
+  bool alive = !too_large(L, lc);
+  if (!alive) ie_lost = ops[lo1].ie;
+  for (turn = 0; alive && turn < n_turns; turn++) {
+    alive = run_ops(L, ops, lo1, hi1, lc, ie_lost);
+    if (alive && hi2 > lo2) alive = run_ops(L, ops, lo2, hi2, lc, ie_lost);
+    if (!alive) break;
+  }
+  p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
+  p.state[i] = L.state;
+  if (L.state != BMX_ALIVE) {
+    if (p.ix_lost) p.ix_lost[i] = ie_lost;
+    if (p.turn_lost) p.turn_lost[i] = turn;
+    atomicAdd(n_lost, 1ull);
+  }
+}
+
+// Batched grid tracking: blockIdx.y = variant; each lane builds its initial
+// condition from the polar grid, tracks n_turns full turns of its variant and
+// writes the number of surviving turns.
+struct GridDev {
+  int n_angle, n_amp, n_pz, n_ic;
+  double amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min, angle_max;
+};
+
+__global__ __launch_bounds__(256) void bmx_track_grid_kernel(const DevOp* __restrict__ ops,
+                                                             const int32_t* __restrict__ var_op_begin,
+                                                             LatticeConsts lc, GridDev g, int n_turns,
+                                                             int32_t* __restrict__ survival) {
+  int v = blockIdx.y;
+  int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ic >= g.n_ic) return;
+  // ic = (i_pz * n_angle + i_angle) * n_amp + i_amp
+  int i_amp = ic % g.n_amp;
+  int i_ang = (ic / g.n_amp) % g.n_angle;
+  int i_pz = ic / (g.n_amp * g.n_angle);
+  double amp = g.amp_max_x * (double)(i_amp + 1) / (double)g.n_amp;
+  double th = (g.n_angle > 1) ? g.angle_min + (g.angle_max - g.angle_min) * (double)i_ang / (double)(g.n_angle - 1)
+                              : g.angle_min;
+  double pz0 = (g.n_pz > 1) ? g.pz_min + (g.pz_max - g.pz_min) * (double)i_pz / (double)(g.n_pz - 1) : g.pz_min;
+  Lane L;
+  L.x = amp * cos(th);
+  L.y = amp * g.amp_ratio_y * sin(th);
+  L.px = 0.0; L.py = 0.0; L.z = 0.0; L.pz = pz0;
+  {
+    double pc = lc.p0c * (1.0 + pz0);
+    L.beta = (pz0 == 0.0) ? lc.beta0 : pc / sqrt(pc * pc + lc.mc2 * lc.mc2);
+  }
+  L.state = BMX_ALIVE;
+  refresh_invariants(L, lc);
+  int lo = var_op_begin[v], hi = var_op_begin[v + 1];
+  int ie_lost = -1, turn = 0;
+  bool alive = !too_large(L, lc);
+  for (turn = 0; alive && turn < n_turns; turn++) {
+    alive = run_ops(L, ops, lo, hi, lc, ie_lost);
+    if (!alive) break;
+  }
+  survival[(int64_t)v * g.n_ic + ic] = alive ? n_turns : turn;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct bmx_ctx {
+  int device;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  std::string err;
+};
+
+struct bmx_lattice {
+  int device;
+  int n_variants;
+  std::vector<int32_t> var_op_begin;            // n_variants + 1
+  std::vector<std::vector<int32_t>> ele_op_begin;  // per variant, n_ele + 1 (absolute op indices)
+  std::vector<int32_t> var_n_ele;
+  DevOp* d_ops;
+  int32_t* d_var_op_begin;
+  int64_t n_ops;
+  LatticeConsts lc;
+};
+
+static int set_err(bmx_ctx* ctx, const std::string& s) {
+  if (ctx) ctx->err = s;
+  return -1;
+}
+
+#define HIP_TRY(ctx, call)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return set_err((ctx), std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" {
+
+int32_t bmx_abi_version(void) { return BMX_ABI_VERSION; }
+
+int32_t bmx_sizeof(int32_t which) {
+  switch (which) {
+    case 0: return (int32_t)sizeof(bmx_ele_rec);
+    case 1: return (int32_t)sizeof(bmx_particles);
+    case 2: return (int32_t)sizeof(bmx_stats);
+    case 3: return (int32_t)sizeof(bmx_grid);
+    case 4: return (int32_t)sizeof(bmx_config);
+  }
+  return -1;
+}
+
+int32_t bmx_ctx_create(bmx_ctx** out, int32_t device, uint32_t flags) {
+  (void)flags;
+  if (!out) return -1;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -2;
+  if (device < 0 || device >= n) return -3;
+  if (hipSetDevice(device) != hipSuccess) return -4;
+  bmx_ctx* c = new bmx_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return -5;
+  }
+  *out = c;
+  return 0;
+}
+
+void bmx_ctx_destroy(bmx_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipEventDestroy(ctx->ev0);
+  hipEventDestroy(ctx->ev1);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* bmx_last_error(const bmx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_total, const int32_t* variant_offsets,
+                           int32_t n_variants, const bmx_ref* ref, const bmx_config* cfg, bmx_lattice** out) {
+  if (!ctx || !eles || !ref || !out || n_ele_total <= 0 || n_variants <= 0) return set_err(ctx, "invalid arguments");
+  *out = nullptr;
+  bmx_config c = cfg ? *cfg : bmx_config{1, 0, 1e3};
+  std::vector<int32_t> offs;
+  if (variant_offsets) offs.assign(variant_offsets, variant_offsets + n_variants + 1);
+  else if (n_variants == 1) offs = {0, n_ele_total};
+  else return set_err(ctx, "variant_offsets is required for n_variants > 1");
+  for (int v = 0; v < n_variants; v++)
+    if (offs[v] < 0 || offs[v + 1] <= offs[v] || offs[v + 1] > n_ele_total) return set_err(ctx, "bad variant_offsets");
+  bmx_lattice* lat = new bmx_lattice();
+  lat->device = ctx->device;
+  lat->n_variants = n_variants;
+  std::vector<DevOp> prog;
+  for (int v = 0; v < n_variants; v++) {
+    lat->var_op_begin.push_back((int32_t)prog.size());
+    std::vector<int32_t> eb;
+    LatticeConsts lc;
+    std::memset(&lc, 0, sizeof(lc));
+    std::string err;
+    if (!compile_variant(eles + offs[v], offs[v + 1] - offs[v], *ref, c, prog, eb, &lc, &err)) {
+      delete lat;
+      return set_err(ctx, "variant " + std::to_string(v) + ": " + err);
+    }
+    lc.max_aperture_limit = c.max_aperture_limit;
+    lc.aperture_limit_on = c.aperture_limit_on;
+    if (v == 0) lat->lc = lc;
+    else if (lc.p0c != lat->lc.p0c || lc.mc2 != lat->lc.mc2) {
+      delete lat;
+      return set_err(ctx, "all variants must share the reference energy and species");
+    }
+    lat->ele_op_begin.push_back(eb);
+    lat->var_n_ele.push_back(offs[v + 1] - offs[v]);
+  }
+  lat->var_op_begin.push_back((int32_t)prog.size());
+  lat->n_ops = (int64_t)prog.size();
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (hipMalloc(&lat->d_ops, prog.size() * sizeof(DevOp)) != hipSuccess ||
+      hipMalloc(&lat->d_var_op_begin, lat->var_op_begin.size() * sizeof(int32_t)) != hipSuccess) {
+    delete lat;
+    return set_err(ctx, "hipMalloc of the element program failed");
+  }
+  HIP_TRY(ctx, hipMemcpy(lat->d_ops, prog.data(), prog.size() * sizeof(DevOp), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(lat->d_var_op_begin, lat->var_op_begin.data(), lat->var_op_begin.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  *out = lat;
+  return 0;
+}
+
+void bmx_lattice_destroy(bmx_lattice* lat) {
+  if (!lat) return;
+  hipSetDevice(lat->device);
+  hipFree(lat->d_ops);
+  hipFree(lat->d_var_op_begin);
+  delete lat;
+}
+
+int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
+                        int32_t direction, bmx_particles* p, bmx_stats* out) {
+  if (!ctx || !lat || !p) return set_err(ctx, "invalid arguments");
+  if (direction != 1) return set_err(ctx, "only direction = +1 is supported on the device path");
+  if (n_turns < 0) return set_err(ctx, "n_turns < 0");
+  if (!p->x || !p->px || !p->y || !p->py || !p->z || !p->pz || !p->beta || !p->state)
+    return set_err(ctx, "particle arrays x..pz, beta, state are required");
+  int n_ele = lat->var_n_ele[0];
+  if (ix_start < 0 || ix_start > n_ele || ix_end < 0 || ix_end > n_ele) return set_err(ctx, "ix_start/ix_end out of range");
+  const std::vector<int32_t>& eb = lat->ele_op_begin[0];
+  // Bmad element numbering: elements ix_start+1 .. ix_end are array entries ix_start .. ix_end-1.
+  int lo1, hi1, lo2 = 0, hi2 = 0;
+  if (ix_end > ix_start) { lo1 = eb[ix_start]; hi1 = eb[ix_end]; }
+  else { lo1 = eb[ix_start]; hi1 = eb[n_ele]; lo2 = eb[0]; hi2 = eb[ix_end]; }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int64_t n = p->n;
+  if (out) std::memset(out, 0, sizeof(*out));
+  if (n <= 0 || n_turns == 0) return 0;
+
+  SoA s;
+  std::vector<void*> owned;
+  auto dev_arr = [&](void* host, size_t bytes, bool required, void** dptr) -> int {
+    if (!host) { *dptr = nullptr; return required ? -1 : 0; }
+    if (p->on_device) { *dptr = host; return 0; }
+    if (hipMalloc(dptr, bytes) != hipSuccess) return -1;
+    owned.push_back(*dptr);
+    if (hipMemcpyAsync(*dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return -1;
+    return 0;
+  };
+  size_t bd = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
+  void* dp[10];
+  void* hp[10] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost};
+  size_t by[10] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi};
+  int rc = 0;
+  for (int k = 0; k < 10 && rc == 0; k++) rc = dev_arr(hp[k], by[k], k < 8, &dp[k]);
+  unsigned long long* d_lost = nullptr;
+  if (rc == 0 && hipMalloc(&d_lost, sizeof(unsigned long long)) != hipSuccess) rc = -1;
+  if (rc != 0) {
+    for (void* q : owned) hipFree(q);
+    if (d_lost) hipFree(d_lost);
+    return set_err(ctx, "device allocation or upload of the particle arrays failed");
+  }
+  hipMemsetAsync(d_lost, 0, sizeof(unsigned long long), ctx->stream);
+  s.x = (double*)dp[0]; s.px = (double*)dp[1]; s.y = (double*)dp[2]; s.py = (double*)dp[3];
+  s.z = (double*)dp[4]; s.pz = (double*)dp[5]; s.beta = (double*)dp[6];
+  s.state = (int32_t*)dp[7]; s.ix_lost = (int32_t*)dp[8]; s.turn_lost = (int32_t*)dp[9];
+
+  const int block = 256;
+  int64_t grid = (n + block - 1) / block;
+  hipEventRecord(ctx->ev0, ctx->stream);
+  hipLaunchKernelGGL(bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block), 0, ctx->stream, lat->d_ops, lo1, hi1,
+                     lo2, hi2, lat->lc, n_turns, s, n, d_lost);
+  hipEventRecord(ctx->ev1, ctx->stream);
+  hipError_t le = hipGetLastError();
+  unsigned long long h_lost = 0;
+  hipMemcpyAsync(&h_lost, d_lost, sizeof(h_lost), hipMemcpyDeviceToHost, ctx->stream);
+  if (!p->on_device) {
+    for (int k = 0; k < 10; k++)
+      if (hp[k]) hipMemcpyAsync(hp[k], dp[k], by[k], hipMemcpyDeviceToHost, ctx->stream);
+  }
+  hipError_t se = hipStreamSynchronize(ctx->stream);
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  for (void* q : owned) hipFree(q);
+  hipFree(d_lost);
+  if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
+  if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
+  if (out) {
+    out->n_lost = (int64_t)h_lost;
+    out->kernel_ms = ms;
+  }
+  return 0;
+}
+
+int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* grid, int32_t n_turns,
+                       int32_t* survival_turn, double* da_area, bmx_stats* out) {
+  if (!ctx || !lat || !grid || !survival_turn) return set_err(ctx, "invalid arguments");
+  if (grid->n_angle <= 0 || grid->n_amp <= 0 || grid->n_pz <= 0 || n_turns < 0) return set_err(ctx, "bad grid");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  GridDev g;
+  g.n_angle = grid->n_angle; g.n_amp = grid->n_amp; g.n_pz = grid->n_pz;
+  g.n_ic = grid->n_angle * grid->n_amp * grid->n_pz;
+  g.amp_max_x = grid->amp_max_x; g.amp_ratio_y = grid->amp_ratio_y;
+  g.pz_min = grid->pz_min; g.pz_max = grid->pz_max;
+  g.angle_min = grid->angle_min; g.angle_max = grid->angle_max;
+  int64_t total = (int64_t)g.n_ic * lat->n_variants;
+  int32_t* d_surv = nullptr;
+  HIP_TRY(ctx, hipMalloc(&d_surv, total * sizeof(int32_t)));
+  const int block = 256;
+  dim3 gd((unsigned)((g.n_ic + block - 1) / block), (unsigned)lat->n_variants);
+  hipEventRecord(ctx->ev0, ctx->stream);
+  hipLaunchKernelGGL(bmx_track_grid_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin, lat->lc, g,
+                     n_turns, d_surv);
+  hipEventRecord(ctx->ev1, ctx->stream);
+  hipError_t le = hipGetLastError();
+  hipMemcpyAsync(survival_turn, d_surv, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t se = hipStreamSynchronize(ctx->stream);
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  hipFree(d_surv);
+  if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
+  if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
+  // DA area per variant: boundary r(theta) = amplitude of the last IC of the
+  // contiguous surviving run from the smallest amplitude, on the pz = middle
+  // slice; area = 1/2 sum r_i r_{i+1} sin(dtheta) (polar-grid trapezoids).
+  if (da_area) {
+    int ipz = g.n_pz / 2;
+    double dth = (g.n_angle > 1) ? (g.angle_max - g.angle_min) / (g.n_angle - 1) : 0.0;
+    for (int v = 0; v < lat->n_variants; v++) {
+      std::vector<double> r(g.n_angle, 0.0);
+      for (int ia = 0; ia < g.n_angle; ia++) {
+        int last = -1;
+        for (int k = 0; k < g.n_amp; k++) {
+          int ic = (ipz * g.n_angle + ia) * g.n_amp + k;
+          if (survival_turn[(int64_t)v * g.n_ic + ic] >= n_turns) last = k;
+          else break;
+        }
+        double amp = (last < 0) ? 0.0 : g.amp_max_x * (double)(last + 1) / g.n_amp;
+        double th = g.angle_min + dth * ia;
+        double ax = amp * cos(th), ay = amp * g.amp_ratio_y * sin(th);
+        r[ia] = sqrt(ax * ax + ay * ay);
+      }
+      double a = 0;
+      for (int ia = 0; ia + 1 < g.n_angle; ia++) a += 0.5 * r[ia] * r[ia + 1] * sin(dth);
+      da_area[v] = a;
+    }
+  }
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->kernel_ms = ms;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,232 @@
+/*
+ * bmx.h -- C ABI of libbmx, an MI355X (gfx950) multi-particle tracker for
+ * Bmad's bmad_standard element-by-element tracking.
+ *
+ * Everything that crosses this boundary is plain C: fixed-layout structs,
+ * pointers and sizes.  No C++ or torch types appear here.  The Fortran side
+ * binds these entry points with ISO_C_BINDING (bmad-ecosystem_amd/fortran).
+ *
+ * Coordinates are Bmad's canonical 6-D phase space (x, px, y, py, z, pz),
+ * pz = dp/p0, stored structure-of-arrays.  Particle loss is reported only
+ * through state[] with Bmad's codes (bmad/modules/bmad_struct.f90:522-527);
+ * a negative return code is an error (bad input, device failure), never a loss.
+ *
+ * Entry point  <->  reference interface it replaces
+ *   bmx_track_bunch  track_bunch / track1_bunch / track1_bunch_hom
+ *                    (bmad/multiparticle/beam_mod.f90:103-170, :238-386;
+ *                     bmad/multiparticle/beam_utils.f90:32-78) reached through
+ *                    track1_bunch_hook_ptr (bmad/modules/bmad_routine_interface.f90:3927-3936)
+ *   bmx_track_grid   track_many through track_many_hook_ptr
+ *                    (bmad/code/track_many.f90:50-157,
+ *                     bmad/modules/bmad_routine_interface.f90:3949-3959) as used by
+ *                    dynamic_aperture_scan (bmad/modules/dynamic_aperture_mod.f90:27-207)
+ *   bmx_lattice_upload  replaces the per-particle multipole_ele_to_ab calls
+ *                    (bmad/code/multipole_ele_to_ab.f90:31-235) by one hoisting pass
+ */
+#ifndef BMX_H
+#define BMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMX_ABI_VERSION 1
+#define BMX_N_POLE 22 /* n_pole_maxx + 1, bmad/modules/bmad_struct.f90 */
+
+/* ---- particle state codes (bmad/modules/bmad_struct.f90:520-527) ---- */
+enum {
+  BMX_NOT_SET = -1,
+  BMX_PRE_BORN = 0,
+  BMX_ALIVE = 1,
+  BMX_LOST = 2,
+  BMX_LOST_NEG_X = 3,
+  BMX_LOST_POS_X = 4,
+  BMX_LOST_NEG_Y = 5,
+  BMX_LOST_POS_Y = 6,
+  BMX_LOST_Z = 7,
+  BMX_LOST_PZ = 8
+};
+
+/* ---- element keys: the bmad_standard element mix this library tracks ---- */
+enum {
+  BMX_KEY_MARKER = 0,
+  BMX_KEY_DRIFT = 1,
+  BMX_KEY_QUADRUPOLE = 2,
+  BMX_KEY_SBEND = 3,
+  BMX_KEY_SEXTUPOLE = 4,
+  BMX_KEY_OCTUPOLE = 5,
+  BMX_KEY_THICK_MULTIPOLE = 6,
+  BMX_KEY_MULTIPOLE = 7,    /* thin KnL/Tn multipole: a[] = KnL, b[] = Tn */
+  BMX_KEY_AB_MULTIPOLE = 8, /* thin An/Bn multipole */
+  BMX_KEY_RFCAVITY = 9,
+  BMX_KEY_HKICKER = 10,
+  BMX_KEY_VKICKER = 11,
+  BMX_KEY_KICKER = 12,
+  BMX_KEY_INSTRUMENT = 13,
+  BMX_KEY_MONITOR = 14,
+  BMX_KEY_ECOLLIMATOR = 15,
+  BMX_KEY_RCOLLIMATOR = 16,
+  BMX_KEY_PIPE = 17,
+  BMX_N_KEY = 18
+};
+
+/* fringe_type codes (bmad/modules/bmad_struct.f90:27,167-168) */
+enum {
+  BMX_FRINGE_NONE = 1,
+  BMX_FRINGE_SOFT_EDGE_ONLY = 2,
+  BMX_FRINGE_HARD_EDGE_ONLY = 3,
+  BMX_FRINGE_FULL = 4,
+  BMX_FRINGE_SAD_FULL = 5,
+  BMX_FRINGE_LINEAR_EDGE = 6,
+  BMX_FRINGE_BASIC_BEND = 7
+};
+
+/* element-end codes for fringe_at / aperture_at / coupler_at
+ * (bmad/modules/bmad_struct.f90:372) */
+enum { BMX_ENTRANCE_END = 1, BMX_EXIT_END = 2, BMX_BOTH_ENDS = 3, BMX_NO_END = 4 };
+
+/* aperture_type codes (bmad/modules/bmad_struct.f90:157) */
+enum { BMX_APERTURE_RECTANGULAR = 2, BMX_APERTURE_ELLIPTICAL = 3 };
+
+/* particle species */
+enum {
+  BMX_ELECTRON = 1,
+  BMX_POSITRON = 2,
+  BMX_PROTON = 3,
+  BMX_ANTIPROTON = 4,
+  BMX_MUON = 5,
+  BMX_ANTIMUON = 6
+};
+
+/*
+ * One lattice element after Bmad's bookkeeping (attribute_bookkeeper), in the
+ * element frame: the "_tot" attributes of the reference are the plain fields
+ * here.  Layout is fixed (all doubles 8-byte aligned); the Fortran shim fills
+ * it from ele%value(...).  ds_step = l / num_steps.
+ */
+typedef struct bmx_ele_rec {
+  int32_t key;              /* BMX_KEY_* */
+  int32_t is_on;            /* ele%is_on */
+  int32_t orientation;      /* ele%orientation, +1 or -1 */
+  int32_t num_steps;        /* >= 1 */
+  int32_t fringe_type;      /* BMX_FRINGE_* */
+  int32_t fringe_at;        /* BMX_*_END */
+  int32_t aperture_type;    /* BMX_APERTURE_* */
+  int32_t aperture_at;      /* BMX_*_END */
+  int32_t field_master;     /* multipoles given in field units (T m^-n) */
+  int32_t scale_multipoles; /* ele%scale_multipoles (default 1) */
+  int32_t has_mag_poles;    /* associated(ele%a_pole) */
+  int32_t has_elec_poles;   /* associated(ele%a_pole_elec) */
+  int32_t coupler_at;       /* BMX_*_END, rfcavity only */
+  int32_t exact_multipoles; /* 1 = off (only supported value) */
+  int32_t reserved[2];
+  double l, k1, k2, k3, g, dg, e1, e2, fint, fintx, hgap, hgapx;
+  double tilt, ref_tilt;
+  double x_offset, y_offset, z_offset, x_pitch, y_pitch;
+  double hkick, vkick, kick;
+  double voltage, rf_frequency, phi0, phi0_multipass;
+  double coupler_strength, coupler_angle, coupler_phase;
+  double x1_limit, x2_limit, y1_limit, y2_limit;
+  double p0c, e_tot, p0c_start, e_tot_start, delta_ref_time, ref_time;
+  double r0_mag, r0_elec;
+  /* collimator phase-space limits (ecollimator / rcollimator) */
+  double px_aperture_width2, px_aperture_center, py_aperture_width2, py_aperture_center;
+  double z_aperture_width2, z_aperture_center, pz_aperture_width2, pz_aperture_center;
+  double a[BMX_N_POLE], b[BMX_N_POLE];           /* magnetic An/Bn (MULTIPOLE: KnL/Tn) */
+  double a_elec[BMX_N_POLE], b_elec[BMX_N_POLE]; /* electric An/Bn */
+} bmx_ele_rec;
+
+/* Reference particle of the branch (lat%param%particle). */
+typedef struct bmx_ref {
+  int32_t species;  /* BMX_ELECTRON, ... */
+  int32_t reserved;
+} bmx_ref;
+
+/* Global switches (subset of bmad_com, bmad/modules/bmad_struct.f90:2296-2343). */
+typedef struct bmx_config {
+  int32_t aperture_limit_on;     /* default 1 */
+  int32_t reserved;
+  double max_aperture_limit;     /* default 1e3 m */
+} bmx_config;
+
+/*
+ * Particle store, structure-of-arrays.  x..pz, beta and state are required;
+ * ix_lost and turn_lost may be NULL.  beta is the particle velocity / c and is
+ * carried exactly as coord_struct%beta (energy kicks update it).
+ * on_device = 1: pointers are device (HBM) addresses; 0: host memory.
+ */
+typedef struct bmx_particles {
+  double *x, *px, *y, *py, *z, *pz;
+  double *beta;
+  int32_t *state;
+  int32_t *ix_lost;    /* element index (0-based) where the particle was lost, -1 if alive */
+  int32_t *turn_lost;  /* turn index where the particle was lost, -1 if alive */
+  int64_t n;
+  int32_t on_device;
+  int32_t reserved;
+} bmx_particles;
+
+/* Per-call statistics returned by the tracking entry points. */
+typedef struct bmx_stats {
+  int64_t n_live;           /* particles still alive after the call */
+  int64_t n_lost;           /* particles lost during the call */
+  double kernel_ms;         /* device time of the tracking kernel(s), HIP events */
+  double pet;               /* live particle-element-turns processed */
+} bmx_stats;
+
+/* Batched dynamic-aperture grid: n_var lattice variants, each with the same
+ * polar grid of initial conditions n_angle x n_amp x n_pz around the
+ * variant's closed orbit (taken as zero here). */
+typedef struct bmx_grid {
+  int32_t n_angle, n_amp, n_pz, reserved;
+  double amp_max_x;         /* largest x amplitude [m]; y amplitude scales by amp_ratio_y */
+  double amp_ratio_y;
+  double pz_min, pz_max;
+  double angle_min, angle_max; /* polar angle range [rad], default (0, pi) */
+} bmx_grid;
+
+typedef struct bmx_ctx bmx_ctx;
+typedef struct bmx_lattice bmx_lattice;
+
+/* Library version (BMX_ABI_VERSION). */
+int32_t bmx_abi_version(void);
+
+/* sizeof of the ABI structs, for binding-side layout checks:
+ * which = 0 bmx_ele_rec, 1 bmx_particles, 2 bmx_stats, 3 bmx_grid, 4 bmx_config. */
+int32_t bmx_sizeof(int32_t which);
+
+/* Create a context on HIP device `device`.  flags: reserved, pass 0. */
+int32_t bmx_ctx_create(bmx_ctx **out, int32_t device, uint32_t flags);
+void bmx_ctx_destroy(bmx_ctx *ctx);
+const char *bmx_last_error(const bmx_ctx *ctx);
+
+/* Compile and upload a lattice.  eles holds n_variants consecutive element
+ * lists; variant v is eles[variant_offsets[v] .. variant_offsets[v+1]).
+ * variant_offsets has n_variants+1 entries (NULL with n_variants = 1 means the
+ * whole array).  Replaces the per-particle multipole_ele_to_ab bookkeeping. */
+int32_t bmx_lattice_upload(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele_total,
+                           const int32_t *variant_offsets, int32_t n_variants,
+                           const bmx_ref *ref, const bmx_config *cfg, bmx_lattice **out);
+void bmx_lattice_destroy(bmx_lattice *lat);
+
+/* Track a bunch through elements ix_start+1 .. ix_end of variant 0 with
+ * track_bunch wrap-around semantics (ix_end <= ix_start wraps through the
+ * end of the lattice), repeated n_turns times.  direction must be +1.
+ * Equivalent to n_turns calls of track_bunch(lat, bunch, ele_start, ele_end). */
+int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
+                        int32_t n_turns, int32_t direction, bmx_particles *p, bmx_stats *out);
+
+/* Track every variant's grid of initial conditions for n_turns full turns in
+ * one launch.  survival_turn[v*n_ic + i] = number of turns survived
+ * (n_turns if alive at the end), da_area[v] = polar-grid DA area [m^2].
+ * Host output arrays. */
+int32_t bmx_track_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *grid, int32_t n_turns,
+                       int32_t *survival_turn, double *da_area, bmx_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BMX_H */

@@ -1,0 +1,147 @@
+"""Extract golden vectors for the bmad_standard path from the reference's
+regression tests into small JSON fixtures under tests/golden/.
+
+Run in a container that has the reference tree (default /root/reference);
+the JSON files it writes are committed and are all the tests read.
+
+Sources (relative to the bmad-ecosystem root):
+  regression_tests/tracking_method_test/output.correct  (Bmad_Standard rows)
+  regression_tests/tracking_method_test/tracking_method_test.bmad (element attributes, start orbit)
+  regression_tests/fringe_test/output.correct, fringe_test.bmad
+  regression_tests/aperture_test/output.correct, aperture_test.bmad
+
+The element attributes below are transcribed from tracking_method_test.bmad
+(lines cited) into this build's element-record vocabulary; defaults are the
+ones bmad/parsing/set_ele_defaults.f90 applies.
+"""
+import json
+import math
+import os
+import re
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+PI = math.pi
+
+# tracking_method_test.bmad: element definitions used by the Bmad_Standard rows
+# this build covers.  q0 / sb0 base attributes (lines 62, 79) are merged in.
+Q0 = dict(a={0: 0.01, 1: 0.1}, vkick=-0.0001, b_elec={0: -2e3}, a_elec={1: 2e4})
+SB0 = dict(a={0: 0.01, 1: 0.1}, vkick=-0.0001, b_elec={0: -2e3}, a_elec={1: 2e4}, r0_mag=2, r0_elec=3)
+ELEMENTS = {
+    # line 18
+    "AB_MULTIPOLE1": dict(key="ab_multipole", a={2: 1e2}, b={2: 1e2}, x_offset=0.02, tilt=0.1),
+    # line 25
+    "DRIFT1": dict(key="drift", l=4.5, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
+    # line 27
+    "ECOLLIMATOR1": dict(key="ecollimator", l=4.5, x_limit=0.2, y_limit=0.1, num_steps=10, x_offset=0.02,
+                         y_pitch=0.001, tilt=0.1, hkick=0.01, vkick=0.002),
+    # line 40
+    "HKICKER1": dict(key="hkicker", l=4.5, kick=0.005, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
+    # line 41
+    "INSTRUMENT1": dict(key="instrument", l=4.5, num_steps=10),
+    # line 42
+    "KICKER1": dict(key="kicker", l=4.5, b={0: 0.002}, vkick=0.003, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                    tilt=0.1),
+    # line 56
+    "MONITOR1": dict(key="monitor", l=4.5, num_steps=10),
+    # line 57: t1 with no value = pi/4
+    "MULTIPOLE1": dict(key="multipole", knl={1: 1.0, 3: 4.5}, tn={1: PI / 4, 3: 0.31 * PI}, x_offset=0.05),
+    # line 59: bare tilt = pi/8
+    "OCTUPOLE1": dict(key="octupole", l=4.5, k3=1e3, tilt=PI / 8, num_steps=10, x_offset=0.01, y_pitch=0.001),
+    # lines 62-67: bare tilt = pi/4
+    "QUADRUPOLE1": dict(key="quadrupole", l=0.6, k1=1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001, **Q0),
+    "QUADRUPOLE2": dict(key="quadrupole", l=0.6, k1=-1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001, **Q0),
+    "QUADRUPOLE3": dict(key="quadrupole", l=0.6, k1=1e-20, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        **Q0),
+    "QUADRUPOLE4": dict(key="quadrupole", l=0.6, k1=1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        fringe_type="full", **Q0),
+    "QUADRUPOLE5": dict(key="quadrupole", l=0.6, k1=-1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        fringe_type="full", **Q0),
+    # line 68
+    "RCOLLIMATOR1": dict(key="rcollimator", l=4.5, x_limit=0.2, y_limit=0.1, num_steps=10, x_offset=0.02,
+                         y_pitch=0.001, tilt=0.1),
+    # lines 69-70
+    "RFCAVITY1": dict(key="rfcavity", l=4.5, rf_frequency=500e6, voltage=1e5, phi0=0.1, num_steps=10, x_offset=0.05,
+                      y_pitch=0.001, tilt=0.1),
+    # lines 73-74 (zero length)
+    "RFCAVITY3": dict(key="rfcavity", rf_frequency=500e6, voltage=1e5, phi0=0.1, num_steps=10, x_offset=0.05,
+                      y_pitch=0.001, tilt=0.1, a={2: 0.02}),
+    # line 84
+    "SBEND3": dict(key="sbend", l=0.6, k1=0.3, k2=3, num_steps=10, x_offset=0.05, y_pitch=0.001, ref_tilt=0.1, **SB0),
+    # line 91
+    "SBEND7": dict(key="sbend", l=0.6, g=0.01, e1=0.1, e2=0.2, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                   ref_tilt=0.1, **SB0),
+    # line 93: bare tilt = pi/6
+    "SEXTUPOLE1": dict(key="sextupole", l=0.6, k2=1, tilt=PI / 6, num_steps=10, x_offset=0.05, y_pitch=0.001),
+    # line 129: no num_steps -> ds_step = default_ds_step 0.2 -> nint(22.5) = 23 steps
+    "THICK_MULTIPOLE1": dict(key="thick_multipole", l=4.5, a={2: 1}, b={3: 20}, x_offset=0.02, tilt=0.1,
+                             num_steps=23),
+    # line 130
+    "VKICKER1": dict(key="vkicker", l=4.5, kick=0.005, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
+}
+
+VARIANTS = {"": (1, 1, False), "-Anti_O": (1, -1, False), "-Anti_D": (-1, 1, False), "-Anti_OD": (-1, -1, False),
+            "-Abs_Time": (1, 1, True)}
+
+
+def tracking_method():
+    path = os.path.join(REF, "regression_tests/tracking_method_test/output.correct")
+    rows = []
+    pat = re.compile(r'^"([A-Z0-9_]+)((?:-Anti_OD|-Anti_O|-Anti_D|-Abs_Time)?): Bmad_Standard"\s+ABS\s+(\S+)\s+(.*)$')
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = pat.match(line.strip())
+            if not m or m.group(1) not in ELEMENTS:
+                continue
+            vals = [float(v) for v in m.group(4).split()]
+            rows.append(dict(name=m.group(1), variant=m.group(2), line=lineno, tol=float(m.group(3)), vec=vals[:6],
+                             col7=vals[6]))
+    return dict(
+        source="regression_tests/tracking_method_test/output.correct",
+        lattice="regression_tests/tracking_method_test/tracking_method_test.bmad",
+        # tracking_method_test.bmad:9-14 (start orbit), :141 (l1[e_tot] = 1e6); default species positron
+        particle="positron", e_tot=1e6, start=[0.001, 0.002, 0.003, 0.004, 0.005, 0.006],
+        printed_digits=11, elements=ELEMENTS, variants={k: list(v) for k, v in VARIANTS.items()}, rows=rows)
+
+
+def fringe():
+    path = os.path.join(REF, "regression_tests/fringe_test/output.correct")
+    out = {}
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = re.match(r'^"([A-Za-z0-9-]+)"\s+ABS\s+(\S+)\s+(.*)$', line.strip())
+            if m:
+                out[m.group(1)] = dict(line=lineno, tol=float(m.group(2)), vals=[float(v) for v in m.group(3).split()])
+    # fringe_test.bmad: electron, e_tot 1e6, start (0.01, 0.02, 0.03, 0.04, 0, 0.5)
+    # b: sbend, e1 = 0.1, e2 = -0.2, k1 = 0.1, g = 0.1, fint = 0.5, hgap = 0.4 (fintx, hgapx default to fint, hgap)
+    return dict(source="regression_tests/fringe_test/output.correct", particle="electron", e_tot=1e6,
+                start=[0.01, 0.02, 0.03, 0.04, 0.0, 0.5],
+                element=dict(key="sbend", e1=0.1, e2=-0.2, k1=0.1, g=0.1, fint=0.5, hgap=0.4, fintx=0.5, hgapx=0.4),
+                rows=out)
+
+
+def aperture():
+    path = os.path.join(REF, "regression_tests/aperture_test/output.correct")
+    out = {}
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = re.match(r'^"((?:EC|RC)_[SU]_[A-Za-z]+[-+])"\s+ABS\s+(\S+)\s+(.*)$', line.strip())
+            if m:
+                out[m.group(1)] = dict(line=lineno, tol=float(m.group(2)), vals=[float(v) for v in m.group(3).split()])
+    # aperture_test.bmad: antimuon at p0c = 3.094353005e9; ec / rc attributes (lines 22-35)
+    coll = dict(px_aperture_width2=0.01, px_aperture_center=0.01, py_aperture_width2=0.015, z_aperture_width2=0.2,
+                z_aperture_center=-0.6, pz_aperture_width2=0.03, pz_aperture_center=0.02, x1_limit=1e-10,
+                x2_limit=0.02, y_limit=0.015)
+    return dict(source="regression_tests/aperture_test/output.correct", particle="antimuon", p0c=3.094353005e9,
+                elements={"EC": dict(key="ecollimator", **coll), "RC": dict(key="rcollimator", **coll)}, rows=out)
+
+
+if __name__ == "__main__":
+    for name, fn in (("tracking_method_bmad_standard.json", tracking_method), ("fringe_test.json", fringe),
+                     ("aperture_test.json", aperture)):
+        d = fn()
+        with open(os.path.join(HERE, name), "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        print(name, len(d["rows"]))
