@@ -1,0 +1,142 @@
+"""Benchmark: FP64 multi-particle tracking on one MI355X.
+
+Workload (one GPU): the 4000-element ring with synchrotron motion -- FODO
+cells with thick sextupoles (4 kick-drift slices), thin octupoles, sector bends
+with hard-edge fringe and 8 x 1 MV 500 MHz rf cavities, 6 GeV electrons,
+1,000,000 Gaussian particles (eps_x 10 nm, eps_y 0.1 nm, sigma_z 1 mm,
+sigma_pz 1e-3, matched to the periodic Twiss), elliptical 20 x 10 mm
+aperture, FP64.  One step = TURNS_PER_STEP turns of the whole bunch through
+the ring in one bmx_track_bunch call; the bunch stays resident in HBM and is
+tracked on from step to step, as the long-term tracking turn loop does.
+
+Printed: one JSON line with the metric (particle-turns/s), the roofline of the
+tracking kernel (algorithmic FP64 FLOPs from the reference's operation count
+per particle-element-turn / kernel time from HIP events on the library's
+stream) and the CPU baseline (the oracle's reference-structured OpenMP
+tracking on a bounded sample, timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bmad-ecosystem_amd"))
+
+import numpy as np  # noqa: E402
+
+TURNS_PER_STEP = 10
+N_PARTICLES = 1_000_000
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_baseline(eles, tw, budget_s=15.0):
+    """Oracle (reference loop structure: element-outer, OpenMP over particles,
+    bmad/multiparticle/beam_utils.f90:69-74) on a bounded sample of the same
+    workload: sample particles x full ring x 1 turn, repeated until ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from bmx import abi, beam
+
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 64))
+    n = 4096
+    coords = beam.matched_gaussian(n, tw, 10e-9, 0.1e-9, 1e-3, 1e-3, seed=20261203)
+    arr = oracle.abi_array(eles)
+    turns = 0
+    t0 = time.perf_counter()
+    while True:
+        oracle.track_bunch(arr, abi.ELECTRON, 1, coords, n_threads=threads)
+        turns += 1
+        if time.perf_counter() - t0 > budget_s or turns >= 50:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * turns / dt, "unit": "particle-turns/s", "cores": threads, "kind": "port",
+            "sample": f"{n} particles x {len(eles)} elements x {turns} turns (1 turn per call), "
+                      f"element-outer OpenMP loop, {dt:.1f} s",
+            "pet_per_s": n * turns * len(eles) / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--particles", type=int, default=N_PARTICLES)
+    ap.add_argument("--turns-per-step", type=int, default=TURNS_PER_STEP)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); particle tracking "
+              f"shards by particles with no exchange, see the design notes.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bmx import abi, beam, lattice, tracker
+
+    eles = lattice.ring_c1(n_cells=500, n_rf=8)
+    n_ele = len(eles)
+    tw = beam.periodic_twiss(lattice.ring_c1(n_cells=500, n_rf=0)[:8])
+    coords = beam.matched_gaussian(args.particles, tw, 10e-9, 0.1e-9, 1e-3, 1e-3, seed=20261203)
+
+    with open(os.path.join(ROOT, "tests", "golden", "flops_per_pet.json")) as f:
+        flops = json.load(f)["C1"]
+    flops_per_particle_turn = flops["flops_per_particle_turn"]
+
+    ctx = tracker.Context(0)
+    lat = ctx.lattice(eles, abi.ELECTRON)
+    bunch = tracker.Bunch(coords, lat.p0c, abi.M_ELECTRON, e_tot=lat.e_tot)
+    torch.cuda.synchronize()
+
+    n_live = args.particles
+    for _ in range(args.warmup):
+        st = tracker.track_bunch(lat, bunch, n_turns=args.turns_per_step, sync=False)
+        n_live -= st["n_lost"]
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    particle_turns = 0.0
+    for _ in range(args.steps):
+        st = tracker.track_bunch(lat, bunch, n_turns=args.turns_per_step, sync=False)
+        lost = st["n_lost"]
+        particle_turns += (n_live - 0.5 * lost) * args.turns_per_step
+        n_live -= lost
+        kernel_ms += st["kernel_ms"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    value = particle_turns / wall
+    kernel_s = kernel_ms / 1e3
+    achieved = flops_per_particle_turn * particle_turns / kernel_s / 1e12
+    result = {
+        "metric": "FP64 particle-turns/s",
+        "value": value,
+        "unit": "particle-turns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "4000-element FODO ring with thick sextupoles, thin octupoles, 8 rf cavities; "
+                               "6 GeV e-, 1e6 matched Gaussian particles, elliptical 20x10 mm aperture",
+                   "particles": args.particles, "elements": n_ele, "turns_per_step": args.turns_per_step,
+                   "particle_element_turns_per_s": value * n_ele, "particles_alive_end": int(n_live)},
+        "roofline": {"bound": "fp64-vector", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_pet": flops["flops_per_pet"], "kernel_ms_per_launch": kernel_ms / args.steps},
+    }
+    lat.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(eles, tw)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -238,3 +238,8 @@ def da_variants(n_var=256, seed=20261204, min_cells=63, max_cells=500, n_rf=8):
             offs[("s", i)] = tuple(50e-6 * rng.standard_normal(2))
         out.append(ring_c1(n_cells=nc, n_rf=n_rf, seed=seed + 1000 + v, k1_err=k1e, k2_err=k2e, offsets=offs))
     return out
+
+
+def EleRec_copy(e):
+    """Independent copy of an element record."""
+    return abi.EleRec.from_buffer_copy(e)

@@ -1,0 +1,196 @@
+"""Python front end of libbmx: the track_beam / track_bunch / track_many call
+shape of Bmad (bmad/multiparticle/beam_mod.f90:40-170, bmad/code/track_many.f90)
+over the C ABI.
+
+Particles live in a `Bunch` of device tensors (structure of arrays, FP64 x,
+px, y, py, z, pz, beta; int32 state, ix_lost, turn_lost).  PyTorch is used
+only for device memory; every computation runs in libbmx's HIP kernels.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi, lattice as latmod
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+class BmxError(RuntimeError):
+    pass
+
+
+def _check(ctx, rc, what):
+    if rc != 0:
+        lib = abi.load_library()
+        msg = lib.bmx_last_error(ctx).decode() if ctx else ""
+        raise BmxError(f"{what} failed (rc={rc}): {msg}")
+
+
+class Context:
+    """A libbmx context on one HIP device (bmx_ctx_create)."""
+
+    def __init__(self, device=0):
+        self.lib = abi.load_library()
+        abi.check_layout(self.lib)
+        h = C.c_void_p()
+        rc = self.lib.bmx_ctx_create(C.byref(h), int(device), 0)
+        if rc != 0:
+            raise BmxError(f"bmx_ctx_create(device={device}) failed with rc={rc} (no HIP device?)")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.bmx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lattice(self, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3):
+        return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit)
+
+
+class Lattice:
+    """A compiled, device-resident element program (bmx_lattice_upload)."""
+
+    def __init__(self, ctx, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3):
+        self.ctx = ctx
+        self.species = species
+        arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
+        self.n_ele_total = len(arr)
+        self.p0c = arr[0].p0c
+        self.e_tot = arr[0].e_tot
+        ref = abi.Ref(species, 0)
+        cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit)
+        if variant_offsets is None:
+            offs_p = None
+            n_var = 1
+            self.n_ele = [len(arr)]
+        else:
+            offs = (C.c_int32 * len(variant_offsets))(*variant_offsets)
+            offs_p = offs
+            n_var = len(variant_offsets) - 1
+            self.n_ele = [variant_offsets[i + 1] - variant_offsets[i] for i in range(n_var)]
+        self.n_variants = n_var
+        h = C.c_void_p()
+        rc = ctx.lib.bmx_lattice_upload(ctx.h, arr, len(arr), offs_p, n_var, C.byref(ref), C.byref(cfg), C.byref(h))
+        _check(ctx.h, rc, "bmx_lattice_upload")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.bmx_lattice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Bunch:
+    """Structure-of-arrays particle store in HBM."""
+
+    NAMES = ("x", "px", "y", "py", "z", "pz")
+
+    def __init__(self, coords, p0c, mc2, device="cuda:0", beta=None, e_tot=None):
+        c = torch.as_tensor(np.asarray(coords, dtype=np.float64).reshape(6, -1))
+        self.n = c.shape[1]
+        self.device = torch.device(device)
+        self.cols = [c[i].contiguous().to(self.device) for i in range(6)]
+        if beta is None:
+            pz = c[5]
+            pc = p0c * (1.0 + pz)
+            b = pc / torch.sqrt(pc * pc + mc2 * mc2)
+            if e_tot is not None:  # init_coord: beta = p0c / e_tot when pz == 0 (init_coord.f90:204-208)
+                b = torch.where(pz == 0, torch.full_like(b, p0c / e_tot), b)
+            beta = b
+        self.beta = torch.as_tensor(beta, dtype=torch.float64).contiguous().to(self.device)
+        self.state = torch.ones(self.n, dtype=torch.int32, device=self.device)
+        self.ix_lost = torch.full((self.n,), -1, dtype=torch.int32, device=self.device)
+        self.turn_lost = torch.full((self.n,), -1, dtype=torch.int32, device=self.device)
+
+    def particles_struct(self):
+        p = abi.Particles()
+        for name, t in zip(self.NAMES, self.cols):
+            setattr(p, name, t.data_ptr())
+        p.beta = self.beta.data_ptr()
+        p.state = self.state.data_ptr()
+        p.ix_lost = self.ix_lost.data_ptr()
+        p.turn_lost = self.turn_lost.data_ptr()
+        p.n = self.n
+        p.on_device = 1
+        return p
+
+    def numpy(self):
+        out = {k: t.cpu().numpy() for k, t in zip(self.NAMES, self.cols)}
+        out.update(beta=self.beta.cpu().numpy(), state=self.state.cpu().numpy(), ix_lost=self.ix_lost.cpu().numpy(),
+                   turn_lost=self.turn_lost.cpu().numpy())
+        return out
+
+
+def track_bunch(lat, bunch, n_turns=1, ix_start=0, ix_end=0, direction=1, sync=True):
+    """n_turns passes of track_bunch(lat, bunch, ele_start = ix_start, ele_end = ix_end).
+    Returns bmx_stats as a dict."""
+    if sync:
+        torch.cuda.synchronize(bunch.device)
+    p = bunch.particles_struct()
+    st = abi.Stats()
+    rc = lat.ctx.lib.bmx_track_bunch(lat.ctx.h, lat.h, int(ix_start), int(ix_end), int(n_turns), int(direction),
+                                     C.byref(p), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_track_bunch")
+    return dict(n_lost=st.n_lost, kernel_ms=st.kernel_ms)
+
+
+def track_bunch_host(lat, coords, n_turns=1, beta=None):
+    """Same as track_bunch on host numpy arrays (the library stages them through HBM)."""
+    c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)
+    n = c.shape[1]
+    cols = [np.ascontiguousarray(c[i]) for i in range(6)]
+    if beta is None:
+        pc = lat.p0c * (1.0 + cols[5])
+        m = abi.MASS[lat.species]
+        beta = np.where(cols[5] == 0, lat.p0c / lat.e_tot, pc / np.sqrt(pc * pc + m * m))
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    state = np.ones(n, dtype=np.int32)
+    ix_lost = np.full(n, -1, dtype=np.int32)
+    turn_lost = np.full(n, -1, dtype=np.int32)
+    p = abi.Particles()
+    for name, a in zip(Bunch.NAMES, cols):
+        setattr(p, name, a.ctypes.data)
+    p.beta = beta.ctypes.data
+    p.state = state.ctypes.data
+    p.ix_lost = ix_lost.ctypes.data
+    p.turn_lost = turn_lost.ctypes.data
+    p.n = n
+    p.on_device = 0
+    st = abi.Stats()
+    rc = lat.ctx.lib.bmx_track_bunch(lat.ctx.h, lat.h, 0, 0, int(n_turns), 1, C.byref(p), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_track_bunch")
+    out = {k: v for k, v in zip(Bunch.NAMES, cols)}
+    out.update(beta=beta, state=state, ix_lost=ix_lost, turn_lost=turn_lost, n_lost=st.n_lost, kernel_ms=st.kernel_ms)
+    return out
+
+
+def track_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, pz_min=0.0, pz_max=0.0,
+               angle_min=0.0, angle_max=np.pi):
+    """Batched DA grid over every variant of `lat` (bmx_track_grid).
+    Returns (survival[n_var, n_pz, n_angle, n_amp], da_area[n_var], kernel_ms)."""
+    g = abi.Grid(n_angle, n_amp, n_pz, 0, amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min, angle_max)
+    n_ic = n_angle * n_amp * n_pz
+    surv = np.zeros(lat.n_variants * n_ic, dtype=np.int32)
+    area = np.zeros(lat.n_variants, dtype=np.float64)
+    st = abi.Stats()
+    rc = lat.ctx.lib.bmx_track_grid(lat.ctx.h, lat.h, C.byref(g), int(n_turns),
+                                    surv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    area.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_track_grid")
+    return surv.reshape(lat.n_variants, n_pz, n_angle, n_amp), area, st.kernel_ms

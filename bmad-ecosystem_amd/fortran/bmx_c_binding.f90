@@ -1,0 +1,180 @@
+!+
+! Module bmx_c_binding
+!
+! ISO_C_BINDING interface to libbmx (include/bmx.h): interoperable mirrors of
+! the ABI structs and bind(C) interfaces of every entry point.  A Bmad program
+! uses it to hand bunches to the MI355X tracker from the plugin hooks
+! track1_bunch_hook_ptr (bmad/modules/bmad_routine_interface.f90:3927-3936)
+! and track_many_hook_ptr (:3949-3959).  This module depends on nothing from
+! Bmad, so it compiles on its own; bmx_bmad_hooks.f90 holds the Bmad-side
+! adapters.
+!-
+
+module bmx_c_binding
+
+use, intrinsic :: iso_c_binding
+
+implicit none
+
+integer(c_int32_t), parameter :: bmx_n_pole = 22
+
+! particle states (bmad/modules/bmad_struct.f90:520-527)
+integer(c_int32_t), parameter :: bmx_alive = 1, bmx_lost = 2, bmx_lost_neg_x = 3, bmx_lost_pos_x = 4, &
+                                 bmx_lost_neg_y = 5, bmx_lost_pos_y = 6, bmx_lost_z = 7, bmx_lost_pz = 8
+
+! element keys
+integer(c_int32_t), parameter :: bmx_key_marker = 0, bmx_key_drift = 1, bmx_key_quadrupole = 2, &
+     bmx_key_sbend = 3, bmx_key_sextupole = 4, bmx_key_octupole = 5, bmx_key_thick_multipole = 6, &
+     bmx_key_multipole = 7, bmx_key_ab_multipole = 8, bmx_key_rfcavity = 9, bmx_key_hkicker = 10, &
+     bmx_key_vkicker = 11, bmx_key_kicker = 12, bmx_key_instrument = 13, bmx_key_monitor = 14, &
+     bmx_key_ecollimator = 15, bmx_key_rcollimator = 16, bmx_key_pipe = 17
+
+integer(c_int32_t), parameter :: bmx_electron = 1, bmx_positron = 2, bmx_proton = 3, bmx_antiproton = 4, &
+                                 bmx_muon = 5, bmx_antimuon = 6
+
+type, bind(C) :: bmx_ele_rec
+  integer(c_int32_t) :: key = 0, is_on = 1, orientation = 1, num_steps = 1
+  integer(c_int32_t) :: fringe_type = 1, fringe_at = 3, aperture_type = 2, aperture_at = 2
+  integer(c_int32_t) :: field_master = 0, scale_multipoles = 1, has_mag_poles = 0, has_elec_poles = 0
+  integer(c_int32_t) :: coupler_at = 2, exact_multipoles = 1, reserved(2) = 0
+  real(c_double) :: l = 0, k1 = 0, k2 = 0, k3 = 0, g = 0, dg = 0, e1 = 0, e2 = 0, fint = 0, fintx = 0, hgap = 0, hgapx = 0
+  real(c_double) :: tilt = 0, ref_tilt = 0
+  real(c_double) :: x_offset = 0, y_offset = 0, z_offset = 0, x_pitch = 0, y_pitch = 0
+  real(c_double) :: hkick = 0, vkick = 0, kick = 0
+  real(c_double) :: voltage = 0, rf_frequency = 0, phi0 = 0, phi0_multipass = 0
+  real(c_double) :: coupler_strength = 0, coupler_angle = 0, coupler_phase = 0
+  real(c_double) :: x1_limit = 0, x2_limit = 0, y1_limit = 0, y2_limit = 0
+  real(c_double) :: p0c = 0, e_tot = 0, p0c_start = 0, e_tot_start = 0, delta_ref_time = 0, ref_time = 0
+  real(c_double) :: r0_mag = 0, r0_elec = 0
+  real(c_double) :: px_aperture_width2 = 0, px_aperture_center = 0, py_aperture_width2 = 0, py_aperture_center = 0
+  real(c_double) :: z_aperture_width2 = 0, z_aperture_center = 0, pz_aperture_width2 = 0, pz_aperture_center = 0
+  real(c_double) :: a(0:bmx_n_pole-1) = 0, b(0:bmx_n_pole-1) = 0
+  real(c_double) :: a_elec(0:bmx_n_pole-1) = 0, b_elec(0:bmx_n_pole-1) = 0
+end type
+
+type, bind(C) :: bmx_ref
+  integer(c_int32_t) :: species = bmx_electron, reserved = 0
+end type
+
+type, bind(C) :: bmx_config
+  integer(c_int32_t) :: aperture_limit_on = 1, reserved = 0
+  real(c_double) :: max_aperture_limit = 1d3
+end type
+
+type, bind(C) :: bmx_particles
+  type(c_ptr) :: x = c_null_ptr, px = c_null_ptr, y = c_null_ptr, py = c_null_ptr, z = c_null_ptr, pz = c_null_ptr
+  type(c_ptr) :: beta = c_null_ptr, state = c_null_ptr, ix_lost = c_null_ptr, turn_lost = c_null_ptr
+  integer(c_int64_t) :: n = 0
+  integer(c_int32_t) :: on_device = 0, reserved = 0
+end type
+
+type, bind(C) :: bmx_stats
+  integer(c_int64_t) :: n_live = 0, n_lost = 0
+  real(c_double) :: kernel_ms = 0, pet = 0
+end type
+
+type, bind(C) :: bmx_grid
+  integer(c_int32_t) :: n_angle = 0, n_amp = 0, n_pz = 0, reserved = 0
+  real(c_double) :: amp_max_x = 0, amp_ratio_y = 1, pz_min = 0, pz_max = 0, angle_min = 0, angle_max = 0
+end type
+
+interface
+  function bmx_abi_version() bind(C, name = 'bmx_abi_version') result(v)
+    import
+    integer(c_int32_t) :: v
+  end function
+
+  function bmx_sizeof(which) bind(C, name = 'bmx_sizeof') result(s)
+    import
+    integer(c_int32_t), value :: which
+    integer(c_int32_t) :: s
+  end function
+
+  function bmx_ctx_create(ctx, device, flags) bind(C, name = 'bmx_ctx_create') result(rc)
+    import
+    type(c_ptr) :: ctx
+    integer(c_int32_t), value :: device
+    integer(c_int32_t), value :: flags
+    integer(c_int32_t) :: rc
+  end function
+
+  subroutine bmx_ctx_destroy(ctx) bind(C, name = 'bmx_ctx_destroy')
+    import
+    type(c_ptr), value :: ctx
+  end subroutine
+
+  function bmx_last_error(ctx) bind(C, name = 'bmx_last_error') result(msg)
+    import
+    type(c_ptr), value :: ctx
+    type(c_ptr) :: msg
+  end function
+
+  function bmx_lattice_upload(ctx, eles, n_ele_total, variant_offsets, n_variants, ref, cfg, lat) &
+                              bind(C, name = 'bmx_lattice_upload') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    type(bmx_ele_rec) :: eles(*)
+    integer(c_int32_t), value :: n_ele_total
+    type(c_ptr), value :: variant_offsets
+    integer(c_int32_t), value :: n_variants
+    type(bmx_ref) :: ref
+    type(bmx_config) :: cfg
+    type(c_ptr) :: lat
+    integer(c_int32_t) :: rc
+  end function
+
+  subroutine bmx_lattice_destroy(lat) bind(C, name = 'bmx_lattice_destroy')
+    import
+    type(c_ptr), value :: lat
+  end subroutine
+
+  function bmx_track_bunch(ctx, lat, ix_start, ix_end, n_turns, direction, p, stats) &
+                           bind(C, name = 'bmx_track_bunch') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    integer(c_int32_t), value :: ix_start, ix_end, n_turns, direction
+    type(bmx_particles) :: p
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_track_grid(ctx, lat, grid, n_turns, survival_turn, da_area, stats) &
+                          bind(C, name = 'bmx_track_grid') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    type(bmx_grid) :: grid
+    integer(c_int32_t), value :: n_turns
+    integer(c_int32_t) :: survival_turn(*)
+    real(c_double) :: da_area(*)
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
+end interface
+
+contains
+
+!+
+! Function bmx_error_string(ctx) result(str)
+! Copy of the context's last error message as a Fortran string.
+!-
+function bmx_error_string(ctx) result(str)
+  type(c_ptr), value :: ctx
+  character(:), allocatable :: str
+  character(kind=c_char), pointer :: chars(:)
+  type(c_ptr) :: p
+  integer n
+  interface
+    function c_strlen(s) bind(C, name = 'strlen') result(n)
+      import
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function
+  end interface
+  p = bmx_last_error(ctx)
+  n = int(c_strlen(p))
+  call c_f_pointer(p, chars, [n])
+  allocate(character(n) :: str)
+  str = transfer(chars(1:n), str)
+end function
+
+end module bmx_c_binding

@@ -27,8 +27,20 @@ extern "C" {
 long long oracle_flop_counter = 0;
 }
 #define ORACLE_API extern "C"
+#elif defined(ORACLE_DUAL)
+#define ORACLE_API extern "C"
 #else
 #define ORACLE_API
+#endif
+
+/* Time t is carried for the golden-vector checks only; libbmx does not track
+ * it, so its arithmetic is left out of the FLOP count. */
+#ifdef ORACLE_COUNT_FLOPS
+#define NOCOUNT_BEGIN long long nocount_saved_ = oracle_flop_counter
+#define NOCOUNT_END oracle_flop_counter = nocount_saved_
+#else
+#define NOCOUNT_BEGIN ((void)0)
+#define NOCOUNT_END ((void)0)
 #endif
 
 /* ------------------------------------------------------------------------ */
@@ -550,9 +562,11 @@ static void track_a_drift(o_coord *orb, real length, int ele_orientation, int in
     } else {
       dz = -length / ps_rel;
     }
+    NOCOUNT_BEGIN;
     dt = rel_z_vel * length / (orb->beta * ps_rel * O_C_LIGHT);
     orb->t = orb->t + dt;
     if (time) *time = *time + dt;
+    NOCOUNT_END;
   } else {
     if (include_ref_motion) {
       dz = length * (1 - 1 / ps_rel);
@@ -1552,8 +1566,10 @@ static void track_a_quadrupole(o_coord *orbit, const bmx_ele_rec *ele, int ref_s
 
   if (has_misalign(ele)) offset_particle(ele, ref_species, 0, orbit, 1, 0, -1);
 
+  NOCOUNT_BEGIN;
   orbit->t = start_orb.t + orbit->direction * orbit->time_dir * ele->delta_ref_time +
              (start_orb.vec[4] - orbit->vec[4]) / (orbit->beta * O_C_LIGHT);
+  NOCOUNT_END;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1607,8 +1623,10 @@ static void track_a_thick_multipole(o_coord *orbit, const bmx_ele_rec *ele, int 
 
   if (has_misalign(ele)) offset_particle(ele, ref_species, 0, orbit, 1, 0, -1);
 
+  NOCOUNT_BEGIN;
   orbit->t = start_orb.t + orbit->direction * orbit->time_dir * ele->delta_ref_time +
              (start_orb.vec[4] - orbit->vec[4]) / (orbit->beta * O_C_LIGHT);
+  NOCOUNT_END;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1865,8 +1883,10 @@ static void track_a_bend(o_coord *orbit, const bmx_ele_rec *ele, int ref_species
 
   if (has_misalign(ele)) offset_particle(ele, ref_species, 0, orbit, 1, 0, -1);
 
+  NOCOUNT_BEGIN;
   orbit->t = start_orb.t + orbit->direction * orbit->time_dir * ele->delta_ref_time +
              (start_orb.vec[4] - orbit->vec[4]) / (orbit->beta * O_C_LIGHT);
+  NOCOUNT_END;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2150,13 +2170,14 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
       for (i = 0; i < n; i++) {
         o_coord a, b;
         if (state[i] != BMX_ALIVE) continue;
-        a.vec[0] = x[i]; a.vec[1] = px[i]; a.vec[2] = y[i]; a.vec[3] = py[i]; a.vec[4] = z[i]; a.vec[5] = pz[i];
+        a.vec[0] = PDEP(x[i]); a.vec[1] = PDEP(px[i]); a.vec[2] = PDEP(y[i]); a.vec[3] = PDEP(py[i]);
+        a.vec[4] = PDEP(z[i]); a.vec[5] = PDEP(pz[i]);
         a.species = ref_species;
         a.direction = 1;
         a.time_dir = 1;
         a.state = BMX_ALIVE;
         a.p0c = ele->p0c_start;
-        a.beta = beta[i];
+        a.beta = PDEP(beta[i]);
         a.t = 0;
         a.s = 0;
         a.phase1 = 0;
@@ -2190,12 +2211,13 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
     o_coord a, b;
     int turn, ie;
     if (state[i] != BMX_ALIVE) continue;
-    a.vec[0] = x[i]; a.vec[1] = px[i]; a.vec[2] = y[i]; a.vec[3] = py[i]; a.vec[4] = z[i]; a.vec[5] = pz[i];
+    a.vec[0] = PDEP(x[i]); a.vec[1] = PDEP(px[i]); a.vec[2] = PDEP(y[i]); a.vec[3] = PDEP(py[i]);
+    a.vec[4] = PDEP(z[i]); a.vec[5] = PDEP(pz[i]);
     a.species = ref_species;
     a.direction = 1;
     a.time_dir = 1;
     a.state = BMX_ALIVE;
-    a.beta = beta[i];
+    a.beta = PDEP(beta[i]);
     a.t = 0;
     a.s = 0;
     a.phase1 = 0;
@@ -2219,6 +2241,47 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
   }
   return 0;
 }
+
+#ifdef ORACLE_DUAL
+/* One particle through n_turns of the element list with dual numbers: returns
+ * the final coordinates and the 6x6 Jacobian J[i*6+j] = d out_i / d in_j. */
+ORACLE_API int oracle_jacobian(const bmx_ele_rec *eles, int n_ele, int ref_species, int n_turns, const double *vec,
+                               double beta, double *out, double *J) {
+  o_coord a, b;
+  int turn, ie, i, j;
+  for (i = 0; i < 6; i++) {
+    a.vec[i] = real(vec[i]);
+    a.vec[i].d[i] = 1.0;
+  }
+  a.species = ref_species;
+  a.direction = 1;
+  a.time_dir = 1;
+  a.state = BMX_ALIVE;
+  a.t = 0;
+  a.s = 0;
+  a.phase1 = 0;
+  a.location = O_UPSTREAM;
+  {
+    /* beta as a function of pz: convert_pc_to, so its derivative is carried */
+    real pc = real(eles[0].p0c_start) * (1 + a.vec[5]);
+    real mc2 = mass_of(ref_species);
+    a.beta = (vec[5] == 0) ? real(beta) : pc / sqrt(pc * pc + mc2 * mc2);
+  }
+  for (turn = 0; turn < n_turns; turn++) {
+    for (ie = 0; ie < n_ele; ie++) {
+      a.p0c = eles[ie].p0c_start;
+      track1(&a, &eles[ie], ref_species, &b);
+      a = b;
+      if (a.state != BMX_ALIVE) return a.state;
+    }
+  }
+  for (i = 0; i < 6; i++) {
+    out[i] = a.vec[i].v;
+    for (j = 0; j < 6; j++) J[i * 6 + j] = a.vec[i].d[j];
+  }
+  return 0;
+}
+#endif
 
 #ifdef ORACLE_COUNT_FLOPS
 ORACLE_API long long oracle_flops_read_reset(void) {

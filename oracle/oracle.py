@@ -140,3 +140,37 @@ def abi_array(eles):
 
 def flops_read_reset():
     return lib_count().oracle_flops_read_reset()
+
+
+LIB_DUAL = os.path.join(HERE, "liboracle_dual.so")
+_lib_dual = None
+
+
+def jacobian(eles, species, n_turns, vec):
+    """Exact (dual-number) Jacobian of the oracle's n_turns map at vec.
+    Returns (out6, J 6x6); raises if the particle is lost."""
+    global _lib_dual
+    if _lib_dual is None:
+        if not os.path.exists(LIB_DUAL):
+            build()
+        _lib_dual = C.CDLL(LIB_DUAL)
+        _lib_dual.oracle_jacobian.restype = C.c_int
+        _lib_dual.oracle_jacobian.argtypes = [C.POINTER(abi.EleRec), C.c_int, C.c_int, C.c_int, _DP, C.c_double, _DP, _DP]
+    arr = eles if isinstance(eles, C.Array) else abi_array(eles)
+    v = (C.c_double * 6)(*vec)
+    out = (C.c_double * 6)()
+    J = (C.c_double * 36)()
+    beta0 = arr[0].p0c_start / arr[0].e_tot_start
+    rc = _lib_dual.oracle_jacobian(arr, len(arr), species, n_turns, v, beta0, out, J)
+    if rc != 0:
+        raise RuntimeError(f"particle lost (state {rc}) while forming the Jacobian")
+    return np.array(out[:]), np.array(J[:]).reshape(6, 6)
+
+
+def symp_error(M):
+    """mat_symp_error (sim_utils/matrix/mat_symp_error.f90:31-71): max |M^T S M - S|."""
+    S = np.zeros((6, 6))
+    for k in range(3):
+        S[2 * k, 2 * k + 1] = 1
+        S[2 * k + 1, 2 * k] = -1
+    return float(np.max(np.abs(M.T @ S @ M - S)))

@@ -1,0 +1,43 @@
+"""Shared test helpers: element construction from golden-fixture specs and
+parity metrics."""
+import numpy as np
+
+from bmx import abi, lattice
+
+FRINGE = {"full": abi.FRINGE_FULL, "hard_edge_only": abi.FRINGE_HARD_EDGE_ONLY, "basic_bend": abi.FRINGE_BASIC_BEND,
+          "none": abi.FRINGE_NONE}
+SPECIES = {"positron": abi.POSITRON, "electron": abi.ELECTRON, "antimuon": abi.ANTIMUON, "proton": abi.PROTON}
+NAMES = ("x", "px", "y", "py", "z", "pz")
+
+
+def ele_from_spec(spec, species, e_tot=None, p0c=None):
+    spec = dict(spec)
+    key = spec.pop("key")
+    if "fringe_type" in spec and isinstance(spec["fringe_type"], str):
+        spec["fringe_type"] = FRINGE[spec["fringe_type"]]
+    e = lattice.ele(key, **spec)
+    lattice.set_reference([e], species, e_tot=e_tot, p0c=p0c)
+    return e
+
+
+def printed_match(val, golden, digits=11):
+    """True if val rounds to the golden value printed with `digits` significant digits."""
+    if golden == 0:
+        return abs(val) < 5e-21
+    e = np.floor(np.log10(abs(golden)))
+    half_ulp = 0.5 * 10.0 ** (e - (digits - 1))
+    return abs(val - golden) <= half_ulp * 1.0001
+
+
+def rel_err(gpu, ref, mask=None):
+    """max over coordinates of max|gpu - ref| / max|ref| on the masked particles."""
+    worst = 0.0
+    for k in NAMES:
+        a, b = np.asarray(gpu[k]), np.asarray(ref[k])
+        if mask is not None:
+            a, b = a[mask], b[mask]
+        if a.size == 0:
+            continue
+        scale = np.max(np.abs(b)) + 1e-300
+        worst = max(worst, float(np.max(np.abs(a - b)) / scale))
+    return worst

