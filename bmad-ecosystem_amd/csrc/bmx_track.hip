@@ -72,17 +72,26 @@ __device__ __forceinline__ double d_modulo2(double x, double amp) {
 // ---------------------------------------------------------------------------
 
 // track_a_drift (bmad/low_level/track_a_drift.f90:25-116), forward, with reference motion.
-// One division: t = 1/(ps (ps+1)) gives 1/ps = (ps+1) t and sqrt_one(-pxy2)/ps = -pxy2 t.
+// With u = (px^2 + py^2)/(1+pz)^2 and S = 1/sqrt(1-u) - 1:
+//   x += L px_rel (1 + S),  z += L (beta/beta0 - 1 - S)   [sqrt_one(-u)/ps = -S].
+// For u < 1e-4 (angles below 10 mrad) S is summed as its binomial series to
+// u^5 (truncation < 3e-25), which needs no division or square root; larger
+// angles take the exact form.
 __device__ __forceinline__ void op_drift(Lane& L, double len) {
   double pxr = L.px * L.irp, pyr = L.py * L.irp;
-  double pxy2 = pxr * pxr + pyr * pyr;
-  if (pxy2 >= 1.0) { L.state = BMX_LOST_PZ; return; }
-  double ps = sqrt(1.0 - pxy2);
-  double t = 1.0 / (ps * (ps + 1.0));
-  double lps = len * ((ps + 1.0) * t);
+  double u = pxr * pxr + pyr * pyr;
+  double S;
+  if (u < 1e-4) {
+    S = u * (0.5 + u * (0.375 + u * (0.3125 + u * (0.2734375 + u * 0.24609375))));
+  } else {
+    if (u >= 1.0) { L.state = BMX_LOST_PZ; return; }
+    double ps = sqrt(1.0 - u);
+    S = u / (ps * (ps + 1.0));
+  }
+  double lps = len * (1.0 + S);
   L.x += lps * pxr;
   L.y += lps * pyr;
-  L.z += len * (L.bb - pxy2 * t);
+  L.z += len * (L.bb - S);
 }
 
 // drift with include_ref_motion = false (offset_particle's drift to the edge)
@@ -154,23 +163,43 @@ __device__ __forceinline__ double low_energy_dz(const Lane& L, const LatticeCons
 
 // One quadrupole body step: quad_mat2_calc (bmad/low_level/quad_mat2_calc.f90:22-81)
 // for both planes plus the quadratic z terms (track_a_quadrupole.f90:82-112).
+// With w = k L^2 for a plane of strength k: c = sum w^n/(2n)!, s/L = sum w^n/(2n+1)!
+// (cosh / sinh for k > 0, cos / sin for k < 0).  For |w| <= 1 the series to
+// n = 11 is exact to < 1e-21 and replaces sincos, expm1, sqrt and three
+// divisions; stronger quadrupoles take the exact trigonometric form.
+__device__ __forceinline__ void quad_cs_series(double w, double len, double& c, double& s) {
+  const double ic[12] = {1.0, 1.0 / 2, 1.0 / 24, 1.0 / 720, 1.0 / 40320, 1.0 / 3628800, 1.0 / 479001600,
+                         1.0 / 87178291200.0, 1.0 / 20922789888000.0, 1.0 / 6402373705728000.0,
+                         1.0 / 2432902008176640000.0, 1.0 / 1124000727777607680000.0};
+  const double is[12] = {1.0, 1.0 / 6, 1.0 / 120, 1.0 / 5040, 1.0 / 362880, 1.0 / 39916800, 1.0 / 6227020800.0,
+                         1.0 / 1307674368000.0, 1.0 / 355687428096000.0, 1.0 / 121645100408832000.0,
+                         1.0 / 51090942171709440000.0, 1.0 / 25852016738884976640000.0};
+  double cc = ic[11], ss = is[11];
+#pragma unroll
+  for (int n = 10; n >= 0; n--) {
+    cc = cc * w + ic[n];
+    ss = ss * w + is[n];
+  }
+  c = cc;
+  s = ss * len;
+}
+
 __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeConsts& lc) {
   double len = op.c[0];
   double rel_p = 1.0 + L.pz;
   double k1 = op.c[1] * L.irp;  // charge_dir b1 / (L rel_p)
   double kx = -k1, ky = k1;     // quad_mat2_calc(-k1, ...) for x, quad_mat2_calc(k1, ...) for y
-  double sqrt_k = sqrt(fabs(k1));
-  double skl = sqrt_k * len;
   double cx, sx, cy, sy;
-  if (fabs(skl) < 1e-10) {
-    cx = 1.0 + kx * len * len / 2.0;
-    sx = (1.0 + kx * len * len / 6.0) * len;
-    cy = 1.0 + ky * len * len / 2.0;
-    sy = (1.0 + ky * len * len / 6.0) * len;
+  double w = k1 * len * len;
+  if (fabs(w) <= 1.0) {
+    quad_cs_series(-w, len, cx, sx);
+    quad_cs_series(w, len, cy, sy);
   } else {
+    double sqrt_k = sqrt(fabs(k1));
+    double skl = sqrt_k * len;
     double sn, cs;
     sincos(skl, &sn, &cs);
-    double em1 = expm1(skl);  // cosh, sinh without cancellation at small skl
+    double em1 = expm1(skl);
     double ch = 1.0 + em1 * em1 / (2.0 * (em1 + 1.0));
     double sh = 0.5 * (em1 + em1 / (em1 + 1.0));
     double isk = 1.0 / sqrt_k;

@@ -2161,6 +2161,7 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
   int64_t i;
   int turn, ie;
   (void)n_threads;
+  c_multi_init(); /* lazily built table: build it before the parallel region */
   for (turn = 0; turn < n_turns; turn++) {
     for (ie = 0; ie < n_ele; ie++) {
       const bmx_ele_rec *ele = &eles[ie];
@@ -2204,6 +2205,7 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
                                            double *beta, int *state, int *ix_lost, int *turn_lost, int n_threads) {
   int64_t i;
   (void)n_threads;
+  c_multi_init(); /* lazily built table: build it before the parallel region */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64) num_threads(n_threads > 0 ? n_threads : 1)
 #endif

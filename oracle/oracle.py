@@ -46,8 +46,10 @@ def _setup(lib):
     return lib
 
 
+LIB_FMA = os.path.join(HERE, "liboracle_fma.so")
 _lib = None
 _lib_count = None
+_lib_fma = None
 
 
 def lib():
@@ -57,6 +59,15 @@ def lib():
             build()
         _lib = _setup(C.CDLL(LIB))
     return _lib
+
+
+def lib_fma():
+    global _lib_fma
+    if _lib_fma is None:
+        if not os.path.exists(LIB_FMA):
+            build()
+        _lib_fma = _setup(C.CDLL(LIB_FMA))
+    return _lib_fma
 
 
 def lib_count():
@@ -104,10 +115,12 @@ def beta_of(pz, p0c, mc2):
     return pc / np.sqrt(pc * pc + mc2 * mc2)
 
 
-def track_bunch(eles, species, n_turns, coords, beta=None, n_threads=1, particle_outer=False, counting=False):
+def track_bunch(eles, species, n_turns, coords, beta=None, n_threads=1, particle_outer=False, counting=False,
+                fma=False):
     """Track SoA coords (dict or (6, n) array) through the element array for n_turns.
-    Returns dict with x..pz, beta, state, ix_lost, turn_lost (copies)."""
-    L = lib_count() if counting else lib()
+    Returns dict with x..pz, beta, state, ix_lost, turn_lost (copies).
+    fma=True uses the FMA-contracted build (a second FP evaluation order)."""
+    L = lib_count() if counting else (lib_fma() if fma else lib())
     arr = eles if isinstance(eles, C.Array) else abi_array(eles)
     c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)
     n = c.shape[1]

@@ -29,6 +29,24 @@ def printed_match(val, golden, digits=11):
     return abs(val - golden) <= half_ulp * 1.0001
 
 
+def coord_errors(gpu, ref, mask, path_length):
+    """Per-coordinate parity metric: max|gpu - ref| / scale with scale = max|ref|
+    for x, px, y, py, pz and scale = max(max|z|, 1e-3 * path_length) for z.
+    z is the small difference of path lengths of order the distance tracked,
+    so its attainable accuracy is set by that distance, not by |z|."""
+    out = {}
+    for k in NAMES:
+        a, b = np.asarray(gpu[k])[mask], np.asarray(ref[k])[mask]
+        if a.size == 0:
+            out[k] = 0.0
+            continue
+        scale = np.max(np.abs(b))
+        if k == "z":
+            scale = max(scale, 1e-3 * path_length)
+        out[k] = float(np.max(np.abs(a - b)) / (scale + 1e-300))
+    return out
+
+
 def rel_err(gpu, ref, mask=None):
     """max over coordinates of max|gpu - ref| / max|ref| on the masked particles."""
     worst = 0.0
