@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--particles", type=int, default=N_PARTICLES)
     ap.add_argument("--turns-per-step", type=int, default=TURNS_PER_STEP)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device copy (reference for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); particle tracking "
@@ -74,6 +76,16 @@ def main():
 
     import torch
     from bmx import abi, beam, lattice, tracker
+
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": src.numel() * 8, "copy_calibration_bytes_written":
+                          dst.numel() * 8, "launches": 3}), flush=True)
+        return
 
     eles = lattice.ring_c1(n_cells=500, n_rf=8)
     n_ele = len(eles)

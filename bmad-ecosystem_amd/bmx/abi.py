@@ -79,7 +79,7 @@ class Stats(C.Structure):
 
 
 class Grid(C.Structure):
-    _fields_ = [("n_angle", C.c_int32), ("n_amp", C.c_int32), ("n_pz", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("n_angle", C.c_int32), ("n_amp", C.c_int32), ("n_pz", C.c_int32), ("turns_per_block", C.c_int32),
                 ("amp_max_x", C.c_double), ("amp_ratio_y", C.c_double), ("pz_min", C.c_double),
                 ("pz_max", C.c_double), ("angle_min", C.c_double), ("angle_max", C.c_double)]
 
@@ -91,6 +91,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
     "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_track_bunch", "bmx_track_grid",
+    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_ctx_sync",
 ]
 
 _lib = None
@@ -125,6 +126,21 @@ def load_library(path=None):
     lib.bmx_track_grid.restype = C.c_int32
     lib.bmx_track_grid.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_double), C.POINTER(Stats)]
+    lib.bmx_gemm_bf16.restype = C.c_int32
+    lib.bmx_gemm_bf16.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                  C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                  C.c_int32, C.c_void_p]
+    lib.bmx_colsum_f32.restype = C.c_int32
+    lib.bmx_colsum_f32.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.bmx_adam_step.restype = C.c_int32
+    lib.bmx_adam_step.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32]
+    lib.bmx_f32_to_bf16.restype = C.c_int32
+    lib.bmx_f32_to_bf16.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    lib.bmx_mse_grad.restype = C.c_int32
+    lib.bmx_mse_grad.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.bmx_ctx_sync.restype = C.c_int32
+    lib.bmx_ctx_sync.argtypes = [C.c_void_p]
     if path is None:
         _lib = lib
     return lib

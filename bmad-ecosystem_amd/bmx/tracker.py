@@ -181,10 +181,12 @@ def track_bunch_host(lat, coords, n_turns=1, beta=None):
 
 
 def track_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, pz_min=0.0, pz_max=0.0,
-               angle_min=0.0, angle_max=np.pi):
+               angle_min=0.0, angle_max=np.pi, turns_per_block=0, return_stats=False):
     """Batched DA grid over every variant of `lat` (bmx_track_grid).
-    Returns (survival[n_var, n_pz, n_angle, n_amp], da_area[n_var], kernel_ms)."""
-    g = abi.Grid(n_angle, n_amp, n_pz, 0, amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min, angle_max)
+    Returns (survival[n_var, n_pz, n_angle, n_amp], da_area[n_var], kernel_ms)
+    (plus the stats dict when return_stats)."""
+    g = abi.Grid(n_angle, n_amp, n_pz, int(turns_per_block), amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min,
+                 angle_max)
     n_ic = n_angle * n_amp * n_pz
     surv = np.zeros(lat.n_variants * n_ic, dtype=np.int32)
     area = np.zeros(lat.n_variants, dtype=np.float64)
@@ -193,4 +195,7 @@ def track_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, p
                                     surv.ctypes.data_as(C.POINTER(C.c_int32)),
                                     area.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st))
     _check(lat.ctx.h, rc, "bmx_track_grid")
-    return surv.reshape(lat.n_variants, n_pz, n_angle, n_amp), area, st.kernel_ms
+    out = (surv.reshape(lat.n_variants, n_pz, n_angle, n_amp), area, st.kernel_ms)
+    if return_stats:
+        return out + (dict(pet=st.pet, n_live=st.n_live, n_lost=st.n_lost, kernel_ms=st.kernel_ms),)
+    return out

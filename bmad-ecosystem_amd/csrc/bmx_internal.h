@@ -53,12 +53,24 @@ struct alignas(64) DevOp {
 static_assert(sizeof(DevOp) == 128, "DevOp must be 128 bytes");
 
 // Per-lattice constants shared by every op (one reference particle, constant p0c).
+// Everything the kernels would otherwise derive from these with FP64
+// arithmetic is precomputed here: such values would be hoisted out of the
+// element loop into long-lived vector registers (there is no scalar FP64 ALU),
+// while kernel-argument constants stay in scalar registers.
 struct LatticeConsts {
   double p0c;        // reference momentum [eV]
   double e_tot;      // reference energy [eV]
   double mc2;        // particle mass [eV]
   double beta0;      // p0c / e_tot
   double max_aperture_limit;
+  double mc2sq;      // mc2^2
+  double p0c_sq;     // p0c^2
+  double lez_k;      // mc2 * beta0^2       (low_energy_z_correction branch test)
+  double lez_thr;    // 3e-7 * e_tot
+  double lez_a;      // 3/2 beta0^2
+  double lez_f;      // beta0^2 (2 beta0^2 - (mc2/e_tot)^2 / 2)
+  double lez_me2;    // (mc2/e_tot)^2
+  double two_max_ap; // 2 * max_aperture_limit
   int32_t aperture_limit_on;
   int32_t species;
 };

@@ -237,6 +237,16 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
   lc->mc2 = mc2;
   lc->beta0 = p0c / e_tot;
   lc->species = ref.species;
+  lc->mc2sq = mc2 * mc2;
+  lc->p0c_sq = p0c * p0c;
+  {
+    double b0 = lc->beta0, me = mc2 / e_tot;
+    lc->lez_k = mc2 * b0 * b0;
+    lc->lez_thr = 3e-7 * e_tot;
+    lc->lez_a = 1.5 * b0 * b0;
+    lc->lez_f = b0 * b0 * (2.0 * b0 * b0 - me * me / 2.0);
+    lc->lez_me2 = me * me;
+  }
 
   for (int ie = 0; ie < n_ele; ie++) {
     const bmx_ele_rec& e = eles[ie];

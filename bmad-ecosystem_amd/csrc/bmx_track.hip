@@ -14,6 +14,7 @@
 // per-particle constants between energy kicks).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -54,8 +55,7 @@ __device__ __forceinline__ void refresh_invariants(Lane& L, const LatticeConsts&
   double rel = 1.0 + L.pz;
   L.irp = 1.0 / rel;
   double pc = lc.p0c * rel;
-  double mc2sq = lc.mc2 * lc.mc2;
-  L.bb = d_sqrt_one((mc2sq * (2.0 * L.pz + L.pz * L.pz)) / (pc * pc + mc2sq));
+  L.bb = d_sqrt_one((lc.mc2sq * (2.0 * L.pz + L.pz * L.pz)) / (pc * pc + lc.mc2sq));
 }
 
 // modulo2 (sim_utils/special_functions/modulo2_mod.f90:85-100)
@@ -108,7 +108,7 @@ __device__ __forceinline__ void op_drift_noref(Lane& L, double len) {
 // Thin multipole kick, ab form (bmad/modules/multipole_mod.f90:82-174, :314-401):
 // K = sum_n c_n (x + i y)^n, dpx = -Re K, dpy = Im K, Horner in complex FP64.
 __device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, double& kr, double& ki) {
-  int nh = op.flags;  // Horner steps
+  int nh = __builtin_amdgcn_readfirstlane(op.flags);  // Horner steps
   kr = op.c[2 * nh];
   ki = op.c[2 * nh + 1];
   for (int j = nh - 1; j >= 0; j--) {
@@ -117,7 +117,7 @@ __device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, d
     kr = tr;
     ki = ti;
   }
-  int lo = op.n;  // multiply by (x + i y)^lo
+  int lo = __builtin_amdgcn_readfirstlane(op.n);  // multiply by (x + i y)^lo
   for (int j = 0; j < lo; j++) {
     double tr = kr * x - ki * y;
     double ti = kr * y + ki * x;
@@ -152,13 +152,11 @@ __device__ __forceinline__ void op_kick_ref0(Lane& L, const DevOp& op) {
 
 // low_energy_z_correction (bmad/low_level/low_energy_z_correction.f90:22-58)
 __device__ __forceinline__ double low_energy_dz(const Lane& L, const LatticeConsts& lc, double ds) {
-  double pz = L.pz, b0 = lc.beta0;
-  if (lc.mc2 * (b0 * pz) * (b0 * pz) < 3e-7 * lc.e_tot) {
-    double me = lc.mc2 / lc.e_tot;
-    double f = b0 * b0 * (2.0 * b0 * b0 - me * me / 2.0);
-    return ds * pz * (1.0 - 3.0 * (pz * b0 * b0) / 2.0 + pz * pz * f) * (me * me);
+  double pz = L.pz;
+  if (lc.lez_k * (pz * pz) < lc.lez_thr) {
+    return ds * pz * (1.0 - lc.lez_a * pz + pz * pz * lc.lez_f) * lc.lez_me2;
   }
-  return ds * (L.beta - b0) / b0;
+  return ds * (L.beta - lc.beta0) / lc.beta0;
 }
 
 // One quadrupole body step: quad_mat2_calc (bmad/low_level/quad_mat2_calc.f90:22-81)
@@ -221,36 +219,110 @@ __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeC
 // sinc (sim_utils/special_functions/sinc.f90)
 __device__ __forceinline__ double d_sinc(double x) { return (fabs(x) < 1e-8) ? 1.0 : sin(x) / x; }
 
-// Exact sector-bend body step (track_a_bend.f90:120-209).
+// Short Taylor series for the small angles of the bend body.  Each is written
+// as x + x^3 P(x^2), so the leading term enters exactly and the last addition
+// rounds once: the result is correctly rounded in nearly every case, like the
+// host libm calls the reference makes (those angles enter the exit px through
+// a difference of O(angle) terms, so one ulp there is visible downstream).
+
+// asin(s), |s| <= 0.05, to s^13 (truncation < 1e-19 relative).
+__device__ __forceinline__ double asin_small(double s) {
+  double s2 = s * s;
+  return s + s * s2 * (1.0 / 6 + s2 * (3.0 / 40 + s2 * (5.0 / 112 + s2 * (35.0 / 1152 + s2 * (63.0 / 2816 +
+             s2 * (231.0 / 13312))))));
+}
+
+// sin(e), |e| <= 0.2, to e^11 (truncation < 1e-18 relative).
+__device__ __forceinline__ double sin_small(double e) {
+  double e2 = e * e;
+  return e + e * e2 * (-1.0 / 6 + e2 * (1.0 / 120 - e2 * (1.0 / 5040 - e2 * (1.0 / 362880 - e2 * (1.0 / 39916800)))));
+}
+
+// cos(e), |e| <= 0.2, to e^12 (truncation < 1e-20).
+__device__ __forceinline__ double cos_small(double e) {
+  double e2 = e * e;
+  return 1.0 + e2 * (-0.5 + e2 * (1.0 / 24 - e2 * (1.0 / 720 - e2 * (1.0 / 40320 - e2 * (1.0 / 3628800 -
+               e2 * (1.0 / 479001600))))));
+}
+
+// atan(t), |t| <= 0.2, to t^13 (truncation < 1e-11 * t^14, < 2e-21 relative).
+__device__ __forceinline__ double atan_small(double t) {
+  double t2 = t * t;
+  return t + t * t2 * (-1.0 / 3 + t2 * (1.0 / 5 - t2 * (1.0 / 7 - t2 * (1.0 / 9 - t2 * (1.0 / 11 - t2 * (1.0 / 13))))));
+}
+
+// x / sin(x) = 1 / sinc(x), |x| <= 0.1, to x^14 (truncation < 1e-20).
+__device__ __forceinline__ double inv_sinc_small(double x) {
+  double x2 = x * x;
+  return 1.0 + x2 * (1.0 / 6 + x2 * (7.0 / 360 + x2 * (31.0 / 15120 + x2 * (127.0 / 604800 + x2 * (73.0 / 3421440 +
+         x2 * (1414477.0 / 653837184000.0 + x2 * (8191.0 / 37362124800.0)))))));
+}
+
+// The reference's transcendental calls for the bend entry and exit angles
+// (track_a_bend.f90:172-177, 205-207) for large angles.  Kept out of line:
+// rare, and inlined they would raise the register allocation of the whole
+// element loop.
+struct BendAngles { double phi_1, sin_plus, cos_plus; };
+struct BendExit { double L_p, se; };
+
+__device__ __noinline__ BendAngles bend_entry_exact(double sphi, double angle) {
+  BendAngles b;
+  b.phi_1 = asin(sphi);
+  b.sin_plus = sin(angle + b.phi_1);
+  b.cos_plus = cos(angle + b.phi_1);
+  return b;
+}
+
+__device__ __noinline__ BendExit bend_exit_exact(double phi_1, double L_u, double L_v, double angle, double L_c) {
+  double angle_p = 2.0 * (angle + phi_1 - atan2(L_u, -L_v));
+  return BendExit{L_c / d_sinc(angle_p / 2.0), sin(phi_1 + angle - angle_p)};
+}
+
+// Exact sector-bend body step (track_a_bend.f90:120-209), in the reference's
+// order of operations: phi_1 = asin(px/pt), sin/cos(angle + phi_1), the exit
+// offset xi, angle_p = 2 (angle + phi_1 - atan2(L_u, -L_v)),
+// px = pt sin(phi_1 + angle - angle_p), L_p = L_c / sinc(angle_p / 2).
+// The transcendental calls become the short series above when their
+// arguments are small (the usual case); otherwise the libm calls are made.
 // c: 0 step_len, 1 g, 2 g_tot, 3 angle, 4 cos a, 5 sin a, 6 sinc a, 7 cosc a, 8 sincc a, 9 beta_ref, 10 dg
 __device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeConsts& lc) {
   double sl = op.c[0], g = op.c[1], g_tot = op.c[2], angle = op.c[3];
   double cos_a = op.c[4], sin_a = op.c[5], sinc_a = op.c[6], cosc_a = op.c[7];
-  double x = L.x, px = L.px, y = L.y, py = L.py, z = L.z, pz = L.pz;
+  double x = L.x, px = L.px, py = L.py, pz = L.pz;
   double rel_p = 1.0 + pz;
   if (op.c[10] == 0.0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 && fabs(pz) < 1e-9) {
     double pc = rel_p * lc.p0c;
-    double gam2 = lc.mc2 * lc.mc2 / (pc * pc + lc.mc2 * lc.mc2);
+    double gam2 = lc.mc2sq / (pc * pc + lc.mc2sq);
     double m56 = sl * (gam2 - (g * sl) * (g * sl) * op.c[8]);
     L.x = cos_a * x + sl * sinc_a * px + g * sl * sl * cosc_a * pz;
     L.px = -g * sin_a * x + cos_a * px + g * sl * sinc_a * pz;
-    L.y = y + sl * py;
-    L.z = -g * sl * sinc_a * x - g * sl * sl * cosc_a * px + z + m56 * pz;
+    L.y = L.y + sl * py;
+    L.z = -g * sl * sinc_a * x - g * sl * sl * cosc_a * px + L.z + m56 * pz;
     return;
   }
   double pt = sqrt(rel_p * rel_p - py * py);
   if (fabs(px) > pt) { L.state = BMX_LOST_PZ; return; }
   double ipt = 1.0 / pt;
   double g_p = g_tot * ipt;
-  double phi_1 = asin(px * ipt);
-  double sin_plus, cos_plus;
-  sincos(angle + phi_1, &sin_plus, &cos_plus);
+  double sphi = px / pt;
+  double phi_1, sin_plus, cos_plus;
+  if (fabs(sphi) <= 0.05 && fabs(angle) <= 0.15) {
+    phi_1 = asin_small(sphi);
+    double u = angle + phi_1;
+    sin_plus = sin_small(u);
+    cos_plus = cos_small(u);
+  } else {
+    BendAngles b = bend_entry_exact(sphi, angle);
+    phi_1 = b.phi_1;
+    sin_plus = b.sin_plus;
+    cos_plus = b.cos_plus;
+  }
   double q = (1.0 + g * x) * sl * sinc_a;
   double alpha = 2.0 * (1.0 + g * x) * sin_plus * sl * sinc_a - g_p * q * q;
   double r = cos_plus * cos_plus + g_p * alpha;
   if (r < 0.0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
     L.state = BMX_LOST;
-    L.x = 2.0 * lc.max_aperture_limit;
+    L.x = lc.two_max_ap;
     return;
   }
   double rad = sqrt(r);
@@ -263,11 +335,25 @@ __device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeC
   double L_u = xi;
   double L_v = -(sl * sinc_a + x * sin_a);
   double L_c = sqrt(L_v * L_v + L_u * L_u);
-  double angle_p = 2.0 * (angle + phi_1 - atan2(L_u, -L_v));
-  double L_p = L_c / d_sinc(angle_p / 2.0);
-  L.px = pt * sin(phi_1 + angle - angle_p);
-  L.y = y + py * L_p * ipt;
-  L.z = z + L.beta * sl / op.c[9] - rel_p * L_p * ipt;
+  double L_p, se;
+  bool small = L_v < 0.0 && fabs(L_u) <= -0.2 * L_v;
+  if (small) {
+    double half_p = angle + phi_1 - atan_small(L_u / -L_v);  // angle_p / 2
+    double e = phi_1 + angle - 2.0 * half_p;                 // phi_1 + angle - angle_p
+    small = fabs(half_p) <= 0.1 && fabs(e) <= 0.2;
+    if (small) {
+      L_p = (fabs(half_p) < 1e-8) ? L_c : L_c * inv_sinc_small(half_p);
+      se = sin_small(e);
+    }
+  }
+  if (!small) {
+    BendExit b = bend_exit_exact(phi_1, L_u, L_v, angle, L_c);
+    L_p = b.L_p;
+    se = b.se;
+  }
+  L.px = pt * se;
+  L.y = L.y + py * L_p * ipt;
+  L.z = L.z + L.beta * sl / op.c[9] - rel_p * L_p * ipt;
 }
 
 // sbend_body_with_k1_map (track_a_bend.f90:500-659), forward, orientation +1.
@@ -328,7 +414,7 @@ __device__ __forceinline__ void op_hwang(Lane& L, const DevOp& op) {
   double ef = L.irp;
   double v0 = L.x, v1 = L.px, v2 = L.y, v3 = L.py;
   double dx, dpx, dy, dpy, dz;
-  if (op.flags & F_HWANG_ENTER) {
+  if (__builtin_amdgcn_readfirstlane(op.flags) & F_HWANG_ENTER) {
     dx = (-gt2 * v0 * v0 + gs2 * v2 * v2) * ef / 2.0;
     dpx = (gt * g_tot * (1.0 + 2.0 * te * te) * v2 * v2 / 2.0 + gt2 * (v0 * v1 - v2 * v3) + k1t * (v0 * v0 - v2 * v2)) * ef;
     dy = gt2 * v0 * v2 * ef;
@@ -383,7 +469,7 @@ __device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set) {
     r2 = 0.0 + op.c[3];  // r(3) - ds_center with ds_center = -L/2
   }
   double s_pos;  // s_body (set) or s_lab (unset)
-  if (op.flags & F_OFFSET_PITCH) {
+  if (__builtin_amdgcn_readfirstlane(op.flags) & F_OFFSET_PITCH) {
     const double* w = &op.c[4];
     double n0 = w[0] * r0 + w[1] * r1 + w[2] * r2;
     double n1 = w[3] * r0 + w[4] * r1 + w[5] * r2;
@@ -436,7 +522,7 @@ __device__ __forceinline__ void energy_kick(Lane& L, double dE, const LatticeCon
 // c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f); n = n_slice
 __device__ void op_rfcavity(Lane& L, const DevOp& op, const LatticeConsts& lc) {
   double len = op.c[0], volt = op.c[1], freq = op.c[3], half_t = op.c[4];
-  int n_slice = op.n;
+  int n_slice = __builtin_amdgcn_readfirstlane(op.n);
   double time = -L.z / (L.beta * D_C_LIGHT);
   if (freq != 0.0 && !(time >= -half_t && time < half_t)) time = d_modulo2(time, half_t);
   double phase = d_modulo2(D_TWOPI * (op.c[2] - time * freq), D_PI);
@@ -476,8 +562,8 @@ __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
 
 // aperture part of check_aperture_limit (bmad/low_level/check_aperture_limit.f90:257-309)
 // c: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim, 4 wx, 5 cx, 6 wy, 7 cy
-__device__ __forceinline__ void aperture(Lane& L, const DevOp& op) {
-  if (op.flags & F_APERTURE_ELLIPTICAL) {
+__device__ __forceinline__ void aperture(Lane& L, const DevOp& op, int flags) {
+  if (flags & F_APERTURE_ELLIPTICAL) {
     double xp = L.x - op.c[5], yp = L.y - op.c[7];
     double ax = xp / op.c[4], ay = yp / op.c[6];
     double r = ax * ax + ay * ay;
@@ -494,39 +580,101 @@ __device__ __forceinline__ void aperture(Lane& L, const DevOp& op) {
 }
 
 // ---------------------------------------------------------------------------
-// one element-program pass for one lane; returns false when the lane is lost
+// one op on one lane
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool run_ops(Lane& L, const DevOp* __restrict__ ops, int lo, int hi,
-                                        const LatticeConsts& lc, int& ie_lost) {
-  for (int k = lo; k < hi; k++) {
-    const DevOp& op = ops[k];
-    switch (op.type) {
-      case OP_DRIFT: op_drift(L, op.c[0]); break;
-      case OP_KICK: op_kick(L, op); break;
-      case OP_KICK_REF0: op_kick_ref0(L, op); break;
-      case OP_QUAD: op_quad(L, op, lc); break;
-      case OP_BEND: op_bend(L, op, lc); break;
-      case OP_BEND_K1: op_bend_k1(L, op, lc); break;
-      case OP_BEND_KICK: op_bend_kick(L, op); break;
-      case OP_HWANG: op_hwang(L, op); break;
-      case OP_LINEAR_EDGE: op_linear_edge(L, op); break;
-      case OP_TILT: op_tilt(L, op); break;
-      case OP_OFFSET_SET: op_offset(L, op, true); break;
-      case OP_OFFSET_UNSET: op_offset(L, op, false); break;
-      case OP_TOO_LARGE: too_large(L, lc); break;
-      case OP_RFCAVITY: op_rfcavity(L, op, lc); break;
-      case OP_APERTURE_IN: aperture(L, op); break;
-      case OP_ELE_END:
-        if (L.state == BMX_ALIVE && !too_large(L, lc) && (op.flags & F_HAS_APERTURE)) aperture(L, op);
-        break;
-      default: break;
-    }
-    if (L.state != BMX_ALIVE) {
-      ie_lost = op.ie;
-      return false;
+__device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, const LatticeConsts& lc) {
+  switch (type) {
+    case OP_DRIFT: op_drift(L, op.c[0]); break;
+    case OP_KICK: op_kick(L, op); break;
+    case OP_KICK_REF0: op_kick_ref0(L, op); break;
+    case OP_QUAD: op_quad(L, op, lc); break;
+    case OP_BEND: op_bend(L, op, lc); break;
+    case OP_BEND_K1: op_bend_k1(L, op, lc); break;
+    case OP_BEND_KICK: op_bend_kick(L, op); break;
+    case OP_HWANG: op_hwang(L, op); break;
+    case OP_LINEAR_EDGE: op_linear_edge(L, op); break;
+    case OP_TILT: op_tilt(L, op); break;
+    case OP_OFFSET_SET: op_offset(L, op, true); break;
+    case OP_OFFSET_UNSET: op_offset(L, op, false); break;
+    case OP_TOO_LARGE: too_large(L, lc); break;
+    case OP_RFCAVITY: op_rfcavity(L, op, lc); break;
+    case OP_APERTURE_IN: aperture(L, op, flags); break;
+    case OP_ELE_END:
+      if (L.state == BMX_ALIVE && !too_large(L, lc) && (flags & F_HAS_APERTURE)) aperture(L, op, flags);
+      break;
+    default: break;
+  }
+}
+
+// Element programs are staged through LDS: the workgroup copies OPS_CHUNK
+// records at a time (16 KiB) and every wave reads them at wave-uniform LDS
+// addresses, instead of each CU fetching every record from L2 through its
+// scalar cache on the dependent path of the element loop.
+#ifndef BMX_LDS_OPS
+#define BMX_LDS_OPS 1
+#endif
+constexpr int OPS_CHUNK = 128;
+
+// Walk of `turns` turns over ops [lo1, hi1) then [lo2, hi2) for one lane.
+// The whole workgroup calls it together (it holds barriers).  On return
+// `alive` is false for a lane lost on the way, with ie_lost / lost_turn set;
+// `ran` counts the ops of the lane's last turn it executed (for PET counts).
+__device__ __forceinline__ void walk(Lane& L, bool& alive, int& ie_lost, int& lost_turn, const DevOp* __restrict__ ops,
+                                     int lo1, int hi1, int lo2, int hi2, int turns, const LatticeConsts& lc,
+                                     DevOp* sops) {
+#if BMX_LDS_OPS
+  for (int t = 0; t < turns; t++) {
+    for (int seg = 0; seg < 2; seg++) {
+      int lo = seg ? lo2 : lo1, hi = seg ? hi2 : hi1;
+      for (int c = lo; c < hi; c += OPS_CHUNK) {
+        int m = min(OPS_CHUNK, hi - c);
+        // barrier: the previous chunk is consumed; leave together once the workgroup has no live lane
+        if (!__syncthreads_or(alive)) return;
+        const uint4* src = reinterpret_cast<const uint4*>(ops + c);
+        uint4* dst = reinterpret_cast<uint4*>(sops);
+        constexpr int W = (int)(sizeof(DevOp) / sizeof(uint4));
+        for (int e = threadIdx.x; e < m * W; e += blockDim.x) dst[e] = src[e];
+        __syncthreads();
+        if (alive) {
+          for (int k = 0; k < m; k++) {
+            const DevOp& op = sops[k];
+            int type = __builtin_amdgcn_readfirstlane(op.type);
+            int flags = __builtin_amdgcn_readfirstlane(op.flags);
+            run_op(L, op, type, flags, lc);
+            if (L.state != BMX_ALIVE) {
+              alive = false;
+              ie_lost = op.ie;
+              lost_turn = t;
+              break;
+            }
+          }
+        }
+      }
     }
   }
-  return true;
+#else
+  (void)sops;
+  for (int t = 0; alive && t < turns; t++) {
+    int k = lo1, end = hi1;
+    bool second = hi2 > lo2;
+    while (true) {
+      if (k >= end) {
+        if (!second) break;
+        k = lo2; end = hi2; second = false;
+        continue;
+      }
+      const DevOp& op = ops[k];
+      run_op(L, op, op.type, op.flags, lc);
+      if (L.state != BMX_ALIVE) {
+        alive = false;
+        ie_lost = op.ie;
+        lost_turn = t;
+        break;
+      }
+      k++;
+    }
+  }
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -537,50 +685,63 @@ struct SoA {
   int32_t *state, *ix_lost, *turn_lost;
 };
 
+// Waves per SIMD the register allocation targets (occupancy); the lane
+// program is FP64-latency bound, so occupancy is traded against spills.
+#ifndef BMX_WAVES_PER_EU
+#define BMX_WAVES_PER_EU 4
+#endif
+
 // Multi-turn bunch tracking: segments [lo1, hi1) then [lo2, hi2) per turn.
-__global__ __launch_bounds__(256) void bmx_track_bunch_kernel(const DevOp* __restrict__ ops, int lo1, int hi1,
-                                                              int lo2, int hi2, LatticeConsts lc, int n_turns,
-                                                              SoA p, int64_t n, unsigned long long* __restrict__ n_lost) {
+__global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_bunch_kernel(
+    const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2, LatticeConsts lc, int n_turns, SoA p,
+    int64_t n, unsigned long long* __restrict__ n_lost) {
+  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  bool valid = i < n;
   Lane L;
-  L.state = p.state[i];
-  if (L.state != BMX_ALIVE) return;
-  L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
-  refresh_invariants(L, lc);
-  int ie_lost = -1, turn = 0;
-  bool alive = !too_large(L, lc);
-  if (!alive) ie_lost = ops[lo1].ie;
-  for (turn = 0; alive && turn < n_turns; turn++) {
-    alive = run_ops(L, ops, lo1, hi1, lc, ie_lost);
-    if (alive && hi2 > lo2) alive = run_ops(L, ops, lo2, hi2, lc, ie_lost);
-    if (!alive) break;
+  L.state = valid ? p.state[i] : BMX_LOST;
+  bool entered = L.state == BMX_ALIVE;  // lanes already lost are left untouched
+  bool alive = false;
+  int ie_lost = -1, lost_turn = 0;
+  if (entered) {
+    L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
+    refresh_invariants(L, lc);
+    alive = !too_large(L, lc);
+    if (!alive) ie_lost = ops[lo1].ie;
   }
+  walk(L, alive, ie_lost, lost_turn, ops, lo1, hi1, lo2, hi2, n_turns, lc, sops);
+  if (!entered) return;
   p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
   p.state[i] = L.state;
   if (L.state != BMX_ALIVE) {
     if (p.ix_lost) p.ix_lost[i] = ie_lost;
-    if (p.turn_lost) p.turn_lost[i] = turn;
+    if (p.turn_lost) p.turn_lost[i] = lost_turn;
     atomicAdd(n_lost, 1ull);
   }
 }
 
-// Batched grid tracking: blockIdx.y = variant; each lane builds its initial
-// condition from the polar grid, tracks n_turns full turns of its variant and
-// writes the number of surviving turns.
+// Batched grid tracking in turn blocks with lane compaction.  Particles of
+// variant v occupy slots [v * n_ic, v * n_ic + count[v]) of a structure-of-
+// arrays store; blockIdx.y = variant, so every wave walks one variant's
+// program.  A block tracks up to `turns` turns; survivors are appended to the
+// destination store through one atomic per wave (ballot + lane prefix), lost
+// particles write their survival turn.  The next block launches only over the
+// survivors, so waves stay full as the grid loses particles.
 struct GridDev {
   int n_angle, n_amp, n_pz, n_ic;
   double amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min, angle_max;
 };
 
-__global__ __launch_bounds__(256) void bmx_track_grid_kernel(const DevOp* __restrict__ ops,
-                                                             const int32_t* __restrict__ var_op_begin,
-                                                             LatticeConsts lc, GridDev g, int n_turns,
-                                                             int32_t* __restrict__ survival) {
+struct GridSoA {
+  double *x, *px, *y, *py, *z, *pz, *beta;
+  int32_t* ic;
+};
+
+__global__ __launch_bounds__(256) void bmx_grid_init_kernel(GridDev g, LatticeConsts lc, GridSoA s,
+                                                           int32_t* __restrict__ survival, int n_turns) {
   int v = blockIdx.y;
   int ic = blockIdx.x * blockDim.x + threadIdx.x;
   if (ic >= g.n_ic) return;
-  // ic = (i_pz * n_angle + i_angle) * n_amp + i_amp
   int i_amp = ic % g.n_amp;
   int i_ang = (ic / g.n_amp) % g.n_angle;
   int i_pz = ic / (g.n_amp * g.n_angle);
@@ -588,24 +749,62 @@ __global__ __launch_bounds__(256) void bmx_track_grid_kernel(const DevOp* __rest
   double th = (g.n_angle > 1) ? g.angle_min + (g.angle_max - g.angle_min) * (double)i_ang / (double)(g.n_angle - 1)
                               : g.angle_min;
   double pz0 = (g.n_pz > 1) ? g.pz_min + (g.pz_max - g.pz_min) * (double)i_pz / (double)(g.n_pz - 1) : g.pz_min;
+  int64_t k = (int64_t)v * g.n_ic + ic;
+  s.x[k] = amp * cos(th);
+  s.y[k] = amp * g.amp_ratio_y * sin(th);
+  s.px[k] = 0.0; s.py[k] = 0.0; s.z[k] = 0.0; s.pz[k] = pz0;
+  double pc = lc.p0c * (1.0 + pz0);
+  s.beta[k] = (pz0 == 0.0) ? lc.beta0 : pc / sqrt(pc * pc + lc.mc2sq);
+  s.ic[k] = ic;
+  survival[k] = n_turns;
+}
+
+__global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
+    const DevOp* __restrict__ ops, const int32_t* __restrict__ var_op_begin, const int32_t* __restrict__ var_n_ele,
+    LatticeConsts lc, int n_ic, GridSoA src, GridSoA dst, const int32_t* __restrict__ count_in,
+    int32_t* __restrict__ count_out, int turn0, int turns, int32_t* __restrict__ survival,
+    unsigned long long* __restrict__ pet) {
+  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
+  int v = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = i < count_in[v];
+  bool alive = false;
+  int64_t base = (int64_t)v * n_ic;
   Lane L;
-  L.x = amp * cos(th);
-  L.y = amp * g.amp_ratio_y * sin(th);
-  L.px = 0.0; L.py = 0.0; L.z = 0.0; L.pz = pz0;
-  {
-    double pc = lc.p0c * (1.0 + pz0);
-    L.beta = (pz0 == 0.0) ? lc.beta0 : pc / sqrt(pc * pc + lc.mc2 * lc.mc2);
+  int ic = 0, turn = 0, ie_lost = -1;
+  if (active) {
+    int64_t k = base + i;
+    L.x = src.x[k]; L.px = src.px[k]; L.y = src.y[k]; L.py = src.py[k]; L.z = src.z[k]; L.pz = src.pz[k];
+    L.beta = src.beta[k];
+    ic = src.ic[k];
+    L.state = BMX_ALIVE;
+    refresh_invariants(L, lc);
+    alive = !(turn0 == 0 && too_large(L, lc));
   }
-  L.state = BMX_ALIVE;
-  refresh_invariants(L, lc);
   int lo = var_op_begin[v], hi = var_op_begin[v + 1];
-  int ie_lost = -1, turn = 0;
-  bool alive = !too_large(L, lc);
-  for (turn = 0; alive && turn < n_turns; turn++) {
-    alive = run_ops(L, ops, lo, hi, lc, ie_lost);
-    if (!alive) break;
+  walk(L, alive, ie_lost, turn, ops, lo, hi, 0, 0, turns, lc, sops);
+  if (active && alive) turn = turns;
+  if (active && !alive) survival[base + ic] = turn0 + turn;
+  // live particle-element-turns of this lane
+  unsigned long long my_pet = 0;
+  if (active) my_pet = (unsigned long long)turn * (unsigned long long)var_n_ele[v] + (alive ? 0ull : (unsigned long long)(ie_lost + 1));
+  // wave-level compaction of survivors
+  unsigned long long mask = __ballot(alive);
+  int lane = __lane_id();
+  unsigned long long below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+  int n_alive = __popcll(mask);
+  int first = __ffsll((long long)mask) - 1;
+  int slot0 = 0;
+  if (n_alive > 0 && lane == first) slot0 = atomicAdd(&count_out[v], n_alive);
+  slot0 = __shfl(slot0, first < 0 ? 0 : first);
+  if (alive) {
+    int64_t k = base + slot0 + __popcll(below);
+    dst.x[k] = L.x; dst.px[k] = L.px; dst.y[k] = L.y; dst.py[k] = L.py; dst.z[k] = L.z; dst.pz[k] = L.pz;
+    dst.beta[k] = L.beta;
+    dst.ic[k] = ic;
   }
-  survival[(int64_t)v * g.n_ic + ic] = alive ? n_turns : turn;
+  for (int off = 32; off > 0; off >>= 1) my_pet += __shfl_xor(my_pet, off);
+  if (lane == 0 && my_pet) atomicAdd(pet, my_pet);
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +825,7 @@ struct bmx_lattice {
   std::vector<int32_t> var_n_ele;
   DevOp* d_ops;
   int32_t* d_var_op_begin;
+  int32_t* d_var_n_ele;
   int64_t n_ops;
   LatticeConsts lc;
 };
@@ -640,6 +840,12 @@ static int set_err(bmx_ctx* ctx, const std::string& s) {
     hipError_t e_ = (call);                                                             \
     if (e_ != hipSuccess) return set_err((ctx), std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Shared with bmx_mlp.hip (hidden: not part of the exported ABI).
+extern "C" __attribute__((visibility("hidden"))) hipStream_t bmx_ctx_stream(const bmx_ctx* ctx) { return ctx->stream; }
+extern "C" __attribute__((visibility("hidden"))) int32_t bmx_ctx_set_error(bmx_ctx* ctx, const char* msg) {
+  return set_err(ctx, msg ? msg : "error");
+}
 
 extern "C" {
 
@@ -713,6 +919,7 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
       return set_err(ctx, "variant " + std::to_string(v) + ": " + err);
     }
     lc.max_aperture_limit = c.max_aperture_limit;
+    lc.two_max_ap = 2.0 * c.max_aperture_limit;
     lc.aperture_limit_on = c.aperture_limit_on;
     if (v == 0) lat->lc = lc;
     else if (lc.p0c != lat->lc.p0c || lc.mc2 != lat->lc.mc2) {
@@ -726,12 +933,15 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
   lat->n_ops = (int64_t)prog.size();
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (hipMalloc(&lat->d_ops, prog.size() * sizeof(DevOp)) != hipSuccess ||
-      hipMalloc(&lat->d_var_op_begin, lat->var_op_begin.size() * sizeof(int32_t)) != hipSuccess) {
+      hipMalloc(&lat->d_var_op_begin, lat->var_op_begin.size() * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&lat->d_var_n_ele, lat->var_n_ele.size() * sizeof(int32_t)) != hipSuccess) {
     delete lat;
     return set_err(ctx, "hipMalloc of the element program failed");
   }
   HIP_TRY(ctx, hipMemcpy(lat->d_ops, prog.data(), prog.size() * sizeof(DevOp), hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(lat->d_var_op_begin, lat->var_op_begin.data(), lat->var_op_begin.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(lat->d_var_n_ele, lat->var_n_ele.data(), lat->var_n_ele.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
   *out = lat;
   return 0;
@@ -742,6 +952,7 @@ void bmx_lattice_destroy(bmx_lattice* lat) {
   hipSetDevice(lat->device);
   hipFree(lat->d_ops);
   hipFree(lat->d_var_op_begin);
+  hipFree(lat->d_var_n_ele);
   delete lat;
 }
 
@@ -830,31 +1041,74 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
   g.amp_max_x = grid->amp_max_x; g.amp_ratio_y = grid->amp_ratio_y;
   g.pz_min = grid->pz_min; g.pz_max = grid->pz_max;
   g.angle_min = grid->angle_min; g.angle_max = grid->angle_max;
-  int64_t total = (int64_t)g.n_ic * lat->n_variants;
-  int32_t* d_surv = nullptr;
-  HIP_TRY(ctx, hipMalloc(&d_surv, total * sizeof(int32_t)));
+  int nv = lat->n_variants;
+  int tpb = grid->turns_per_block > 0 ? grid->turns_per_block : 32;
+  int64_t total = (int64_t)g.n_ic * nv;
+  // device buffers: two particle stores (ping-pong), survival, counts, PET counter
+  std::vector<void*> owned;
+  auto dalloc = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    owned.push_back(p);
+    return p;
+  };
+  auto cleanup = [&]() { for (void* q : owned) (void)hipFree(q); };
+  GridSoA st[2];
+  bool ok = true;
+  for (int b = 0; b < 2 && ok; b++) {
+    double** f[7] = {&st[b].x, &st[b].px, &st[b].y, &st[b].py, &st[b].z, &st[b].pz, &st[b].beta};
+    for (int k = 0; k < 7 && ok; k++) ok = (*f[k] = (double*)dalloc(total * sizeof(double))) != nullptr;
+    if (ok) ok = (st[b].ic = (int32_t*)dalloc(total * sizeof(int32_t))) != nullptr;
+  }
+  int32_t* d_surv = ok ? (int32_t*)dalloc(total * sizeof(int32_t)) : nullptr;
+  int32_t* d_count = d_surv ? (int32_t*)dalloc(2 * nv * sizeof(int32_t)) : nullptr;
+  unsigned long long* d_pet = d_count ? (unsigned long long*)dalloc(sizeof(unsigned long long)) : nullptr;
+  if (!d_pet) { cleanup(); return set_err(ctx, "device allocation for the grid failed"); }
+  std::vector<int32_t> h_count(nv, g.n_ic);
+  (void)hipMemcpyAsync(d_count, h_count.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+  (void)hipMemsetAsync(d_pet, 0, sizeof(unsigned long long), ctx->stream);
   const int block = 256;
-  dim3 gd((unsigned)((g.n_ic + block - 1) / block), (unsigned)lat->n_variants);
   hipEventRecord(ctx->ev0, ctx->stream);
-  hipLaunchKernelGGL(bmx_track_grid_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin, lat->lc, g,
-                     n_turns, d_surv);
-  hipEventRecord(ctx->ev1, ctx->stream);
+  hipLaunchKernelGGL(bmx_grid_init_kernel, dim3((unsigned)((g.n_ic + block - 1) / block), (unsigned)nv), dim3(block), 0,
+                     ctx->stream, g, lat->lc, st[0], d_surv, n_turns);
+  int cur = 0, turn0 = 0, max_live = g.n_ic;
   hipError_t le = hipGetLastError();
-  hipMemcpyAsync(survival_turn, d_surv, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  while (le == hipSuccess && turn0 < n_turns && max_live > 0) {
+    int turns = std::min(tpb, n_turns - turn0);
+    int32_t* cin = d_count + (cur ? nv : 0);
+    int32_t* cout = d_count + (cur ? 0 : nv);
+    (void)hipMemsetAsync(cout, 0, nv * sizeof(int32_t), ctx->stream);
+    dim3 gd((unsigned)((max_live + block - 1) / block), (unsigned)nv);
+    hipLaunchKernelGGL(bmx_grid_block_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin,
+                       lat->d_var_n_ele, lat->lc, g.n_ic, st[cur], st[1 - cur], cin, cout, turn0, turns, d_surv, d_pet);
+    le = hipGetLastError();
+    if (le != hipSuccess) break;
+    (void)hipMemcpyAsync(h_count.data(), cout, nv * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) break;
+    max_live = 0;
+    for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
+    cur = 1 - cur;
+    turn0 += turns;
+  }
+  hipEventRecord(ctx->ev1, ctx->stream);
+  unsigned long long h_pet = 0;
+  (void)hipMemcpyAsync(&h_pet, d_pet, sizeof(h_pet), hipMemcpyDeviceToHost, ctx->stream);
+  (void)hipMemcpyAsync(survival_turn, d_surv, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
   hipError_t se = hipStreamSynchronize(ctx->stream);
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  hipFree(d_surv);
+  cleanup();
   if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
   if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
-  // DA area per variant: boundary r(theta) = amplitude of the last IC of the
-  // contiguous surviving run from the smallest amplitude, on the pz = middle
-  // slice; area = 1/2 sum r_i r_{i+1} sin(dtheta) (polar-grid trapezoids).
+  // DA area per variant: on the middle pz slice, the boundary point of each
+  // angle is the (x, y) of the last IC of the contiguous surviving run from
+  // the smallest amplitude; area = the polygon through the origin and those
+  // points in angle order, 1/2 sum (x_i y_{i+1} - x_{i+1} y_i).
   if (da_area) {
     int ipz = g.n_pz / 2;
     double dth = (g.n_angle > 1) ? (g.angle_max - g.angle_min) / (g.n_angle - 1) : 0.0;
-    for (int v = 0; v < lat->n_variants; v++) {
-      std::vector<double> r(g.n_angle, 0.0);
+    for (int v = 0; v < nv; v++) {
+      std::vector<double> bx(g.n_angle, 0.0), by(g.n_angle, 0.0);
       for (int ia = 0; ia < g.n_angle; ia++) {
         int last = -1;
         for (int k = 0; k < g.n_amp; k++) {
@@ -864,17 +1118,22 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
         }
         double amp = (last < 0) ? 0.0 : g.amp_max_x * (double)(last + 1) / g.n_amp;
         double th = g.angle_min + dth * ia;
-        double ax = amp * cos(th), ay = amp * g.amp_ratio_y * sin(th);
-        r[ia] = sqrt(ax * ax + ay * ay);
+        bx[ia] = amp * cos(th);
+        by[ia] = amp * g.amp_ratio_y * sin(th);
       }
       double a = 0;
-      for (int ia = 0; ia + 1 < g.n_angle; ia++) a += 0.5 * r[ia] * r[ia + 1] * sin(dth);
+      for (int ia = 0; ia + 1 < g.n_angle; ia++) a += 0.5 * (bx[ia] * by[ia + 1] - bx[ia + 1] * by[ia]);
       da_area[v] = a;
     }
   }
   if (out) {
     std::memset(out, 0, sizeof(*out));
     out->kernel_ms = ms;
+    out->pet = (double)h_pet;
+    int64_t live = 0;
+    for (int64_t k = 0; k < total; k++) live += (survival_turn[k] >= n_turns);
+    out->n_live = live;
+    out->n_lost = total - live;
   }
   return 0;
 }

@@ -74,7 +74,7 @@ type, bind(C) :: bmx_stats
 end type
 
 type, bind(C) :: bmx_grid
-  integer(c_int32_t) :: n_angle = 0, n_amp = 0, n_pz = 0, reserved = 0
+  integer(c_int32_t) :: n_angle = 0, n_amp = 0, n_pz = 0, turns_per_block = 0
   real(c_double) :: amp_max_x = 0, amp_ratio_y = 1, pz_min = 0, pz_max = 0, angle_min = 0, angle_max = 0
 end type
 

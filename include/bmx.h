@@ -178,9 +178,14 @@ typedef struct bmx_stats {
 
 /* Batched dynamic-aperture grid: n_var lattice variants, each with the same
  * polar grid of initial conditions n_angle x n_amp x n_pz around the
- * variant's closed orbit (taken as zero here). */
+ * variant's closed orbit (taken as zero here).  Initial condition
+ * ic = (i_pz * n_angle + i_angle) * n_amp + i_amp has
+ * x = a cos(th), y = a amp_ratio_y sin(th), px = py = z = 0, pz = pz_i with
+ * a = amp_max_x (i_amp + 1) / n_amp.  Tracking runs in blocks of
+ * turns_per_block turns; survivors are compacted between blocks
+ * (turns_per_block <= 0: 32). */
 typedef struct bmx_grid {
-  int32_t n_angle, n_amp, n_pz, reserved;
+  int32_t n_angle, n_amp, n_pz, turns_per_block;
   double amp_max_x;         /* largest x amplitude [m]; y amplitude scales by amp_ratio_y */
   double amp_ratio_y;
   double pz_min, pz_max;
@@ -224,6 +229,39 @@ int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, 
  * Host output arrays. */
 int32_t bmx_track_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *grid, int32_t n_turns,
                        int32_t *survival_turn, double *da_area, bmx_stats *out);
+
+/* ------------------------------------------------------------------------
+ * Dynamic-aperture surrogate kernels (no counterpart in the reference; they
+ * serve the learned screening of lattice variants).  Device pointers; work
+ * is queued on the context's stream (bmx_ctx_sync waits for it).
+ * ------------------------------------------------------------------------ */
+
+/* C[M x N] = op(A) op(B), BF16 inputs (op = transpose when trans_*), FP32
+ * matrix-core accumulation.  Row-major with leading dimensions lda, ldb, ldc.
+ * epilogue: 0 plain; 1 + bias[n]; 2 pre = acc + bias[n] -> C (FP32, may be
+ * NULL), C_bf16 = SiLU(pre); 3 acc * SiLU'(aux[m, n]).  C (FP32) and/or
+ * C_bf16 receive the result. */
+int32_t bmx_gemm_bf16(bmx_ctx *ctx, int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K,
+                      const void *A, int32_t lda, const void *B, int32_t ldb, float *C, void *C_bf16, int32_t ldc,
+                      const float *bias, int32_t epilogue, const float *aux);
+
+/* out[n] = sum_m X[m * ld + n] (bias gradients). */
+int32_t bmx_colsum_f32(bmx_ctx *ctx, int32_t M, int32_t N, const float *X, int32_t ld, float *out);
+
+/* One Adam step on FP32 master weights w (and their BF16 copy, may be NULL). */
+int32_t bmx_adam_step(bmx_ctx *ctx, int64_t n, float *w, void *w_bf16, const float *g, float *m, float *v, float lr,
+                      float beta1, float beta2, float eps, int32_t step);
+
+/* y = (bf16) x, element-wise. */
+int32_t bmx_f32_to_bf16(bmx_ctx *ctx, int64_t n, const float *x, void *y);
+
+/* Mean-squared-error loss over n predictions: *loss = sum (pred - target)^2 / n,
+ * d = 2 (pred - target) / n in FP32 and (d_bf16) BF16. */
+int32_t bmx_mse_grad(bmx_ctx *ctx, int32_t n, const float *pred, const float *target, float *d, void *d_bf16,
+                     float *loss);
+
+/* Wait for the work queued on the context's stream. */
+int32_t bmx_ctx_sync(bmx_ctx *ctx);
 
 #ifdef __cplusplus
 }

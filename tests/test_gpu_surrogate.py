@@ -1,0 +1,179 @@
+"""GPU tests of the surrogate kernels (bmx_mlp.hip through the libbmx C ABI)
+against the CPU oracle (oracle/mlp_oracle.py) and FP32/FP64 references.
+
+Tolerances: a GEMM of BF16 operands with FP32 accumulation agrees with the
+FP64 product of the same BF16 values to 1e-5 of the largest |C| (FP32
+accumulation order); fused epilogues to 1e-5 relative plus one BF16 ulp on the
+BF16 outputs.  Whole-network gradients agree with the oracle (which rounds to
+BF16 at the same points) to 1e-3 in relative Frobenius norm per tensor: the
+residue is FP32-vs-FP64 accumulation noise plus the occasional BF16 rounding
+that falls the other way; a transposed operand or dropped term is O(1).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from bmx import surrogate
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _bf(a):
+    return torch.from_numpy(np.asarray(a, dtype=np.float32)).to(torch.bfloat16).to(DEV)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 50), (1, 256, 333), (257, 3, 1), (130, 256, 35)])
+def test_gemm_bf16(ctx, ta, tb, M, N, K):
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    a = mo.bf16(rng.standard_normal((K, M) if ta else (M, K)))
+    b = mo.bf16(rng.standard_normal((N, K) if tb else (K, N)))
+    A, B = _bf(a), _bf(b)
+    Cf = torch.full((M, N), np.nan, dtype=torch.float32, device=DEV)
+    Cb = torch.zeros((M, N), dtype=torch.bfloat16, device=DEV)
+    torch.cuda.synchronize()
+    lda = a.shape[1]
+    ldb = b.shape[1]
+    rc = ctx.lib.bmx_gemm_bf16(ctx.h, ta, tb, M, N, K, _p(A), lda, _p(B), ldb, _p(Cf), _p(Cb), N, None, 0, None)
+    assert rc == 0, ctx.lib.bmx_last_error(ctx.h)
+    assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
+    want = (a.T if ta else a) @ (b.T if tb else b)
+    got = Cf.cpu().numpy().astype(np.float64)
+    scale = np.max(np.abs(want)) + 1e-30
+    err = np.max(np.abs(got - want)) / scale
+    print(f"gemm ta={ta} tb={tb} {M}x{N}x{K}: {err:.2e}", flush=True)
+    assert err < 1e-5
+    assert np.array_equal(Cb.float().cpu().numpy().astype(np.float64), mo.bf16(got))
+
+
+@pytest.mark.parametrize("epi", [1, 2, 3])
+def test_gemm_epilogues(ctx, epi):
+    M, N, K = 150, 96, 64
+    rng = np.random.default_rng(epi)
+    a = mo.bf16(rng.standard_normal((M, K)) * 0.3)
+    b = mo.bf16(rng.standard_normal((N, K)) * 0.3)
+    bias = rng.standard_normal(N).astype(np.float32)
+    aux = (rng.standard_normal((M, N)) * 2).astype(np.float32)
+    A, B = _bf(a), _bf(b)
+    tb = torch.from_numpy(bias).to(DEV)
+    taux = torch.from_numpy(aux).to(DEV)
+    Cf = torch.zeros((M, N), dtype=torch.float32, device=DEV)
+    Cb = torch.zeros((M, N), dtype=torch.bfloat16, device=DEV)
+    torch.cuda.synchronize()
+    rc = ctx.lib.bmx_gemm_bf16(ctx.h, 0, 1, M, N, K, _p(A), K, _p(B), K, _p(Cf), _p(Cb), N, _p(tb), epi, _p(taux))
+    assert rc == 0 and ctx.lib.bmx_ctx_sync(ctx.h) == 0
+    acc = a @ b.T
+    if epi == 1:
+        want_f = acc + bias
+        want_b = mo.bf16(want_f)
+    elif epi == 2:
+        want_f = acc + bias
+        want_b = mo.bf16(mo.silu(want_f))
+    else:
+        want_f = acc * mo.silu_grad(aux.astype(np.float64))
+        want_b = mo.bf16(want_f)
+    got_f = Cf.cpu().numpy().astype(np.float64)
+    got_b = Cb.float().cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(got_f - want_f)) <= 1e-5 * np.max(np.abs(want_f))
+    # BF16 outputs: within one BF16 ulp (2^-8 relative) of the FP64 value
+    assert np.all(np.abs(got_b - want_b) <= 2.0 ** -7 * np.abs(want_b) + 1e-30)
+
+
+def test_colsum_mse_adam(ctx):
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((3000, 77)).astype(np.float32)
+    tX = torch.from_numpy(X).to(DEV)
+    out = torch.zeros(77, dtype=torch.float32, device=DEV)
+    torch.cuda.synchronize()
+    assert ctx.lib.bmx_colsum_f32(ctx.h, 3000, 77, _p(tX), 77, _p(out)) == 0
+    assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
+    assert np.allclose(out.cpu().numpy(), X.astype(np.float64).sum(0), rtol=1e-5, atol=1e-4)
+
+    n = 5000
+    p = rng.standard_normal(n).astype(np.float32)
+    t = rng.standard_normal(n).astype(np.float32)
+    tp, tt = torch.from_numpy(p).to(DEV), torch.from_numpy(t).to(DEV)
+    d = torch.zeros(n, dtype=torch.float32, device=DEV)
+    db = torch.zeros(n, dtype=torch.bfloat16, device=DEV)
+    loss = torch.zeros(1, dtype=torch.float32, device=DEV)
+    torch.cuda.synchronize()
+    assert ctx.lib.bmx_mse_grad(ctx.h, n, _p(tp), _p(tt), _p(d), _p(db), _p(loss)) == 0
+    assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
+    e = p.astype(np.float64) - t
+    assert abs(loss.item() - np.mean(e * e)) < 1e-5 * np.mean(e * e)
+    assert np.allclose(d.cpu().numpy(), 2 * e / n, rtol=1e-6, atol=1e-12)
+
+    w = rng.standard_normal(1000).astype(np.float32)
+    wd = torch.from_numpy(w).to(DEV)
+    wb = torch.zeros(1000, dtype=torch.bfloat16, device=DEV)
+    m = torch.zeros(1000, dtype=torch.float32, device=DEV)
+    v = torch.zeros(1000, dtype=torch.float32, device=DEV)
+    w64, m64, v64 = w.astype(np.float64), np.zeros(1000), np.zeros(1000)
+    for step in range(1, 4):
+        g = rng.standard_normal(1000).astype(np.float32)
+        gd = torch.from_numpy(g).to(DEV)
+        torch.cuda.synchronize()
+        assert ctx.lib.bmx_adam_step(ctx.h, 1000, _p(wd), _p(wb), _p(gd), _p(m), _p(v), 1e-2, 0.9, 0.999, 1e-8,
+                                     step) == 0
+        assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
+        mo.adam(w64, g.astype(np.float64), m64, v64, 1e-2, 0.9, 0.999, 1e-8, step)
+    got = wd.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(got - w64)) < 1e-5
+    assert np.array_equal(wb.float().cpu().numpy().astype(np.float64), mo.bf16(got))
+
+
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_network_forward_backward_vs_oracle(ctx, n):
+    dims = [35, 256, 256, 256, 256, 1]
+    params = mo.init_params(dims, seed=5)
+    net = surrogate.MLP(ctx, dims, params=params)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, (n, 35)).astype(np.float32)
+    t = rng.uniform(0, 1, n).astype(np.float32)
+    tx, tt = torch.from_numpy(x).to(DEV), torch.from_numpy(t).to(DEV)
+    torch.cuda.synchronize()
+    y = net.forward(tx)
+    loss = net.backward(tt)
+    net.sync()
+    p32 = [(W.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)) for W, b in params]
+    want_loss, want_g, want_y = mo.loss_and_grads(p32, x.astype(np.float64), t.astype(np.float64))
+    got_y = y.cpu().numpy().astype(np.float64)
+    ey = np.max(np.abs(got_y - want_y)) / np.max(np.abs(want_y))
+    el = abs(loss.item() - want_loss) / want_loss
+    print(f"forward {ey:.2e} loss {el:.2e}", flush=True)
+    assert ey < 1e-3 and el < 1e-4
+    for li, ((gW, gb), (wW, wb)) in enumerate(zip(net.grads(), want_g)):
+        eW = np.linalg.norm(gW - wW) / np.linalg.norm(wW)
+        eb = np.linalg.norm(gb - wb) / np.linalg.norm(wb)
+        print(f"layer {li}: dW {eW:.2e} db {eb:.2e}", flush=True)
+        assert eW < 1e-3 and eb < 1e-3, (li, eW, eb)
+
+
+def test_training_reduces_loss(ctx):
+    """A few hundred Adam steps on a smooth target halve the loss at least."""
+    dims = [3, 64, 64, 1]
+    net = surrogate.MLP(ctx, dims, seed=1)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, (2048, 3)).astype(np.float32)
+    t = (np.sin(2 * x[:, 0]) * np.cos(x[:, 1]) + 0.5 * x[:, 2] ** 2).astype(np.float32)
+    tx, tt = torch.from_numpy(x).to(DEV), torch.from_numpy(t).to(DEV)
+    torch.cuda.synchronize()
+    first = None
+    for i in range(300):
+        loss = net.train_step(tx, tt, 3e-3)
+        if i == 0:
+            net.sync()
+            first = loss.item()
+    net.sync()
+    last = loss.item()
+    print(f"loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.5 * first
