@@ -32,6 +32,7 @@ enum OpType : int32_t {
   OP_APERTURE_IN = 15,   // aperture check at the entrance edge
   OP_ELE_END = 16,       // orbit_too_large + exit aperture + loss bookkeeping
   OP_TILT = 17,          // tilt_coords rotation, c[0] = cos, c[1] = sin
+  OP_KD_SLICES = 18,     // whole kick-drift body of a thick multipole: half kick, n x (drift, kick), last half
 };
 
 // op flags

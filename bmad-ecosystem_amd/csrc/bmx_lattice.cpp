@@ -285,6 +285,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         if (wx == 0) wx = cfg.max_aperture_limit;
         if (wy == 0) wy = cfg.max_aperture_limit;
         op.c[4] = std::max(wx, 0.0); op.c[5] = cx; op.c[6] = std::max(wy, 0.0); op.c[7] = cy;
+        op.c[8] = 1.0 / op.c[4]; op.c[9] = 1.0 / op.c[6];  // for the device's division-free screen
       }
     };
     if (cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_ENTRANCE_END || e.aperture_at == BMX_BOTH_ENDS)) {
@@ -379,10 +380,26 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         int n_step = (ab.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
         double step_len = e.l / n_step, r_step = 1.0 / n_step;
         emit_offset(prog, ie, e, true, true);
-        emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
-        for (int i = 1; i <= n_step; i++) {
-          if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
-          emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
+        int lo = 0;
+        while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
+        if (ab.ix_max >= 0 && step_len != 0 && ab.ix_max - lo < 6) {
+          // one op for the body; the half kicks are the full-step kick x 0.5,
+          // which is exact (the reference scales by r_step / 2)
+          DevOp op = make_op(OP_KD_SLICES, ie);
+          op.n = n_step;
+          op.flags = (ab.ix_max - lo) | (lo << 8);
+          op.c[0] = step_len;
+          for (int n = lo; n <= ab.ix_max; n++) {
+            op.c[1 + 2 * (n - lo)] = r_step * ab.b[n];
+            op.c[2 + 2 * (n - lo)] = r_step * ab.a[n];
+          }
+          prog.push_back(op);
+        } else {
+          emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
+          for (int i = 1; i <= n_step; i++) {
+            if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
+            emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
+          }
         }
         emit_offset(prog, ie, e, false, true);
         break;

@@ -47,6 +47,16 @@ struct Lane {
   int state;
 };
 
+// FP64 VALU instructions take no 64-bit literal, so every series coefficient
+// lives in a register pair.  Left alone, the compiler hoists all of them out
+// of the element loop into VGPRs and spills them to scratch; K() pins a
+// coefficient to an SGPR pair at its point of use instead (an empty asm with
+// an "s" operand that cannot be moved out of the loop).
+__device__ __forceinline__ double K(double c) {
+  asm volatile("" : "+s"(c));
+  return c;
+}
+
 // sqrt_one (sim_utils/special_functions/sqrt_one.f90:18-41)
 __device__ __forceinline__ double d_sqrt_one(double x) { return x / (sqrt(1.0 + x) + 1.0); }
 
@@ -82,7 +92,7 @@ __device__ __forceinline__ void op_drift(Lane& L, double len) {
   double u = pxr * pxr + pyr * pyr;
   double S;
   if (u < 1e-4) {
-    S = u * (0.5 + u * (0.375 + u * (0.3125 + u * (0.2734375 + u * 0.24609375))));
+    S = u * (0.5 + u * (0.375 + u * (K(0.3125) + u * (K(0.2734375) + u * K(0.24609375)))));
   } else {
     if (u >= 1.0) { L.state = BMX_LOST_PZ; return; }
     double ps = sqrt(1.0 - u);
@@ -107,18 +117,18 @@ __device__ __forceinline__ void op_drift_noref(Lane& L, double len) {
 
 // Thin multipole kick, ab form (bmad/modules/multipole_mod.f90:82-174, :314-401):
 // K = sum_n c_n (x + i y)^n, dpx = -Re K, dpy = Im K, Horner in complex FP64.
-__device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, double& kr, double& ki) {
-  int nh = __builtin_amdgcn_readfirstlane(op.flags);  // Horner steps
-  kr = op.c[2 * nh];
-  ki = op.c[2 * nh + 1];
+// c holds nh + 1 complex coefficients for orders lo .. lo + nh.
+__device__ __forceinline__ void kick_horner(const double* c, int nh, int lo, double x, double y, double& kr,
+                                            double& ki) {
+  kr = c[2 * nh];
+  ki = c[2 * nh + 1];
   for (int j = nh - 1; j >= 0; j--) {
-    double tr = kr * x - ki * y + op.c[2 * j];
-    double ti = kr * y + ki * x + op.c[2 * j + 1];
+    double tr = kr * x - ki * y + c[2 * j];
+    double ti = kr * y + ki * x + c[2 * j + 1];
     kr = tr;
     ki = ti;
   }
-  int lo = __builtin_amdgcn_readfirstlane(op.n);  // multiply by (x + i y)^lo
-  for (int j = 0; j < lo; j++) {
+  for (int j = 0; j < lo; j++) {  // multiply by (x + i y)^lo
     double tr = kr * x - ki * y;
     double ti = kr * y + ki * x;
     kr = tr;
@@ -126,11 +136,39 @@ __device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, d
   }
 }
 
+__device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, double& kr, double& ki) {
+  kick_horner(op.c, __builtin_amdgcn_readfirstlane(op.flags), __builtin_amdgcn_readfirstlane(op.n), x, y, kr, ki);
+}
+
 __device__ __forceinline__ void op_kick(Lane& L, const DevOp& op) {
   double kr, ki;
   kick_eval(op, L.x, L.y, kr, ki);
   L.px -= kr;
   L.py += ki;
+}
+
+// Body of a thick multipole (track_a_thick_multipole.f90:91-121): kick x 1/2,
+// then n_step x (drift, kick), the last kick x 1/2.  As in the reference the
+// loop runs on after a drift has set lost_pz; the element end checks state.
+// c: 0 step_len, 1.. full-step kick coefficients; n = n_step; flags = nh | lo << 8
+__device__ __forceinline__ void op_kd_slices(Lane& L, const DevOp& op) {
+  int n_step = __builtin_amdgcn_readfirstlane(op.n);
+  int fl = __builtin_amdgcn_readfirstlane(op.flags);
+  int nh = fl & 0xff, lo = fl >> 8;
+  double len = op.c[0];
+  double kr, ki;
+  kick_horner(&op.c[1], nh, lo, L.x, L.y, kr, ki);
+  L.px -= 0.5 * kr;
+  L.py += 0.5 * ki;
+  for (int i = 1; i <= n_step; i++) {
+    int st = L.state;
+    op_drift(L, len);
+    L.state = (st != BMX_ALIVE) ? st : L.state;  // first loss state wins
+    kick_horner(&op.c[1], nh, lo, L.x, L.y, kr, ki);
+    double h = (i == n_step) ? 0.5 : 1.0;
+    L.px -= h * kr;
+    L.py += h * ki;
+  }
 }
 
 // apply_multipole_kicks of track_a_bend (track_a_bend.f90:338-471): kick scaled by (1 + g x).
@@ -166,20 +204,19 @@ __device__ __forceinline__ double low_energy_dz(const Lane& L, const LatticeCons
 // n = 11 is exact to < 1e-21 and replaces sincos, expm1, sqrt and three
 // divisions; stronger quadrupoles take the exact trigonometric form.
 __device__ __forceinline__ void quad_cs_series(double w, double len, double& c, double& s) {
-  const double ic[12] = {1.0, 1.0 / 2, 1.0 / 24, 1.0 / 720, 1.0 / 40320, 1.0 / 3628800, 1.0 / 479001600,
-                         1.0 / 87178291200.0, 1.0 / 20922789888000.0, 1.0 / 6402373705728000.0,
-                         1.0 / 2432902008176640000.0, 1.0 / 1124000727777607680000.0};
-  const double is[12] = {1.0, 1.0 / 6, 1.0 / 120, 1.0 / 5040, 1.0 / 362880, 1.0 / 39916800, 1.0 / 6227020800.0,
-                         1.0 / 1307674368000.0, 1.0 / 355687428096000.0, 1.0 / 121645100408832000.0,
-                         1.0 / 51090942171709440000.0, 1.0 / 25852016738884976640000.0};
-  double cc = ic[11], ss = is[11];
-#pragma unroll
-  for (int n = 10; n >= 0; n--) {
-    cc = cc * w + ic[n];
-    ss = ss * w + is[n];
-  }
-  c = cc;
-  s = ss * len;
+  double cc = K(1.0 / 1124000727777607680000.0), ss = K(1.0 / 25852016738884976640000.0);
+  cc = cc * w + K(1.0 / 2432902008176640000.0);   ss = ss * w + K(1.0 / 51090942171709440000.0);
+  cc = cc * w + K(1.0 / 6402373705728000.0);      ss = ss * w + K(1.0 / 121645100408832000.0);
+  cc = cc * w + K(1.0 / 20922789888000.0);        ss = ss * w + K(1.0 / 355687428096000.0);
+  cc = cc * w + K(1.0 / 87178291200.0);           ss = ss * w + K(1.0 / 1307674368000.0);
+  cc = cc * w + K(1.0 / 479001600);               ss = ss * w + K(1.0 / 6227020800.0);
+  cc = cc * w + K(1.0 / 3628800);                 ss = ss * w + K(1.0 / 39916800);
+  cc = cc * w + K(1.0 / 40320);                   ss = ss * w + K(1.0 / 362880);
+  cc = cc * w + K(1.0 / 720);                     ss = ss * w + K(1.0 / 5040);
+  cc = cc * w + K(1.0 / 24);                      ss = ss * w + K(1.0 / 120);
+  cc = cc * w + 0.5;                              ss = ss * w + K(1.0 / 6);
+  c = cc * w + 1.0;
+  s = (ss * w + 1.0) * len;
 }
 
 __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeConsts& lc) {
@@ -228,34 +265,34 @@ __device__ __forceinline__ double d_sinc(double x) { return (fabs(x) < 1e-8) ? 1
 // asin(s), |s| <= 0.05, to s^13 (truncation < 1e-19 relative).
 __device__ __forceinline__ double asin_small(double s) {
   double s2 = s * s;
-  return s + s * s2 * (1.0 / 6 + s2 * (3.0 / 40 + s2 * (5.0 / 112 + s2 * (35.0 / 1152 + s2 * (63.0 / 2816 +
-             s2 * (231.0 / 13312))))));
+  return s + s * s2 * (K(1.0 / 6) + s2 * (K(3.0 / 40) + s2 * (K(5.0 / 112) + s2 * (K(35.0 / 1152) + s2 * (K(63.0 / 2816) +
+             s2 * (K(231.0 / 13312)))))));
 }
 
 // sin(e), |e| <= 0.2, to e^11 (truncation < 1e-18 relative).
 __device__ __forceinline__ double sin_small(double e) {
   double e2 = e * e;
-  return e + e * e2 * (-1.0 / 6 + e2 * (1.0 / 120 - e2 * (1.0 / 5040 - e2 * (1.0 / 362880 - e2 * (1.0 / 39916800)))));
+  return e + e * e2 * (K(-1.0 / 6) + e2 * (K(1.0 / 120) - e2 * (K(1.0 / 5040) - e2 * (K(1.0 / 362880) - e2 * (K(1.0 / 39916800))))));
 }
 
 // cos(e), |e| <= 0.2, to e^12 (truncation < 1e-20).
 __device__ __forceinline__ double cos_small(double e) {
   double e2 = e * e;
-  return 1.0 + e2 * (-0.5 + e2 * (1.0 / 24 - e2 * (1.0 / 720 - e2 * (1.0 / 40320 - e2 * (1.0 / 3628800 -
-               e2 * (1.0 / 479001600))))));
+  return 1.0 + e2 * (-0.5 + e2 * (K(1.0 / 24) - e2 * (K(1.0 / 720) - e2 * (K(1.0 / 40320) - e2 * (K(1.0 / 3628800) -
+               e2 * (K(1.0 / 479001600)))))));
 }
 
 // atan(t), |t| <= 0.2, to t^13 (truncation < 1e-11 * t^14, < 2e-21 relative).
 __device__ __forceinline__ double atan_small(double t) {
   double t2 = t * t;
-  return t + t * t2 * (-1.0 / 3 + t2 * (1.0 / 5 - t2 * (1.0 / 7 - t2 * (1.0 / 9 - t2 * (1.0 / 11 - t2 * (1.0 / 13))))));
+  return t + t * t2 * (K(-1.0 / 3) + t2 * (K(1.0 / 5) - t2 * (K(1.0 / 7) - t2 * (K(1.0 / 9) - t2 * (K(1.0 / 11) - t2 * (K(1.0 / 13)))))));
 }
 
 // x / sin(x) = 1 / sinc(x), |x| <= 0.1, to x^14 (truncation < 1e-20).
 __device__ __forceinline__ double inv_sinc_small(double x) {
   double x2 = x * x;
-  return 1.0 + x2 * (1.0 / 6 + x2 * (7.0 / 360 + x2 * (31.0 / 15120 + x2 * (127.0 / 604800 + x2 * (73.0 / 3421440 +
-         x2 * (1414477.0 / 653837184000.0 + x2 * (8191.0 / 37362124800.0)))))));
+  return 1.0 + x2 * (K(1.0 / 6) + x2 * (K(7.0 / 360) + x2 * (K(31.0 / 15120) + x2 * (K(127.0 / 604800) + x2 * (K(73.0 / 3421440) +
+         x2 * (K(1414477.0 / 653837184000.0) + x2 * (K(8191.0 / 37362124800.0))))))));
 }
 
 // The reference's transcendental calls for the bend entry and exit angles
@@ -542,29 +579,34 @@ __device__ void op_rfcavity(Lane& L, const DevOp& op, const LatticeConsts& lc) {
   }
 }
 
-// orbit_too_large (bmad/low_level/orbit_too_large.f90:26-120)
+// orbit_too_large (bmad/low_level/orbit_too_large.f90:26-120).  A straight-
+// line screen of all four conditions runs first; only lanes that fail it take
+// the reference's ordered tests, which pick the loss state.
 __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
   double mal = lc.max_aperture_limit;
+  double rel_p = 1.0 + L.pz;
+  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
+  bool bad = (fabs(L.x) > mal) | (fabs(L.y) > mal) | (rel_p < 0.0) | (f > 0.0);
+  if (!bad) return false;
   if (L.x > mal) { L.state = BMX_LOST_POS_X; return true; }
   if (-L.x > mal) { L.state = BMX_LOST_NEG_X; return true; }
   if (L.y > mal) { L.state = BMX_LOST_POS_Y; return true; }
   if (-L.y > mal) { L.state = BMX_LOST_NEG_Y; return true; }
-  double rel_p = 1.0 + L.pz;
   if (rel_p < 0.0) { L.state = BMX_LOST_PZ; return true; }
-  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
-  if (f > 0.0) {
-    if (fabs(L.px) > fabs(L.py)) L.state = (L.px > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
-    else L.state = (L.py > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
-    return true;
-  }
-  return false;
+  if (fabs(L.px) > fabs(L.py)) L.state = (L.px > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
+  else L.state = (L.py > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
+  return true;
 }
 
 // aperture part of check_aperture_limit (bmad/low_level/check_aperture_limit.f90:257-309)
-// c: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim, 4 wx, 5 cx, 6 wy, 7 cy
+// c: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim, 4 wx, 5 cx, 6 wy, 7 cy, 8 1/wx, 9 1/wy
 __device__ __forceinline__ void aperture(Lane& L, const DevOp& op, int flags) {
   if (flags & F_APERTURE_ELLIPTICAL) {
     double xp = L.x - op.c[5], yp = L.y - op.c[7];
+    // screen with the reciprocal widths: it is within a few ulp of the
+    // reference's r, so r_fast < 1 - 1e-13 decides r < 1 exactly
+    double fx = xp * op.c[8], fy = yp * op.c[9];
+    if (fx * fx + fy * fy < 1.0 - 1e-13) return;
     double ax = xp / op.c[4], ay = yp / op.c[6];
     double r = ax * ax + ay * ay;
     if (r < 1.0) return;
@@ -582,38 +624,53 @@ __device__ __forceinline__ void aperture(Lane& L, const DevOp& op, int flags) {
 // ---------------------------------------------------------------------------
 // one op on one lane
 // ---------------------------------------------------------------------------
+// The dispatch is a run of independent, wave-uniform if-blocks on a one-hot
+// type mask rather than a switch: a switch becomes a branch tree whose join
+// blocks copy the whole lane state between registers on every op, while each
+// if-block joins back with the lane state in place.  The mask comes through
+// readfirstlane, which also keeps the compiler from folding the tests back
+// into a switch.
+#define TB(t) (1u << (t))
 __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, const LatticeConsts& lc) {
-  switch (type) {
-    case OP_DRIFT: op_drift(L, op.c[0]); break;
-    case OP_KICK: op_kick(L, op); break;
-    case OP_KICK_REF0: op_kick_ref0(L, op); break;
-    case OP_QUAD: op_quad(L, op, lc); break;
-    case OP_BEND: op_bend(L, op, lc); break;
-    case OP_BEND_K1: op_bend_k1(L, op, lc); break;
-    case OP_BEND_KICK: op_bend_kick(L, op); break;
-    case OP_HWANG: op_hwang(L, op); break;
-    case OP_LINEAR_EDGE: op_linear_edge(L, op); break;
-    case OP_TILT: op_tilt(L, op); break;
-    case OP_OFFSET_SET: op_offset(L, op, true); break;
-    case OP_OFFSET_UNSET: op_offset(L, op, false); break;
-    case OP_TOO_LARGE: too_large(L, lc); break;
-    case OP_RFCAVITY: op_rfcavity(L, op, lc); break;
-    case OP_APERTURE_IN: aperture(L, op, flags); break;
-    case OP_ELE_END:
-      if (L.state == BMX_ALIVE && !too_large(L, lc) && (flags & F_HAS_APERTURE)) aperture(L, op, flags);
-      break;
-    default: break;
+  unsigned tb = (unsigned)__builtin_amdgcn_readfirstlane((int)(1u << type));
+  if (tb & TB(OP_DRIFT)) op_drift(L, op.c[0]);
+  if (tb & TB(OP_KICK)) op_kick(L, op);
+  if (tb & TB(OP_KD_SLICES)) op_kd_slices(L, op);
+  if (tb & TB(OP_ELE_END)) {
+    if (L.state == BMX_ALIVE && !too_large(L, lc) && (flags & F_HAS_APERTURE)) aperture(L, op, flags);
+  }
+  if (tb & TB(OP_QUAD)) op_quad(L, op, lc);
+  if (tb & TB(OP_BEND)) op_bend(L, op, lc);
+  if (tb & TB(OP_HWANG)) op_hwang(L, op);
+  if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, lc);
+  constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
+                            TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN);
+  if (tb & RARE) {
+    if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
+    if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
+    if (tb & TB(OP_BEND_KICK)) op_bend_kick(L, op);
+    if (tb & TB(OP_LINEAR_EDGE)) op_linear_edge(L, op);
+    if (tb & TB(OP_TILT)) op_tilt(L, op);
+    if (tb & TB(OP_OFFSET_SET)) op_offset(L, op, true);
+    if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false);
+    if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
+    if (tb & TB(OP_APERTURE_IN)) aperture(L, op, flags);
   }
 }
+#undef TB
 
 // Element programs are staged through LDS: the workgroup copies OPS_CHUNK
-// records at a time (16 KiB) and every wave reads them at wave-uniform LDS
+// records at a time and every wave reads them at wave-uniform LDS
 // addresses, instead of each CU fetching every record from L2 through its
-// scalar cache on the dependent path of the element loop.
+// scalar cache on the dependent path of the element loop.  256 records are
+// 32 KiB: four workgroups of 256 lanes (occupancy 4) fit in a CU's 160 KiB.
 #ifndef BMX_LDS_OPS
 #define BMX_LDS_OPS 1
 #endif
-constexpr int OPS_CHUNK = 128;
+#ifndef BMX_OPS_CHUNK
+#define BMX_OPS_CHUNK 256
+#endif
+constexpr int OPS_CHUNK = BMX_OPS_CHUNK;
 
 // Walk of `turns` turns over ops [lo1, hi1) then [lo2, hi2) for one lane.
 // The whole workgroup calls it together (it holds barriers).  On return
