@@ -90,7 +90,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 # every entry point declared in include/bmx.h
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
-    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_track_bunch", "bmx_track_grid",
+    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
+    "bmx_track_ics", "bmx_closed_orbit",
     "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_ctx_sync",
 ]
 
@@ -123,9 +124,18 @@ def load_library(path=None):
     lib.bmx_track_bunch.restype = C.c_int32
     lib.bmx_track_bunch.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.POINTER(Particles), C.POINTER(Stats)]
+    lib.bmx_track_multi.restype = C.c_int32
+    lib.bmx_track_multi.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(Particles),
+                                    C.POINTER(Stats)]
     lib.bmx_track_grid.restype = C.c_int32
-    lib.bmx_track_grid.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
-                                   C.POINTER(C.c_double), C.POINTER(Stats)]
+    lib.bmx_track_grid.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Grid), C.POINTER(C.c_double), C.c_int32,
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(Stats)]
+    lib.bmx_track_ics.restype = C.c_int32
+    lib.bmx_track_ics.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32,
+                                  C.c_int32, C.POINTER(C.c_int32), C.POINTER(Stats)]
+    lib.bmx_closed_orbit.restype = C.c_int32
+    lib.bmx_closed_orbit.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                     C.POINTER(Stats)]
     lib.bmx_gemm_bf16.restype = C.c_int32
     lib.bmx_gemm_bf16.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                   C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,

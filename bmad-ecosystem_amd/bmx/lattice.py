@@ -220,10 +220,11 @@ def ring_c1(n_cells=500, n_rf=8, seed=20261203, species=abi.ELECTRON, e_tot=6e9,
     return set_reference(eles, species, e_tot=e_tot)
 
 
-def da_variants(n_var=256, seed=20261204, min_cells=63, max_cells=500, n_rf=8):
+def da_variants(n_var=256, seed=20261204, min_cells=63, max_cells=500, n_rf=8, offsets_scale=1.0):
     """Error-seeded C1-style variants of uneven length (63..500 cells = 504..4000
     elements): quad k1 x (1 + 1e-3 g), sextupole k2 x (1 + 5e-3 g), x/y offsets
-    50 um g on quadrupoles and sextupoles.  Returns (list of element lists)."""
+    offsets_scale x 50 um g on quadrupoles and sextupoles.  Returns (list of
+    element lists)."""
     rng = np.random.default_rng(seed)
     out = []
     for v in range(n_var):
@@ -233,9 +234,9 @@ def da_variants(n_var=256, seed=20261204, min_cells=63, max_cells=500, n_rf=8):
         k2e = 5e-3 * rng.standard_normal(ns)
         offs = {}
         for i in range(nq):
-            offs[("q", i)] = tuple(50e-6 * rng.standard_normal(2))
+            offs[("q", i)] = tuple(offsets_scale * 50e-6 * rng.standard_normal(2))
         for i in range(ns):
-            offs[("s", i)] = tuple(50e-6 * rng.standard_normal(2))
+            offs[("s", i)] = tuple(offsets_scale * 50e-6 * rng.standard_normal(2))
         out.append(ring_c1(n_cells=nc, n_rf=n_rf, seed=seed + 1000 + v, k1_err=k1e, k2_err=k2e, offsets=offs))
     return out
 

@@ -211,3 +211,316 @@ def _init_params(dims, seed):
             W *= 0.1
         ps.append((W, np.zeros(dims[i + 1])))
     return ps
+
+
+# ---------------------------------------------------------------------------
+# Dynamic-aperture scan with the surrogate
+# ---------------------------------------------------------------------------
+
+N_KNOBS = 32
+KNOB_RANGE = 0.2
+
+
+class ScanSpec:
+    """Polar grid of a DA scan (the bmx_track_grid layout): n_angle angles in
+    [0, pi], n_amp amplitudes amp_max_x (k + 1) / n_amp, n_pz momentum offsets
+    in [pz_min, pz_max], n_turns turns."""
+
+    def __init__(self, n_turns=256, n_angle=16, n_amp=32, n_pz=3, amp_max_x=0.02, amp_ratio_y=0.5, pz_min=-0.01,
+                 pz_max=0.01, n_cells=50):
+        self.n_turns, self.n_angle, self.n_amp, self.n_pz = n_turns, n_angle, n_amp, n_pz
+        self.amp_max_x, self.amp_ratio_y, self.pz_min, self.pz_max = amp_max_x, amp_ratio_y, pz_min, pz_max
+        self.n_cells = n_cells
+
+    @property
+    def n_ic(self):
+        return self.n_angle * self.n_amp * self.n_pz
+
+    def angles(self):
+        return np.pi * np.arange(self.n_angle) / max(self.n_angle - 1, 1)
+
+    def pzs(self):
+        return np.linspace(self.pz_min, self.pz_max, self.n_pz) if self.n_pz > 1 else np.array([self.pz_min])
+
+    def offsets(self):
+        """[n_pz, n_angle, n_amp, 3] = (x, y, pz) of every grid IC."""
+        a = self.amp_max_x * (np.arange(self.n_amp) + 1) / self.n_amp
+        th = self.angles()
+        pz = self.pzs()
+        out = np.zeros((self.n_pz, self.n_angle, self.n_amp, 3))
+        out[..., 0] = np.cos(th)[None, :, None] * a[None, None, :]
+        out[..., 1] = self.amp_ratio_y * np.sin(th)[None, :, None] * a[None, None, :]
+        out[..., 2] = pz[:, None, None]
+        return out
+
+
+def knob_lattice(knobs, spec, n_rf=0):
+    """C1-cell ring of spec.n_cells cells whose sextupole i has k2 x (1 + knobs[i % 32])."""
+    from . import lattice as latmod
+    ns = 2 * spec.n_cells
+    k2e = np.array([knobs[i % N_KNOBS] for i in range(ns)])
+    return latmod.ring_c1(n_cells=spec.n_cells, n_rf=n_rf, k2_err=k2e)
+
+
+def upload_variants(ctx, knob_sets, spec):
+    """One lattice upload holding a knob_lattice per knob set.  The element
+    records of the nominal ring are replicated with numpy and only the
+    sextupole k2 fields are rewritten (k2 = k2_nominal x (1 + knob), the same
+    product knob_lattice forms)."""
+    import ctypes as Cc
+    from . import abi
+    from . import lattice as latmod
+    knob_sets = np.asarray(knob_sets, dtype=np.float64)
+    base = knob_lattice(np.zeros(N_KNOBS), spec)
+    arr0 = latmod.to_array(base)
+    size = Cc.sizeof(abi.EleRec)
+    n_ele = len(base)
+    rec = np.frombuffer(arr0, dtype=np.uint8).reshape(n_ele, size)
+    sext = np.array([i for i, e in enumerate(base) if e.key == abi.KEY_SEXTUPOLE])
+    k2_nom = np.array([base[i].k2 for i in sext])
+    fam = np.arange(len(sext)) % N_KNOBS
+    n_var = len(knob_sets)
+    big = np.ascontiguousarray(np.tile(rec, (n_var, 1)))
+    k2_off = abi.EleRec.k2.offset
+    k2 = big[:, k2_off:k2_off + 8].copy().view(np.float64).reshape(n_var, n_ele)
+    k2[:, sext] = k2_nom[None, :] * (1.0 + knob_sets[:, fam])
+    big[:, k2_off:k2_off + 8] = k2.reshape(-1, 1).view(np.uint8)
+    arr = (abi.EleRec * (n_var * n_ele)).from_buffer(big)
+    offs = [v * n_ele for v in range(n_var + 1)]
+    lat = ctx.lattice(arr, abi.ELECTRON, variant_offsets=offs)
+    lat._records = big  # the ctypes array views this buffer
+    return lat
+
+
+def features(knob_sets, spec):
+    """Network inputs [n_lat * n_ic, 35]: knobs / 0.2, x0 / amp_max, y0 / (amp_max ratio_y), pz / max|pz|."""
+    knob_sets = np.asarray(knob_sets, dtype=np.float64)
+    off = spec.offsets().reshape(-1, 3)
+    pzs = max(abs(spec.pz_min), abs(spec.pz_max), 1e-30)
+    geo = np.stack([off[:, 0] / spec.amp_max_x, off[:, 1] / (spec.amp_max_x * spec.amp_ratio_y), off[:, 2] / pzs], 1)
+    n_lat = len(knob_sets)
+    X = np.empty((n_lat, spec.n_ic, N_KNOBS + 3), dtype=np.float32)
+    X[:, :, :N_KNOBS] = (knob_sets / KNOB_RANGE)[:, None, :]
+    X[:, :, N_KNOBS:] = geo[None]
+    return X.reshape(-1, N_KNOBS + 3)
+
+
+def target(survival, n_turns):
+    """log survival turns, scaled to [0, 1]."""
+    return (np.log1p(np.asarray(survival, dtype=np.float64)) / np.log1p(n_turns)).astype(np.float32)
+
+
+def plain_scan(ctx, knob_sets, spec, turns_per_block=0):
+    """Full-grid FP64 tracking of every knob set in one batched launch.
+    Returns (survival [n_lat, n_pz, n_angle, n_amp], area [n_lat], timing dict
+    with total / upload (lattice build + upload) / track seconds)."""
+    import time
+    from . import tracker
+    t0 = time.perf_counter()
+    lat = upload_variants(ctx, knob_sets, spec)
+    t1 = time.perf_counter()
+    surv, _, _ = tracker.track_grid(lat, spec.n_turns, spec.n_angle, spec.n_amp, spec.n_pz, spec.amp_max_x,
+                                    spec.amp_ratio_y, spec.pz_min, spec.pz_max, turns_per_block=turns_per_block)
+    t2 = time.perf_counter()
+    lat.close()
+    area = np.array([slice_area(s[spec.n_pz // 2], spec) for s in surv])
+    return surv, area, dict(total=t2 - t0, upload=t1 - t0, track=t2 - t1)
+
+
+def slice_area(surv_slice, spec):
+    """DA area of one pz slice survival[n_angle, n_amp] (bmx_track_grid's definition)."""
+    from . import da
+    bx, by, _ = da.grid_boundary(surv_slice, spec.n_turns, spec.amp_max_x, spec.amp_ratio_y, spec.angles())
+    return da.polygon_area(bx, by)
+
+
+def area_from_runs(runs, spec):
+    """Area from per-angle surviving-prefix lengths runs[n_angle]."""
+    from . import da
+    a = spec.amp_max_x * np.asarray(runs) / spec.n_amp
+    th = spec.angles()
+    return da.polygon_area(a * np.cos(th), a * spec.amp_ratio_y * np.sin(th))
+
+
+def train(net, X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, lr_final=2e-4):
+    """Minibatch Adam on (X, Y) (numpy float32) with a cosine-decayed learning
+    rate; every epoch is one shuffled pass.  Returns the per-epoch mean losses."""
+    dev = net.dev
+    Xd = torch.from_numpy(np.ascontiguousarray(X)).to(dev)
+    Yd = torch.from_numpy(np.ascontiguousarray(Y)).to(dev)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    n = Xd.shape[0]
+    steps = epochs * max(1, n // batch)
+    step = 0
+    losses = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(dev)
+        acc = torch.zeros(1, device=dev)
+        nb = 0
+        for b in range(0, n - batch + 1, batch):
+            idx = perm[b:b + batch]
+            xb = Xd.index_select(0, idx).contiguous()
+            yb = Yd.index_select(0, idx).contiguous()
+            torch.cuda.synchronize(dev)
+            frac = step / max(steps - 1, 1)
+            cur = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac))
+            loss = net.train_step(xb, yb, float(cur))
+            net.sync()
+            acc += loss
+            nb += 1
+            step += 1
+        losses.append(float(acc.item()) / max(nb, 1))
+    return losses
+
+
+def predict(net, X, batch=1 << 16):
+    """Network output for X (numpy float32 [n, 35]) -> numpy float32 [n]."""
+    out = np.empty(len(X), dtype=np.float32)
+    Xd = torch.from_numpy(np.ascontiguousarray(X)).to(net.dev)
+    for b in range(0, len(X), batch):
+        xb = Xd[b:b + batch].contiguous()
+        torch.cuda.synchronize(net.dev)
+        y = net.forward(xb)
+        net.sync()
+        out[b:b + len(xb)] = y.cpu().numpy()
+    return out
+
+
+def predict_knobs(net, knob_sets, spec, lat_chunk=None):
+    """Network output for every grid IC of every knob set, building the inputs
+    one chunk of lattices at a time: [n_lat, n_pz, n_angle, n_amp] float32."""
+    knob_sets = np.asarray(knob_sets)
+    n_lat = len(knob_sets)
+    if lat_chunk is None:
+        lat_chunk = max(1, (1 << 21) // spec.n_ic)
+    out = np.empty((n_lat, spec.n_ic), dtype=np.float32)
+    for a in range(0, n_lat, lat_chunk):
+        b = min(n_lat, a + lat_chunk)
+        out[a:b] = predict(net, features(knob_sets[a:b], spec)).reshape(b - a, spec.n_ic)
+    return out.reshape(n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
+
+
+def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, alive_frac=0.5, turns_per_block=0, max_rounds=64, refine=6):
+    """Surrogate-screened DA scan.  Every ray (lattice, pz, angle) gets a
+    predicted boundary index from the network; the initial conditions whose
+    amplitude lies within +-band of the predicted boundary amplitude are
+    re-tracked in FP64 (bmx_track_ics, one launch for all rays).  Where the
+    re-tracked band does not pin the first loss -- untracked amplitudes remain
+    between the highest tracked survivor and the first tracked loss (or the
+    grid edge) -- the next round tracks up to `refine` evenly spaced points of
+    that gap, until every ray is resolved: plain FP64 tracking is the fallback
+    and the judge of every boundary.  Returns (runs [n_lat, n_pz, n_angle], area [n_lat],
+    timing dict (total / predict / upload / track seconds), n_tracked)."""
+    import time
+    from . import tracker
+    t0 = time.perf_counter()
+    knob_sets = np.asarray(knob_sets)
+    n_lat = len(knob_sets)
+    N = spec.n_turns
+    pred = predict_knobs(net, knob_sets, spec)
+    thr = np.log1p(alive_frac * N) / np.log1p(N)
+    alive_p = pred >= thr
+    kb = np.cumprod(alive_p, axis=-1).sum(-1)  # predicted surviving-prefix length
+    amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
+    # band in index space: amplitudes within +-band of the boundary amplitude
+    a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
+    lo = np.searchsorted(amps, a_b * (1 - band), side="left")
+    hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
+    lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
+    hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
+    shape = (n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
+    known = np.full(shape, -1, dtype=np.int64)  # tracked survival turns, -1 = not tracked
+    t_pred = time.perf_counter()
+    lat = upload_variants(ctx, knob_sets, spec)
+    t_up = time.perf_counter()
+    off = spec.offsets()
+    kk = np.arange(spec.n_amp)
+    n_tracked = 0
+    rounds = []
+    next_mask = (kk >= lo[..., None]) & (kk <= hi[..., None])
+    for _ in range(max_rounds):
+        idx = np.argwhere(next_mask & (known < 0))  # sorted by lattice
+        if len(idx) == 0:
+            break
+        n_tracked += len(idx)
+        o = off[idx[:, 1], idx[:, 2], idx[:, 3]]
+        counts = np.bincount(idx[:, 0], minlength=n_lat)
+        splits = np.cumsum(counts)[:-1]
+        per = []
+        for part in np.split(o, splits):
+            c = np.zeros((6, len(part)))
+            c[0], c[2], c[5] = part[:, 0], part[:, 1], part[:, 2]
+            per.append(c)
+        tr0 = time.perf_counter()
+        surv = tracker.track_ics(lat, per, N, turns_per_block)
+        rounds.append((len(idx), time.perf_counter() - tr0))
+        known[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]] = np.concatenate(surv)
+        # resolution per ray: b = first tracked loss (n_amp if none), a = the
+        # highest tracked survivor below it (-1 if none).  All amplitudes in
+        # (a, b) are untracked; the ray is resolved when none are left.  The
+        # next round tracks up to `refine` evenly spaced points of (a, b).
+        tracked = known >= 0
+        lostm = tracked & (known < N)
+        has_lost = lostm.any(-1)
+        bnd = np.where(has_lost, np.argmax(lostm, -1), spec.n_amp)
+        alive_below = tracked & (known >= N) & (kk < bnd[..., None])
+        a_top = np.where(alive_below.any(-1), spec.n_amp - 1 - np.argmax(alive_below[..., ::-1], -1), -1)
+        gap = bnd - a_top - 1
+        j = np.arange(1, refine + 1)
+        pts = a_top[..., None] + np.ceil((gap[..., None] + 1) * j / (refine + 1)).astype(np.int64)
+        pts = np.where((pts > a_top[..., None]) & (pts < bnd[..., None]), pts, -1)
+        sel = np.zeros(shape, dtype=bool)
+        li, pi, ai, ji = np.nonzero(pts >= 0)
+        sel[li, pi, ai, pts[li, pi, ai, ji]] = True
+        next_mask = sel
+    lat.close()
+    # surviving prefix: untracked amplitudes below the first tracked loss count as alive
+    lostm = (known >= 0) & (known < N)
+    runs = np.where(lostm.any(-1), np.argmax(lostm, -1), spec.n_amp)
+    area = np.array([area_from_runs(runs[v, spec.n_pz // 2], spec) for v in range(n_lat)])
+    t1 = time.perf_counter()
+    timing = dict(total=t1 - t0, predict=t_pred - t0, upload=t_up - t_pred, track=t1 - t_up, rounds=rounds)
+    return runs, area, timing, n_tracked
+
+
+def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed=20261205, band=0.10):
+    """Surrogate-screened vs plain DA scans of n_test unseen knob settings:
+    trains a fresh 4 x 256 network on a plain scan of n_train settings, then
+    scans the test settings both ways.  Returns a dict of timings,
+    lattices/hour (tracking + screening; lattice construction and upload are
+    excluded from both), FP64-tracked IC counts and the DA-area error
+    statistics."""
+    import time
+    rng = np.random.default_rng(seed)
+    k_train = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_train, N_KNOBS))
+    k_test = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_test, N_KNOBS))
+    s_tr, _, t_gen = plain_scan(ctx, k_train, spec)
+    t0 = time.perf_counter()
+    net = MLP(ctx, [N_KNOBS + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
+    X = features(k_train, spec)
+    Y = target(s_tr.reshape(-1), spec.n_turns)
+    del s_tr
+    losses = train(net, X, Y, epochs=epochs, batch=batch)
+    del X, Y
+    t_train = time.perf_counter() - t0
+    s_te, a_plain, t_plain = plain_scan(ctx, k_test, spec)
+    plain_runs = np.cumprod(s_te >= spec.n_turns, axis=-1).sum(-1)
+    del s_te
+    runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band)
+    err = np.abs(a_sur - a_plain) / np.maximum(a_plain, 1e-300)
+    plain_s = t_plain["track"]
+    sur_s = t_sur["predict"] + t_sur["track"]
+    per_lat_gain = plain_s / n_test - sur_s / n_test
+    return {
+        "n_train": n_train, "n_test": n_test, "ics_per_lattice": spec.n_ic, "train_samples": n_train * spec.n_ic,
+        "train_scan_s": t_gen["track"], "train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
+        "plain_s": plain_s, "surrogate_s": sur_s,
+        "plain_lattices_per_hour": n_test / plain_s * 3600, "surrogate_lattices_per_hour": n_test / sur_s * 3600,
+        "speedup": plain_s / sur_s,
+        "ics_tracked_plain": n_test * spec.n_ic, "ics_tracked_surrogate": int(n_tracked),
+        "rays_identical": float(np.mean(runs == plain_runs)),
+        "da_area_err_mean": float(err.mean()), "da_area_err_max": float(err.max()),
+        "frac_within_2pct": float(np.mean(err <= 0.02)),
+        "rounds": [[int(n), round(t, 4)] for n, t in t_sur["rounds"]], "predict_s": t_sur["predict"],
+        "break_even_lattices": (t_gen["track"] + t_train) / per_lat_gain if per_lat_gain > 0 else None,
+    }

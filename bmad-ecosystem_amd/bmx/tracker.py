@@ -153,8 +153,19 @@ def track_bunch(lat, bunch, n_turns=1, ix_start=0, ix_end=0, direction=1, sync=T
 def track_bunch_host(lat, coords, n_turns=1, beta=None):
     """Same as track_bunch on host numpy arrays (the library stages them through HBM)."""
     c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)
-    n = c.shape[1]
     cols = [np.ascontiguousarray(c[i]) for i in range(6)]
+    p, extra = _host_particles(lat, cols, beta)
+    st = abi.Stats()
+    rc = lat.ctx.lib.bmx_track_bunch(lat.ctx.h, lat.h, 0, 0, int(n_turns), 1, C.byref(p), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_track_bunch")
+    out = {k: v for k, v in zip(Bunch.NAMES, cols)}
+    out.update(extra)
+    out.update(n_lost=st.n_lost, kernel_ms=st.kernel_ms)
+    return out
+
+
+def _host_particles(lat, cols, beta=None):
+    n = len(cols[0])
     if beta is None:
         pc = lat.p0c * (1.0 + cols[5])
         m = abi.MASS[lat.species]
@@ -172,17 +183,72 @@ def track_bunch_host(lat, coords, n_turns=1, beta=None):
     p.turn_lost = turn_lost.ctypes.data
     p.n = n
     p.on_device = 0
+    return p, dict(beta=beta, state=state, ix_lost=ix_lost, turn_lost=turn_lost)
+
+
+def track_multi(lat, coords_per_variant, n_turns=1):
+    """Track one host coordinate set [6, n_v] per variant of `lat` in one launch
+    (bmx_track_multi).  Returns a list of per-variant result dicts (x..pz,
+    beta, state, ix_lost, turn_lost)."""
+    if len(coords_per_variant) != lat.n_variants:
+        raise ValueError("one coordinate array per variant is required")
+    blocks = [np.asarray(c, dtype=np.float64).reshape(6, -1) for c in coords_per_variant]
+    offs = np.zeros(lat.n_variants + 1, dtype=np.int64)
+    offs[1:] = np.cumsum([b.shape[1] for b in blocks])
+    allc = np.concatenate(blocks, axis=1) if offs[-1] else np.zeros((6, 0))
+    cols = [np.ascontiguousarray(allc[i]) for i in range(6)]
+    p, extra = _host_particles(lat, cols)
     st = abi.Stats()
-    rc = lat.ctx.lib.bmx_track_bunch(lat.ctx.h, lat.h, 0, 0, int(n_turns), 1, C.byref(p), C.byref(st))
-    _check(lat.ctx.h, rc, "bmx_track_bunch")
-    out = {k: v for k, v in zip(Bunch.NAMES, cols)}
-    out.update(beta=beta, state=state, ix_lost=ix_lost, turn_lost=turn_lost, n_lost=st.n_lost, kernel_ms=st.kernel_ms)
+    rc = lat.ctx.lib.bmx_track_multi(lat.ctx.h, lat.h, offs.ctypes.data_as(C.POINTER(C.c_int64)), int(n_turns),
+                                     C.byref(p), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_track_multi")
+    out = []
+    for v in range(lat.n_variants):
+        a, b = offs[v], offs[v + 1]
+        d = {k: col[a:b].copy() for k, col in zip(Bunch.NAMES, cols)}
+        d.update({k: val[a:b].copy() for k, val in extra.items()})
+        out.append(d)
+    return out
+
+
+def closed_orbit(lat, n_dim=4, guess=None):
+    """Closed orbit of every variant (bmx_closed_orbit).  guess: [n_var, 6]
+    (pz and, for n_dim = 4, z are held).  Returns (orbit [n_var, 6], status)."""
+    orb = np.zeros((lat.n_variants, 6)) if guess is None else np.array(guess, dtype=np.float64).reshape(lat.n_variants, 6)
+    orb = np.ascontiguousarray(orb)
+    status = np.zeros(lat.n_variants, dtype=np.int32)
+    st = abi.Stats()
+    rc = lat.ctx.lib.bmx_closed_orbit(lat.ctx.h, lat.h, int(n_dim), orb.ctypes.data_as(C.POINTER(C.c_double)),
+                                      status.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_closed_orbit")
+    return orb, status
+
+
+def track_ics(lat, ics_per_variant, n_turns, turns_per_block=0, return_stats=False):
+    """Survival turns of explicit initial conditions, one [6, n_v] host array
+    per variant (bmx_track_ics).  Returns a list of int32 arrays."""
+    blocks = [np.asarray(c, dtype=np.float64).reshape(6, -1) for c in ics_per_variant]
+    if len(blocks) != lat.n_variants:
+        raise ValueError("one initial-condition array per variant is required")
+    offs = np.zeros(lat.n_variants + 1, dtype=np.int64)
+    offs[1:] = np.cumsum([b.shape[1] for b in blocks])
+    allc = np.ascontiguousarray(np.concatenate(blocks, axis=1))
+    surv = np.zeros(int(offs[-1]), dtype=np.int32)
+    st = abi.Stats()
+    rc = lat.ctx.lib.bmx_track_ics(lat.ctx.h, lat.h, allc.ctypes.data_as(C.POINTER(C.c_double)),
+                                   offs.ctypes.data_as(C.POINTER(C.c_int64)), int(n_turns), int(turns_per_block),
+                                   surv.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_track_ics")
+    out = [surv[offs[v]:offs[v + 1]].copy() for v in range(lat.n_variants)]
+    if return_stats:
+        return out, dict(pet=st.pet, n_live=st.n_live, n_lost=st.n_lost, kernel_ms=st.kernel_ms)
     return out
 
 
 def track_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, pz_min=0.0, pz_max=0.0,
-               angle_min=0.0, angle_max=np.pi, turns_per_block=0, return_stats=False):
+               angle_min=0.0, angle_max=np.pi, turns_per_block=0, ref_orbit=None, return_stats=False):
     """Batched DA grid over every variant of `lat` (bmx_track_grid).
+    ref_orbit: None or [n_var, n_pz, 6] reference orbits the offsets are added to.
     Returns (survival[n_var, n_pz, n_angle, n_amp], da_area[n_var], kernel_ms)
     (plus the stats dict when return_stats)."""
     g = abi.Grid(n_angle, n_amp, n_pz, int(turns_per_block), amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min,
@@ -190,8 +256,12 @@ def track_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, p
     n_ic = n_angle * n_amp * n_pz
     surv = np.zeros(lat.n_variants * n_ic, dtype=np.int32)
     area = np.zeros(lat.n_variants, dtype=np.float64)
+    ref_p = None
+    if ref_orbit is not None:
+        ref = np.ascontiguousarray(np.asarray(ref_orbit, dtype=np.float64).reshape(lat.n_variants, n_pz, 6))
+        ref_p = ref.ctypes.data_as(C.POINTER(C.c_double))
     st = abi.Stats()
-    rc = lat.ctx.lib.bmx_track_grid(lat.ctx.h, lat.h, C.byref(g), int(n_turns),
+    rc = lat.ctx.lib.bmx_track_grid(lat.ctx.h, lat.h, C.byref(g), ref_p, int(n_turns),
                                     surv.ctypes.data_as(C.POINTER(C.c_int32)),
                                     area.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st))
     _check(lat.ctx.h, rc, "bmx_track_grid")

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/bmx.h"
@@ -748,16 +749,14 @@ struct SoA {
 #define BMX_WAVES_PER_EU 4
 #endif
 
-// Multi-turn bunch tracking: segments [lo1, hi1) then [lo2, hi2) per turn.
-__global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_bunch_kernel(
-    const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2, LatticeConsts lc, int n_turns, SoA p,
-    int64_t n, unsigned long long* __restrict__ n_lost) {
-  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < n;
+// One lane of a bunch: load, `n_turns` turns over [lo1, hi1) then [lo2, hi2),
+// store.  Lanes already lost on entry are left untouched.
+__device__ __forceinline__ void track_lane(int64_t i, bool valid, const DevOp* __restrict__ ops, int lo1, int hi1,
+                                           int lo2, int hi2, const LatticeConsts& lc, int n_turns, SoA& p,
+                                           unsigned long long* __restrict__ n_lost, DevOp* sops) {
   Lane L;
   L.state = valid ? p.state[i] : BMX_LOST;
-  bool entered = L.state == BMX_ALIVE;  // lanes already lost are left untouched
+  bool entered = L.state == BMX_ALIVE;
   bool alive = false;
   int ie_lost = -1, lost_turn = 0;
   if (entered) {
@@ -775,6 +774,26 @@ __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_bunch_kernel(
     if (p.turn_lost) p.turn_lost[i] = lost_turn;
     atomicAdd(n_lost, 1ull);
   }
+}
+
+// Multi-turn bunch tracking: segments [lo1, hi1) then [lo2, hi2) per turn.
+__global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_bunch_kernel(
+    const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2, LatticeConsts lc, int n_turns, SoA p,
+    int64_t n, unsigned long long* __restrict__ n_lost) {
+  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  track_lane(i, i < n, ops, lo1, hi1, lo2, hi2, lc, n_turns, p, n_lost, sops);
+}
+
+// Particles of several variants in one launch: particles [off[v], off[v+1])
+// are tracked through variant v (blockIdx.y = v, whole turns).
+__global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_multi_kernel(
+    const DevOp* __restrict__ ops, const int32_t* __restrict__ var_op_begin, const int64_t* __restrict__ off,
+    LatticeConsts lc, int n_turns, SoA p, unsigned long long* __restrict__ n_lost) {
+  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
+  int v = blockIdx.y;
+  int64_t i = off[v] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  track_lane(i, i < off[v + 1], ops, var_op_begin[v], var_op_begin[v + 1], 0, 0, lc, n_turns, p, n_lost, sops);
 }
 
 // Batched grid tracking in turn blocks with lane compaction.  Particles of
@@ -795,6 +814,7 @@ struct GridSoA {
 };
 
 __global__ __launch_bounds__(256) void bmx_grid_init_kernel(GridDev g, LatticeConsts lc, GridSoA s,
+                                                           const double* __restrict__ ref_orbit,
                                                            int32_t* __restrict__ survival, int n_turns) {
   int v = blockIdx.y;
   int ic = blockIdx.x * blockDim.x + threadIdx.x;
@@ -806,13 +826,33 @@ __global__ __launch_bounds__(256) void bmx_grid_init_kernel(GridDev g, LatticeCo
   double th = (g.n_angle > 1) ? g.angle_min + (g.angle_max - g.angle_min) * (double)i_ang / (double)(g.n_angle - 1)
                               : g.angle_min;
   double pz0 = (g.n_pz > 1) ? g.pz_min + (g.pz_max - g.pz_min) * (double)i_pz / (double)(g.n_pz - 1) : g.pz_min;
+  double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, pz0};
+  if (ref_orbit) {  // offsets are taken from the reference orbit of this variant and pz slice
+    const double* o = ref_orbit + ((int64_t)v * g.n_pz + i_pz) * 6;
+    for (int k = 0; k < 6; k++) r[k] = o[k];
+  }
   int64_t k = (int64_t)v * g.n_ic + ic;
-  s.x[k] = amp * cos(th);
-  s.y[k] = amp * g.amp_ratio_y * sin(th);
-  s.px[k] = 0.0; s.py[k] = 0.0; s.z[k] = 0.0; s.pz[k] = pz0;
-  double pc = lc.p0c * (1.0 + pz0);
-  s.beta[k] = (pz0 == 0.0) ? lc.beta0 : pc / sqrt(pc * pc + lc.mc2sq);
+  s.x[k] = r[0] + amp * cos(th);
+  s.y[k] = r[2] + amp * g.amp_ratio_y * sin(th);
+  s.px[k] = r[1]; s.py[k] = r[3]; s.z[k] = r[4]; s.pz[k] = r[5];
+  double pc = lc.p0c * (1.0 + r[5]);
+  s.beta[k] = (r[5] == 0.0) ? lc.beta0 : pc / sqrt(pc * pc + lc.mc2sq);
   s.ic[k] = ic;
+  survival[k] = n_turns;
+}
+
+// Explicit initial conditions already placed in slots [v * stride, v * stride + count[v]):
+// beta from pz (init_coord), slot ids, survival = n_turns.
+__global__ __launch_bounds__(256) void bmx_ics_init_kernel(LatticeConsts lc, GridSoA s, const int32_t* __restrict__ count,
+                                                          int stride, int32_t* __restrict__ survival, int n_turns) {
+  int v = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count[v]) return;
+  int64_t k = (int64_t)v * stride + i;
+  double pz = s.pz[k];
+  double pc = lc.p0c * (1.0 + pz);
+  s.beta[k] = (pz == 0.0) ? lc.beta0 : pc / sqrt(pc * pc + lc.mc2sq);
+  s.ic[k] = i;
   survival[k] = n_turns;
 }
 
@@ -1013,13 +1053,80 @@ void bmx_lattice_destroy(bmx_lattice* lat) {
   delete lat;
 }
 
+}  // extern "C"
+
+// Device views of a bmx_particles: host arrays are uploaded into buffers the
+// call owns; results go back and the buffers are freed in finish_particles.
+struct Staged {
+  SoA s;
+  void* dp[10];
+  void* hp[10];
+  size_t by[10];
+  std::vector<void*> owned;
+  unsigned long long* d_lost = nullptr;
+};
+
+static int stage_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st) {
+  int64_t n = p->n;
+  size_t bd = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
+  void* hp[10] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost};
+  size_t by[10] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi};
+  int rc = 0;
+  for (int k = 0; k < 10; k++) {
+    st.hp[k] = hp[k];
+    st.by[k] = by[k];
+    st.dp[k] = nullptr;
+    if (rc || !hp[k]) continue;
+    if (p->on_device) { st.dp[k] = hp[k]; continue; }
+    if (hipMalloc(&st.dp[k], by[k]) != hipSuccess) { rc = -1; continue; }
+    st.owned.push_back(st.dp[k]);
+    if (hipMemcpyAsync(st.dp[k], hp[k], by[k], hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = -1;
+  }
+  if (rc == 0 && hipMalloc(&st.d_lost, sizeof(unsigned long long)) != hipSuccess) rc = -1;
+  if (rc != 0) {
+    for (void* q : st.owned) (void)hipFree(q);
+    if (st.d_lost) (void)hipFree(st.d_lost);
+    return set_err(ctx, "device allocation or upload of the particle arrays failed");
+  }
+  (void)hipMemsetAsync(st.d_lost, 0, sizeof(unsigned long long), ctx->stream);
+  st.s.x = (double*)st.dp[0]; st.s.px = (double*)st.dp[1]; st.s.y = (double*)st.dp[2]; st.s.py = (double*)st.dp[3];
+  st.s.z = (double*)st.dp[4]; st.s.pz = (double*)st.dp[5]; st.s.beta = (double*)st.dp[6];
+  st.s.state = (int32_t*)st.dp[7]; st.s.ix_lost = (int32_t*)st.dp[8]; st.s.turn_lost = (int32_t*)st.dp[9];
+  return 0;
+}
+
+static int finish_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st, hipError_t le, bmx_stats* out) {
+  unsigned long long h_lost = 0;
+  (void)hipMemcpyAsync(&h_lost, st.d_lost, sizeof(h_lost), hipMemcpyDeviceToHost, ctx->stream);
+  if (!p->on_device)
+    for (int k = 0; k < 10; k++)
+      if (st.hp[k]) (void)hipMemcpyAsync(st.hp[k], st.dp[k], st.by[k], hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t se = hipStreamSynchronize(ctx->stream);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  for (void* q : st.owned) (void)hipFree(q);
+  (void)hipFree(st.d_lost);
+  if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
+  if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
+  if (out) {
+    out->n_lost = (int64_t)h_lost;
+    out->kernel_ms = ms;
+  }
+  return 0;
+}
+
+static bool particles_ok(const bmx_particles* p) {
+  return p->x && p->px && p->y && p->py && p->z && p->pz && p->beta && p->state;
+}
+
+extern "C" {
+
 int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
                         int32_t direction, bmx_particles* p, bmx_stats* out) {
   if (!ctx || !lat || !p) return set_err(ctx, "invalid arguments");
   if (direction != 1) return set_err(ctx, "only direction = +1 is supported on the device path");
   if (n_turns < 0) return set_err(ctx, "n_turns < 0");
-  if (!p->x || !p->px || !p->y || !p->py || !p->z || !p->pz || !p->beta || !p->state)
-    return set_err(ctx, "particle arrays x..pz, beta, state are required");
+  if (!particles_ok(p)) return set_err(ctx, "particle arrays x..pz, beta, state are required");
   int n_ele = lat->var_n_ele[0];
   if (ix_start < 0 || ix_start > n_ele || ix_end < 0 || ix_end > n_ele) return set_err(ctx, "ix_start/ix_end out of range");
   const std::vector<int32_t>& eb = lat->ele_op_begin[0];
@@ -1031,64 +1138,144 @@ int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, 
   int64_t n = p->n;
   if (out) std::memset(out, 0, sizeof(*out));
   if (n <= 0 || n_turns == 0) return 0;
-
-  SoA s;
-  std::vector<void*> owned;
-  auto dev_arr = [&](void* host, size_t bytes, bool required, void** dptr) -> int {
-    if (!host) { *dptr = nullptr; return required ? -1 : 0; }
-    if (p->on_device) { *dptr = host; return 0; }
-    if (hipMalloc(dptr, bytes) != hipSuccess) return -1;
-    owned.push_back(*dptr);
-    if (hipMemcpyAsync(*dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return -1;
-    return 0;
-  };
-  size_t bd = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
-  void* dp[10];
-  void* hp[10] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost};
-  size_t by[10] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi};
-  int rc = 0;
-  for (int k = 0; k < 10 && rc == 0; k++) rc = dev_arr(hp[k], by[k], k < 8, &dp[k]);
-  unsigned long long* d_lost = nullptr;
-  if (rc == 0 && hipMalloc(&d_lost, sizeof(unsigned long long)) != hipSuccess) rc = -1;
-  if (rc != 0) {
-    for (void* q : owned) hipFree(q);
-    if (d_lost) hipFree(d_lost);
-    return set_err(ctx, "device allocation or upload of the particle arrays failed");
-  }
-  hipMemsetAsync(d_lost, 0, sizeof(unsigned long long), ctx->stream);
-  s.x = (double*)dp[0]; s.px = (double*)dp[1]; s.y = (double*)dp[2]; s.py = (double*)dp[3];
-  s.z = (double*)dp[4]; s.pz = (double*)dp[5]; s.beta = (double*)dp[6];
-  s.state = (int32_t*)dp[7]; s.ix_lost = (int32_t*)dp[8]; s.turn_lost = (int32_t*)dp[9];
-
+  Staged st;
+  if (stage_particles(ctx, p, st) != 0) return -1;
   const int block = 256;
   int64_t grid = (n + block - 1) / block;
-  hipEventRecord(ctx->ev0, ctx->stream);
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
   hipLaunchKernelGGL(bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block), 0, ctx->stream, lat->d_ops, lo1, hi1,
-                     lo2, hi2, lat->lc, n_turns, s, n, d_lost);
-  hipEventRecord(ctx->ev1, ctx->stream);
+                     lo2, hi2, lat->lc, n_turns, st.s, n, st.d_lost);
   hipError_t le = hipGetLastError();
-  unsigned long long h_lost = 0;
-  hipMemcpyAsync(&h_lost, d_lost, sizeof(h_lost), hipMemcpyDeviceToHost, ctx->stream);
-  if (!p->on_device) {
-    for (int k = 0; k < 10; k++)
-      if (hp[k]) hipMemcpyAsync(hp[k], dp[k], by[k], hipMemcpyDeviceToHost, ctx->stream);
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  return finish_particles(ctx, p, st, le, out);
+}
+
+int32_t bmx_track_multi(bmx_ctx* ctx, const bmx_lattice* lat, const int64_t* part_offsets, int32_t n_turns,
+                        bmx_particles* p, bmx_stats* out) {
+  if (!ctx || !lat || !p || !part_offsets) return set_err(ctx, "invalid arguments");
+  if (n_turns < 0) return set_err(ctx, "n_turns < 0");
+  if (!particles_ok(p)) return set_err(ctx, "particle arrays x..pz, beta, state are required");
+  int nv = lat->n_variants;
+  if (part_offsets[0] != 0 || part_offsets[nv] != p->n) return set_err(ctx, "part_offsets must run from 0 to p->n");
+  int64_t max_count = 0;
+  for (int v = 0; v < nv; v++) {
+    if (part_offsets[v + 1] < part_offsets[v]) return set_err(ctx, "part_offsets must be non-decreasing");
+    max_count = std::max(max_count, part_offsets[v + 1] - part_offsets[v]);
   }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (out) std::memset(out, 0, sizeof(*out));
+  if (p->n <= 0 || n_turns == 0) return 0;
+  Staged st;
+  if (stage_particles(ctx, p, st) != 0) return -1;
+  int64_t* d_off = nullptr;
+  if (hipMalloc(&d_off, (nv + 1) * sizeof(int64_t)) != hipSuccess) {
+    finish_particles(ctx, p, st, hipSuccess, nullptr);
+    return set_err(ctx, "hipMalloc of the variant offsets failed");
+  }
+  (void)hipMemcpyAsync(d_off, part_offsets, (nv + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream);
+  const int block = 256;
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
+  hipLaunchKernelGGL(bmx_track_multi_kernel, dim3((unsigned)((max_count + block - 1) / block), (unsigned)nv),
+                     dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin, d_off, lat->lc, n_turns, st.s,
+                     st.d_lost);
+  hipError_t le = hipGetLastError();
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  int rc = finish_particles(ctx, p, st, le, out);
+  (void)hipFree(d_off);
+  return rc;
+}
+
+}  // extern "C"
+
+// Survival tracking of per-variant particle stores in turn blocks with lane
+// compaction (bmx_grid_block_kernel), shared by the polar grid and explicit
+// initial conditions.  The caller fills store 0 and the survival array through
+// `init` (a launcher run on the stream after the buffers exist).
+struct SurvivalRun {
+  int nv = 0, stride = 0;
+  GridSoA st[2];
+  int32_t* d_surv = nullptr;
+  int32_t* d_count = nullptr;
+  unsigned long long* d_pet = nullptr;
+  std::vector<void*> owned;
+  ~SurvivalRun() {
+    for (void* q : owned) (void)hipFree(q);
+  }
+  bool alloc(int n_var, int n_stride) {
+    nv = n_var;
+    stride = n_stride;
+    int64_t total = (int64_t)nv * stride;
+    auto dalloc = [&](size_t bytes) -> void* {
+      void* p = nullptr;
+      if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+      owned.push_back(p);
+      return p;
+    };
+    for (int b = 0; b < 2; b++) {
+      double** f[7] = {&st[b].x, &st[b].px, &st[b].y, &st[b].py, &st[b].z, &st[b].pz, &st[b].beta};
+      for (int k = 0; k < 7; k++)
+        if (!(*f[k] = (double*)dalloc(total * sizeof(double)))) return false;
+      if (!(st[b].ic = (int32_t*)dalloc(total * sizeof(int32_t)))) return false;
+    }
+    return (d_surv = (int32_t*)dalloc(total * sizeof(int32_t))) && (d_count = (int32_t*)dalloc(2 * nv * sizeof(int32_t))) &&
+           (d_pet = (unsigned long long*)dalloc(sizeof(unsigned long long)));
+  }
+};
+
+// Runs the turn blocks; counts[v] = initial particles of variant v.  Fills the
+// host survival array (nv * stride), PET and timing.
+static int run_survival(bmx_ctx* ctx, const bmx_lattice* lat, SurvivalRun& R, std::vector<int32_t> h_count,
+                        int n_turns, int tpb, int32_t* h_surv, bmx_stats* out,
+                        const std::function<void()>& init) {
+  const int block = 256;
+  int nv = R.nv;
+  (void)hipMemcpyAsync(R.d_count, h_count.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+  (void)hipMemsetAsync(R.d_pet, 0, sizeof(unsigned long long), ctx->stream);
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
+  init();
+  int cur = 0, turn0 = 0, max_live = 0;
+  for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
+  hipError_t le = hipGetLastError();
+  while (le == hipSuccess && turn0 < n_turns && max_live > 0) {
+    int turns = std::min(tpb, n_turns - turn0);
+    int32_t* cin = R.d_count + (cur ? nv : 0);
+    int32_t* cout = R.d_count + (cur ? 0 : nv);
+    (void)hipMemsetAsync(cout, 0, nv * sizeof(int32_t), ctx->stream);
+    dim3 gd((unsigned)((max_live + block - 1) / block), (unsigned)nv);
+    hipLaunchKernelGGL(bmx_grid_block_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin,
+                       lat->d_var_n_ele, lat->lc, R.stride, R.st[cur], R.st[1 - cur], cin, cout, turn0, turns,
+                       R.d_surv, R.d_pet);
+    le = hipGetLastError();
+    if (le != hipSuccess) break;
+    (void)hipMemcpyAsync(h_count.data(), cout, nv * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) break;
+    max_live = 0;
+    for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
+    cur = 1 - cur;
+    turn0 += turns;
+  }
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  unsigned long long h_pet = 0;
+  int64_t total = (int64_t)nv * R.stride;
+  (void)hipMemcpyAsync(&h_pet, R.d_pet, sizeof(h_pet), hipMemcpyDeviceToHost, ctx->stream);
+  (void)hipMemcpyAsync(h_surv, R.d_surv, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
   hipError_t se = hipStreamSynchronize(ctx->stream);
   float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  for (void* q : owned) hipFree(q);
-  hipFree(d_lost);
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
   if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
   if (out) {
-    out->n_lost = (int64_t)h_lost;
+    std::memset(out, 0, sizeof(*out));
     out->kernel_ms = ms;
+    out->pet = (double)h_pet;
   }
   return 0;
 }
 
-int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* grid, int32_t n_turns,
-                       int32_t* survival_turn, double* da_area, bmx_stats* out) {
+extern "C" {
+
+int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* grid, const double* ref_orbit,
+                       int32_t n_turns, int32_t* survival_turn, double* da_area, bmx_stats* out) {
   if (!ctx || !lat || !grid || !survival_turn) return set_err(ctx, "invalid arguments");
   if (grid->n_angle <= 0 || grid->n_amp <= 0 || grid->n_pz <= 0 || n_turns < 0) return set_err(ctx, "bad grid");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -1101,66 +1288,26 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
   int nv = lat->n_variants;
   int tpb = grid->turns_per_block > 0 ? grid->turns_per_block : 32;
   int64_t total = (int64_t)g.n_ic * nv;
-  // device buffers: two particle stores (ping-pong), survival, counts, PET counter
-  std::vector<void*> owned;
-  auto dalloc = [&](size_t bytes) -> void* {
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    owned.push_back(p);
-    return p;
-  };
-  auto cleanup = [&]() { for (void* q : owned) (void)hipFree(q); };
-  GridSoA st[2];
-  bool ok = true;
-  for (int b = 0; b < 2 && ok; b++) {
-    double** f[7] = {&st[b].x, &st[b].px, &st[b].y, &st[b].py, &st[b].z, &st[b].pz, &st[b].beta};
-    for (int k = 0; k < 7 && ok; k++) ok = (*f[k] = (double*)dalloc(total * sizeof(double))) != nullptr;
-    if (ok) ok = (st[b].ic = (int32_t*)dalloc(total * sizeof(int32_t))) != nullptr;
+  SurvivalRun R;
+  double* d_ref = nullptr;
+  if (!R.alloc(nv, g.n_ic)) return set_err(ctx, "device allocation for the grid failed");
+  if (ref_orbit) {
+    size_t bytes = (size_t)nv * g.n_pz * 6 * sizeof(double);
+    if (hipMalloc(&d_ref, bytes) != hipSuccess) return set_err(ctx, "device allocation for the grid failed");
+    R.owned.push_back(d_ref);
+    (void)hipMemcpyAsync(d_ref, ref_orbit, bytes, hipMemcpyHostToDevice, ctx->stream);
   }
-  int32_t* d_surv = ok ? (int32_t*)dalloc(total * sizeof(int32_t)) : nullptr;
-  int32_t* d_count = d_surv ? (int32_t*)dalloc(2 * nv * sizeof(int32_t)) : nullptr;
-  unsigned long long* d_pet = d_count ? (unsigned long long*)dalloc(sizeof(unsigned long long)) : nullptr;
-  if (!d_pet) { cleanup(); return set_err(ctx, "device allocation for the grid failed"); }
   std::vector<int32_t> h_count(nv, g.n_ic);
-  (void)hipMemcpyAsync(d_count, h_count.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
-  (void)hipMemsetAsync(d_pet, 0, sizeof(unsigned long long), ctx->stream);
   const int block = 256;
-  hipEventRecord(ctx->ev0, ctx->stream);
-  hipLaunchKernelGGL(bmx_grid_init_kernel, dim3((unsigned)((g.n_ic + block - 1) / block), (unsigned)nv), dim3(block), 0,
-                     ctx->stream, g, lat->lc, st[0], d_surv, n_turns);
-  int cur = 0, turn0 = 0, max_live = g.n_ic;
-  hipError_t le = hipGetLastError();
-  while (le == hipSuccess && turn0 < n_turns && max_live > 0) {
-    int turns = std::min(tpb, n_turns - turn0);
-    int32_t* cin = d_count + (cur ? nv : 0);
-    int32_t* cout = d_count + (cur ? 0 : nv);
-    (void)hipMemsetAsync(cout, 0, nv * sizeof(int32_t), ctx->stream);
-    dim3 gd((unsigned)((max_live + block - 1) / block), (unsigned)nv);
-    hipLaunchKernelGGL(bmx_grid_block_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin,
-                       lat->d_var_n_ele, lat->lc, g.n_ic, st[cur], st[1 - cur], cin, cout, turn0, turns, d_surv, d_pet);
-    le = hipGetLastError();
-    if (le != hipSuccess) break;
-    (void)hipMemcpyAsync(h_count.data(), cout, nv * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) break;
-    max_live = 0;
-    for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
-    cur = 1 - cur;
-    turn0 += turns;
-  }
-  hipEventRecord(ctx->ev1, ctx->stream);
-  unsigned long long h_pet = 0;
-  (void)hipMemcpyAsync(&h_pet, d_pet, sizeof(h_pet), hipMemcpyDeviceToHost, ctx->stream);
-  (void)hipMemcpyAsync(survival_turn, d_surv, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
-  hipError_t se = hipStreamSynchronize(ctx->stream);
-  float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  cleanup();
-  if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
-  if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
+  int rc = run_survival(ctx, lat, R, h_count, n_turns, tpb, survival_turn, out, [&]() {
+    hipLaunchKernelGGL(bmx_grid_init_kernel, dim3((unsigned)((g.n_ic + block - 1) / block), (unsigned)nv), dim3(block),
+                       0, ctx->stream, g, lat->lc, R.st[0], d_ref, R.d_surv, n_turns);
+  });
+  if (rc != 0) return rc;
   // DA area per variant: on the middle pz slice, the boundary point of each
-  // angle is the (x, y) of the last IC of the contiguous surviving run from
-  // the smallest amplitude; area = the polygon through the origin and those
-  // points in angle order, 1/2 sum (x_i y_{i+1} - x_{i+1} y_i).
+  // angle is the (x, y) offset of the last IC of the contiguous surviving run
+  // from the smallest amplitude; area = the polygon through the origin and
+  // those points in angle order, 1/2 sum (x_i y_{i+1} - x_{i+1} y_i).
   if (da_area) {
     int ipz = g.n_pz / 2;
     double dth = (g.n_angle > 1) ? (g.angle_max - g.angle_min) / (g.n_angle - 1) : 0.0;
@@ -1184,13 +1331,157 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
     }
   }
   if (out) {
-    std::memset(out, 0, sizeof(*out));
-    out->kernel_ms = ms;
-    out->pet = (double)h_pet;
     int64_t live = 0;
     for (int64_t k = 0; k < total; k++) live += (survival_turn[k] >= n_turns);
     out->n_live = live;
     out->n_lost = total - live;
+  }
+  return 0;
+}
+
+int32_t bmx_track_ics(bmx_ctx* ctx, const bmx_lattice* lat, const double* ics, const int64_t* ic_offsets,
+                      int32_t n_turns, int32_t turns_per_block, int32_t* survival_turn, bmx_stats* out) {
+  if (!ctx || !lat || !ics || !ic_offsets || !survival_turn || n_turns < 0) return set_err(ctx, "invalid arguments");
+  int nv = lat->n_variants;
+  if (ic_offsets[0] != 0) return set_err(ctx, "ic_offsets[0] must be 0");
+  int64_t n = ic_offsets[nv], stride = 0;
+  for (int v = 0; v < nv; v++) {
+    if (ic_offsets[v + 1] < ic_offsets[v]) return set_err(ctx, "ic_offsets must be non-decreasing");
+    stride = std::max(stride, ic_offsets[v + 1] - ic_offsets[v]);
+  }
+  if (stride > INT32_MAX / 2) return set_err(ctx, "too many initial conditions per variant");
+  if (out) std::memset(out, 0, sizeof(*out));
+  if (n == 0) return 0;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  SurvivalRun R;
+  if (!R.alloc(nv, (int)stride)) return set_err(ctx, "device allocation for the initial conditions failed");
+  // padded host staging of x..pz, [v * stride + i]
+  int64_t total = (int64_t)nv * stride;
+  std::vector<double> pad(total);
+  double* dst[6] = {R.st[0].x, R.st[0].px, R.st[0].y, R.st[0].py, R.st[0].z, R.st[0].pz};
+  for (int c = 0; c < 6; c++) {
+    std::fill(pad.begin(), pad.end(), 0.0);
+    for (int v = 0; v < nv; v++)
+      std::memcpy(&pad[(size_t)v * stride], ics + (size_t)c * n + ic_offsets[v],
+                  (size_t)(ic_offsets[v + 1] - ic_offsets[v]) * sizeof(double));
+    if (hipMemcpy(dst[c], pad.data(), total * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return set_err(ctx, "upload of the initial conditions failed");
+  }
+  std::vector<int32_t> h_count(nv);
+  for (int v = 0; v < nv; v++) h_count[v] = (int32_t)(ic_offsets[v + 1] - ic_offsets[v]);
+  std::vector<int32_t> h_surv(total);
+  const int block = 256;
+  int rc = run_survival(ctx, lat, R, h_count, n_turns, turns_per_block > 0 ? turns_per_block : 32, h_surv.data(), out,
+                        [&]() {
+                          hipLaunchKernelGGL(bmx_ics_init_kernel,
+                                             dim3((unsigned)((stride + block - 1) / block), (unsigned)nv), dim3(block),
+                                             0, ctx->stream, lat->lc, R.st[0], R.d_count, (int)stride, R.d_surv, n_turns);
+                        });
+  if (rc != 0) return rc;
+  int64_t live = 0;
+  for (int v = 0; v < nv; v++)
+    for (int64_t i = ic_offsets[v]; i < ic_offsets[v + 1]; i++) {
+      survival_turn[i] = h_surv[(size_t)v * stride + (i - ic_offsets[v])];
+      live += survival_turn[i] >= n_turns;
+    }
+  if (out) {
+    out->n_live = live;
+    out->n_lost = n - live;
+  }
+  return 0;
+}
+
+
+
+// closed_orbit_calc (bmad/code/closed_orbit_calc.f90:81-...) in the form the
+// batched tracker serves: Newton iteration x <- x + (I - M)^-1 (T(x) - x) on
+// the one-turn map T of every variant at once, with M from one-sided finite
+// differences (one launch tracks 1 + n_dim probes per variant).  n_dim = 4
+// keeps pz (and z) of the guess fixed; n_dim = 6 closes all six (rf on).
+int32_t bmx_closed_orbit(bmx_ctx* ctx, const bmx_lattice* lat, int32_t n_dim, double* orbit, int32_t* status,
+                         bmx_stats* out) {
+  if (!ctx || !lat || !orbit || !status || (n_dim != 4 && n_dim != 6)) return set_err(ctx, "invalid arguments");
+  int nv = lat->n_variants;
+  int np = 1 + n_dim;
+  int64_t n = (int64_t)nv * np;
+  std::vector<int64_t> off(nv + 1);
+  for (int v = 0; v <= nv; v++) off[v] = (int64_t)v * np;
+  std::vector<double> c(6 * n), beta(n);
+  std::vector<int32_t> state(n), ixl(n), tl(n);
+  std::vector<int> done(nv, 0);
+  for (int v = 0; v < nv; v++) status[v] = 1;
+  const double h = 1e-8;
+  double mc2 = lat->lc.mc2, p0c = lat->lc.p0c;
+  if (out) std::memset(out, 0, sizeof(*out));
+  for (int it = 0; it < 40; it++) {
+    for (int v = 0; v < nv; v++)
+      for (int j = 0; j < np; j++) {
+        int64_t k = off[v] + j;
+        for (int d = 0; d < 6; d++) c[(size_t)d * n + k] = orbit[v * 6 + d] + ((j > 0 && d == j - 1) ? h : 0.0);
+        double pz = c[(size_t)5 * n + k], pc = p0c * (1.0 + pz);
+        beta[k] = pc / std::sqrt(pc * pc + mc2 * mc2);
+        state[k] = BMX_ALIVE;
+      }
+    bmx_particles p;
+    std::memset(&p, 0, sizeof(p));
+    p.x = &c[0]; p.px = &c[(size_t)n]; p.y = &c[(size_t)2 * n]; p.py = &c[(size_t)3 * n];
+    p.z = &c[(size_t)4 * n]; p.pz = &c[(size_t)5 * n];
+    p.beta = beta.data(); p.state = state.data(); p.ix_lost = ixl.data(); p.turn_lost = tl.data();
+    p.n = n;
+    p.on_device = 0;
+    bmx_stats st;
+    if (bmx_track_multi(ctx, lat, off.data(), 1, &p, &st) != 0) return -1;
+    if (out) out->kernel_ms += st.kernel_ms;
+    int n_open = 0;
+    for (int v = 0; v < nv; v++) {
+      if (done[v]) continue;
+      bool lost = false;
+      for (int j = 0; j < np; j++) lost |= state[off[v] + j] != BMX_ALIVE;
+      if (lost) { status[v] = 2; done[v] = 1; continue; }
+      // residual r = T(x) - x and Jacobian columns over the first n_dim coordinates
+      double r[6], A[6][7];
+      double rmax = 0;
+      for (int d = 0; d < n_dim; d++) {
+        r[d] = c[(size_t)d * n + off[v]] - orbit[v * 6 + d];
+        rmax = std::max(rmax, std::fabs(r[d]));
+      }
+      if (rmax < 1e-15) { status[v] = 0; done[v] = 1; continue; }
+      for (int d = 0; d < n_dim; d++) {
+        for (int j = 0; j < n_dim; j++) {
+          double mdj = (c[(size_t)d * n + off[v] + 1 + j] - c[(size_t)d * n + off[v]]) / h;
+          A[d][j] = (d == j ? 1.0 : 0.0) - mdj;  // (I - M)
+        }
+        A[d][n_dim] = r[d];
+      }
+      // Gaussian elimination with partial pivoting: (I - M) dx = r
+      bool sing = false;
+      for (int col = 0; col < n_dim && !sing; col++) {
+        int piv = col;
+        for (int row = col + 1; row < n_dim; row++)
+          if (std::fabs(A[row][col]) > std::fabs(A[piv][col])) piv = row;
+        if (std::fabs(A[piv][col]) < 1e-14) { sing = true; break; }
+        for (int k = 0; k <= n_dim; k++) std::swap(A[col][k], A[piv][k]);
+        for (int row = col + 1; row < n_dim; row++) {
+          double f = A[row][col] / A[col][col];
+          for (int k = col; k <= n_dim; k++) A[row][k] -= f * A[col][k];
+        }
+      }
+      if (sing) { status[v] = 3; done[v] = 1; continue; }
+      double dx[6];
+      for (int row = n_dim - 1; row >= 0; row--) {
+        double acc = A[row][n_dim];
+        for (int k = row + 1; k < n_dim; k++) acc -= A[row][k] * dx[k];
+        dx[row] = acc / A[row][row];
+      }
+      double dmax = 0;
+      for (int d = 0; d < n_dim; d++) {
+        orbit[v * 6 + d] += dx[d];
+        dmax = std::max(dmax, std::fabs(dx[d]));
+      }
+      if (dmax < 1e-15) { status[v] = 0; done[v] = 1; continue; }
+      n_open++;
+    }
+    if (n_open == 0) break;
   }
   return 0;
 }

@@ -138,14 +138,47 @@ interface
     integer(c_int32_t) :: rc
   end function
 
-  function bmx_track_grid(ctx, lat, grid, n_turns, survival_turn, da_area, stats) &
+  function bmx_track_multi(ctx, lat, part_offsets, n_turns, p, stats) bind(C, name = 'bmx_track_multi') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    integer(c_int64_t) :: part_offsets(*)
+    integer(c_int32_t), value :: n_turns
+    type(bmx_particles) :: p
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_track_grid(ctx, lat, grid, ref_orbit, n_turns, survival_turn, da_area, stats) &
                           bind(C, name = 'bmx_track_grid') result(rc)
     import
     type(c_ptr), value :: ctx, lat
     type(bmx_grid) :: grid
+    type(c_ptr), value :: ref_orbit      ! c_null_ptr or c_loc of real(c_double) :: orb(6, n_pz, n_variants)
     integer(c_int32_t), value :: n_turns
     integer(c_int32_t) :: survival_turn(*)
     real(c_double) :: da_area(*)
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_track_ics(ctx, lat, ics, ic_offsets, n_turns, turns_per_block, survival_turn, stats) &
+                         bind(C, name = 'bmx_track_ics') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    real(c_double) :: ics(*)              ! (n, 6): x(1:n), px(1:n), ... pz(1:n)
+    integer(c_int64_t) :: ic_offsets(*)
+    integer(c_int32_t), value :: n_turns, turns_per_block
+    integer(c_int32_t) :: survival_turn(*)
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_closed_orbit(ctx, lat, n_dim, orbit, status, stats) bind(C, name = 'bmx_closed_orbit') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    integer(c_int32_t), value :: n_dim
+    real(c_double) :: orbit(*)            ! (6, n_variants)
+    integer(c_int32_t) :: status(*)
     type(bmx_stats) :: stats
     integer(c_int32_t) :: rc
   end function

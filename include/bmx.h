@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define BMX_ABI_VERSION 1
+#define BMX_ABI_VERSION 2
 #define BMX_N_POLE 22 /* n_pole_maxx + 1, bmad/modules/bmad_struct.f90 */
 
 /* ---- particle state codes (bmad/modules/bmad_struct.f90:520-527) ---- */
@@ -177,13 +177,14 @@ typedef struct bmx_stats {
 } bmx_stats;
 
 /* Batched dynamic-aperture grid: n_var lattice variants, each with the same
- * polar grid of initial conditions n_angle x n_amp x n_pz around the
- * variant's closed orbit (taken as zero here).  Initial condition
- * ic = (i_pz * n_angle + i_angle) * n_amp + i_amp has
- * x = a cos(th), y = a amp_ratio_y sin(th), px = py = z = 0, pz = pz_i with
- * a = amp_max_x (i_amp + 1) / n_amp.  Tracking runs in blocks of
- * turns_per_block turns; survivors are compacted between blocks
- * (turns_per_block <= 0: 32). */
+ * polar grid of initial conditions n_angle x n_amp x n_pz.  Initial condition
+ * ic = (i_pz * n_angle + i_angle) * n_amp + i_amp is the reference orbit of
+ * (variant, pz slice) plus the offset x = a cos(th), y = a amp_ratio_y sin(th)
+ * with a = amp_max_x (i_amp + 1) / n_amp (dynamic_aperture_point,
+ * bmad/modules/dynamic_aperture_mod.f90:333-336).  Without a reference orbit
+ * it is (0, 0, 0, 0, 0, pz_i), pz_i evenly spaced over [pz_min, pz_max].
+ * Tracking runs in blocks of turns_per_block turns; survivors are compacted
+ * between blocks (turns_per_block <= 0: 32). */
 typedef struct bmx_grid {
   int32_t n_angle, n_amp, n_pz, turns_per_block;
   double amp_max_x;         /* largest x amplitude [m]; y amplitude scales by amp_ratio_y */
@@ -223,12 +224,39 @@ void bmx_lattice_destroy(bmx_lattice *lat);
 int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
                         int32_t n_turns, int32_t direction, bmx_particles *p, bmx_stats *out);
 
-/* Track every variant's grid of initial conditions for n_turns full turns in
- * one launch.  survival_turn[v*n_ic + i] = number of turns survived
- * (n_turns if alive at the end), da_area[v] = polar-grid DA area [m^2].
- * Host output arrays. */
-int32_t bmx_track_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *grid, int32_t n_turns,
-                       int32_t *survival_turn, double *da_area, bmx_stats *out);
+/* Track particles of several lattice variants in one launch: particles
+ * [part_offsets[v], part_offsets[v+1]) go through variant v for n_turns whole
+ * turns; part_offsets has n_variants + 1 entries from 0 to p->n.  Replaces one
+ * track_many / track_bunch call per lattice (bmad/code/track_many.f90,
+ * bmad/multiparticle/beam_mod.f90:40-170) when many lattices are tracked. */
+int32_t bmx_track_multi(bmx_ctx *ctx, const bmx_lattice *lat, const int64_t *part_offsets, int32_t n_turns,
+                        bmx_particles *p, bmx_stats *out);
+
+/* Track every variant's grid of initial conditions for n_turns full turns.
+ * ref_orbit: NULL, or n_variants x n_pz x 6 reference orbits (closed orbit
+ * plus the pz offset, as dynamic_aperture_scan forms ap_scan%ref_orb,
+ * bmad/modules/dynamic_aperture_mod.f90:130-140).  survival_turn[v*n_ic + i]
+ * = turns survived (n_turns if alive at the end); da_area[v] = area of the
+ * polygon through the origin and the per-angle boundary offsets of the middle
+ * pz slice [m^2] (may be NULL).  Host output arrays. */
+int32_t bmx_track_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *grid, const double *ref_orbit,
+                       int32_t n_turns, int32_t *survival_turn, double *da_area, bmx_stats *out);
+
+/* Survival turns of explicit initial conditions: ics is 6 x n host SoA
+ * (x[n], px[n], y[n], py[n], z[n], pz[n]) with n = ic_offsets[n_variants];
+ * ICs [ic_offsets[v], ic_offsets[v+1]) are tracked through variant v.  The
+ * per-IC survival loop of dynamic_aperture_point
+ * (bmad/modules/dynamic_aperture_mod.f90:339-349), batched. */
+int32_t bmx_track_ics(bmx_ctx *ctx, const bmx_lattice *lat, const double *ics, const int64_t *ic_offsets,
+                      int32_t n_turns, int32_t turns_per_block, int32_t *survival_turn, bmx_stats *out);
+
+/* Closed orbit of every variant (closed_orbit_calc, bmad/code/closed_orbit_calc.f90:81):
+ * orbit is n_variants x 6, the guess on entry and the closed orbit on return.
+ * n_dim = 4 closes x, px, y, py at the guess's pz; n_dim = 6 closes all six
+ * (rf on).  status[v]: 0 converged, 1 not converged, 2 a probe was lost,
+ * 3 singular one-turn matrix. */
+int32_t bmx_closed_orbit(bmx_ctx *ctx, const bmx_lattice *lat, int32_t n_dim, double *orbit, int32_t *status,
+                         bmx_stats *out);
 
 /* ------------------------------------------------------------------------
  * Dynamic-aperture surrogate kernels (no counterpart in the reference; they

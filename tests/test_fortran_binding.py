@@ -40,4 +40,4 @@ def test_fortran_binding_layout(tmp_path):
     for line in out[:5]:
         a, b = line.split()
         assert a == b, out
-    assert out[5].strip() == "1"
+    assert out[5].strip() == "2"

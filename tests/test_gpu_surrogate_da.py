@@ -1,0 +1,53 @@
+"""End-to-end surrogate DA scan on the GPU: survival data from the batched FP64
+grid tracker, the MLP trained by libbmx's own forward/backward/Adam kernels,
+and the screened scan (prediction, FP64 re-tracking of the boundary band,
+extension fallback) on lattices the network has not seen, judged against
+plain full-grid FP64 tracking of the same lattices.
+
+What the trained network gains over the method without it: with the same
+band search, an untrained network (random initial weights) needs many more
+FP64-tracked initial conditions, because its predicted boundaries are far off
+and the fallback has to walk to the true ones.
+"""
+import numpy as np
+import pytest
+
+from bmx import surrogate
+
+pytestmark = pytest.mark.gpu
+
+
+def test_surrogate_da_scan_end_to_end(ctx):
+    spec = surrogate.ScanSpec(n_turns=64, n_angle=8, n_amp=24, n_pz=3, amp_max_x=0.02, amp_ratio_y=0.5,
+                              pz_min=-0.002, pz_max=0.002, n_cells=30)
+    rng = np.random.default_rng(2026)
+    k_train = rng.uniform(-0.2, 0.2, (96, surrogate.N_KNOBS))
+    k_test = rng.uniform(-0.2, 0.2, (12, surrogate.N_KNOBS))
+    surv_tr, area_tr, t_tr = surrogate.plain_scan(ctx, k_train, spec)
+    print(f"training scan: 96 lattices in {t_tr['total']:.2f} s; DA area {area_tr.min():.3e} .. {area_tr.max():.3e}",
+          flush=True)
+    assert area_tr.max() > 1.2 * area_tr.min(), "the knobs should move the DA"
+    X = surrogate.features(k_train, spec)
+    Y = surrogate.target(surv_tr.reshape(-1), spec.n_turns)
+    net = surrogate.MLP(ctx, [35, 256, 256, 256, 256, 1], seed=3)
+    losses = surrogate.train(net, X, Y, epochs=40, batch=2048, lr=2e-3)
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+    assert losses[-1] < 0.3 * losses[0]
+
+    surv_te, area_plain, t_plain = surrogate.plain_scan(ctx, k_test, spec)
+    runs, area_s, t_s, n_tr = surrogate.surrogate_scan(ctx, net, k_test, spec)
+    err = np.abs(area_s - area_plain) / area_plain
+    full = len(k_test) * spec.n_ic
+    print(f"surrogate: tracked {n_tr} of {full} ICs; DA-area error mean {err.mean():.4f} max {err.max():.4f}",
+          flush=True)
+    # the screened boundary equals the plain one wherever survival is monotonic along the ray
+    plain_runs = np.cumprod(surv_te >= spec.n_turns, axis=-1).sum(-1)
+    agree = np.mean(runs == plain_runs)
+    print(f"rays with identical boundary: {agree:.3f}", flush=True)
+    assert err.mean() <= 0.02
+    assert n_tr < 0.6 * full
+
+    untrained = surrogate.MLP(ctx, [35, 256, 256, 256, 256, 1], seed=4)
+    _, area_u, _, n_u = surrogate.surrogate_scan(ctx, untrained, k_test, spec)
+    print(f"untrained network: tracked {n_u} ICs", flush=True)
+    assert n_tr < n_u

@@ -1,0 +1,112 @@
+"""Host logic of the surrogate-screened DA scan (bmx.surrogate.surrogate_scan),
+with the FP64 tracker and the network replaced by CPU stand-ins: a synthetic
+survival field whose true boundary is known, and predictions off by a chosen
+number of grid steps.  The band search with its extension fallback must
+recover the true surviving-prefix length of every ray however far off the
+prediction is, and track fewer initial conditions than the full grid when the
+prediction is close.  (The GPU end-to-end run is in test_gpu_surrogate_da.py.)"""
+import numpy as np
+import pytest
+
+from bmx import surrogate
+
+
+class _FakeLat:
+    def close(self):
+        pass
+
+
+def _run(monkeypatch, truth, pred_shift, islands=None):
+    spec = surrogate.ScanSpec(n_turns=100, n_angle=6, n_amp=40, n_pz=3)
+    n_lat = truth.shape[0]
+    N = spec.n_turns
+    amp_k = np.arange(spec.n_amp)
+
+    def surv_of(v, ip, ia, k):
+        alive = k < truth[v, ip, ia]
+        if islands is not None and (v, ip, ia, k) in islands:
+            alive = False
+        return N if alive else 7
+
+    # predicted survival: alive below truth + shift
+    def fake_predict(net, X):
+        out = np.zeros((n_lat, spec.n_pz, spec.n_angle, spec.n_amp), dtype=np.float32)
+        kb = np.clip(truth + pred_shift, 0, spec.n_amp)
+        out[...] = np.where(amp_k[None, None, None, :] < kb[..., None], 1.0, 0.0)
+        return out.reshape(-1)
+
+    tracked = []
+
+    def fake_track_ics(lat, per, n_turns, tpb=0):
+        off = spec.offsets()
+        res = []
+        for v, c in enumerate(per):
+            s = np.zeros(c.shape[1], dtype=np.int32)
+            for j in range(c.shape[1]):
+                # recover (ip, ia, k) from the coordinates
+                d = np.abs(off[..., 0] - c[0, j]) + np.abs(off[..., 1] - c[2, j]) + np.abs(off[..., 2] - c[5, j])
+                ip, ia, k = np.unravel_index(np.argmin(d), d.shape)
+                s[j] = surv_of(v, ip, ia, k)
+            tracked.append(c.shape[1])
+            res.append(s)
+        return res
+
+    monkeypatch.setattr(surrogate, "predict", fake_predict)
+    monkeypatch.setattr(surrogate, "upload_variants", lambda ctx, k, s: _FakeLat())
+    from bmx import tracker
+    monkeypatch.setattr(tracker, "track_ics", fake_track_ics)
+    runs, area, _, n_tracked = surrogate.surrogate_scan(None, None, np.zeros((n_lat, 32)), spec)
+    return runs, n_tracked, spec
+
+
+@pytest.mark.parametrize("shift", [0, 2, -3, 15, -15])
+def test_band_search_recovers_true_boundary(monkeypatch, shift):
+    rng = np.random.default_rng(abs(shift) + 1)
+    truth = rng.integers(5, 35, size=(4, 3, 6))
+    runs, n_tracked, spec = _run(monkeypatch, truth, shift)
+    assert np.array_equal(runs, truth)
+    full = truth.size * spec.n_amp
+    if abs(shift) <= 2:
+        assert n_tracked < 0.35 * full, (n_tracked, full)
+
+
+def test_boundary_at_grid_edges(monkeypatch):
+    truth = np.zeros((2, 3, 6), dtype=np.int64)
+    truth[0] = 40  # survives the whole grid
+    truth[1] = 0   # lost at the smallest amplitude
+    runs, _, _ = _run(monkeypatch, truth, 0)
+    assert np.array_equal(runs, truth)
+
+
+def test_island_below_band_is_missed_by_design(monkeypatch):
+    """An isolated loss far below the predicted band is not tracked (the price
+    of screening); the DA-area error measured against plain tracking is what
+    bounds it."""
+    truth = np.full((1, 3, 6), 30)
+    runs, _, _ = _run(monkeypatch, truth, 0, islands={(0, 1, 2, 5)})
+    assert runs[0, 1, 2] == 30
+
+
+def test_replicated_knob_lattices_equal_built_ones():
+    """upload_variants' numpy replication gives, byte for byte, the element
+    records knob_lattice builds for each knob set."""
+    import ctypes as C
+    from bmx import abi, lattice
+
+    captured = {}
+
+    class _Ctx:
+        def lattice(self, arr, species, variant_offsets=None):
+            captured["arr"] = bytes(memoryview(arr).cast("B"))
+            captured["offs"] = variant_offsets
+            return _FakeLat()
+
+    spec = surrogate.ScanSpec(n_cells=12)
+    rng = np.random.default_rng(5)
+    knobs = rng.uniform(-0.2, 0.2, (3, surrogate.N_KNOBS))
+    surrogate.upload_variants(_Ctx(), knobs, spec)
+    size = C.sizeof(abi.EleRec)
+    for v in range(3):
+        want = bytes(memoryview(lattice.to_array(surrogate.knob_lattice(knobs[v], spec))).cast("B"))
+        a, b = captured["offs"][v] * size, captured["offs"][v + 1] * size
+        assert captured["arr"][a:b] == want
