@@ -58,6 +58,31 @@ def cpu_baseline(eles, tw, budget_s=15.0):
             "pet_per_s": n * turns * len(eles) / dt}
 
 
+DA_SPEC = dict(n_turns=512, n_angle=64, n_amp=128, n_pz=5, amp_max_x=0.01, amp_ratio_y=0.5, pz_min=-0.002,
+               pz_max=0.002, n_cells=50)
+
+
+def da_scan(ctx, n_train=512, n_test=512, epochs=8):
+    """DA lattices/hour, surrogate-screened vs plain FP64 grid tracking, on
+    unseen sextupole-knob settings of a 50-cell (400-element) ring: polar grid
+    64 angles x 128 amplitudes (0..10 mm, y at half the x amplitude) x 5 pz in
+    [-0.2 %, +0.2 %], 512 turns.  The network (4 x 256 SiLU, BF16 MFMA) is
+    trained first on a plain scan of n_train settings; that one-time cost and
+    the number of scanned lattices that repays it are reported beside the
+    rates."""
+    from bmx import surrogate
+    spec = surrogate.ScanSpec(**DA_SPEC)
+    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs)
+    keep = ("plain_lattices_per_hour", "surrogate_lattices_per_hour", "speedup", "da_area_err_mean",
+            "da_area_err_max", "frac_within_2pct", "rays_identical", "ics_tracked_plain", "ics_tracked_surrogate",
+            "pet_plain", "pet_surrogate", "plain_s", "surrogate_s", "train_scan_s", "train_s", "break_even_lattices",
+            "pred_boundary_within_band", "mlp_forward_tflops", "mlp_train_step_tflops", "n_train", "n_test")
+    out = {k: r[k] for k in keep}
+    out["workload"] = ("50-cell ring (400 elements), 32 sextupole-family knobs U(+-20%), grid 64 x 128 x 5 "
+                       "(40,960 ICs/lattice), 512 turns, FP64; band +-10% of the predicted boundary re-tracked")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +91,7 @@ def main():
     ap.add_argument("--particles", type=int, default=N_PARTICLES)
     ap.add_argument("--turns-per-step", type=int, default=TURNS_PER_STEP)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-da", action="store_true", help="skip the surrogate-vs-plain DA scan comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy (reference for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -144,6 +170,8 @@ def main():
                      "flops_per_pet": flops["flops_per_pet"], "kernel_ms_per_launch": kernel_ms / args.steps},
     }
     lat.close()
+    if not args.no_da:
+        result["da_scan"] = da_scan(ctx)
     ctx.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(eles, tw)

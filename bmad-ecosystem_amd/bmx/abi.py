@@ -91,8 +91,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
     "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
-    "bmx_track_ics", "bmx_closed_orbit",
-    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_ctx_sync",
+    "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit",
+    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_ctx_sync",
 ]
 
 _lib = None
@@ -133,6 +133,8 @@ def load_library(path=None):
     lib.bmx_track_ics.restype = C.c_int32
     lib.bmx_track_ics.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32,
                                   C.c_int32, C.POINTER(C.c_int32), C.POINTER(Stats)]
+    lib.bmx_bunch_moments.restype = C.c_int32
+    lib.bmx_bunch_moments.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(C.c_double)]
     lib.bmx_closed_orbit.restype = C.c_int32
     lib.bmx_closed_orbit.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
                                      C.POINTER(Stats)]
@@ -149,6 +151,9 @@ def load_library(path=None):
     lib.bmx_f32_to_bf16.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
     lib.bmx_mse_grad.restype = C.c_int32
     lib.bmx_mse_grad.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.bmx_da_features.restype = C.c_int32
+    lib.bmx_da_features.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_float,
+                                    C.c_void_p, C.c_void_p]
     lib.bmx_ctx_sync.restype = C.c_int32
     lib.bmx_ctx_sync.argtypes = [C.c_void_p]
     if path is None:

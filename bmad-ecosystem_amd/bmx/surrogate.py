@@ -139,11 +139,17 @@ class MLP:
         """x: FP32 device tensor [n, dims[0]] (torch work on it must be
         complete).  Returns the FP32 device tensor y [n] (valid after sync())."""
         n = x.shape[0]
-        if x.dtype != torch.float32 or not x.is_contiguous() or x.shape[1] != self.dims[0]:
-            raise ValueError("x must be a contiguous float32 [n, n_in] tensor")
+        if x.dtype not in (torch.float32, torch.bfloat16) or not x.is_contiguous() or x.shape[1] != self.dims[0]:
+            raise ValueError("x must be a contiguous float32 or bfloat16 [n, n_in] tensor")
         b = self._buffers(n)
-        self._check(self.lib.bmx_f32_to_bf16(self.ctx.h, n * self.dims[0], _p(x), _p(b["xb"])), "bmx_f32_to_bf16")
-        h = b["xb"]
+        if x.dtype == torch.bfloat16:
+            h = x
+            b["xb_ext"] = x
+        else:
+            self._check(self.lib.bmx_f32_to_bf16(self.ctx.h, n * self.dims[0], _p(x), _p(b["xb"])),
+                        "bmx_f32_to_bf16")
+            h = b["xb"]
+            b["xb_ext"] = b["xb"]
         L = len(self.layout) - 1
         for l in range(L):
             wo, bo, n_in, n_out = self.layout[l]
@@ -164,7 +170,7 @@ class MLP:
                                           _p(self.loss)), "bmx_mse_grad")
         # head: dW = d^T h_L, db = sum d, dh_L = d W_L -> dpre_{L-1}
         wo, bo, n_in, n_out = self.layout[L]
-        h_in = b["h"][L - 1] if L > 0 else b["xb"]
+        h_in = b["h"][L - 1] if L > 0 else b["xb_ext"]
         self._gemm(1, 0, 1, n_in, n, _p(b["db"]), 1, _p(h_in), n_in, self._gwptr(L), None, n_in, None, 0, None)
         self._check(self.lib.bmx_colsum_f32(self.ctx.h, n, 1, _p(b["d"]), 1, self._bptr(L, self.g)), "bmx_colsum_f32")
         if L == 0:
@@ -174,7 +180,7 @@ class MLP:
                    n_in, None, 3, _p(b["pre"][L - 1]))
         for l in range(L - 1, -1, -1):
             wo, bo, n_in, n_out = self.layout[l]
-            h_in = b["h"][l - 1] if l > 0 else b["xb"]
+            h_in = b["h"][l - 1] if l > 0 else b["xb_ext"]
             dp, dpb = b["dpre"][cur], b["dpre_b"][cur]
             self._gemm(1, 0, n_out, n_in, n, _p(dpb), n_out, _p(h_in), n_in, self._gwptr(l), None, n_in, None, 0,
                        None)
@@ -293,11 +299,10 @@ def upload_variants(ctx, knob_sets, spec):
 
 
 def features(knob_sets, spec):
-    """Network inputs [n_lat * n_ic, 35]: knobs / 0.2, x0 / amp_max, y0 / (amp_max ratio_y), pz / max|pz|."""
+    """Network inputs [n_lat * n_ic, 35] (float32, host): knobs / 0.2 then the
+    grid inputs of geometry(); the same rows bmx_da_features forms on the device."""
     knob_sets = np.asarray(knob_sets, dtype=np.float64)
-    off = spec.offsets().reshape(-1, 3)
-    pzs = max(abs(spec.pz_min), abs(spec.pz_max), 1e-30)
-    geo = np.stack([off[:, 0] / spec.amp_max_x, off[:, 1] / (spec.amp_max_x * spec.amp_ratio_y), off[:, 2] / pzs], 1)
+    geo = geometry(spec)
     n_lat = len(knob_sets)
     X = np.empty((n_lat, spec.n_ic, N_KNOBS + 3), dtype=np.float32)
     X[:, :, :N_KNOBS] = (knob_sets / KNOB_RANGE)[:, None, :]
@@ -305,9 +310,22 @@ def features(knob_sets, spec):
     return X.reshape(-1, N_KNOBS + 3)
 
 
-def target(survival, n_turns):
-    """log survival turns, scaled to [0, 1]."""
-    return (np.log1p(np.asarray(survival, dtype=np.float64)) / np.log1p(n_turns)).astype(np.float32)
+TARGET_GAMMA = 4.0
+
+
+def target(survival, n_turns, gamma=TARGET_GAMMA):
+    """Network target: log survival turns scaled to [0, 1] and raised to the
+    power gamma, y = (log(1 + s) / log(1 + N))^gamma -- a monotone map of log
+    survival that spreads the top of the range (full survival at 1, a
+    particle lost after N/5 turns at 0.74^gamma), so the fit concentrates on
+    the survive-every-turn boundary the scan needs."""
+    y = np.log1p(np.asarray(survival, dtype=np.float64)) / np.log1p(n_turns)
+    return (y ** gamma).astype(np.float32)
+
+
+def survival_of(y, n_turns, gamma=TARGET_GAMMA):
+    """Inverse of target(): predicted survival turns."""
+    return np.expm1(np.clip(y, 0, None) ** (1.0 / gamma) * np.log1p(n_turns))
 
 
 def plain_scan(ctx, knob_sets, spec, turns_per_block=0):
@@ -319,12 +337,13 @@ def plain_scan(ctx, knob_sets, spec, turns_per_block=0):
     t0 = time.perf_counter()
     lat = upload_variants(ctx, knob_sets, spec)
     t1 = time.perf_counter()
-    surv, _, _ = tracker.track_grid(lat, spec.n_turns, spec.n_angle, spec.n_amp, spec.n_pz, spec.amp_max_x,
-                                    spec.amp_ratio_y, spec.pz_min, spec.pz_max, turns_per_block=turns_per_block)
+    surv, _, _, st = tracker.track_grid(lat, spec.n_turns, spec.n_angle, spec.n_amp, spec.n_pz, spec.amp_max_x,
+                                        spec.amp_ratio_y, spec.pz_min, spec.pz_max, turns_per_block=turns_per_block,
+                                        return_stats=True)
     t2 = time.perf_counter()
     lat.close()
     area = np.array([slice_area(s[spec.n_pz // 2], spec) for s in surv])
-    return surv, area, dict(total=t2 - t0, upload=t1 - t0, track=t2 - t1)
+    return surv, area, dict(total=t2 - t0, upload=t1 - t0, track=t2 - t1, pet=st["pet"])
 
 
 def slice_area(surv_slice, spec):
@@ -386,74 +405,92 @@ def predict(net, X, batch=1 << 16):
     return out
 
 
+def step_fit_boundary(pred, alive_level):
+    """Boundary index of every ray from its predicted profile pred[..., n_amp]:
+    the k minimising sum_{j<k} (p_j - alive_level)^2 + sum_{j>=k} (p_j - mean_{j>=k} p)^2,
+    the least-squares fit of a step from the surviving level to a free lower
+    level (uses the whole profile, so one mispredicted point does not move it).
+    Returns k in 0..n_amp (n_amp: the whole ray survives)."""
+    p = np.asarray(pred, dtype=np.float64)
+    K = p.shape[-1]
+    pre = np.concatenate([np.zeros(p.shape[:-1] + (1,)), np.cumsum((p - alive_level) ** 2, -1)], -1)  # k = 0..K
+    rs = np.cumsum(p[..., ::-1], -1)[..., ::-1]
+    rq = np.cumsum((p * p)[..., ::-1], -1)[..., ::-1]
+    zero = np.zeros(p.shape[:-1] + (1,))
+    suf_s = np.concatenate([rs, zero], -1)
+    suf_q = np.concatenate([rq, zero], -1)
+    cnt = (K - np.arange(K + 1)).astype(np.float64)
+    suf = suf_q - np.where(cnt > 0, suf_s * suf_s / np.maximum(cnt, 1), 0.0)
+    cost = pre + suf
+    return K - np.argmin(cost[..., ::-1], -1)  # ties (a flat profile) resolve to the larger k
+
+
+def geometry(spec):
+    """[n_ic, 3] grid inputs: x0 / amp_max, y0 / (amp_max ratio_y), pz / max|pz|."""
+    off = spec.offsets().reshape(-1, 3)
+    pzs = max(abs(spec.pz_min), abs(spec.pz_max), 1e-30)
+    return np.stack([off[:, 0] / spec.amp_max_x, off[:, 1] / (spec.amp_max_x * spec.amp_ratio_y), off[:, 2] / pzs], 1)
+
+
 def predict_knobs(net, knob_sets, spec, lat_chunk=None):
-    """Network output for every grid IC of every knob set, building the inputs
-    one chunk of lattices at a time: [n_lat, n_pz, n_angle, n_amp] float32."""
+    """Network output for every grid IC of every knob set
+    [n_lat, n_pz, n_angle, n_amp] float32.  The inputs are formed on the
+    device (bmx_da_features), one chunk of lattices at a time."""
     knob_sets = np.asarray(knob_sets)
     n_lat = len(knob_sets)
     if lat_chunk is None:
         lat_chunk = max(1, (1 << 21) // spec.n_ic)
-    out = np.empty((n_lat, spec.n_ic), dtype=np.float32)
+    dev = net.dev
+    geo = torch.from_numpy(geometry(spec).astype(np.float32)).to(dev)
+    kd = torch.from_numpy(knob_sets.astype(np.float32)).to(dev)
+    width = N_KNOBS + 3
+    out = torch.empty(n_lat * spec.n_ic, dtype=torch.float32, device=dev)
+    X = torch.empty(min(lat_chunk, n_lat) * spec.n_ic, width, dtype=torch.bfloat16, device=dev)
+    torch.cuda.synchronize(dev)
     for a in range(0, n_lat, lat_chunk):
         b = min(n_lat, a + lat_chunk)
-        out[a:b] = predict(net, features(knob_sets[a:b], spec)).reshape(b - a, spec.n_ic)
-    return out.reshape(n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
+        rows = (b - a) * spec.n_ic
+        xb = X[:rows]
+        rc = net.lib.bmx_da_features(net.ctx.h, b - a, spec.n_ic, N_KNOBS, 3,
+                                     C.c_void_p(kd.data_ptr() + a * N_KNOBS * 4), KNOB_RANGE, _p(geo), _p(xb))
+        net._check(rc, "bmx_da_features")
+        y = net.forward(xb)
+        net.sync()
+        out[a * spec.n_ic:b * spec.n_ic].copy_(y)
+        torch.cuda.synchronize(dev)
+    return out.cpu().numpy().reshape(n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
 
 
-def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, alive_frac=0.5, turns_per_block=0, max_rounds=64, refine=6):
-    """Surrogate-screened DA scan.  Every ray (lattice, pz, angle) gets a
-    predicted boundary index from the network; the initial conditions whose
-    amplitude lies within +-band of the predicted boundary amplitude are
-    re-tracked in FP64 (bmx_track_ics, one launch for all rays).  Where the
-    re-tracked band does not pin the first loss -- untracked amplitudes remain
-    between the highest tracked survivor and the first tracked loss (or the
-    grid edge) -- the next round tracks up to `refine` evenly spaced points of
-    that gap, until every ray is resolved: plain FP64 tracking is the fallback
-    and the judge of every boundary.  Returns (runs [n_lat, n_pz, n_angle], area [n_lat],
-    timing dict (total / predict / upload / track seconds), n_tracked)."""
+def _resolve_rays(ctx, knob_sets, spec, lo, hi, refine, turns_per_block, max_rounds):
+    """FP64 rounds of surrogate_scan for one chunk of lattices on context ctx:
+    the band [lo, hi] first, then gap refinement until every ray is resolved.
+    Returns (known survival [n_lat, n_pz, n_angle, n_amp] (-1 = not tracked),
+    rounds [(n_ics, seconds, pet)])."""
     import time
     from . import tracker
-    t0 = time.perf_counter()
-    knob_sets = np.asarray(knob_sets)
     n_lat = len(knob_sets)
     N = spec.n_turns
-    pred = predict_knobs(net, knob_sets, spec)
-    thr = np.log1p(alive_frac * N) / np.log1p(N)
-    alive_p = pred >= thr
-    kb = np.cumprod(alive_p, axis=-1).sum(-1)  # predicted surviving-prefix length
-    amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
-    # band in index space: amplitudes within +-band of the boundary amplitude
-    a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
-    lo = np.searchsorted(amps, a_b * (1 - band), side="left")
-    hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
-    lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
-    hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
     shape = (n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
-    known = np.full(shape, -1, dtype=np.int64)  # tracked survival turns, -1 = not tracked
-    t_pred = time.perf_counter()
+    known = np.full(shape, -1, dtype=np.int64)
     lat = upload_variants(ctx, knob_sets, spec)
-    t_up = time.perf_counter()
     off = spec.offsets()
     kk = np.arange(spec.n_amp)
-    n_tracked = 0
     rounds = []
     next_mask = (kk >= lo[..., None]) & (kk <= hi[..., None])
     for _ in range(max_rounds):
         idx = np.argwhere(next_mask & (known < 0))  # sorted by lattice
         if len(idx) == 0:
             break
-        n_tracked += len(idx)
         o = off[idx[:, 1], idx[:, 2], idx[:, 3]]
         counts = np.bincount(idx[:, 0], minlength=n_lat)
-        splits = np.cumsum(counts)[:-1]
         per = []
-        for part in np.split(o, splits):
+        for part in np.split(o, np.cumsum(counts)[:-1]):
             c = np.zeros((6, len(part)))
             c[0], c[2], c[5] = part[:, 0], part[:, 1], part[:, 2]
             per.append(c)
         tr0 = time.perf_counter()
-        surv = tracker.track_ics(lat, per, N, turns_per_block)
-        rounds.append((len(idx), time.perf_counter() - tr0))
+        surv, st = tracker.track_ics(lat, per, N, turns_per_block, return_stats=True)
+        rounds.append((len(idx), time.perf_counter() - tr0, st["pet"]))
         known[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]] = np.concatenate(surv)
         # resolution per ray: b = first tracked loss (n_amp if none), a = the
         # highest tracked survivor below it (-1 if none).  All amplitudes in
@@ -465,6 +502,9 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, alive_frac=0.5, turns_p
         bnd = np.where(has_lost, np.argmax(lostm, -1), spec.n_amp)
         alive_below = tracked & (known >= N) & (kk < bnd[..., None])
         a_top = np.where(alive_below.any(-1), spec.n_amp - 1 - np.argmax(alive_below[..., ::-1], -1), -1)
+        if refine <= 0:  # the whole gap in one round
+            next_mask = (kk > a_top[..., None]) & (kk < bnd[..., None])
+            continue
         gap = bnd - a_top - 1
         j = np.arange(1, refine + 1)
         pts = a_top[..., None] + np.ceil((gap[..., None] + 1) * j / (refine + 1)).astype(np.int64)
@@ -474,22 +514,117 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, alive_frac=0.5, turns_p
         sel[li, pi, ai, pts[li, pi, ai, ji]] = True
         next_mask = sel
     lat.close()
+    return known, rounds
+
+
+def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_rounds=64, refine=6, workers=1):
+    """Surrogate-screened DA scan.  Every ray (lattice, pz, angle) gets a
+    predicted boundary index from the network (step_fit_boundary of its
+    predicted profile); the initial conditions whose amplitude lies within
+    +-band of the predicted boundary amplitude are re-tracked in FP64
+    (bmx_track_ics, one launch for all rays of a chunk).  Where the re-tracked
+    band does not pin the first loss -- untracked amplitudes remain between
+    the highest tracked survivor and the first tracked loss (or the grid
+    edge) -- the next round tracks up to `refine` evenly spaced points of the
+    gap (refine = 0: the whole gap in one round), until every ray is resolved: plain FP64 tracking is the fallback and the judge of
+    every boundary.  workers > 1 splits the lattices into that
+    many chunks resolved concurrently, each on its own context (HIP stream),
+    so the latency-bound late rounds of one chunk overlap the others' work.
+    Returns (runs [n_lat, n_pz, n_angle], area [n_lat], timing dict
+    (total / predict / track seconds, rounds), n_tracked)."""
+    import threading
+    import time
+    from . import tracker
+    t0 = time.perf_counter()
+    knob_sets = np.asarray(knob_sets)
+    n_lat = len(knob_sets)
+    N = spec.n_turns
+    pred = predict_knobs(net, knob_sets, spec)
+    kb = step_fit_boundary(pred, 1.0)
+    amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
+    # band in index space: amplitudes within +-band of the boundary amplitude
+    a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
+    lo = np.searchsorted(amps, a_b * (1 - band), side="left")
+    hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
+    lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
+    hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
+    t_pred = time.perf_counter()
+    workers = max(1, min(workers, n_lat))
+    bounds = np.linspace(0, n_lat, workers + 1).astype(int)
+    results = [None] * workers
+    errors = []
+
+    def work(w, c):
+        try:
+            a, b = bounds[w], bounds[w + 1]
+            results[w] = _resolve_rays(c, knob_sets[a:b], spec, lo[a:b], hi[a:b], refine, turns_per_block, max_rounds)
+        except Exception as e:  # re-raised in the caller
+            errors.append(e)
+
+    if workers == 1:
+        work(0, ctx)
+    else:
+        ctxs = [tracker.Context(ctx.device) for _ in range(workers)]
+        th = [threading.Thread(target=work, args=(w, ctxs[w])) for w in range(workers)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for c in ctxs:
+            c.close()
+    if errors:
+        raise errors[0]
+    known = np.concatenate([r[0] for r in results], 0)
+    rounds = [x for r in results for x in r[1]]
+    n_tracked = int((known >= 0).sum())
     # surviving prefix: untracked amplitudes below the first tracked loss count as alive
     lostm = (known >= 0) & (known < N)
     runs = np.where(lostm.any(-1), np.argmax(lostm, -1), spec.n_amp)
     area = np.array([area_from_runs(runs[v, spec.n_pz // 2], spec) for v in range(n_lat)])
     t1 = time.perf_counter()
-    timing = dict(total=t1 - t0, predict=t_pred - t0, upload=t_up - t_pred, track=t1 - t_up, rounds=rounds)
+    timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb)
     return runs, area, timing, n_tracked
 
 
-def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed=20261205, band=0.10):
+def inference_throughput(net, n=1 << 18, reps=10):
+    """Dense MFMA throughput of the network (algorithmic BF16 flops of the
+    GEMMs / wall time between stream syncs) for forward passes and training
+    steps at batch n."""
+    import time
+    dims = net.dims
+    flop = 2 * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+    x = torch.rand(n, dims[0], device=net.dev, dtype=torch.float32)
+    t = torch.rand(n, device=net.dev, dtype=torch.float32)
+    torch.cuda.synchronize(net.dev)
+    net.forward(x)
+    net.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(x)
+    net.sync()
+    fwd = (time.perf_counter() - t0) / reps
+    saved = (net.w.clone(), net.wb.clone(), net.m.clone(), net.v.clone(), net.step_count)
+    net.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(x, t, 0.0)
+    net.sync()
+    trn = (time.perf_counter() - t0) / reps
+    net.w.copy_(saved[0]); net.wb.copy_(saved[1]); net.m.copy_(saved[2]); net.v.copy_(saved[3])
+    net.step_count = saved[4]
+    torch.cuda.synchronize(net.dev)
+    return {"forward_tflops": n * flop / fwd / 1e12, "train_tflops": 3 * n * flop / trn / 1e12,
+            "forward_ms": fwd * 1e3, "train_step_ms": trn * 1e3, "batch": n}
+
+
+def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed=20261205, band=0.10,
+                  gamma=TARGET_GAMMA, workers=1, refine=6):
     """Surrogate-screened vs plain DA scans of n_test unseen knob settings:
     trains a fresh 4 x 256 network on a plain scan of n_train settings, then
     scans the test settings both ways.  Returns a dict of timings,
-    lattices/hour (tracking + screening; lattice construction and upload are
-    excluded from both), FP64-tracked IC counts and the DA-area error
-    statistics."""
+    lattices/hour (lattice upload + tracking for plain; upload + inference +
+    band search + FP64 re-tracking for the surrogate), FP64-tracked IC counts
+    and the DA-area error statistics."""
     import time
     rng = np.random.default_rng(seed)
     k_train = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_train, N_KNOBS))
@@ -498,7 +633,7 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
     t0 = time.perf_counter()
     net = MLP(ctx, [N_KNOBS + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
     X = features(k_train, spec)
-    Y = target(s_tr.reshape(-1), spec.n_turns)
+    Y = target(s_tr.reshape(-1), spec.n_turns, gamma)
     del s_tr
     losses = train(net, X, Y, epochs=epochs, batch=batch)
     del X, Y
@@ -506,11 +641,12 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
     s_te, a_plain, t_plain = plain_scan(ctx, k_test, spec)
     plain_runs = np.cumprod(s_te >= spec.n_turns, axis=-1).sum(-1)
     del s_te
-    runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band)
+    runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band, workers=workers, refine=refine)
     err = np.abs(a_sur - a_plain) / np.maximum(a_plain, 1e-300)
-    plain_s = t_plain["track"]
-    sur_s = t_sur["predict"] + t_sur["track"]
+    plain_s = t_plain["total"]
+    sur_s = t_sur["total"]
     per_lat_gain = plain_s / n_test - sur_s / n_test
+    inf = inference_throughput(net)
     return {
         "n_train": n_train, "n_test": n_test, "ics_per_lattice": spec.n_ic, "train_samples": n_train * spec.n_ic,
         "train_scan_s": t_gen["track"], "train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
@@ -519,8 +655,12 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
         "speedup": plain_s / sur_s,
         "ics_tracked_plain": n_test * spec.n_ic, "ics_tracked_surrogate": int(n_tracked),
         "rays_identical": float(np.mean(runs == plain_runs)),
+        "pred_boundary_err_mean": float(np.mean(np.abs(t_sur["kb"] - plain_runs))),
+        "pred_boundary_within_band": float(np.mean(np.abs(t_sur["kb"] - plain_runs) <= np.maximum(1, band * plain_runs))),
         "da_area_err_mean": float(err.mean()), "da_area_err_max": float(err.max()),
         "frac_within_2pct": float(np.mean(err <= 0.02)),
-        "rounds": [[int(n), round(t, 4)] for n, t in t_sur["rounds"]], "predict_s": t_sur["predict"],
-        "break_even_lattices": (t_gen["track"] + t_train) / per_lat_gain if per_lat_gain > 0 else None,
+        "rounds": [[int(n), round(t, 4), float(p)] for n, t, p in t_sur["rounds"]], "predict_s": t_sur["predict"],
+        "pet_plain": t_plain["pet"], "pet_surrogate": float(sum(p for _, _, p in t_sur["rounds"])),
+        "break_even_lattices": (t_gen["total"] + t_train) / per_lat_gain if per_lat_gain > 0 else None,
+        "mlp_forward_tflops": inf["forward_tflops"], "mlp_train_step_tflops": inf["train_tflops"],
     }

@@ -150,6 +150,24 @@ def track_bunch(lat, bunch, n_turns=1, ix_start=0, ix_end=0, direction=1, sync=T
     return dict(n_lost=st.n_lost, kernel_ms=st.kernel_ms)
 
 
+def bunch_moments(ctx, bunch):
+    """(n_live, sums[6], second moments[6, 6]) over the live particles of a
+    device Bunch (bmx_bunch_moments)."""
+    torch.cuda.synchronize(bunch.device)
+    p = bunch.particles_struct()
+    m = (C.c_double * 28)()
+    rc = ctx.lib.bmx_bunch_moments(ctx.h, C.byref(p), m)
+    _check(ctx.h, rc, "bmx_bunch_moments")
+    a = np.array(m[:])
+    sq = np.zeros((6, 6))
+    k = 7
+    for r in range(6):
+        for c in range(r, 6):
+            sq[r, c] = sq[c, r] = a[k]
+            k += 1
+    return int(round(a[0])), a[1:7], sq
+
+
 def track_bunch_host(lat, coords, n_turns=1, beta=None):
     """Same as track_bunch on host numpy arrays (the library stages them through HBM)."""
     c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)

@@ -11,7 +11,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
+#include <utility>
 
 #include "../../include/bmx.h"
 
@@ -30,18 +33,67 @@ __device__ __forceinline__ float silu_grad(float x) {
   return s * (1.0f + x * (1.0f - s));
 }
 
+// Epilogue on the FP32 accumulator v of C(m, n):
+//   0: out = v
+//   1: out = v + bias[n]
+//   2: pre = v + bias[n] -> C (fp32, if given); C_bf16 = silu(pre)
+//   3: out = v * silu'(aux[m, n])   (aux = pre-activation, fp32, pitch ldc)
+__device__ __forceinline__ void epilogue_store(float v, int64_t o, int n, int epi, float* __restrict__ C,
+                                               __bf16* __restrict__ Cb, const float* __restrict__ bias,
+                                               const float* __restrict__ aux) {
+  if (epi == 1 || epi == 2) v += bias[n];
+  if (epi == 3) v *= silu_grad(aux[o]);
+  if (C) C[o] = v;
+  if (Cb) Cb[o] = (__bf16)((epi == 2) ? silu(v) : v);
+}
+
+typedef __attribute__((__vector_size__(8 * sizeof(unsigned short)))) unsigned short u16x8_t;
+
+// Stage a 64 x 32 operand tile into LDS as T[r][k] (r = row of the output
+// tile side, k = reduction index), from a row-major matrix whose element
+// (r, k) lives at P[k * ld + r] when KMAJOR (r contiguous) or P[r * ld + k]
+// (k contiguous).  VEC: 16-byte loads (the caller checked alignment and that
+// the contiguous extent is a multiple of 8).
+template <bool KMAJOR, bool VEC>
+__device__ __forceinline__ void stage_tile(__bf16* __restrict__ T, const __bf16* __restrict__ P, int ld, int r0,
+                                           int R, int k0, int K, int tid) {
+  if (VEC) {
+    if (!KMAJOR) {  // k contiguous: thread -> (r = tid / 4, 8 k's)
+      int r = tid >> 2, kk = (tid & 3) * 8;
+      int gr = r0 + r, gk = k0 + kk;
+      u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (gr < R && gk < K) v = *reinterpret_cast<const u16x8_t*>(P + (int64_t)gr * ld + gk);
+      *reinterpret_cast<u16x8_t*>(T + r * LDA_S + kk) = v;
+    } else {  // r contiguous: thread -> (k = tid / 8, 8 r's)
+      int kk = tid >> 3, r = (tid & 7) * 8;
+      int gr = r0 + r, gk = k0 + kk;
+      u16x8_t v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (gr < R && gk < K) v = *reinterpret_cast<const u16x8_t*>(P + (int64_t)gk * ld + gr);
+      unsigned short* t = reinterpret_cast<unsigned short*>(T);
+      for (int j = 0; j < 8; j++) t[(r + j) * LDA_S + kk] = v[j];
+    }
+  } else {
+    for (int e = tid; e < BM * BK; e += 256) {
+      int r = KMAJOR ? e % BM : e / BK, kk = KMAJOR ? e / BM : e % BK;
+      int gr = r0 + r, gk = k0 + kk;
+      __bf16 v = (__bf16)0.0f;
+      if (gr < R && gk < K) v = KMAJOR ? P[(int64_t)gk * ld + gr] : P[(int64_t)gr * ld + gk];
+      T[r * LDA_S + kk] = v;
+    }
+  }
+}
+
 // C[M x N] = op(A)[M x K] op(B)[K x N] with op = transpose when TA / TB.
-// A element (m, k) is A[k * lda + m] when TA else A[m * lda + k]; same for B.
-// Epilogue on acc (+ beta_acc * C_old when C_old != nullptr):
-//   0: out = acc
-//   1: out = acc + bias[n]
-//   2: pre = acc + bias[n] -> C (fp32, if given); C_bf16 = silu(pre)
-//   3: out = acc * silu'(aux[m, n])   (aux = pre-activation, fp32, pitch ldc)
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const __bf16* __restrict__ A, int lda,
-                                                       const __bf16* __restrict__ B, int ldb, float* __restrict__ C,
-                                                       __bf16* __restrict__ Cb, int ldc, const float* __restrict__ bias,
-                                                       int epi, const float* __restrict__ aux) {
+// A element (m, k) is A[k * lda + m] when TA else A[m * lda + k]; B element
+// (k, n) is B[n * ldb + k] when TB else B[k * ldb + n].  blockIdx.z = K
+// split: with ksplit > 1 each split writes its raw partial tile to
+// W[z][m][n] and splitk_reduce_kernel applies the epilogue.
+template <bool TA, bool TB, bool VA, bool VB>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, int k_chunk, const __bf16* __restrict__ A,
+                                                       int lda, const __bf16* __restrict__ B, int ldb,
+                                                       float* __restrict__ C, __bf16* __restrict__ Cb, int ldc,
+                                                       const float* __restrict__ bias, int epi,
+                                                       const float* __restrict__ aux, float* __restrict__ W) {
   __shared__ __bf16 As[BM * LDA_S];
   __shared__ __bf16 Bs[BN * LDB_S];
   const int tid = threadIdx.x;
@@ -49,38 +101,21 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
   const int wave = tid >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // wave quadrant
+  const int kb = blockIdx.z * k_chunk, ke = min(K, kb + k_chunk);
   f32x4_t acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    // stage the A tile (BM x BK) as As[m][k] and the B tile (BK x BN) as
-    // Bs[n][k]; consecutive threads walk the operand's contiguous index.
-    for (int e = tid; e < BM * BK; e += 256) {
-      int m = TA ? e % BM : e / BK, k = TA ? e / BM : e % BK;
-      int gm = m0 + m, gk = k0 + k;
-      __bf16 v = (__bf16)0.0f;
-      if (gm < M && gk < K) v = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
-      As[m * LDA_S + k] = v;
-    }
-    for (int e = tid; e < BN * BK; e += 256) {
-      int n = TB ? e / BK : e % BN, k = TB ? e % BK : e / BN;
-      int gn = n0 + n, gk = k0 + k;
-      __bf16 v = (__bf16)0.0f;
-      if (gn < N && gk < K) v = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
-      Bs[n * LDB_S + k] = v;
-    }
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    stage_tile<TA, VA>(As, A, lda, m0, M, k0, ke, tid);
+    stage_tile<!TB, VB>(Bs, B, ldb, n0, N, k0, ke, tid);
     __syncthreads();
     // lane -> (row / column r = lane % 16, k group q = lane / 16 holding k = 8q .. 8q+7)
     const int r = lane & 15, q = lane >> 4;
     bf16x8_t af[2], bfr[2];
     for (int i = 0; i < 2; i++) {
-      const __bf16* pa = &As[(wm + 16 * i + r) * LDA_S + 8 * q];
-      const __bf16* pb = &Bs[(wn + 16 * i + r) * LDB_S + 8 * q];
-      for (int t = 0; t < 8; t++) {
-        af[i][t] = pa[t];
-        bfr[i][t] = pb[t];
-      }
+      af[i] = *reinterpret_cast<const bf16x8_t*>(&As[(wm + 16 * i + r) * LDA_S + 8 * q]);
+      bfr[i] = *reinterpret_cast<const bf16x8_t*>(&Bs[(wn + 16 * i + r) * LDB_S + 8 * q]);
     }
     for (int i = 0; i < 2; i++)
       for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
@@ -95,17 +130,23 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
         int n = n0 + wn + 16 * j + cn;
         if (m >= M || n >= N) continue;
         float v = acc[i][j][t];
-        int64_t o = (int64_t)m * ldc + n;
-        if (epi == 1 || epi == 2) v += bias[n];
-        if (epi == 3) v *= silu_grad(aux[o]);
-        if (epi == 2) {
-          if (C) C[o] = v;
-          if (Cb) Cb[o] = (__bf16)silu(v);
-        } else {
-          if (C) C[o] = v;
-          if (Cb) Cb[o] = (__bf16)v;
-        }
+        if (W) W[((int64_t)blockIdx.z * M + m) * N + n] = v;
+        else epilogue_store(v, (int64_t)m * ldc + n, n, epi, C, Cb, bias, aux);
       }
+}
+
+// Sum of the K-split partial tiles (in split order, so the result does not
+// depend on scheduling) followed by the epilogue.
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int ksplit, const float* __restrict__ W,
+                                                           float* __restrict__ C, __bf16* __restrict__ Cb, int ldc,
+                                                           const float* __restrict__ bias, int epi,
+                                                           const float* __restrict__ aux) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)M * N) return;
+  int m = (int)(i / N), n = (int)(i % N);
+  float v = 0.f;
+  for (int z = 0; z < ksplit; z++) v += W[(int64_t)z * M * N + i];
+  epilogue_store(v, (int64_t)m * ldc + n, n, epi, C, Cb, bias, aux);
 }
 
 // column sums of a row-major fp32 matrix: out[n] = sum_m X[m * ld + n]
@@ -170,11 +211,67 @@ __global__ __launch_bounds__(1024) void mse_kernel(int n, const float* __restric
   if (threadIdx.x == 0) *loss = red[0] * inv;
 }
 
+// Surrogate inputs of a DA scan, straight to BF16: row l * n_ic + i holds the
+// knob vector of lattice l scaled by 1 / knob_range, then the n_geo grid
+// coordinates of IC i.
+__global__ __launch_bounds__(256) void da_features_kernel(int64_t n_rows, int n_ic, int n_knob, int n_geo,
+                                                         const float* __restrict__ knobs, float inv_range,
+                                                         const float* __restrict__ geo, __bf16* __restrict__ X) {
+  int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (row >= n_rows) return;
+  int64_t l = row / n_ic;
+  int i = (int)(row % n_ic);
+  int w = n_knob + n_geo;
+  __bf16* x = X + row * w;
+  for (int k = 0; k < n_knob; k++) x[k] = (__bf16)(knobs[l * n_knob + k] * inv_range);
+  for (int g = 0; g < n_geo; g++) x[n_knob + g] = (__bf16)geo[(int64_t)i * n_geo + g];
+}
+
 }  // namespace
 
 // defined in bmx_track.hip (hidden visibility)
 extern "C" hipStream_t bmx_ctx_stream(const bmx_ctx* ctx);
 extern "C" int32_t bmx_ctx_set_error(bmx_ctx* ctx, const char* msg);
+
+// per-context split-K workspace, grown on demand, released by bmx_ctx_destroy
+static std::mutex g_ws_mu;
+static std::unordered_map<const bmx_ctx*, std::pair<float*, size_t>> g_ws;
+
+static float* splitk_workspace(const bmx_ctx* ctx, size_t floats) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto& e = g_ws[ctx];
+  if (e.second < floats) {
+    if (e.first) (void)hipFree(e.first);
+    e.first = nullptr;
+    e.second = 0;
+    if (hipMalloc(&e.first, floats * sizeof(float)) != hipSuccess) return nullptr;
+    e.second = floats;
+  }
+  return e.first;
+}
+
+template <bool TA, bool TB>
+static void launch_gemm(dim3 grid, hipStream_t s, bool va, bool vb, int M, int N, int K, int k_chunk, const __bf16* a,
+                        int lda, const __bf16* b, int ldb, float* C, __bf16* cb, int ldc, const float* bias, int epi,
+                        const float* aux, float* W) {
+#define BMX_GEMM_LAUNCH(VA_, VB_)                                                                                \
+  hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, VA_, VB_>), grid, dim3(256), 0, s, M, N, K, k_chunk, a, lda, b, ldb, \
+                     C, cb, ldc, bias, epi, aux, W)
+  if (va && vb) BMX_GEMM_LAUNCH(true, true);
+  else if (va) BMX_GEMM_LAUNCH(true, false);
+  else if (vb) BMX_GEMM_LAUNCH(false, true);
+  else BMX_GEMM_LAUNCH(false, false);
+#undef BMX_GEMM_LAUNCH
+}
+
+// Called by bmx_ctx_destroy (hidden: not part of the exported ABI).
+extern "C" __attribute__((visibility("hidden"))) void bmx_mlp_release(const bmx_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto it = g_ws.find(ctx);
+  if (it == g_ws.end()) return;
+  if (it->second.first) (void)hipFree(it->second.first);
+  g_ws.erase(it);
+}
 
 extern "C" {
 
@@ -186,23 +283,40 @@ int32_t bmx_gemm_bf16(bmx_ctx* ctx, int32_t trans_a, int32_t trans_b, int32_t M,
   if ((epilogue == 1 || epilogue == 2) && !bias) return bmx_ctx_set_error(ctx, "bmx_gemm_bf16: bias required");
   if (epilogue == 3 && !aux) return bmx_ctx_set_error(ctx, "bmx_gemm_bf16: aux required");
   if (epilogue < 0 || epilogue > 3) return bmx_ctx_set_error(ctx, "bmx_gemm_bf16: bad epilogue");
-  dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
   hipStream_t s = bmx_ctx_stream(ctx);
   const __bf16* a = (const __bf16*)A;
   const __bf16* b = (const __bf16*)B;
   __bf16* cb = (__bf16*)C_bf16;
+  // 16-byte staging needs the contiguous extent and leading dimension in
+  // multiples of 8 elements and 16-byte aligned base pointers
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  bool va = al16(A) && (lda % 8 == 0) && ((trans_a ? M : K) % 8 == 0);
+  bool vb = al16(B) && (ldb % 8 == 0) && ((trans_b ? K : N) % 8 == 0);
+  int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  // split K when the output has too few tiles to fill the chip
+  int ksplit = 1;
+  if (tiles < 512 && K >= 2048) ksplit = std::min(std::min(K / 512, (1024 + tiles - 1) / tiles), 64);
+  int k_chunk = ((K + ksplit - 1) / ksplit + BK - 1) / BK * BK;
+  ksplit = (K + k_chunk - 1) / k_chunk;
+  float* W = nullptr;
+  if (ksplit > 1) {
+    W = splitk_workspace(ctx, (size_t)ksplit * M * N);
+    if (!W) return bmx_ctx_set_error(ctx, "bmx_gemm_bf16: split-K workspace allocation failed");
+  }
+  dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM), (unsigned)ksplit);
   if (!trans_a && !trans_b)
-    hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, cb, ldc,
-                       bias, epilogue, aux);
+    launch_gemm<false, false>(grid, s, va, vb, M, N, K, k_chunk, a, lda, b, ldb, C, cb, ldc, bias, epilogue, aux, W);
   else if (!trans_a && trans_b)
-    hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, cb, ldc,
-                       bias, epilogue, aux);
+    launch_gemm<false, true>(grid, s, va, vb, M, N, K, k_chunk, a, lda, b, ldb, C, cb, ldc, bias, epilogue, aux, W);
   else if (trans_a && !trans_b)
-    hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, cb, ldc,
-                       bias, epilogue, aux);
+    launch_gemm<true, false>(grid, s, va, vb, M, N, K, k_chunk, a, lda, b, ldb, C, cb, ldc, bias, epilogue, aux, W);
   else
-    hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, cb, ldc,
-                       bias, epilogue, aux);
+    launch_gemm<true, true>(grid, s, va, vb, M, N, K, k_chunk, a, lda, b, ldb, C, cb, ldc, bias, epilogue, aux, W);
+  if (ksplit > 1) {
+    int64_t total = (int64_t)M * N;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, M, N, ksplit, W,
+                       C, cb, ldc, bias, epilogue, aux);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bmx_ctx_set_error(ctx, hipGetErrorString(e));
   return 0;
@@ -239,6 +353,17 @@ int32_t bmx_mse_grad(bmx_ctx* ctx, int32_t n, const float* pred, const float* ta
     return bmx_ctx_set_error(ctx, "bmx_mse_grad: invalid arguments");
   hipLaunchKernelGGL(mse_kernel, dim3(1), dim3(1024), 0, bmx_ctx_stream(ctx), n, pred, target, d, (__bf16*)d_bf16,
                      loss);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+}
+
+int32_t bmx_da_features(bmx_ctx* ctx, int32_t n_lat, int32_t n_ic, int32_t n_knob, int32_t n_geo,
+                        const float* knobs, float knob_range, const float* geo, void* X_bf16) {
+  if (!ctx || !knobs || !geo || !X_bf16 || n_lat <= 0 || n_ic <= 0 || n_knob < 0 || n_geo < 0 || knob_range <= 0)
+    return bmx_ctx_set_error(ctx, "bmx_da_features: invalid arguments");
+  int64_t rows = (int64_t)n_lat * n_ic;
+  hipLaunchKernelGGL(da_features_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, bmx_ctx_stream(ctx), rows,
+                     n_ic, n_knob, n_geo, knobs, 1.0f / knob_range, geo, (__bf16*)X_bf16);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
 }

@@ -904,6 +904,54 @@ __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
   if (lane == 0 && my_pet) atomicAdd(pet, my_pet);
 }
 
+// Bunch moments over live particles (the sums of ltt_calc_bunch_data,
+// bsim/modules/lt_tracking_mod.f90:2391-2402): out[0] = n_live,
+// out[1..6] = sum vec_i, out[7..27] = sum vec_i vec_j (i <= j, row-major upper
+// triangle).  Two passes with a fixed reduction order, so the sums do not
+// depend on scheduling: per-block partials, then one block over the partials.
+constexpr int N_MOM = 28;
+
+__device__ __forceinline__ void block_reduce_moments(double (&a)[N_MOM], double* red) {
+  // red: N_MOM x 256 doubles of LDS
+  int t = threadIdx.x;
+  for (int k = 0; k < N_MOM; k++) red[k * 256 + t] = a[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int k = 0; k < N_MOM; k++) red[k * 256 + t] += red[k * 256 + t + w];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void bmx_moments_partial_kernel(SoA p, int64_t n, double* __restrict__ partial) {
+  extern __shared__ double red[];
+  double a[N_MOM];
+  for (int k = 0; k < N_MOM; k++) a[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    if (p.state[i] != BMX_ALIVE) continue;
+    double v[6] = {p.x[i], p.px[i], p.y[i], p.py[i], p.z[i], p.pz[i]};
+    a[0] += 1.0;
+    int k = 7;
+    for (int r = 0; r < 6; r++) {
+      a[1 + r] += v[r];
+      for (int c = r; c < 6; c++) a[k++] += v[r] * v[c];
+    }
+  }
+  block_reduce_moments(a, red);
+  if (threadIdx.x < N_MOM) partial[(int64_t)blockIdx.x * N_MOM + threadIdx.x] = red[threadIdx.x * 256];
+}
+
+__global__ __launch_bounds__(256) void bmx_moments_final_kernel(const double* __restrict__ partial, int n_part,
+                                                               double* __restrict__ out) {
+  extern __shared__ double red[];
+  double a[N_MOM];
+  for (int k = 0; k < N_MOM; k++) a[k] = 0.0;
+  for (int b = threadIdx.x; b < n_part; b += 256)
+    for (int k = 0; k < N_MOM; k++) a[k] += partial[(int64_t)b * N_MOM + k];
+  block_reduce_moments(a, red);
+  if (threadIdx.x < N_MOM) out[threadIdx.x] = red[threadIdx.x * 256];
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -940,6 +988,8 @@ static int set_err(bmx_ctx* ctx, const std::string& s) {
 
 // Shared with bmx_mlp.hip (hidden: not part of the exported ABI).
 extern "C" __attribute__((visibility("hidden"))) hipStream_t bmx_ctx_stream(const bmx_ctx* ctx) { return ctx->stream; }
+// defined in bmx_mlp.hip: frees the context's GEMM workspace
+extern "C" void bmx_mlp_release(const bmx_ctx* ctx);
 extern "C" __attribute__((visibility("hidden"))) int32_t bmx_ctx_set_error(bmx_ctx* ctx, const char* msg) {
   return set_err(ctx, msg ? msg : "error");
 }
@@ -980,6 +1030,7 @@ int32_t bmx_ctx_create(bmx_ctx** out, int32_t device, uint32_t flags) {
 
 void bmx_ctx_destroy(bmx_ctx* ctx) {
   if (!ctx) return;
+  bmx_mlp_release(ctx);
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   hipEventDestroy(ctx->ev0);
@@ -1392,6 +1443,39 @@ int32_t bmx_track_ics(bmx_ctx* ctx, const bmx_lattice* lat, const double* ics, c
 }
 
 
+
+int32_t bmx_bunch_moments(bmx_ctx* ctx, const bmx_particles* p, double* moments) {
+  if (!ctx || !p || !moments || !particles_ok(p)) return set_err(ctx, "invalid arguments");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (int k = 0; k < N_MOM; k++) moments[k] = 0.0;
+  if (p->n <= 0) return 0;
+  Staged st;
+  bmx_particles q = *p;
+  q.ix_lost = nullptr;
+  q.turn_lost = nullptr;
+  if (stage_particles(ctx, &q, st) != 0) return -1;
+  const int blocks = 1024;
+  double *d_part = nullptr, *d_out = nullptr;
+  if (hipMalloc(&d_part, (size_t)blocks * N_MOM * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_out, N_MOM * sizeof(double)) != hipSuccess) {
+    if (d_part) (void)hipFree(d_part);
+    st.hp[0] = nullptr;  // nothing to copy back
+    q.on_device = 1;
+    finish_particles(ctx, &q, st, hipSuccess, nullptr);
+    return set_err(ctx, "hipMalloc for the moment reduction failed");
+  }
+  size_t lds = (size_t)N_MOM * 256 * sizeof(double);
+  hipLaunchKernelGGL(bmx_moments_partial_kernel, dim3(blocks), dim3(256), lds, ctx->stream, st.s, p->n, d_part);
+  hipLaunchKernelGGL(bmx_moments_final_kernel, dim3(1), dim3(256), lds, ctx->stream, d_part, blocks, d_out);
+  hipError_t le = hipGetLastError();
+  (void)hipMemcpyAsync(moments, d_out, N_MOM * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  // no coordinates change: skip the copy-back of host arrays
+  q.on_device = 1;
+  int rc = finish_particles(ctx, &q, st, le, nullptr);
+  (void)hipFree(d_part);
+  (void)hipFree(d_out);
+  return rc;
+}
 
 // closed_orbit_calc (bmad/code/closed_orbit_calc.f90:81-...) in the form the
 // batched tracker serves: Newton iteration x <- x + (I - M)^-1 (T(x) - x) on

@@ -173,6 +173,14 @@ interface
     integer(c_int32_t) :: rc
   end function
 
+  function bmx_bunch_moments(ctx, p, moments) bind(C, name = 'bmx_bunch_moments') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    type(bmx_particles) :: p
+    real(c_double) :: moments(28)
+    integer(c_int32_t) :: rc
+  end function
+
   function bmx_closed_orbit(ctx, lat, n_dim, orbit, status, stats) bind(C, name = 'bmx_closed_orbit') result(rc)
     import
     type(c_ptr), value :: ctx, lat

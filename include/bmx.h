@@ -250,6 +250,12 @@ int32_t bmx_track_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *gri
 int32_t bmx_track_ics(bmx_ctx *ctx, const bmx_lattice *lat, const double *ics, const int64_t *ic_offsets,
                       int32_t n_turns, int32_t turns_per_block, int32_t *survival_turn, bmx_stats *out);
 
+/* Sums over the live particles of a bunch, the moments of ltt_calc_bunch_data
+ * (bsim/modules/lt_tracking_mod.f90:2391-2402): moments[0] = n_live,
+ * moments[1..6] = sum vec_i, moments[7..27] = sum vec_i vec_j for i <= j
+ * (upper triangle, row by row).  Deterministic reduction order. */
+int32_t bmx_bunch_moments(bmx_ctx *ctx, const bmx_particles *p, double *moments);
+
 /* Closed orbit of every variant (closed_orbit_calc, bmad/code/closed_orbit_calc.f90:81):
  * orbit is n_variants x 6, the guess on entry and the closed orbit on return.
  * n_dim = 4 closes x, px, y, py at the guess's pz; n_dim = 6 closes all six
@@ -287,6 +293,12 @@ int32_t bmx_f32_to_bf16(bmx_ctx *ctx, int64_t n, const float *x, void *y);
  * d = 2 (pred - target) / n in FP32 and (d_bf16) BF16. */
 int32_t bmx_mse_grad(bmx_ctx *ctx, int32_t n, const float *pred, const float *target, float *d, void *d_bf16,
                      float *loss);
+
+/* Surrogate inputs of a DA scan in BF16, row l * n_ic + i = (knobs[l, :] /
+ * knob_range, geo[i, :]) for n_lat lattices x n_ic grid points (row width
+ * n_knob + n_geo).  Device pointers. */
+int32_t bmx_da_features(bmx_ctx *ctx, int32_t n_lat, int32_t n_ic, int32_t n_knob, int32_t n_geo,
+                        const float *knobs, float knob_range, const float *geo, void *X_bf16);
 
 /* Wait for the work queued on the context's stream. */
 int32_t bmx_ctx_sync(bmx_ctx *ctx);

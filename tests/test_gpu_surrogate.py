@@ -32,7 +32,8 @@ def _bf(a):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 50), (1, 256, 333), (257, 3, 1), (130, 256, 35)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 50), (1, 256, 333), (257, 3, 1), (130, 256, 35),
+                                   (256, 256, 8192), (64, 128, 4096), (1, 256, 5000)])
 def test_gemm_bf16(ctx, ta, tb, M, N, K):
     rng = np.random.default_rng(M * 1000 + N * 10 + K)
     a = mo.bf16(rng.standard_normal((K, M) if ta else (M, K)))
@@ -177,3 +178,22 @@ def test_training_reduces_loss(ctx):
     last = loss.item()
     print(f"loss {first:.4f} -> {last:.4f}", flush=True)
     assert last < 0.5 * first
+
+
+def test_da_features_match_host_rows(ctx):
+    """bmx_da_features forms, in BF16, exactly the rows surrogate.features
+    builds on the host."""
+    spec = surrogate.ScanSpec(n_turns=64, n_angle=5, n_amp=7, n_pz=3)
+    rng = np.random.default_rng(3)
+    knobs = rng.uniform(-0.2, 0.2, (4, surrogate.N_KNOBS))
+    want = mo.bf16(surrogate.features(knobs, spec))
+    geo = torch.from_numpy(surrogate.geometry(spec).astype(np.float32)).to(DEV)
+    kd = torch.from_numpy(knobs.astype(np.float32)).to(DEV)
+    X = torch.zeros(4 * spec.n_ic, surrogate.N_KNOBS + 3, dtype=torch.bfloat16, device=DEV)
+    torch.cuda.synchronize()
+    rc = ctx.lib.bmx_da_features(ctx.h, 4, spec.n_ic, surrogate.N_KNOBS, 3, _p(kd), surrogate.KNOB_RANGE, _p(geo),
+                                 _p(X))
+    assert rc == 0 and ctx.lib.bmx_ctx_sync(ctx.h) == 0
+    got = X.float().cpu().numpy().astype(np.float64)
+    # the host rows are float32 knobs / 0.2; the device scales float32 knobs by 1 / 0.2
+    assert np.max(np.abs(got - want)) <= 2.0 ** -8 * np.max(np.abs(want))

@@ -16,7 +16,7 @@ class _FakeLat:
         pass
 
 
-def _run(monkeypatch, truth, pred_shift, islands=None):
+def _run(monkeypatch, truth, pred_shift, islands=None, refine=0):
     spec = surrogate.ScanSpec(n_turns=100, n_angle=6, n_amp=40, n_pz=3)
     n_lat = truth.shape[0]
     N = spec.n_turns
@@ -29,15 +29,15 @@ def _run(monkeypatch, truth, pred_shift, islands=None):
         return N if alive else 7
 
     # predicted survival: alive below truth + shift
-    def fake_predict(net, X):
+    def fake_predict(net, knob_sets, spec_):
         out = np.zeros((n_lat, spec.n_pz, spec.n_angle, spec.n_amp), dtype=np.float32)
         kb = np.clip(truth + pred_shift, 0, spec.n_amp)
         out[...] = np.where(amp_k[None, None, None, :] < kb[..., None], 1.0, 0.0)
-        return out.reshape(-1)
+        return out
 
     tracked = []
 
-    def fake_track_ics(lat, per, n_turns, tpb=0):
+    def fake_track_ics(lat, per, n_turns, tpb=0, return_stats=False):
         off = spec.offsets()
         res = []
         for v, c in enumerate(per):
@@ -49,21 +49,22 @@ def _run(monkeypatch, truth, pred_shift, islands=None):
                 s[j] = surv_of(v, ip, ia, k)
             tracked.append(c.shape[1])
             res.append(s)
-        return res
+        return (res, dict(pet=0.0)) if return_stats else res
 
-    monkeypatch.setattr(surrogate, "predict", fake_predict)
+    monkeypatch.setattr(surrogate, "predict_knobs", fake_predict)
     monkeypatch.setattr(surrogate, "upload_variants", lambda ctx, k, s: _FakeLat())
     from bmx import tracker
     monkeypatch.setattr(tracker, "track_ics", fake_track_ics)
-    runs, area, _, n_tracked = surrogate.surrogate_scan(None, None, np.zeros((n_lat, 32)), spec)
+    runs, area, _, n_tracked = surrogate.surrogate_scan(None, None, np.zeros((n_lat, 32)), spec, refine=refine)
     return runs, n_tracked, spec
 
 
+@pytest.mark.parametrize("refine", [0, 6])
 @pytest.mark.parametrize("shift", [0, 2, -3, 15, -15])
-def test_band_search_recovers_true_boundary(monkeypatch, shift):
+def test_band_search_recovers_true_boundary(monkeypatch, shift, refine):
     rng = np.random.default_rng(abs(shift) + 1)
     truth = rng.integers(5, 35, size=(4, 3, 6))
-    runs, n_tracked, spec = _run(monkeypatch, truth, shift)
+    runs, n_tracked, spec = _run(monkeypatch, truth, shift, refine=refine)
     assert np.array_equal(runs, truth)
     full = truth.size * spec.n_amp
     if abs(shift) <= 2:
@@ -110,3 +111,11 @@ def test_replicated_knob_lattices_equal_built_ones():
         want = bytes(memoryview(lattice.to_array(surrogate.knob_lattice(knobs[v], spec))).cast("B"))
         a, b = captured["offs"][v] * size, captured["offs"][v + 1] * size
         assert captured["arr"][a:b] == want
+
+
+def test_step_fit_boundary():
+    p = np.array([[1, 1, 1, 0.98, 1, 0.3, 0.2, 0.1],   # step at 5
+                  [1, 1, 1, 1, 1, 1, 1, 1],            # survives the whole ray
+                  [0.2, 0.1, 0.1, 0, 0, 0, 0, 0],      # lost from the start
+                  [1, 1, 0.2, 1, 1, 0.1, 0.1, 0.1]])   # one mispredicted point does not move it
+    assert list(surrogate.step_fit_boundary(p, 1.0)) == [5, 8, 0, 5]
