@@ -84,6 +84,10 @@ class Grid(C.Structure):
                 ("pz_max", C.c_double), ("angle_min", C.c_double), ("angle_max", C.c_double)]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("turns_per_block", C.c_int32), ("graph_depth", C.c_int32), ("probe_pet_per_s", C.c_double)]
+
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 
@@ -91,7 +95,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
     "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
-    "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit",
+    "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
+    "bmx_tune_grid",
     "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_ctx_sync",
 ]
 
@@ -135,6 +140,13 @@ def load_library(path=None):
                                   C.c_int32, C.POINTER(C.c_int32), C.POINTER(Stats)]
     lib.bmx_bunch_moments.restype = C.c_int32
     lib.bmx_bunch_moments.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(C.c_double)]
+    lib.bmx_ctx_set_schedule.restype = C.c_int32
+    lib.bmx_ctx_set_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
+    lib.bmx_ctx_get_schedule.restype = C.c_int32
+    lib.bmx_ctx_get_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
+    lib.bmx_tune_grid.restype = C.c_int32
+    lib.bmx_tune_grid.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Grid), C.POINTER(C.c_double), C.c_int32,
+                                  C.POINTER(Schedule)]
     lib.bmx_closed_orbit.restype = C.c_int32
     lib.bmx_closed_orbit.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
                                      C.POINTER(Stats)]
@@ -163,7 +175,7 @@ def load_library(path=None):
 
 def check_layout(lib):
     """Compare ctypes struct sizes with the compiled library's."""
-    sizes = {0: EleRec, 1: Particles, 2: Stats, 3: Grid, 4: Config}
+    sizes = {0: EleRec, 1: Particles, 2: Stats, 3: Grid, 4: Config, 5: Schedule}
     bad = {k: (C.sizeof(v), lib.bmx_sizeof(k)) for k, v in sizes.items() if C.sizeof(v) != lib.bmx_sizeof(k)}
     if bad:
         raise RuntimeError(f"ABI layout mismatch (ctypes, library): {bad}")

@@ -53,6 +53,16 @@ class Context:
         except Exception:
             pass
 
+    def schedule(self):
+        """The context's turn-block schedule (bmx_ctx_get_schedule) as a dict."""
+        sc = abi.Schedule()
+        self.lib.bmx_ctx_get_schedule(self.h, C.byref(sc))
+        return dict(turns_per_block=sc.turns_per_block, graph_depth=sc.graph_depth, probe_pet_per_s=sc.probe_pet_per_s)
+
+    def set_schedule(self, turns_per_block=32, graph_depth=1):
+        sc = abi.Schedule(int(turns_per_block), int(graph_depth), 0.0)
+        _check(self.h, self.lib.bmx_ctx_set_schedule(self.h, C.byref(sc)), "bmx_ctx_set_schedule")
+
     def lattice(self, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3):
         return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit)
 
@@ -261,6 +271,21 @@ def track_ics(lat, ics_per_variant, n_turns, turns_per_block=0, return_stats=Fal
     if return_stats:
         return out, dict(pet=st.pet, n_live=st.n_live, n_lost=st.n_lost, kernel_ms=st.kernel_ms)
     return out
+
+
+def tune_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, pz_min=0.0, pz_max=0.0,
+              angle_min=0.0, angle_max=np.pi, ref_orbit=None):
+    """Autotune the context's turn-block schedule for this scan (bmx_tune_grid);
+    returns the chosen schedule as a dict."""
+    g = abi.Grid(n_angle, n_amp, n_pz, 0, amp_max_x, amp_ratio_y, pz_min, pz_max, angle_min, angle_max)
+    ref_p = None
+    if ref_orbit is not None:
+        ref = np.ascontiguousarray(np.asarray(ref_orbit, dtype=np.float64).reshape(lat.n_variants, n_pz, 6))
+        ref_p = ref.ctypes.data_as(C.POINTER(C.c_double))
+    sc = abi.Schedule()
+    rc = lat.ctx.lib.bmx_tune_grid(lat.ctx.h, lat.h, C.byref(g), ref_p, int(n_turns), C.byref(sc))
+    _check(lat.ctx.h, rc, "bmx_tune_grid")
+    return dict(turns_per_block=sc.turns_per_block, graph_depth=sc.graph_depth, probe_pet_per_s=sc.probe_pet_per_s)
 
 
 def track_grid(lat, n_turns, n_angle, n_amp, n_pz, amp_max_x, amp_ratio_y=1.0, pz_min=0.0, pz_max=0.0,

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -859,9 +861,15 @@ __global__ __launch_bounds__(256) void bmx_ics_init_kernel(LatticeConsts lc, Gri
 __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
     const DevOp* __restrict__ ops, const int32_t* __restrict__ var_op_begin, const int32_t* __restrict__ var_n_ele,
     LatticeConsts lc, int n_ic, GridSoA src, GridSoA dst, const int32_t* __restrict__ count_in,
-    int32_t* __restrict__ count_out, int turn0, int turns, int32_t* __restrict__ survival,
-    unsigned long long* __restrict__ pet) {
+    int32_t* __restrict__ count_out, const int32_t* __restrict__ d_turn0, int turns_per_block, int n_turns,
+    int32_t* __restrict__ survival, unsigned long long* __restrict__ pet) {
   __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
+  // the block's first turn lives in device memory so that captured graphs of
+  // several blocks replay unchanged; every thread reads the same value, so a
+  // finished scan leaves the whole grid together
+  const int turn0 = *d_turn0;
+  const int turns = min(turns_per_block, n_turns - turn0);
+  if (turns <= 0) return;
   int v = blockIdx.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool active = i < count_in[v];
@@ -902,6 +910,15 @@ __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
   }
   for (int off = 32; off > 0; off >>= 1) my_pet += __shfl_xor(my_pet, off);
   if (lane == 0 && my_pet) atomicAdd(pet, my_pet);
+}
+
+// Step bookkeeping between turn blocks: clears the count buffer the next
+// block appends survivors to and advances the block's first turn.  A kernel,
+// not a memset, so that it can sit in a captured graph.
+__global__ __launch_bounds__(256) void bmx_grid_advance_kernel(int32_t* __restrict__ count_next, int nv,
+                                                              int32_t* __restrict__ d_turn0, int turns) {
+  for (int v = threadIdx.x; v < nv; v += blockDim.x) count_next[v] = 0;
+  if (threadIdx.x == 0) *d_turn0 += turns;
 }
 
 // Bunch moments over live particles (the sums of ltt_calc_bunch_data,
@@ -960,6 +977,7 @@ struct bmx_ctx {
   hipStream_t stream;
   hipEvent_t ev0, ev1;
   std::string err;
+  bmx_schedule sched;  // turn-block schedule used when a call passes turns_per_block <= 0
 };
 
 struct bmx_lattice {
@@ -1005,6 +1023,7 @@ int32_t bmx_sizeof(int32_t which) {
     case 2: return (int32_t)sizeof(bmx_stats);
     case 3: return (int32_t)sizeof(bmx_grid);
     case 4: return (int32_t)sizeof(bmx_config);
+    case 5: return (int32_t)sizeof(bmx_schedule);
   }
   return -1;
 }
@@ -1019,6 +1038,9 @@ int32_t bmx_ctx_create(bmx_ctx** out, int32_t device, uint32_t flags) {
   if (hipSetDevice(device) != hipSuccess) return -4;
   bmx_ctx* c = new bmx_ctx();
   c->device = device;
+  c->sched.turns_per_block = 32;
+  c->sched.graph_depth = 1;
+  c->sched.probe_pet_per_s = 0.0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -1247,6 +1269,7 @@ struct SurvivalRun {
   GridSoA st[2];
   int32_t* d_surv = nullptr;
   int32_t* d_count = nullptr;
+  int32_t* d_turn0 = nullptr;
   unsigned long long* d_pet = nullptr;
   std::vector<void*> owned;
   ~SurvivalRun() {
@@ -1269,42 +1292,77 @@ struct SurvivalRun {
       if (!(st[b].ic = (int32_t*)dalloc(total * sizeof(int32_t)))) return false;
     }
     return (d_surv = (int32_t*)dalloc(total * sizeof(int32_t))) && (d_count = (int32_t*)dalloc(2 * nv * sizeof(int32_t))) &&
-           (d_pet = (unsigned long long*)dalloc(sizeof(unsigned long long)));
+           (d_turn0 = (int32_t*)dalloc(sizeof(int32_t))) && (d_pet = (unsigned long long*)dalloc(sizeof(unsigned long long)));
   }
 };
 
 // Runs the turn blocks; counts[v] = initial particles of variant v.  Fills the
-// host survival array (nv * stride), PET and timing.
+// host survival array (nv * stride), PET and timing.  Blocks alternate the
+// parity p: store p -> store 1-p, counts p -> counts 1-p, then the advance
+// kernel clears counts p and moves the first turn on.  graph_depth > 1
+// captures that many (an even number of) steps into one hipGraph, replayed
+// until the scan ends; the host reads the live counts after every replay and
+// re-captures with a smaller launch once the live particles fall below 3/4 of
+// the captured size.  graph_depth <= 1 launches and synchronises block by block.
 static int run_survival(bmx_ctx* ctx, const bmx_lattice* lat, SurvivalRun& R, std::vector<int32_t> h_count,
-                        int n_turns, int tpb, int32_t* h_surv, bmx_stats* out,
+                        int n_turns, int tpb, int graph_depth, int32_t* h_surv, bmx_stats* out,
                         const std::function<void()>& init) {
   const int block = 256;
   int nv = R.nv;
+  if (graph_depth > 1 && (graph_depth & 1)) graph_depth++;
   (void)hipMemcpyAsync(R.d_count, h_count.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+  (void)hipMemsetAsync(R.d_count + nv, 0, nv * sizeof(int32_t), ctx->stream);
   (void)hipMemsetAsync(R.d_pet, 0, sizeof(unsigned long long), ctx->stream);
+  (void)hipMemsetAsync(R.d_turn0, 0, sizeof(int32_t), ctx->stream);
   (void)hipEventRecord(ctx->ev0, ctx->stream);
   init();
   int cur = 0, turn0 = 0, max_live = 0;
   for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
+  auto step = [&](int p, int live) {
+    dim3 gd((unsigned)((live + block - 1) / block), (unsigned)nv);
+    hipLaunchKernelGGL(bmx_grid_block_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin,
+                       lat->d_var_n_ele, lat->lc, R.stride, R.st[p], R.st[1 - p], R.d_count + p * nv,
+                       R.d_count + (1 - p) * nv, R.d_turn0, tpb, n_turns, R.d_surv, R.d_pet);
+    hipLaunchKernelGGL(bmx_grid_advance_kernel, dim3(1), dim3(256), 0, ctx->stream, R.d_count + p * nv, nv, R.d_turn0,
+                       tpb);
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_live = 0;
   hipError_t le = hipGetLastError();
   while (le == hipSuccess && turn0 < n_turns && max_live > 0) {
-    int turns = std::min(tpb, n_turns - turn0);
-    int32_t* cin = R.d_count + (cur ? nv : 0);
-    int32_t* cout = R.d_count + (cur ? 0 : nv);
-    (void)hipMemsetAsync(cout, 0, nv * sizeof(int32_t), ctx->stream);
-    dim3 gd((unsigned)((max_live + block - 1) / block), (unsigned)nv);
-    hipLaunchKernelGGL(bmx_grid_block_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin,
-                       lat->d_var_n_ele, lat->lc, R.stride, R.st[cur], R.st[1 - cur], cin, cout, turn0, turns,
-                       R.d_surv, R.d_pet);
-    le = hipGetLastError();
-    if (le != hipSuccess) break;
-    (void)hipMemcpyAsync(h_count.data(), cout, nv * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (graph_depth > 1) {
+      if (!exec || max_live < (graph_live * 3) / 4) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        le = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (le != hipSuccess) break;
+        for (int k = 0; k < graph_depth; k++) step((cur + k) & 1, max_live);
+        le = hipStreamEndCapture(ctx->stream, &graph);
+        if (le == hipSuccess) le = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (le != hipSuccess) break;
+        graph_live = max_live;
+      }
+      le = hipGraphLaunch(exec, ctx->stream);
+      if (le != hipSuccess) break;
+      // an even number of steps: the next block reads counts `cur` again
+    } else {
+      step(cur, max_live);
+      le = hipGetLastError();
+      if (le != hipSuccess) break;
+      cur = 1 - cur;
+    }
+    (void)hipMemcpyAsync(h_count.data(), R.d_count + cur * nv, nv * sizeof(int32_t), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    (void)hipMemcpyAsync(&turn0, R.d_turn0, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) break;
     max_live = 0;
     for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
-    cur = 1 - cur;
-    turn0 += turns;
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   (void)hipEventRecord(ctx->ev1, ctx->stream);
   unsigned long long h_pet = 0;
   int64_t total = (int64_t)nv * R.stride;
@@ -1313,7 +1371,7 @@ static int run_survival(bmx_ctx* ctx, const bmx_lattice* lat, SurvivalRun& R, st
   hipError_t se = hipStreamSynchronize(ctx->stream);
   float ms = 0;
   (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
+  if (le != hipSuccess) return set_err(ctx, std::string("turn blocks: ") + hipGetErrorString(le));
   if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
   if (out) {
     std::memset(out, 0, sizeof(*out));
@@ -1337,7 +1395,7 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
   g.pz_min = grid->pz_min; g.pz_max = grid->pz_max;
   g.angle_min = grid->angle_min; g.angle_max = grid->angle_max;
   int nv = lat->n_variants;
-  int tpb = grid->turns_per_block > 0 ? grid->turns_per_block : 32;
+  int tpb = grid->turns_per_block > 0 ? grid->turns_per_block : ctx->sched.turns_per_block;
   int64_t total = (int64_t)g.n_ic * nv;
   SurvivalRun R;
   double* d_ref = nullptr;
@@ -1350,7 +1408,7 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
   }
   std::vector<int32_t> h_count(nv, g.n_ic);
   const int block = 256;
-  int rc = run_survival(ctx, lat, R, h_count, n_turns, tpb, survival_turn, out, [&]() {
+  int rc = run_survival(ctx, lat, R, h_count, n_turns, tpb, ctx->sched.graph_depth, survival_turn, out, [&]() {
     hipLaunchKernelGGL(bmx_grid_init_kernel, dim3((unsigned)((g.n_ic + block - 1) / block), (unsigned)nv), dim3(block),
                        0, ctx->stream, g, lat->lc, R.st[0], d_ref, R.d_surv, n_turns);
   });
@@ -1422,7 +1480,8 @@ int32_t bmx_track_ics(bmx_ctx* ctx, const bmx_lattice* lat, const double* ics, c
   for (int v = 0; v < nv; v++) h_count[v] = (int32_t)(ic_offsets[v + 1] - ic_offsets[v]);
   std::vector<int32_t> h_surv(total);
   const int block = 256;
-  int rc = run_survival(ctx, lat, R, h_count, n_turns, turns_per_block > 0 ? turns_per_block : 32, h_surv.data(), out,
+  int rc = run_survival(ctx, lat, R, h_count, n_turns, turns_per_block > 0 ? turns_per_block : ctx->sched.turns_per_block,
+                        ctx->sched.graph_depth, h_surv.data(), out,
                         [&]() {
                           hipLaunchKernelGGL(bmx_ics_init_kernel,
                                              dim3((unsigned)((stride + block - 1) / block), (unsigned)nv), dim3(block),
@@ -1443,6 +1502,79 @@ int32_t bmx_track_ics(bmx_ctx* ctx, const bmx_lattice* lat, const double* ics, c
 }
 
 
+
+int32_t bmx_ctx_set_schedule(bmx_ctx* ctx, const bmx_schedule* s) {
+  if (!ctx || !s || s->turns_per_block <= 0 || s->graph_depth < 0) return set_err(ctx, "invalid schedule");
+  ctx->sched = *s;
+  return 0;
+}
+
+int32_t bmx_ctx_get_schedule(const bmx_ctx* ctx, bmx_schedule* s) {
+  if (!ctx || !s) return -1;
+  *s = ctx->sched;
+  return 0;
+}
+
+int32_t bmx_tune_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* grid, const double* ref_orbit,
+                      int32_t n_turns, bmx_schedule* best) {
+  if (!ctx || !lat || !grid || !best || n_turns <= 0 || grid->n_angle <= 0 || grid->n_amp <= 0 || grid->n_pz <= 0)
+    return set_err(ctx, "invalid arguments");
+  // sparse sample of the grid: every 4th angle and amplitude, every pz slice, every variant
+  int nv = lat->n_variants;
+  std::vector<std::array<double, 3>> per;  // (x offset, y offset, pz slice)
+  double dth = (grid->n_angle > 1) ? (grid->angle_max - grid->angle_min) / (grid->n_angle - 1) : 0.0;
+  for (int ip = 0; ip < grid->n_pz; ip++)
+    for (int ia = 0; ia < grid->n_angle; ia += 4)
+      for (int k = 0; k < grid->n_amp; k += 4) {
+        double amp = grid->amp_max_x * (double)(k + 1) / grid->n_amp, th = grid->angle_min + dth * ia;
+        per.push_back({amp * cos(th), amp * grid->amp_ratio_y * sin(th), (double)ip});
+      }
+  int64_t m = (int64_t)per.size(), n = m * nv;
+  std::vector<double> ics(6 * n);
+  std::vector<int64_t> off(nv + 1);
+  for (int v = 0; v <= nv; v++) off[v] = (int64_t)v * m;
+  for (int v = 0; v < nv; v++)
+    for (int64_t j = 0; j < m; j++) {
+      int ip = (int)per[j][2];
+      double pz0 = (grid->n_pz > 1) ? grid->pz_min + (grid->pz_max - grid->pz_min) * ip / (grid->n_pz - 1) : grid->pz_min;
+      double r[6] = {0, 0, 0, 0, 0, pz0};
+      if (ref_orbit)
+        for (int c = 0; c < 6; c++) r[c] = ref_orbit[((int64_t)v * grid->n_pz + ip) * 6 + c];
+      r[0] += per[j][0];
+      r[2] += per[j][1];
+      for (int c = 0; c < 6; c++) ics[(size_t)c * n + off[v] + j] = r[c];
+    }
+  std::vector<int32_t> surv(n);
+  const int tpbs[6] = {4, 8, 16, 32, 64, 128};
+  const int depths[2] = {1, 8};
+  int probe = std::min(n_turns, 64);
+  bmx_schedule saved = ctx->sched, pick = saved;
+  double best_rate = -1;
+  for (int d : depths)
+    for (int t : tpbs) {
+      if (t > probe && t != tpbs[0]) continue;
+      ctx->sched.turns_per_block = t;
+      ctx->sched.graph_depth = d;
+      bmx_stats st;
+      auto t0 = std::chrono::steady_clock::now();
+      int rc = bmx_track_ics(ctx, lat, ics.data(), off.data(), probe, t, surv.data(), &st);
+      double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (rc != 0) {
+        ctx->sched = saved;
+        return rc;
+      }
+      double rate = st.pet / std::max(wall, 1e-9);
+      if (rate > best_rate) {
+        best_rate = rate;
+        pick.turns_per_block = t;
+        pick.graph_depth = d;
+        pick.probe_pet_per_s = rate;
+      }
+    }
+  ctx->sched = pick;
+  *best = pick;
+  return 0;
+}
 
 int32_t bmx_bunch_moments(bmx_ctx* ctx, const bmx_particles* p, double* moments) {
   if (!ctx || !p || !moments || !particles_ok(p)) return set_err(ctx, "invalid arguments");

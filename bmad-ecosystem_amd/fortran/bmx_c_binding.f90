@@ -78,6 +78,11 @@ type, bind(C) :: bmx_grid
   real(c_double) :: amp_max_x = 0, amp_ratio_y = 1, pz_min = 0, pz_max = 0, angle_min = 0, angle_max = 0
 end type
 
+type, bind(C) :: bmx_schedule
+  integer(c_int32_t) :: turns_per_block = 32, graph_depth = 1
+  real(c_double) :: probe_pet_per_s = 0
+end type
+
 interface
   function bmx_abi_version() bind(C, name = 'bmx_abi_version') result(v)
     import
@@ -178,6 +183,30 @@ interface
     type(c_ptr), value :: ctx
     type(bmx_particles) :: p
     real(c_double) :: moments(28)
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_ctx_set_schedule(ctx, s) bind(C, name = 'bmx_ctx_set_schedule') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    type(bmx_schedule) :: s
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_ctx_get_schedule(ctx, s) bind(C, name = 'bmx_ctx_get_schedule') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    type(bmx_schedule) :: s
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_tune_grid(ctx, lat, grid, ref_orbit, n_turns, best) bind(C, name = 'bmx_tune_grid') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    type(bmx_grid) :: grid
+    type(c_ptr), value :: ref_orbit
+    integer(c_int32_t), value :: n_turns
+    type(bmx_schedule) :: best
     integer(c_int32_t) :: rc
   end function
 

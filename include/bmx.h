@@ -193,6 +193,17 @@ typedef struct bmx_grid {
   double angle_min, angle_max; /* polar angle range [rad], default (0, pi) */
 } bmx_grid;
 
+/* Turn-block schedule of the survival scans (bmx_track_grid, bmx_track_ics):
+ * turns_per_block turns per kernel launch, survivors compacted between
+ * blocks; graph_depth > 1 captures that many blocks (rounded up to even) into
+ * a hipGraph replayed between host synchronisations, 1 launches block by
+ * block.  probe_pet_per_s: the rate bmx_tune_grid measured for it. */
+typedef struct bmx_schedule {
+  int32_t turns_per_block;
+  int32_t graph_depth;
+  double probe_pet_per_s;
+} bmx_schedule;
+
 typedef struct bmx_ctx bmx_ctx;
 typedef struct bmx_lattice bmx_lattice;
 
@@ -200,7 +211,7 @@ typedef struct bmx_lattice bmx_lattice;
 int32_t bmx_abi_version(void);
 
 /* sizeof of the ABI structs, for binding-side layout checks:
- * which = 0 bmx_ele_rec, 1 bmx_particles, 2 bmx_stats, 3 bmx_grid, 4 bmx_config. */
+ * which = 0 bmx_ele_rec, 1 bmx_particles, 2 bmx_stats, 3 bmx_grid, 4 bmx_config, 5 bmx_schedule. */
 int32_t bmx_sizeof(int32_t which);
 
 /* Create a context on HIP device `device`.  flags: reserved, pass 0. */
@@ -242,13 +253,28 @@ int32_t bmx_track_multi(bmx_ctx *ctx, const bmx_lattice *lat, const int64_t *par
 int32_t bmx_track_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *grid, const double *ref_orbit,
                        int32_t n_turns, int32_t *survival_turn, double *da_area, bmx_stats *out);
 
-/* Survival turns of explicit initial conditions: ics is 6 x n host SoA
+/* Survival turns of explicit initial conditions (turns_per_block <= 0: the
+ * context's schedule): ics is 6 x n host SoA
  * (x[n], px[n], y[n], py[n], z[n], pz[n]) with n = ic_offsets[n_variants];
  * ICs [ic_offsets[v], ic_offsets[v+1]) are tracked through variant v.  The
  * per-IC survival loop of dynamic_aperture_point
  * (bmad/modules/dynamic_aperture_mod.f90:339-349), batched. */
 int32_t bmx_track_ics(bmx_ctx *ctx, const bmx_lattice *lat, const double *ics, const int64_t *ic_offsets,
                       int32_t n_turns, int32_t turns_per_block, int32_t *survival_turn, bmx_stats *out);
+
+/* The context's turn-block schedule (default 32 turns per block, no graph);
+ * a scan call passing turns_per_block <= 0 uses it. */
+int32_t bmx_ctx_set_schedule(bmx_ctx *ctx, const bmx_schedule *s);
+int32_t bmx_ctx_get_schedule(const bmx_ctx *ctx, bmx_schedule *s);
+
+/* Autotuner of the turn-block schedule (no counterpart in the reference):
+ * times every candidate (turns_per_block in {4, 8, 16, 32, 64, 128} x
+ * graph_depth in {1, 8}) on a sparse sample of this grid (every 4th angle and
+ * amplitude, all variants and pz slices) for min(n_turns, 64) turns, keeps
+ * the highest live particle-element-turns per second of wall time, stores it
+ * as the context's schedule and returns it in *best. */
+int32_t bmx_tune_grid(bmx_ctx *ctx, const bmx_lattice *lat, const bmx_grid *grid, const double *ref_orbit,
+                      int32_t n_turns, bmx_schedule *best);
 
 /* Sums over the live particles of a bunch, the moments of ltt_calc_bunch_data
  * (bsim/modules/lt_tracking_mod.f90:2391-2402): moments[0] = n_live,

@@ -162,3 +162,51 @@ def test_dynamic_aperture_scan_matches_oracle_bisection(ctx):
         assert np.array_equal(g[k][~band], o1[k][~band]), k
     assert np.all(g["x"][:, :, 0] > 1e-4)
     lat.close()
+
+
+def test_graph_turn_blocks_match_eager(ctx):
+    """Survival from hipGraph-captured turn blocks (several replays, with
+    re-capture as the live count falls) equals block-by-block launches, for
+    several schedules, on the grid and on explicit initial conditions."""
+    vs, offs = _variants(n_var=3, min_cells=6, max_cells=14)
+    lat = _upload(ctx, vs, offs)
+    kw = dict(n_angle=8, n_amp=16, n_pz=3, amp_max_x=0.02, amp_ratio_y=0.5, pz_min=-0.005, pz_max=0.005)
+    saved = ctx.schedule()
+    results = []
+    for tpb, depth in [(4, 1), (4, 8), (16, 8), (32, 2), (7, 4)]:
+        ctx.set_schedule(tpb, depth)
+        surv, area, _ = tracker.track_grid(lat, 60, turns_per_block=0, **kw)
+        results.append((surv, area))
+        print(f"schedule ({tpb}, {depth}): alive {np.sum(surv >= 60)}", flush=True)
+    for surv, area in results[1:]:
+        assert np.array_equal(surv, results[0][0])
+        assert np.array_equal(area, results[0][1])
+    rng = np.random.default_rng(8)
+    ics = [np.vstack([rng.uniform(-0.01, 0.01, 300), np.zeros(300), rng.uniform(-0.005, 0.005, 300), np.zeros(300),
+                      np.zeros(300), rng.uniform(-0.005, 0.005, 300)]) for _ in range(3)]
+    ctx.set_schedule(5, 1)
+    eager = tracker.track_ics(lat, ics, 45)
+    ctx.set_schedule(5, 6)
+    graph = tracker.track_ics(lat, ics, 45)
+    for a, b in zip(eager, graph):
+        assert np.array_equal(a, b)
+    ctx.set_schedule(saved["turns_per_block"], saved["graph_depth"])
+    lat.close()
+
+
+def test_tune_grid_picks_and_stores_a_schedule(ctx):
+    vs, offs = _variants(n_var=2, min_cells=6, max_cells=10)
+    lat = _upload(ctx, vs, offs)
+    kw = dict(n_angle=8, n_amp=16, n_pz=3, amp_max_x=0.02, amp_ratio_y=0.5, pz_min=-0.005, pz_max=0.005)
+    saved = ctx.schedule()
+    best = tracker.tune_grid(lat, 40, **kw)
+    print("tuned:", best, flush=True)
+    assert best["turns_per_block"] in (4, 8, 16, 32, 64, 128) and best["graph_depth"] in (1, 8)
+    assert best["probe_pet_per_s"] > 0
+    assert ctx.schedule()["turns_per_block"] == best["turns_per_block"]
+    tuned, _, _ = tracker.track_grid(lat, 40, turns_per_block=0, **kw)
+    ctx.set_schedule(32, 1)
+    ref, _, _ = tracker.track_grid(lat, 40, turns_per_block=0, **kw)
+    assert np.array_equal(tuned, ref)
+    ctx.set_schedule(saved["turns_per_block"], saved["graph_depth"])
+    lat.close()
