@@ -30,9 +30,10 @@ enum OpType : int32_t {
   OP_TOO_LARGE = 13,     // orbit_too_large check inside an element (early return)
   OP_RFCAVITY = 14,      // standing-wave cavity body (n_slice energy kicks and drifts)
   OP_APERTURE_IN = 15,   // aperture check at the entrance edge
-  OP_ELE_END = 16,       // orbit_too_large + exit aperture + loss bookkeeping
+  OP_ELE_END = 16,       // no body: only the element-end work of F_END
   OP_TILT = 17,          // tilt_coords rotation, c[0] = cos, c[1] = sin
   OP_KD_SLICES = 18,     // whole kick-drift body of a thick multipole: half kick, n x (drift, kick), last half
+  OP_SBEND_FUSED = 19,   // hard edge (e = 0, no fringe integral), orbit_too_large, exact body, orbit_too_large, hard edge
 };
 
 // op flags
@@ -40,9 +41,17 @@ enum : int32_t {
   F_APERTURE_ELLIPTICAL = 1,
   F_HWANG_ENTER = 2,      // OP_HWANG / OP_HARD_MULT_EDGE at the first track edge
   F_OFFSET_PITCH = 4,     // OP_OFFSET_*: general rotation, else pure translation
-  F_HAS_APERTURE = 16,    // OP_ELE_END: check the exit aperture
   F_OFFSET_MOVE = 32,     // OP_OFFSET_*: has a translation/pitch part
+  F_EDGE_EXIT = 64,       // OP_SBEND_FUSED: exit edge present (F_HWANG_ENTER: entrance edge present)
+  // the element ends after this op's body: orbit_too_large, then (with
+  // F_END_APERTURE) the exit aperture with its 10 constants at c[END_C ..]
+  F_END = 1 << 16,
+  F_END_APERTURE = 1 << 17,
+  F_END_ELLIPTICAL = 1 << 18,
 };
+
+// first of the folded aperture constants of an op carrying F_END
+constexpr int END_C = 4;
 
 struct alignas(64) DevOp {
   int32_t type;   // OpType

@@ -182,6 +182,38 @@ void emit_kicks(std::vector<DevOp>& prog, int ie, const AB& ab, double scale, in
   }
 }
 
+// A Hwang edge with e = 0 and no fringe integral: tan(e) = 0 and sec(e) = 1,
+// so gt, gt2, k1 tan(e) and the fringe factor vanish and gs2 = g_tot.
+bool simple_edge(const DevOp& op) {
+  return op.type == OP_HWANG && op.c[0] == 0 && op.c[1] == 0 && op.c[3] == 0 && op.c[4] == 0 && op.c[6] == 0 &&
+         op.c[2] == op.c[5];
+}
+
+// Replace [HWANG] TOO_LARGE BEND TOO_LARGE [HWANG] at prog[b ..] by one
+// OP_SBEND_FUSED when both edges are simple: the body constants stay in
+// c[0 .. 10], the entrance edge's g_tot goes to c[11], minus the exit
+// edge's to c[12] (the exit kick is the entrance kick with the sign of g_tot
+// flipped).
+void fuse_sbend(std::vector<DevOp>& prog, size_t b) {
+  size_t n = prog.size() - b, i = b;
+  if (n < 3 || n > 5) return;
+  bool in_edge = prog[i].type == OP_HWANG;
+  if (in_edge && !simple_edge(prog[i])) return;
+  i += in_edge;
+  if (i + 3 > prog.size() || prog[i].type != OP_TOO_LARGE || prog[i + 1].type != OP_BEND ||
+      prog[i + 2].type != OP_TOO_LARGE)
+    return;
+  size_t tail = i + 3;
+  bool out_edge = tail < prog.size();
+  if (out_edge && (tail + 1 != prog.size() || !simple_edge(prog[tail]))) return;
+  DevOp f = prog[i + 1];
+  f.type = OP_SBEND_FUSED;
+  if (in_edge) { f.flags |= F_HWANG_ENTER; f.c[11] = prog[b].c[5]; }
+  if (out_edge) { f.flags |= F_EDGE_EXIT; f.c[12] = -prog[tail].c[5]; }
+  prog.resize(b);
+  prog.push_back(f);
+}
+
 bool is_misaligned(const bmx_ele_rec& e) {
   return e.x_offset != 0 || e.y_offset != 0 || e.z_offset != 0 || e.x_pitch != 0 || e.y_pitch != 0;
 }
@@ -438,6 +470,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           op.c[6] = te;
           prog.push_back(op);
         };
+        size_t body_begin = prog.size();
         emit_edge(true);
         prog.push_back(make_op(OP_TOO_LARGE, ie));
         AB ab;
@@ -482,6 +515,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         }
         prog.push_back(make_op(OP_TOO_LARGE, ie));
         emit_edge(false);
+        fuse_sbend(prog, body_begin);
         break;
       }
       case BMX_KEY_RFCAVITY: {
@@ -511,12 +545,28 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         return fail("element key not supported");
     }
 
-    DevOp end = make_op(OP_ELE_END, ie);
-    if (cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS)) {
-      end.flags = F_HAS_APERTURE | ap_flags;
-      ap_consts(end);
+    // Element end (orbit_too_large, exit aperture): folded into the element's
+    // last op when that op leaves c[END_C ..] free, else an OP_ELE_END of its own.
+    bool exit_ap = cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS);
+    DevOp ap = make_op(OP_ELE_END, ie);
+    if (exit_ap) ap_consts(ap);
+    int end_flags = F_END | (exit_ap ? (F_END_APERTURE | ((ap_flags & F_APERTURE_ELLIPTICAL) ? F_END_ELLIPTICAL : 0)) : 0);
+    DevOp* last = (prog.size() > (size_t)ele_op_begin.back()) ? &prog.back() : nullptr;
+    bool foldable = false;
+    if (last && last->ie == ie && !(last->flags & F_END)) {
+      switch (last->type) {
+        case OP_DRIFT: case OP_QUAD: case OP_TILT: foldable = true; break;       // c[0], c[1]
+        case OP_KICK: foldable = 2 * (last->flags + 1) <= END_C; break;            // c[0 .. 2 nh + 1]
+        case OP_KD_SLICES: foldable = 1 + 2 * ((last->flags & 0xff) + 1) <= END_C; break;  // c[0 .. 2 nh + 2]
+        default: break;
+      }
     }
-    prog.push_back(end);
+    DevOp end = make_op(OP_ELE_END, ie);
+    DevOp* tgt = foldable ? last : &end;
+    tgt->flags |= end_flags;
+    if (exit_ap)
+      for (int k = 0; k < 10; k++) tgt->c[END_C + k] = ap.c[k];
+    if (!foldable) prog.push_back(end);
   }
   ele_op_begin.push_back((int32_t)prog.size());
   (void)kCLight;

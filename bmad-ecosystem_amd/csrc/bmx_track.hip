@@ -140,7 +140,8 @@ __device__ __forceinline__ void kick_horner(const double* c, int nh, int lo, dou
 }
 
 __device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, double& kr, double& ki) {
-  kick_horner(op.c, __builtin_amdgcn_readfirstlane(op.flags), __builtin_amdgcn_readfirstlane(op.n), x, y, kr, ki);
+  kick_horner(op.c, __builtin_amdgcn_readfirstlane(op.flags) & 0xff, __builtin_amdgcn_readfirstlane(op.n), x, y, kr,
+              ki);
 }
 
 __device__ __forceinline__ void op_kick(Lane& L, const DevOp& op) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ void op_kick(Lane& L, const DevOp& op) {
 __device__ __forceinline__ void op_kd_slices(Lane& L, const DevOp& op) {
   int n_step = __builtin_amdgcn_readfirstlane(op.n);
   int fl = __builtin_amdgcn_readfirstlane(op.flags);
-  int nh = fl & 0xff, lo = fl >> 8;
+  int nh = fl & 0xff, lo = (fl >> 8) & 0xff;
   double len = op.c[0];
   double kr, ki;
   kick_horner(&op.c[1], nh, lo, L.x, L.y, kr, ki);
@@ -603,25 +604,54 @@ __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
 
 // aperture part of check_aperture_limit (bmad/low_level/check_aperture_limit.f90:257-309)
 // c: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim, 4 wx, 5 cx, 6 wy, 7 cy, 8 1/wx, 9 1/wy
-__device__ __forceinline__ void aperture(Lane& L, const DevOp& op, int flags) {
-  if (flags & F_APERTURE_ELLIPTICAL) {
-    double xp = L.x - op.c[5], yp = L.y - op.c[7];
+__device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptical) {
+  if (elliptical) {
+    double xp = L.x - c[5], yp = L.y - c[7];
     // screen with the reciprocal widths: it is within a few ulp of the
     // reference's r, so r_fast < 1 - 1e-13 decides r < 1 exactly
-    double fx = xp * op.c[8], fy = yp * op.c[9];
+    double fx = xp * c[8], fy = yp * c[9];
     if (fx * fx + fy * fy < 1.0 - 1e-13) return;
-    double ax = xp / op.c[4], ay = yp / op.c[6];
+    double ax = xp / c[4], ay = yp / c[6];
     double r = ax * ax + ay * ay;
     if (r < 1.0) return;
     if (fabs(ax) > fabs(ay)) L.state = (xp > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
     else L.state = (yp > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
     return;
   }
-  double x1 = op.c[0], x2 = op.c[1], y1 = op.c[2], y2 = op.c[3], uf = 0.0, f;
+  double x1 = c[0], x2 = c[1], y1 = c[2], y2 = c[3], uf = 0.0, f;
   if (x1 != 0.0 && L.x < x1) { f = fabs((L.x - x1) / (x2 - x1)); if (f > uf) { L.state = BMX_LOST_NEG_X; uf = f; } }
   if (x2 != 0.0 && L.x > x2) { f = fabs((L.x - x2) / (x2 - x1)); if (f > uf) { L.state = BMX_LOST_POS_X; uf = f; } }
   if (y1 != 0.0 && L.y < y1) { f = fabs((L.y - y1) / (y2 - y1)); if (f > uf) { L.state = BMX_LOST_NEG_Y; uf = f; } }
   if (y2 != 0.0 && L.y > y2) { f = fabs((L.y - y2) / (y2 - y1)); if (f > uf) { L.state = BMX_LOST_POS_Y; uf = f; } }
+}
+
+// Hwang edge kick with e = 0 and no fringe integral (fringe_mod.f90:240-418
+// with tan(e) = 0, sec(e) = 1, fint_gap = 0): of the polynomial only the gs2
+// and g_tot terms remain, gs2 = g_tot.  gs = g_tot at the entrance, -g_tot at
+// the exit (the two edges differ only in those terms' signs).
+__device__ __forceinline__ void hwang_simple(Lane& L, double gs) {
+  double ef = L.irp, v1 = L.px, v2 = L.y;
+  double a = gs * v2 * v2;
+  L.x += a * ef * 0.5;
+  L.py -= gs * v1 * v2 * ef;
+  L.z -= ef * ef * 0.5 * v1 * a;
+}
+
+// Sector bend.  OP_BEND: the body step alone.  OP_SBEND_FUSED: the whole
+// element of track_a_bend (track_a_bend.f90:57-113) for simple edges and one
+// body step: entrance edge, orbit_too_large, body, orbit_too_large, exit
+// edge, returning at the first loss as the reference does.
+// c: 0 .. 10 as op_bend, 11 entrance-edge g_tot, 12 minus the exit-edge g_tot
+__device__ __forceinline__ void op_sbend(Lane& L, const DevOp& op, int flags, bool fused, const LatticeConsts& lc) {
+  if (fused) {
+    if (flags & F_HWANG_ENTER) hwang_simple(L, op.c[11]);
+    if (too_large(L, lc)) return;
+  }
+  op_bend(L, op, lc);
+  if (fused) {
+    if (L.state != BMX_ALIVE || too_large(L, lc)) return;
+    if (flags & F_EDGE_EXIT) hwang_simple(L, op.c[12]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -639,11 +669,8 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   if (tb & TB(OP_DRIFT)) op_drift(L, op.c[0]);
   if (tb & TB(OP_KICK)) op_kick(L, op);
   if (tb & TB(OP_KD_SLICES)) op_kd_slices(L, op);
-  if (tb & TB(OP_ELE_END)) {
-    if (L.state == BMX_ALIVE && !too_large(L, lc) && (flags & F_HAS_APERTURE)) aperture(L, op, flags);
-  }
   if (tb & TB(OP_QUAD)) op_quad(L, op, lc);
-  if (tb & TB(OP_BEND)) op_bend(L, op, lc);
+  if (tb & (TB(OP_BEND) | TB(OP_SBEND_FUSED))) op_sbend(L, op, flags, (tb & TB(OP_SBEND_FUSED)) != 0, lc);
   if (tb & TB(OP_HWANG)) op_hwang(L, op);
   if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, lc);
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
@@ -657,7 +684,13 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_OFFSET_SET)) op_offset(L, op, true);
     if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false);
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
-    if (tb & TB(OP_APERTURE_IN)) aperture(L, op, flags);
+    if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
+  }
+  // element end (an OP_ELE_END, or folded into the element's last op):
+  // orbit_too_large, then the exit aperture, for a particle still alive
+  if (flags & F_END) {
+    if (L.state == BMX_ALIVE && !too_large(L, lc) && (flags & F_END_APERTURE))
+      aperture(L, op.c + END_C, flags & F_END_ELLIPTICAL);
   }
 }
 #undef TB
@@ -667,74 +700,87 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
 // addresses, instead of each CU fetching every record from L2 through its
 // scalar cache on the dependent path of the element loop.  256 records are
 // 32 KiB: four workgroups of 256 lanes (occupancy 4) fit in a CU's 160 KiB.
-#ifndef BMX_LDS_OPS
-#define BMX_LDS_OPS 1
-#endif
 #ifndef BMX_OPS_CHUNK
 #define BMX_OPS_CHUNK 256
 #endif
 constexpr int OPS_CHUNK = BMX_OPS_CHUNK;
 
-// Walk of `turns` turns over ops [lo1, hi1) then [lo2, hi2) for one lane.
-// The whole workgroup calls it together (it holds barriers).  On return
-// `alive` is false for a lane lost on the way, with ie_lost / lost_turn set;
-// `ran` counts the ops of the lane's last turn it executed (for PET counts).
-__device__ __forceinline__ void walk(Lane& L, bool& alive, int& ie_lost, int& lost_turn, const DevOp* __restrict__ ops,
-                                     int lo1, int hi1, int lo2, int hi2, int turns, const LatticeConsts& lc,
-                                     DevOp* sops) {
-#if BMX_LDS_OPS
+// Particles per lane.  Each lane carries NP particles through the element
+// program together, so the op dispatch, the LDS reads of the op constants
+// and the loop control are paid once for NP particles, and the NP particle
+// updates of an op sit in one basic block where they can be interleaved.
+#ifndef BMX_NP
+#define BMX_NP 1
+#endif
+
+// The particles of one lane.  live[j]: still tracked.  A particle is lost at
+// the op whose step set its state: on_loss() gets its coordinates at that
+// moment, then it is parked at the reference orbit (coordinates zero, alive
+// state) so that it keeps taking the ordinary paths of every op while the
+// lane's other particles go on; its results are never read again.
+template <int NP>
+struct Pack {
+  Lane L[NP];
+  bool live[NP];
+  int ie_lost[NP], lost_turn[NP];
+  __device__ bool any() const {
+    bool a = false;
+    for (int j = 0; j < NP; j++) a |= live[j];
+    return a;
+  }
+};
+
+__device__ __forceinline__ void park(Lane& L, const LatticeConsts& lc) {
+  L.x = 0.0; L.px = 0.0; L.y = 0.0; L.py = 0.0; L.z = 0.0; L.pz = 0.0;
+  L.beta = lc.beta0;
+  L.irp = 1.0;
+  L.bb = 0.0;
+  L.state = BMX_ALIVE;
+}
+
+// Walk of `turns` turns over ops [lo1, hi1) then [lo2, hi2) for one lane's
+// particles.  The whole workgroup calls it together (it holds barriers); the
+// element program comes through LDS one chunk at a time.
+template <int NP, class OnLoss>
+__device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2,
+                                     int turns, const LatticeConsts& lc, DevOp* sops, OnLoss on_loss) {
   for (int t = 0; t < turns; t++) {
     for (int seg = 0; seg < 2; seg++) {
       int lo = seg ? lo2 : lo1, hi = seg ? hi2 : hi1;
       for (int c = lo; c < hi; c += OPS_CHUNK) {
         int m = min(OPS_CHUNK, hi - c);
-        // barrier: the previous chunk is consumed; leave together once the workgroup has no live lane
-        if (!__syncthreads_or(alive)) return;
+        // barrier: the previous chunk is consumed; leave together once the workgroup has no live particle
+        if (!__syncthreads_or(P.any())) return;
         const uint4* src = reinterpret_cast<const uint4*>(ops + c);
         uint4* dst = reinterpret_cast<uint4*>(sops);
         constexpr int W = (int)(sizeof(DevOp) / sizeof(uint4));
         for (int e = threadIdx.x; e < m * W; e += blockDim.x) dst[e] = src[e];
         __syncthreads();
-        if (alive) {
+        if (P.any()) {
           for (int k = 0; k < m; k++) {
             const DevOp& op = sops[k];
             int type = __builtin_amdgcn_readfirstlane(op.type);
             int flags = __builtin_amdgcn_readfirstlane(op.flags);
-            run_op(L, op, type, flags, lc);
-            if (L.state != BMX_ALIVE) {
-              alive = false;
-              ie_lost = op.ie;
-              lost_turn = t;
-              break;
+#pragma unroll
+            for (int j = 0; j < NP; j++) run_op(P.L[j], op, type, flags, lc);
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+              if (P.L[j].state != BMX_ALIVE) {
+                if (P.live[j]) {
+                  P.live[j] = false;
+                  P.ie_lost[j] = op.ie;
+                  P.lost_turn[j] = t;
+                  on_loss(j, P.L[j], t, op.ie);
+                }
+                park(P.L[j], lc);
+              }
             }
+            if (!P.any()) break;
           }
         }
       }
     }
   }
-#else
-  (void)sops;
-  for (int t = 0; alive && t < turns; t++) {
-    int k = lo1, end = hi1;
-    bool second = hi2 > lo2;
-    while (true) {
-      if (k >= end) {
-        if (!second) break;
-        k = lo2; end = hi2; second = false;
-        continue;
-      }
-      const DevOp& op = ops[k];
-      run_op(L, op, op.type, op.flags, lc);
-      if (L.state != BMX_ALIVE) {
-        alive = false;
-        ie_lost = op.ie;
-        lost_turn = t;
-        break;
-      }
-      k++;
-    }
-  }
-#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -748,33 +794,58 @@ struct SoA {
 // Waves per SIMD the register allocation targets (occupancy); the lane
 // program is FP64-latency bound, so occupancy is traded against spills.
 #ifndef BMX_WAVES_PER_EU
-#define BMX_WAVES_PER_EU 4
+#define BMX_WAVES_PER_EU (BMX_NP == 1 ? 4 : 2)
 #endif
 
-// One lane of a bunch: load, `n_turns` turns over [lo1, hi1) then [lo2, hi2),
-// store.  Lanes already lost on entry are left untouched.
-__device__ __forceinline__ void track_lane(int64_t i, bool valid, const DevOp* __restrict__ ops, int lo1, int hi1,
-                                           int lo2, int hi2, const LatticeConsts& lc, int n_turns, SoA& p,
-                                           unsigned long long* __restrict__ n_lost, DevOp* sops) {
-  Lane L;
-  L.state = valid ? p.state[i] : BMX_LOST;
-  bool entered = L.state == BMX_ALIVE;
-  bool alive = false;
-  int ie_lost = -1, lost_turn = 0;
-  if (entered) {
-    L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
-    refresh_invariants(L, lc);
-    alive = !too_large(L, lc);
-    if (!alive) ie_lost = ops[lo1].ie;
+// NP particles of a bunch per lane: particle j of lane `tid` in workgroup
+// `blk` is base + j * 256 + tid (coalesced per j).  Load, `n_turns` turns
+// over [lo1, hi1) then [lo2, hi2), store.  Particles already lost on entry are
+// left untouched; a particle lost on the way is stored at its loss.
+template <int NP>
+__device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const DevOp* __restrict__ ops, int lo1,
+                                            int hi1, int lo2, int hi2, const LatticeConsts& lc, int n_turns, SoA& p,
+                                            unsigned long long* __restrict__ n_lost, DevOp* sops) {
+  Pack<NP> P;
+  int64_t idx[NP];
+  bool entered[NP];
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    int64_t i = base + (int64_t)j * blockDim.x + threadIdx.x;
+    idx[j] = i;
+    entered[j] = i < end && p.state[i] == BMX_ALIVE;
+    P.live[j] = false;
+    P.ie_lost[j] = -1;
+    P.lost_turn[j] = 0;
+    park(P.L[j], lc);
+    if (entered[j]) {
+      Lane& L = P.L[j];
+      L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
+      refresh_invariants(L, lc);
+      P.live[j] = !too_large(L, lc);
+      if (!P.live[j]) {  // lost before the first element
+        p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
+        p.state[i] = L.state;
+        if (p.ix_lost) p.ix_lost[i] = ops[lo1].ie;
+        if (p.turn_lost) p.turn_lost[i] = 0;
+        atomicAdd(n_lost, 1ull);
+        park(L, lc);
+      }
+    }
   }
-  walk(L, alive, ie_lost, lost_turn, ops, lo1, hi1, lo2, hi2, n_turns, lc, sops);
-  if (!entered) return;
-  p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
-  p.state[i] = L.state;
-  if (L.state != BMX_ALIVE) {
-    if (p.ix_lost) p.ix_lost[i] = ie_lost;
-    if (p.turn_lost) p.turn_lost[i] = lost_turn;
+  walk<NP>(P, ops, lo1, hi1, lo2, hi2, n_turns, lc, sops, [&](int j, const Lane& L, int t, int ie) {
+    int64_t i = idx[j];
+    p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
+    p.state[i] = L.state;
+    if (p.ix_lost) p.ix_lost[i] = ie;
+    if (p.turn_lost) p.turn_lost[i] = t;
     atomicAdd(n_lost, 1ull);
+  });
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    if (!P.live[j]) continue;
+    int64_t i = idx[j];
+    const Lane& L = P.L[j];
+    p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
   }
 }
 
@@ -782,9 +853,9 @@ __device__ __forceinline__ void track_lane(int64_t i, bool valid, const DevOp* _
 __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_bunch_kernel(
     const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2, LatticeConsts lc, int n_turns, SoA p,
     int64_t n, unsigned long long* __restrict__ n_lost) {
-  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  track_lane(i, i < n, ops, lo1, hi1, lo2, hi2, lc, n_turns, p, n_lost, sops);
+  __shared__ DevOp sops[OPS_CHUNK];
+  int64_t base = (int64_t)blockIdx.x * blockDim.x * BMX_NP;
+  track_lanes<BMX_NP>(base, n, ops, lo1, hi1, lo2, hi2, lc, n_turns, p, n_lost, sops);
 }
 
 // Particles of several variants in one launch: particles [off[v], off[v+1])
@@ -792,10 +863,11 @@ __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_bunch_kernel(
 __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_track_multi_kernel(
     const DevOp* __restrict__ ops, const int32_t* __restrict__ var_op_begin, const int64_t* __restrict__ off,
     LatticeConsts lc, int n_turns, SoA p, unsigned long long* __restrict__ n_lost) {
-  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
+  __shared__ DevOp sops[OPS_CHUNK];
   int v = blockIdx.y;
-  int64_t i = off[v] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  track_lane(i, i < off[v + 1], ops, var_op_begin[v], var_op_begin[v + 1], 0, 0, lc, n_turns, p, n_lost, sops);
+  int64_t base = off[v] + (int64_t)blockIdx.x * blockDim.x * BMX_NP;
+  track_lanes<BMX_NP>(base, off[v + 1], ops, var_op_begin[v], var_op_begin[v + 1], 0, 0, lc, n_turns, p, n_lost,
+                      sops);
 }
 
 // Batched grid tracking in turn blocks with lane compaction.  Particles of
@@ -863,50 +935,73 @@ __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
     LatticeConsts lc, int n_ic, GridSoA src, GridSoA dst, const int32_t* __restrict__ count_in,
     int32_t* __restrict__ count_out, const int32_t* __restrict__ d_turn0, int turns_per_block, int n_turns,
     int32_t* __restrict__ survival, unsigned long long* __restrict__ pet) {
-  __shared__ DevOp sops[BMX_LDS_OPS ? OPS_CHUNK : 1];
+  __shared__ DevOp sops[OPS_CHUNK];
   // the block's first turn lives in device memory so that captured graphs of
   // several blocks replay unchanged; every thread reads the same value, so a
   // finished scan leaves the whole grid together
   const int turn0 = *d_turn0;
   const int turns = min(turns_per_block, n_turns - turn0);
   if (turns <= 0) return;
+  constexpr int NP = BMX_NP;
   int v = blockIdx.y;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = i < count_in[v];
-  bool alive = false;
   int64_t base = (int64_t)v * n_ic;
-  Lane L;
-  int ic = 0, turn = 0, ie_lost = -1;
-  if (active) {
-    int64_t k = base + i;
-    L.x = src.x[k]; L.px = src.px[k]; L.y = src.y[k]; L.py = src.py[k]; L.z = src.z[k]; L.pz = src.pz[k];
-    L.beta = src.beta[k];
-    ic = src.ic[k];
-    L.state = BMX_ALIVE;
-    refresh_invariants(L, lc);
-    alive = !(turn0 == 0 && too_large(L, lc));
+  int n_in = count_in[v];
+  Pack<NP> P;
+  int slot[NP], ic[NP];
+  bool active[NP];
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    slot[j] = (blockIdx.x * NP + j) * blockDim.x + threadIdx.x;
+    active[j] = slot[j] < n_in;
+    ic[j] = 0;
+    P.live[j] = false;
+    P.ie_lost[j] = -1;
+    P.lost_turn[j] = 0;
+    park(P.L[j], lc);
+    if (active[j]) {
+      int64_t k = base + slot[j];
+      Lane& L = P.L[j];
+      L.x = src.x[k]; L.px = src.px[k]; L.y = src.y[k]; L.py = src.py[k]; L.z = src.z[k]; L.pz = src.pz[k];
+      L.beta = src.beta[k];
+      ic[j] = src.ic[k];
+      refresh_invariants(L, lc);
+      P.live[j] = !(turn0 == 0 && too_large(L, lc));
+      if (!P.live[j]) {
+        survival[base + ic[j]] = turn0;
+        park(L, lc);
+      }
+    }
   }
   int lo = var_op_begin[v], hi = var_op_begin[v + 1];
-  walk(L, alive, ie_lost, turn, ops, lo, hi, 0, 0, turns, lc, sops);
-  if (active && alive) turn = turns;
-  if (active && !alive) survival[base + ic] = turn0 + turn;
+  walk<NP>(P, ops, lo, hi, 0, 0, turns, lc, sops,
+           [&](int j, const Lane&, int t, int) { survival[base + ic[j]] = turn0 + t; });
   // live particle-element-turns of this lane
   unsigned long long my_pet = 0;
-  if (active) my_pet = (unsigned long long)turn * (unsigned long long)var_n_ele[v] + (alive ? 0ull : (unsigned long long)(ie_lost + 1));
-  // wave-level compaction of survivors
-  unsigned long long mask = __ballot(alive);
+#pragma unroll
+  for (int j = 0; j < NP; j++)
+    if (active[j]) {
+      unsigned long long tr = P.live[j] ? (unsigned long long)turns : (unsigned long long)P.lost_turn[j];
+      my_pet += tr * (unsigned long long)var_n_ele[v] + (P.live[j] ? 0ull : (unsigned long long)(P.ie_lost[j] + 1));
+    }
+  // wave-level compaction of survivors, one ballot per particle slot
   int lane = __lane_id();
-  unsigned long long below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-  int n_alive = __popcll(mask);
-  int first = __ffsll((long long)mask) - 1;
-  int slot0 = 0;
-  if (n_alive > 0 && lane == first) slot0 = atomicAdd(&count_out[v], n_alive);
-  slot0 = __shfl(slot0, first < 0 ? 0 : first);
-  if (alive) {
-    int64_t k = base + slot0 + __popcll(below);
-    dst.x[k] = L.x; dst.px[k] = L.px; dst.y[k] = L.y; dst.py[k] = L.py; dst.z[k] = L.z; dst.pz[k] = L.pz;
-    dst.beta[k] = L.beta;
-    dst.ic[k] = ic;
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    bool keep = active[j] && P.live[j];
+    unsigned long long mask = __ballot(keep);
+    unsigned long long below = (lane == 0) ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    int n_alive = __popcll(mask);
+    int first = __ffsll((long long)mask) - 1;
+    int slot0 = 0;
+    if (n_alive > 0 && lane == first) slot0 = atomicAdd(&count_out[v], n_alive);
+    slot0 = __shfl(slot0, first < 0 ? 0 : first);
+    if (keep) {
+      int64_t k = base + slot0 + __popcll(below);
+      const Lane& L = P.L[j];
+      dst.x[k] = L.x; dst.px[k] = L.px; dst.y[k] = L.y; dst.py[k] = L.py; dst.z[k] = L.z; dst.pz[k] = L.pz;
+      dst.beta[k] = L.beta;
+      dst.ic[k] = ic[j];
+    }
   }
   for (int off = 32; off > 0; off >>= 1) my_pet += __shfl_xor(my_pet, off);
   if (lane == 0 && my_pet) atomicAdd(pet, my_pet);
@@ -1214,7 +1309,7 @@ int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, 
   Staged st;
   if (stage_particles(ctx, p, st) != 0) return -1;
   const int block = 256;
-  int64_t grid = (n + block - 1) / block;
+  int64_t grid = (n + block * BMX_NP - 1) / (block * BMX_NP);
   (void)hipEventRecord(ctx->ev0, ctx->stream);
   hipLaunchKernelGGL(bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block), 0, ctx->stream, lat->d_ops, lo1, hi1,
                      lo2, hi2, lat->lc, n_turns, st.s, n, st.d_lost);
@@ -1248,7 +1343,8 @@ int32_t bmx_track_multi(bmx_ctx* ctx, const bmx_lattice* lat, const int64_t* par
   (void)hipMemcpyAsync(d_off, part_offsets, (nv + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream);
   const int block = 256;
   (void)hipEventRecord(ctx->ev0, ctx->stream);
-  hipLaunchKernelGGL(bmx_track_multi_kernel, dim3((unsigned)((max_count + block - 1) / block), (unsigned)nv),
+  hipLaunchKernelGGL(bmx_track_multi_kernel,
+                     dim3((unsigned)((max_count + block * BMX_NP - 1) / (block * BMX_NP)), (unsigned)nv),
                      dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin, d_off, lat->lc, n_turns, st.s,
                      st.d_lost);
   hipError_t le = hipGetLastError();
@@ -1319,7 +1415,7 @@ static int run_survival(bmx_ctx* ctx, const bmx_lattice* lat, SurvivalRun& R, st
   int cur = 0, turn0 = 0, max_live = 0;
   for (int v = 0; v < nv; v++) max_live = std::max(max_live, h_count[v]);
   auto step = [&](int p, int live) {
-    dim3 gd((unsigned)((live + block - 1) / block), (unsigned)nv);
+    dim3 gd((unsigned)((live + block * BMX_NP - 1) / (block * BMX_NP)), (unsigned)nv);
     hipLaunchKernelGGL(bmx_grid_block_kernel, gd, dim3(block), 0, ctx->stream, lat->d_ops, lat->d_var_op_begin,
                        lat->d_var_n_ele, lat->lc, R.stride, R.st[p], R.st[1 - p], R.d_count + p * nv,
                        R.d_count + (1 - p) * nv, R.d_turn0, tpb, n_turns, R.d_surv, R.d_pet);
