@@ -50,15 +50,71 @@ struct Lane {
   int state;
 };
 
-// FP64 VALU instructions take no 64-bit literal, so every series coefficient
-// lives in a register pair.  Left alone, the compiler hoists all of them out
-// of the element loop into VGPRs and spills them to scratch; K() pins a
-// coefficient to an SGPR pair at its point of use instead (an empty asm with
-// an "s" operand that cannot be moved out of the loop).
-__device__ __forceinline__ double K(double c) {
-  asm volatile("" : "+s"(c));
+// Series coefficients.  FP64 VALU instructions take no 64-bit literal, so a
+// coefficient has to sit in a register pair.  Left alone, the compiler hoists
+// every one of them out of the element loop into VGPRs and spills them.
+// Instead each series reads its coefficients at the point of use through the
+// scalar cache: series() hands out the table address through an empty asm,
+// so the loads cannot be hoisted and become a few s_load_dwordx8 into
+// SGPRs, and sg() keeps each loaded value in its SGPR pair, where the FMA
+// takes it as an operand.  Per block: the highest-order coefficient last.
+typedef __attribute__((address_space(4))) const double cdouble;
+enum : int {
+  CS_DRIFT = 0,   // 5/16, 35/128, 63/256: binomial series of 1/sqrt(1-u) - 1 from u^3
+  CS_QUAD = 3,    // (4m)!, (4m+2)!, (4m+1)!, (4m+3)! reciprocals, m = 0..5
+  CS_ASIN = 27,   // asin(s) = s + s^3 P(s^2)
+  CS_SIN = 33,    // sin(e) = e + e^3 P(e^2)
+  CS_COS = 38,    // cos(e) = 1 - e^2/2 + e^4 P(e^2)
+  CS_ATAN = 43,   // atan(t) = t + t^3 P(t^2)
+  CS_ISINC = 49,  // x / sin(x) = 1 + x^2 P(x^2)
+  CS_END = 56
+};
+__constant__ double kSeries[CS_END] = {
+    // CS_DRIFT
+    0.3125, 0.2734375, 0.24609375,
+    // CS_QUAD
+    1.0, 0.5, 1.0, 1.0 / 6,
+    1.0 / 24, 1.0 / 720, 1.0 / 120, 1.0 / 5040,
+    1.0 / 40320, 1.0 / 3628800, 1.0 / 362880, 1.0 / 39916800,
+    1.0 / 479001600, 1.0 / 87178291200.0, 1.0 / 6227020800.0, 1.0 / 1307674368000.0,
+    1.0 / 20922789888000.0, 1.0 / 6402373705728000.0, 1.0 / 355687428096000.0, 1.0 / 121645100408832000.0,
+    1.0 / 2432902008176640000.0, 1.0 / 1124000727777607680000.0, 1.0 / 51090942171709440000.0,
+    1.0 / 25852016738884976640000.0,
+    // CS_ASIN
+    1.0 / 6, 3.0 / 40, 5.0 / 112, 35.0 / 1152, 63.0 / 2816, 231.0 / 13312,
+    // CS_SIN
+    -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880, -1.0 / 39916800,
+    // CS_COS
+    1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800, 1.0 / 479001600,
+    // CS_ATAN
+    -1.0 / 3, 1.0 / 5, -1.0 / 7, 1.0 / 9, -1.0 / 11, 1.0 / 13,
+    // CS_ISINC
+    1.0 / 6, 7.0 / 360, 31.0 / 15120, 127.0 / 604800, 73.0 / 3421440, 1414477.0 / 653837184000.0,
+    8191.0 / 37362124800.0,
+};
+
+__device__ __forceinline__ const cdouble* series(int block) {
+  const cdouble* p = (const cdouble*)kSeries + block;
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+__device__ __forceinline__ double sg(double c) {
+  asm("" : "+s"(c));
   return c;
 }
+
+// c[0] + x (c[1] + x (... + x c[N-1]))
+template <int N>
+__device__ __forceinline__ double poly(const cdouble* c, double x) {
+  double a = sg(c[N - 1]);
+#pragma unroll
+  for (int i = N - 2; i >= 0; i--) a = a * x + sg(c[i]);
+  return a;
+}
+
+// true on every lane of the wave when b holds on any active lane
+__device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
 // sqrt_one (sim_utils/special_functions/sqrt_one.f90:18-41)
 __device__ __forceinline__ double d_sqrt_one(double x) { return x / (sqrt(1.0 + x) + 1.0); }
@@ -90,21 +146,36 @@ __device__ __forceinline__ double d_modulo2(double x, double amp) {
 // For u < 1e-4 (angles below 10 mrad) S is summed as its binomial series to
 // u^5 (truncation < 3e-25), which needs no division or square root; larger
 // angles take the exact form.
-__device__ __forceinline__ void op_drift(Lane& L, double len) {
-  double pxr = L.px * L.irp, pyr = L.py * L.irp;
-  double u = pxr * pxr + pyr * pyr;
-  double S;
-  if (u < 1e-4) {
-    S = u * (0.5 + u * (0.375 + u * (K(0.3125) + u * (K(0.2734375) + u * K(0.24609375)))));
-  } else {
-    if (u >= 1.0) { L.state = BMX_LOST_PZ; return; }
-    double ps = sqrt(1.0 - u);
-    S = u / (ps * (ps + 1.0));
-  }
+__device__ __forceinline__ void drift_apply(Lane& L, double len, double pxr, double pyr, double S) {
   double lps = len * (1.0 + S);
   L.x += lps * pxr;
   L.y += lps * pyr;
   L.z += len * (L.bb - S);
+}
+
+// The series is evaluated for every lane; only a wave holding a lane at
+// u >= 1e-4 takes the exact form (and the lost_pz test, u >= 1, which leaves
+// the particle where it is) for those lanes.
+__device__ __forceinline__ void op_drift(Lane& L, double len) {
+  double pxr = L.px * L.irp, pyr = L.py * L.irp;
+  double u = pxr * pxr + pyr * pyr;
+  double S = u * (0.5 + u * (0.375 + u * poly<3>(series(CS_DRIFT), u)));
+  bool wide = u >= 1e-4;
+  if (wave_any(wide)) {
+    bool move = true;
+    if (wide) {
+      if (u >= 1.0) {
+        L.state = BMX_LOST_PZ;
+        move = false;
+      } else {
+        double ps = sqrt(1.0 - u);
+        S = u / (ps * (ps + 1.0));
+      }
+    }
+    if (move) drift_apply(L, len, pxr, pyr, S);
+    return;
+  }
+  drift_apply(L, len, pxr, pyr, S);
 }
 
 // drift with include_ref_motion = false (offset_particle's drift to the edge)
@@ -118,24 +189,33 @@ __device__ __forceinline__ void op_drift_noref(Lane& L, double len) {
   L.z += -len * ips;
 }
 
+// (kr + i ki) *= (ar + i ai)
+__device__ __forceinline__ void cmul(double& kr, double& ki, double ar, double ai) {
+  double tr = kr * ar - ki * ai;
+  double ti = kr * ai + ki * ar;
+  kr = tr;
+  ki = ti;
+}
+
 // Thin multipole kick, ab form (bmad/modules/multipole_mod.f90:82-174, :314-401):
 // K = sum_n c_n (x + i y)^n, dpx = -Re K, dpy = Im K, Horner in complex FP64.
-// c holds nh + 1 complex coefficients for orders lo .. lo + nh.
+// c holds nh + 1 complex coefficients for orders lo .. lo + nh; the common
+// factor (x + i y)^lo is applied as powers of (x + i y)^2, without a loop for
+// lo <= 3 (nh and lo are wave-uniform).
 __device__ __forceinline__ void kick_horner(const double* c, int nh, int lo, double x, double y, double& kr,
                                             double& ki) {
   kr = c[2 * nh];
   ki = c[2 * nh + 1];
   for (int j = nh - 1; j >= 0; j--) {
-    double tr = kr * x - ki * y + c[2 * j];
-    double ti = kr * y + ki * x + c[2 * j + 1];
-    kr = tr;
-    ki = ti;
+    cmul(kr, ki, x, y);
+    kr += c[2 * j];
+    ki += c[2 * j + 1];
   }
-  for (int j = 0; j < lo; j++) {  // multiply by (x + i y)^lo
-    double tr = kr * x - ki * y;
-    double ti = kr * y + ki * x;
-    kr = tr;
-    ki = ti;
+  if (lo & 1) cmul(kr, ki, x, y);
+  if (lo >= 2) {
+    double wr = x * x - y * y, wi = 2.0 * x * y;  // (x + i y)^2
+    cmul(kr, ki, wr, wi);
+    for (int m = lo - 2; m >= 2; m -= 2) cmul(kr, ki, wr, wi);
   }
 }
 
@@ -206,21 +286,24 @@ __device__ __forceinline__ double low_energy_dz(const Lane& L, const LatticeCons
 // With w = k L^2 for a plane of strength k: c = sum w^n/(2n)!, s/L = sum w^n/(2n+1)!
 // (cosh / sinh for k > 0, cos / sin for k < 0).  For |w| <= 1 the series to
 // n = 11 is exact to < 1e-21 and replaces sincos, expm1, sqrt and three
-// divisions; stronger quadrupoles take the exact trigonometric form.
-__device__ __forceinline__ void quad_cs_series(double w, double len, double& c, double& s) {
-  double cc = K(1.0 / 1124000727777607680000.0), ss = K(1.0 / 25852016738884976640000.0);
-  cc = cc * w + K(1.0 / 2432902008176640000.0);   ss = ss * w + K(1.0 / 51090942171709440000.0);
-  cc = cc * w + K(1.0 / 6402373705728000.0);      ss = ss * w + K(1.0 / 121645100408832000.0);
-  cc = cc * w + K(1.0 / 20922789888000.0);        ss = ss * w + K(1.0 / 355687428096000.0);
-  cc = cc * w + K(1.0 / 87178291200.0);           ss = ss * w + K(1.0 / 1307674368000.0);
-  cc = cc * w + K(1.0 / 479001600);               ss = ss * w + K(1.0 / 6227020800.0);
-  cc = cc * w + K(1.0 / 3628800);                 ss = ss * w + K(1.0 / 39916800);
-  cc = cc * w + K(1.0 / 40320);                   ss = ss * w + K(1.0 / 362880);
-  cc = cc * w + K(1.0 / 720);                     ss = ss * w + K(1.0 / 5040);
-  cc = cc * w + K(1.0 / 24);                      ss = ss * w + K(1.0 / 120);
-  cc = cc * w + 0.5;                              ss = ss * w + K(1.0 / 6);
-  c = cc * w + 1.0;
-  s = (ss * w + 1.0) * len;
+// divisions; stronger quadrupoles take the exact trigonometric form.  The two
+// planes have w and -w: with v = w^2 each series splits into even and odd
+// parts in v, c(+-w) = E(v) +- w O(v), so both planes cost one pass.
+__device__ __forceinline__ void quad_cs_pair(double w, double len, double& cp, double& sp, double& cm, double& sm) {
+  const cdouble* q = series(CS_QUAD);
+  double v = w * w;
+  double ec = sg(q[20]), oc = sg(q[21]), es = sg(q[22]), os = sg(q[23]);
+#pragma unroll
+  for (int m = 4; m >= 0; m--) {
+    ec = ec * v + sg(q[4 * m]);
+    oc = oc * v + sg(q[4 * m + 1]);
+    es = es * v + sg(q[4 * m + 2]);
+    os = os * v + sg(q[4 * m + 3]);
+  }
+  cp = ec + w * oc;
+  cm = ec - w * oc;
+  sp = (es + w * os) * len;
+  sm = (es - w * os) * len;
 }
 
 __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeConsts& lc) {
@@ -231,8 +314,7 @@ __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeC
   double cx, sx, cy, sy;
   double w = k1 * len * len;
   if (fabs(w) <= 1.0) {
-    quad_cs_series(-w, len, cx, sx);
-    quad_cs_series(w, len, cy, sy);
+    quad_cs_pair(w, len, cy, sy, cx, sx);
   } else {
     double sqrt_k = sqrt(fabs(k1));
     double skl = sqrt_k * len;
@@ -269,34 +351,31 @@ __device__ __forceinline__ double d_sinc(double x) { return (fabs(x) < 1e-8) ? 1
 // asin(s), |s| <= 0.05, to s^13 (truncation < 1e-19 relative).
 __device__ __forceinline__ double asin_small(double s) {
   double s2 = s * s;
-  return s + s * s2 * (K(1.0 / 6) + s2 * (K(3.0 / 40) + s2 * (K(5.0 / 112) + s2 * (K(35.0 / 1152) + s2 * (K(63.0 / 2816) +
-             s2 * (K(231.0 / 13312)))))));
+  return s + s * s2 * poly<6>(series(CS_ASIN), s2);
 }
 
 // sin(e), |e| <= 0.2, to e^11 (truncation < 1e-18 relative).
 __device__ __forceinline__ double sin_small(double e) {
   double e2 = e * e;
-  return e + e * e2 * (K(-1.0 / 6) + e2 * (K(1.0 / 120) - e2 * (K(1.0 / 5040) - e2 * (K(1.0 / 362880) - e2 * (K(1.0 / 39916800))))));
+  return e + e * e2 * poly<5>(series(CS_SIN), e2);
 }
 
 // cos(e), |e| <= 0.2, to e^12 (truncation < 1e-20).
 __device__ __forceinline__ double cos_small(double e) {
   double e2 = e * e;
-  return 1.0 + e2 * (-0.5 + e2 * (K(1.0 / 24) - e2 * (K(1.0 / 720) - e2 * (K(1.0 / 40320) - e2 * (K(1.0 / 3628800) -
-               e2 * (K(1.0 / 479001600)))))));
+  return 1.0 + e2 * (-0.5 + e2 * poly<5>(series(CS_COS), e2));
 }
 
 // atan(t), |t| <= 0.2, to t^13 (truncation < 1e-11 * t^14, < 2e-21 relative).
 __device__ __forceinline__ double atan_small(double t) {
   double t2 = t * t;
-  return t + t * t2 * (K(-1.0 / 3) + t2 * (K(1.0 / 5) - t2 * (K(1.0 / 7) - t2 * (K(1.0 / 9) - t2 * (K(1.0 / 11) - t2 * (K(1.0 / 13)))))));
+  return t + t * t2 * poly<6>(series(CS_ATAN), t2);
 }
 
 // x / sin(x) = 1 / sinc(x), |x| <= 0.1, to x^14 (truncation < 1e-20).
 __device__ __forceinline__ double inv_sinc_small(double x) {
   double x2 = x * x;
-  return 1.0 + x2 * (K(1.0 / 6) + x2 * (K(7.0 / 360) + x2 * (K(31.0 / 15120) + x2 * (K(127.0 / 604800) + x2 * (K(73.0 / 3421440) +
-         x2 * (K(1414477.0 / 653837184000.0) + x2 * (K(8191.0 / 37362124800.0))))))));
+  return 1.0 + x2 * poly<7>(series(CS_ISINC), x2);
 }
 
 // The reference's transcendental calls for the bend entry and exit angles
@@ -654,6 +733,33 @@ __device__ __forceinline__ void op_sbend(Lane& L, const DevOp& op, int flags, bo
   }
 }
 
+// Element end (an OP_ELE_END, or folded into the element's last op):
+// orbit_too_large, then the exit aperture, for a particle still alive.  A
+// straight-line screen decides "alive, inside every limit" for the whole
+// wave; only when some lane fails it do the reference's ordered tests run.
+// The elliptical screen is the one of aperture(); a NaN coordinate fails
+// every screen and takes the ordered tests, as in the reference.
+// c: the aperture constants (as aperture()).
+__device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, const LatticeConsts& lc) {
+  double mal = lc.max_aperture_limit;
+  double rel_p = 1.0 + L.pz;
+  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
+  bool inside = (fabs(L.x) <= mal) & (fabs(L.y) <= mal) & (rel_p >= 0.0) & (f <= 0.0);
+  if (flags & F_END_APERTURE) {
+    if (flags & F_END_ELLIPTICAL) {
+      double fx = (L.x - c[5]) * c[8], fy = (L.y - c[7]) * c[9];
+      inside &= fx * fx + fy * fy < 1.0 - 1e-13;
+    } else {
+      inside &= (c[0] == 0.0 || L.x >= c[0]) & (c[1] == 0.0 || L.x <= c[1]) & (c[2] == 0.0 || L.y >= c[2]) &
+                (c[3] == 0.0 || L.y <= c[3]);
+    }
+  }
+  bool check = (L.state == BMX_ALIVE) & !inside;
+  if (wave_any(check)) {
+    if (check && !too_large(L, lc) && (flags & F_END_APERTURE)) aperture(L, c, flags & F_END_ELLIPTICAL);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // one op on one lane
 // ---------------------------------------------------------------------------
@@ -686,12 +792,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
     if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
   }
-  // element end (an OP_ELE_END, or folded into the element's last op):
-  // orbit_too_large, then the exit aperture, for a particle still alive
-  if (flags & F_END) {
-    if (L.state == BMX_ALIVE && !too_large(L, lc) && (flags & F_END_APERTURE))
-      aperture(L, op.c + END_C, flags & F_END_ELLIPTICAL);
-  }
+  if (flags & F_END) ele_end(L, op.c + END_C, flags, lc);
 }
 #undef TB
 
