@@ -508,7 +508,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
                                  : (angle - std::sin(angle)) / (angle * x2);
             op.c[0] = step_len; op.c[1] = g; op.c[2] = g_tot; op.c[3] = angle;
             op.c[4] = std::cos(angle); op.c[5] = std::sin(angle); op.c[6] = sinc_a; op.c[7] = cosc_a;
-            op.c[8] = sincc_a; op.c[9] = lc->beta0; op.c[10] = dg;
+            op.c[8] = sincc_a; op.c[9] = lc->beta0; op.c[10] = dg; op.c[13] = step_len / lc->beta0;
             prog.push_back(op);
           }
           if (ix_mag_max > -1) emit_kicks(prog, ie, abk, (n == n_step ? 0.5 : 1.0) * r_step, OP_BEND_KICK, g);

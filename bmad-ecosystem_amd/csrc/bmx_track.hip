@@ -398,33 +398,74 @@ __device__ __noinline__ BendExit bend_exit_exact(double phi_1, double L_u, doubl
   return BendExit{L_c / d_sinc(angle_p / 2.0), sin(phi_1 + angle - angle_p)};
 }
 
+// Square root, reciprocal and quotient for operands known to lie well inside
+// the normal range (the bend's common path checks that they do): the IEEE
+// sequences the compiler emits for sqrt() and '/', without their range
+// scaling and special-value fix-ups.  From v_rsq / v_rcp (relative error
+// below 2^-22) two Newton steps and an FMA residual correction give the
+// correctly rounded result.
+__device__ __forceinline__ double sqrt_mid(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
+__device__ __forceinline__ double rcp_iter(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rcp_mid(double b) {
+  double r = rcp_iter(b);
+  return fma(fma(-b, r, 1.0), r, r);
+}
+
+__device__ __forceinline__ double div_mid(double a, double b) {
+  double r = rcp_iter(b);
+  double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+
 // Exact sector-bend body step (track_a_bend.f90:120-209), in the reference's
 // order of operations: phi_1 = asin(px/pt), sin/cos(angle + phi_1), the exit
 // offset xi, angle_p = 2 (angle + phi_1 - atan2(L_u, -L_v)),
 // px = pt sin(phi_1 + angle - angle_p), L_p = L_c / sinc(angle_p / 2).
-// The transcendental calls become the short series above when their
-// arguments are small (the usual case); otherwise the libm calls are made.
+// This is the general form, with every branch of the reference and the libm
+// calls for large angles; a wave runs it when any of its lanes leaves the
+// common path of op_bend.  px/pt is taken as px * (1/pt) and beta sl /
+// beta_ref as beta (sl / beta_ref), each within an ulp of the reference's
+// quotient, so that on the common path both forms give the same bits.
 // c: 0 step_len, 1 g, 2 g_tot, 3 angle, 4 cos a, 5 sin a, 6 sinc a, 7 cosc a, 8 sincc a, 9 beta_ref, 10 dg
-__device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeConsts& lc) {
-  double sl = op.c[0], g = op.c[1], g_tot = op.c[2], angle = op.c[3];
-  double cos_a = op.c[4], sin_a = op.c[5], sinc_a = op.c[6], cosc_a = op.c[7];
+__device__ __forceinline__ Lane bend_general(Lane L, const double* c, double p0c, double mc2sq, double mal,
+                                          double two_mal) {
+  double sl = c[0], g = c[1], g_tot = c[2], angle = c[3];
+  double cos_a = c[4], sin_a = c[5], sinc_a = c[6], cosc_a = c[7];
   double x = L.x, px = L.px, py = L.py, pz = L.pz;
   double rel_p = 1.0 + pz;
-  if (op.c[10] == 0.0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 && fabs(pz) < 1e-9) {
-    double pc = rel_p * lc.p0c;
-    double gam2 = lc.mc2sq / (pc * pc + lc.mc2sq);
-    double m56 = sl * (gam2 - (g * sl) * (g * sl) * op.c[8]);
+  if (c[10] == 0.0 && fabs(x * g) < 1e-9 && fabs(px) < 1e-9 && fabs(py) < 1e-9 && fabs(pz) < 1e-9) {
+    double pc = rel_p * p0c;
+    double gam2 = mc2sq / (pc * pc + mc2sq);
+    double m56 = sl * (gam2 - (g * sl) * (g * sl) * c[8]);
     L.x = cos_a * x + sl * sinc_a * px + g * sl * sl * cosc_a * pz;
     L.px = -g * sin_a * x + cos_a * px + g * sl * sinc_a * pz;
     L.y = L.y + sl * py;
     L.z = -g * sl * sinc_a * x - g * sl * sl * cosc_a * px + L.z + m56 * pz;
-    return;
+    return L;
   }
   double pt = sqrt(rel_p * rel_p - py * py);
-  if (fabs(px) > pt) { L.state = BMX_LOST_PZ; return; }
+  if (fabs(px) > pt) { L.state = BMX_LOST_PZ; return L; }
   double ipt = 1.0 / pt;
   double g_p = g_tot * ipt;
-  double sphi = px / pt;
+  double sphi = px * ipt;
   double phi_1, sin_plus, cos_plus;
   if (fabs(sphi) <= 0.05 && fabs(angle) <= 0.15) {
     phi_1 = asin_small(sphi);
@@ -442,8 +483,8 @@ __device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeC
   double r = cos_plus * cos_plus + g_p * alpha;
   if (r < 0.0 || (fabs(g_p) < 1e-5 && fabs(cos_plus) < 1e-5)) {
     L.state = BMX_LOST;
-    L.x = lc.two_max_ap;
-    return;
+    L.x = two_mal;
+    return L;
   }
   double rad = sqrt(r);
   double xi;
@@ -451,7 +492,7 @@ __device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeC
   else xi = (rad - cos_plus) / g_p;
   double xn = x * cos_a - sl * sl * g * cosc_a + xi;
   L.x = xn;
-  if (fabs(xn) > lc.max_aperture_limit) { L.state = BMX_LOST; return; }
+  if (fabs(xn) > mal) { L.state = BMX_LOST; return L; }
   double L_u = xi;
   double L_v = -(sl * sinc_a + x * sin_a);
   double L_c = sqrt(L_v * L_v + L_u * L_u);
@@ -473,7 +514,54 @@ __device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeC
   }
   L.px = pt * se;
   L.y = L.y + py * L_p * ipt;
-  L.z = L.z + L.beta * sl / op.c[9] - rel_p * L_p * ipt;
+  L.z = L.z + L.beta * c[13] - rel_p * L_p * ipt;
+  return L;
+}
+
+// The common path of the bend body: small entry and exit angles (the series
+// of asin, sin, cos, atan, 1/sinc), cos(angle + phi_1) > 0, no loss, and the
+// square roots and quotients in mid range.  It is the general form's own
+// sequence on that path (L_c / sinc(h) for |h| < 1e-8 is L_c exactly, as in
+// the reference), without its branches.  The whole wave takes it unless some
+// lane leaves it; then the whole wave takes bend_general.
+// c: as bend_general, 13 step_len / beta_ref
+__device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeConsts& lc) {
+  double sl = op.c[0], g = op.c[1], g_tot = op.c[2], angle = op.c[3];
+  double cos_a = op.c[4], sin_a = op.c[5], sinc_a = op.c[6], cosc_a = op.c[7];
+  double x = L.x, px = L.px, py = L.py, pz = L.pz;
+  double rel_p = 1.0 + pz;
+  double p2 = rel_p * rel_p - py * py;
+  double pt = sqrt_mid(p2);
+  double ipt = rcp_mid(pt);
+  double g_p = g_tot * ipt;
+  double sphi = px * ipt;
+  double phi_1 = asin_small(sphi);
+  double u = angle + phi_1;
+  double sin_plus = sin_small(u), cos_plus = cos_small(u);
+  double q = (1.0 + g * x) * sl * sinc_a;
+  double alpha = 2.0 * (1.0 + g * x) * sin_plus * sl * sinc_a - g_p * q * q;
+  double r = cos_plus * cos_plus + g_p * alpha;
+  double xi = div_mid(alpha, sqrt_mid(r) + cos_plus);
+  double xn = x * cos_a - sl * sl * g * cosc_a + xi;
+  double L_u = xi;
+  double L_v = -(sl * sinc_a + x * sin_a);
+  double L_c2 = L_v * L_v + L_u * L_u;
+  double half_p = angle + phi_1 - atan_small(div_mid(L_u, -L_v));
+  double e = phi_1 + angle - 2.0 * half_p;
+  bool paraxial = (op.c[10] == 0.0) & (fabs(x * g) < 1e-9) & (fabs(px) < 1e-9) & (fabs(py) < 1e-9) & (fabs(pz) < 1e-9);
+  bool common = !paraxial & (p2 > 0.25) & (p2 < 4.0) & (fabs(sphi) <= 0.05) & (fabs(angle) <= 0.15) & (r >= 0.25) &
+                (r <= 4.0) & (fabs(xn) <= lc.max_aperture_limit) & (L_v < -1e-100) & (L_c2 < 1e200) &
+                (fabs(L_u) <= -0.2 * L_v) & (fabs(half_p) <= 0.1) & (fabs(e) <= 0.2);
+  if (wave_any(!common)) {
+    L = bend_general(L, op.c, lc.p0c, lc.mc2sq, lc.max_aperture_limit, lc.two_max_ap);
+    return;
+  }
+  double L_p = sqrt_mid(L_c2) * inv_sinc_small(half_p);
+  double se = sin_small(e);
+  L.x = xn;
+  L.px = pt * se;
+  L.y = L.y + py * L_p * ipt;
+  L.z = L.z + L.beta * op.c[13] - rel_p * L_p * ipt;
 }
 
 // sbend_body_with_k1_map (track_a_bend.f90:500-659), forward, orientation +1.
