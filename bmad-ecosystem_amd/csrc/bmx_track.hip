@@ -156,10 +156,10 @@ __device__ __forceinline__ void drift_apply(Lane& L, double len, double pxr, dou
 // The series is evaluated for every lane; only a wave holding a lane at
 // u >= 1e-4 takes the exact form (and the lost_pz test, u >= 1, which leaves
 // the particle where it is) for those lanes.
-__device__ __forceinline__ void op_drift(Lane& L, double len) {
+__device__ __forceinline__ void drift_c(Lane& L, double len, double d0, double d1, double d2) {
   double pxr = L.px * L.irp, pyr = L.py * L.irp;
   double u = pxr * pxr + pyr * pyr;
-  double S = u * (0.5 + u * (0.375 + u * poly<3>(series(CS_DRIFT), u)));
+  double S = u * (0.5 + u * (0.375 + u * (d0 + u * (d1 + u * d2))));
   bool wide = u >= 1e-4;
   if (wave_any(wide)) {
     bool move = true;
@@ -176,6 +176,11 @@ __device__ __forceinline__ void op_drift(Lane& L, double len) {
     return;
   }
   drift_apply(L, len, pxr, pyr, S);
+}
+
+__device__ __forceinline__ void op_drift(Lane& L, double len) {
+  const cdouble* d = series(CS_DRIFT);
+  drift_c(L, len, sg(d[0]), sg(d[1]), sg(d[2]));
 }
 
 // drift with include_ref_motion = false (offset_particle's drift to the edge)
@@ -235,11 +240,45 @@ __device__ __forceinline__ void op_kick(Lane& L, const DevOp& op) {
 // then n_step x (drift, kick), the last kick x 1/2.  As in the reference the
 // loop runs on after a drift has set lost_pz; the element end checks state.
 // c: 0 step_len, 1.. full-step kick coefficients; n = n_step; flags = nh | lo << 8
+// The body for a single multipole order lo = LO (2: sextupole, 3: octupole)
+// with the kick of kick_horner written out: the coefficients and the drift
+// series are read once for all slices.  cb, ca: full-step b, a; the half
+// kicks use 0.5 cb, 0.5 ca, which scales the kick exactly by 1/2.
+template <int LO>
+__device__ __forceinline__ void kd_single(Lane& L, double len, int n_step, double cb, double ca) {
+  const cdouble* d = series(CS_DRIFT);
+  double d0 = sg(d[0]), d1 = sg(d[1]), d2 = sg(d[2]);
+  auto kick = [&](double b, double a) {
+    double x = L.x, y = L.y;
+    double kr = b, ki = a;
+    if (LO & 1) cmul(kr, ki, x, y);
+    cmul(kr, ki, x * x - y * y, 2.0 * x * y);
+    L.px -= kr;
+    L.py += ki;
+  };
+  kick(0.5 * cb, 0.5 * ca);
+  for (int i = 1; i <= n_step; i++) {
+    int st = L.state;
+    drift_c(L, len, d0, d1, d2);
+    L.state = (st != BMX_ALIVE) ? st : L.state;  // first loss state wins
+    if (i == n_step) kick(0.5 * cb, 0.5 * ca);
+    else kick(cb, ca);
+  }
+}
+
 __device__ __forceinline__ void op_kd_slices(Lane& L, const DevOp& op) {
   int n_step = __builtin_amdgcn_readfirstlane(op.n);
   int fl = __builtin_amdgcn_readfirstlane(op.flags);
   int nh = fl & 0xff, lo = (fl >> 8) & 0xff;
   double len = op.c[0];
+  if (nh == 0 && lo == 2) {
+    kd_single<2>(L, len, n_step, op.c[1], op.c[2]);
+    return;
+  }
+  if (nh == 0 && lo == 3) {
+    kd_single<3>(L, len, n_step, op.c[1], op.c[2]);
+    return;
+  }
   double kr, ki;
   kick_horner(&op.c[1], nh, lo, L.x, L.y, kr, ki);
   L.px -= 0.5 * kr;
