@@ -895,17 +895,24 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
 // blocks copy the whole lane state between registers on every op, while each
 // if-block joins back with the lane state in place.  The mask comes through
 // readfirstlane, which also keeps the compiler from folding the tests back
-// into a switch.
+// into a switch.  The tests are grouped so that an op passes about half of
+// them.
 #define TB(t) (1u << (t))
 __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, const LatticeConsts& lc) {
   unsigned tb = (unsigned)__builtin_amdgcn_readfirstlane((int)(1u << type));
-  if (tb & TB(OP_DRIFT)) op_drift(L, op.c[0]);
-  if (tb & TB(OP_KICK)) op_kick(L, op);
-  if (tb & TB(OP_KD_SLICES)) op_kd_slices(L, op);
-  if (tb & TB(OP_QUAD)) op_quad(L, op, lc);
-  if (tb & (TB(OP_BEND) | TB(OP_SBEND_FUSED))) op_sbend(L, op, flags, (tb & TB(OP_SBEND_FUSED)) != 0, lc);
-  if (tb & TB(OP_HWANG)) op_hwang(L, op);
-  if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, lc);
+  if (tb & (TB(OP_DRIFT) | TB(OP_KICK) | TB(OP_KD_SLICES))) {
+    if (tb & TB(OP_DRIFT)) op_drift(L, op.c[0]);
+    if (tb & TB(OP_KICK)) op_kick(L, op);
+    if (tb & TB(OP_KD_SLICES)) op_kd_slices(L, op);
+  }
+  if (tb & (TB(OP_QUAD) | TB(OP_BEND) | TB(OP_SBEND_FUSED))) {
+    if (tb & TB(OP_QUAD)) op_quad(L, op, lc);
+    if (tb & (TB(OP_BEND) | TB(OP_SBEND_FUSED))) op_sbend(L, op, flags, (tb & TB(OP_SBEND_FUSED)) != 0, lc);
+  }
+  if (tb & (TB(OP_HWANG) | TB(OP_RFCAVITY))) {
+    if (tb & TB(OP_HWANG)) op_hwang(L, op);
+    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, lc);
+  }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN);
   if (tb & RARE) {
@@ -968,7 +975,10 @@ __device__ __forceinline__ void park(Lane& L, const LatticeConsts& lc) {
 
 // Walk of `turns` turns over ops [lo1, hi1) then [lo2, hi2) for one lane's
 // particles.  The whole workgroup calls it together (it holds barriers); the
-// element program comes through LDS one chunk at a time.
+// element program comes through LDS one chunk at a time.  Lanes without a
+// live particle sit out each chunk; a lane whose last particle is lost
+// mid-chunk carries the parked particle to the end of the chunk (an exit
+// test per op costs more than that).
 template <int NP, class OnLoss>
 __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2,
                                      int turns, const LatticeConsts& lc, DevOp* sops, OnLoss on_loss) {
@@ -1003,7 +1013,6 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
                 park(P.L[j], lc);
               }
             }
-            if (!P.any()) break;
           }
         }
       }
