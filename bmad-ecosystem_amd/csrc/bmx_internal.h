@@ -44,14 +44,19 @@ enum : int32_t {
   F_OFFSET_MOVE = 32,     // OP_OFFSET_*: has a translation/pitch part
   F_EDGE_EXIT = 64,       // OP_SBEND_FUSED: exit edge present (F_HWANG_ENTER: entrance edge present)
   // the element ends after this op's body: orbit_too_large, then (with
-  // F_END_APERTURE) the exit aperture with its 10 constants at c[END_C ..]
+  // F_END_APERTURE) the exit aperture with its constants at c[END_C ..]
   F_END = 1 << 16,
   F_END_APERTURE = 1 << 17,
   F_END_ELLIPTICAL = 1 << 18,
+  // after the body (and before the element end), for a particle still alive:
+  // offset_particle unset of a pure transverse offset, x += c[SHIFT_C],
+  // y += c[SHIFT_C + 1] (the element's OP_OFFSET_UNSET, folded)
+  F_SHIFT_OUT = 1 << 19,
 };
 
-// first of the folded aperture constants of an op carrying F_END
-constexpr int END_C = 4;
+// aperture constants of an op carrying F_END (6 slots, see aperture())
+constexpr int END_C = 6;
+constexpr int SHIFT_C = 12;
 
 struct alignas(64) DevOp {
   int32_t type;   // OpType

@@ -253,6 +253,25 @@ void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool se
 
 }  // namespace
 
+namespace {
+
+// Number of leading constants c[0 ..] an op's body uses (14 when not
+// listed); F_SHIFT_OUT offsets sit apart, at c[SHIFT_C ..].
+int used_consts(const DevOp& op) {
+  int nh = op.flags & 0xff;
+  switch (op.type) {
+    case OP_ELE_END: return 0;
+    case OP_DRIFT: return 1;
+    case OP_QUAD: case OP_TILT: return 2;
+    case OP_KICK: return 2 * (nh + 1);
+    case OP_KD_SLICES: return 1 + 2 * (nh + 1);
+    case OP_RFCAVITY: return 5;
+    default: return 14;
+  }
+}
+
+}  // namespace
+
 // Compile one variant.  Returns false and fills *err for unsupported input.
 bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, const bmx_config& cfg,
                      std::vector<DevOp>& prog, std::vector<int32_t>& ele_op_begin, LatticeConsts* lc,
@@ -309,21 +328,27 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     if (e.aperture_type == BMX_APERTURE_ELLIPTICAL && ap_set &&
         ((e.x1_limit == 0) != (e.x2_limit == 0) || (e.y1_limit == 0) != (e.y2_limit == 0)))
       return fail("elliptical aperture needs all four limits");
-    auto ap_consts = [&](DevOp& op) {
+    // aperture constants at c[k0 ..]: rectangular x1, x2, y1, y2; elliptical
+    // (elliptical_params_setup) cx, cy, 1/wx, 1/wy, wx, wy -- the reciprocal
+    // widths for the device's division-free screen
+    auto ap_consts = [&](DevOp& op, int k0) {
       double x1 = -e.x1_limit, x2 = e.x2_limit, y1 = -e.y1_limit, y2 = e.y2_limit;
-      op.c[0] = x1; op.c[1] = x2; op.c[2] = y1; op.c[3] = y2;
-      if (e.aperture_type == BMX_APERTURE_ELLIPTICAL) {  // elliptical_params_setup
+      double* c = op.c + k0;
+      if (e.aperture_type == BMX_APERTURE_ELLIPTICAL) {
         double wx = (x2 - x1) / 2, cx = (x2 + x1) / 2, wy = (y2 - y1) / 2, cy = (y2 + y1) / 2;
         if (wx == 0) wx = cfg.max_aperture_limit;
         if (wy == 0) wy = cfg.max_aperture_limit;
-        op.c[4] = std::max(wx, 0.0); op.c[5] = cx; op.c[6] = std::max(wy, 0.0); op.c[7] = cy;
-        op.c[8] = 1.0 / op.c[4]; op.c[9] = 1.0 / op.c[6];  // for the device's division-free screen
+        wx = std::max(wx, 0.0);
+        wy = std::max(wy, 0.0);
+        c[0] = cx; c[1] = cy; c[2] = 1.0 / wx; c[3] = 1.0 / wy; c[4] = wx; c[5] = wy;
+      } else {
+        c[0] = x1; c[1] = x2; c[2] = y1; c[3] = y2;
       }
     };
     if (cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_ENTRANCE_END || e.aperture_at == BMX_BOTH_ENDS)) {
       DevOp op = make_op(OP_APERTURE_IN, ie);
       op.flags = ap_flags;
-      ap_consts(op);
+      ap_consts(op, 0);
       prog.push_back(op);
     }
 
@@ -546,27 +571,29 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     }
 
     // Element end (orbit_too_large, exit aperture): folded into the element's
-    // last op when that op leaves c[END_C ..] free, else an OP_ELE_END of its own.
+    // last op when that op leaves c[END_C ..] free, else an OP_ELE_END.  A
+    // pure-translation offset_particle unset before it folds into the op
+    // before that (F_SHIFT_OUT).
     bool exit_ap = cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS);
-    DevOp ap = make_op(OP_ELE_END, ie);
-    if (exit_ap) ap_consts(ap);
     int end_flags = F_END | (exit_ap ? (F_END_APERTURE | ((ap_flags & F_APERTURE_ELLIPTICAL) ? F_END_ELLIPTICAL : 0)) : 0);
-    DevOp* last = (prog.size() > (size_t)ele_op_begin.back()) ? &prog.back() : nullptr;
-    bool foldable = false;
-    if (last && last->ie == ie && !(last->flags & F_END)) {
-      switch (last->type) {
-        case OP_DRIFT: case OP_QUAD: case OP_TILT: foldable = true; break;       // c[0], c[1]
-        case OP_KICK: foldable = 2 * (last->flags + 1) <= END_C; break;            // c[0 .. 2 nh + 1]
-        case OP_KD_SLICES: foldable = 1 + 2 * ((last->flags & 0xff) + 1) <= END_C; break;  // c[0 .. 2 nh + 2]
-        default: break;
+    size_t first = (size_t)ele_op_begin.back();
+    if (prog.size() >= first + 2) {
+      DevOp& u = prog.back();
+      DevOp& body = prog[prog.size() - 2];
+      if (u.type == OP_OFFSET_UNSET && !(u.flags & F_OFFSET_PITCH) && u.c[2] == 0.0 && used_consts(body) <= SHIFT_C) {
+        body.flags |= F_SHIFT_OUT;
+        body.c[SHIFT_C] = u.c[0];
+        body.c[SHIFT_C + 1] = u.c[1];
+        prog.pop_back();
       }
     }
-    DevOp end = make_op(OP_ELE_END, ie);
-    DevOp* tgt = foldable ? last : &end;
-    tgt->flags |= end_flags;
-    if (exit_ap)
-      for (int k = 0; k < 10; k++) tgt->c[END_C + k] = ap.c[k];
-    if (!foldable) prog.push_back(end);
+    DevOp* last = (prog.size() > first) ? &prog.back() : nullptr;
+    if (!last || used_consts(*last) > END_C) {
+      prog.push_back(make_op(OP_ELE_END, ie));
+      last = &prog.back();
+    }
+    last->flags |= end_flags;
+    if (exit_ap) ap_consts(*last, END_C);
   }
   ele_op_begin.push_back((int32_t)prog.size());
   (void)kCLight;

@@ -113,6 +113,11 @@ __device__ __forceinline__ double poly(const cdouble* c, double x) {
   return a;
 }
 
+// Keeps the compiler from merging the lane-dependent tests of a block into
+// the wave-uniform test that guards it (which would evaluate them, and mask
+// exec, on every op whether the block runs or not).
+__device__ __forceinline__ void uniform_block() { asm volatile(""); }
+
 // true on every lane of the wave when b holds on any active lane
 __device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
@@ -809,15 +814,16 @@ __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
 }
 
 // aperture part of check_aperture_limit (bmad/low_level/check_aperture_limit.f90:257-309)
-// c: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim, 4 wx, 5 cx, 6 wy, 7 cy, 8 1/wx, 9 1/wy
+// c, rectangular: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim;
+// elliptical: 0 cx, 1 cy, 2 1/wx, 3 1/wy, 4 wx, 5 wy
 __device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptical) {
   if (elliptical) {
-    double xp = L.x - c[5], yp = L.y - c[7];
+    double xp = L.x - c[0], yp = L.y - c[1];
     // screen with the reciprocal widths: it is within a few ulp of the
     // reference's r, so r_fast < 1 - 1e-13 decides r < 1 exactly
-    double fx = xp * c[8], fy = yp * c[9];
+    double fx = xp * c[2], fy = yp * c[3];
     if (fx * fx + fy * fy < 1.0 - 1e-13) return;
-    double ax = xp / c[4], ay = yp / c[6];
+    double ax = xp / c[4], ay = yp / c[5];
     double r = ax * ax + ay * ay;
     if (r < 1.0) return;
     if (fabs(ax) > fabs(ay)) L.state = (xp > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
@@ -874,7 +880,7 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
   bool inside = (fabs(L.x) <= mal) & (fabs(L.y) <= mal) & (rel_p >= 0.0) & (f <= 0.0);
   if (flags & F_END_APERTURE) {
     if (flags & F_END_ELLIPTICAL) {
-      double fx = (L.x - c[5]) * c[8], fy = (L.y - c[7]) * c[9];
+      double fx = (L.x - c[0]) * c[2], fy = (L.y - c[1]) * c[3];
       inside &= fx * fx + fy * fy < 1.0 - 1e-13;
     } else {
       inside &= (c[0] == 0.0 || L.x >= c[0]) & (c[1] == 0.0 || L.x <= c[1]) & (c[2] == 0.0 || L.y >= c[2]) &
@@ -925,6 +931,19 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false);
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
     if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
+  }
+  if (flags & F_SHIFT_OUT) {  // offset_particle unset, pure translation (op_offset without pitch)
+    uniform_block();
+    double rel_p = 1.0 + L.pz;
+    double pzz = rel_p * rel_p - L.px * L.px - L.py * L.py;
+    if (L.state == BMX_ALIVE) {
+      if (pzz <= 0.0) {
+        L.state = BMX_LOST_PZ;
+      } else {
+        L.x += op.c[SHIFT_C];
+        L.y += op.c[SHIFT_C + 1];
+      }
+    }
   }
   if (flags & F_END) ele_end(L, op.c + END_C, flags, lc);
 }
