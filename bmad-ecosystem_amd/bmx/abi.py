@@ -97,7 +97,7 @@ EXPORTS = [
     "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
     "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
-    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_ctx_sync",
+    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_step_fit", "bmx_ctx_sync",
 ]
 
 _lib = None
@@ -166,6 +166,8 @@ def load_library(path=None):
     lib.bmx_da_features.restype = C.c_int32
     lib.bmx_da_features.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_float,
                                     C.c_void_p, C.c_void_p]
+    lib.bmx_step_fit.restype = C.c_int32
+    lib.bmx_step_fit.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_float, C.c_void_p]
     lib.bmx_ctx_sync.restype = C.c_int32
     lib.bmx_ctx_sync.argtypes = [C.c_void_p]
     if path is None:

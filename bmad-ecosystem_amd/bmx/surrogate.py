@@ -461,60 +461,106 @@ def predict_knobs(net, knob_sets, spec, lat_chunk=None):
     return out.cpu().numpy().reshape(n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
 
 
+def predict_boundaries(net, knob_sets, spec, lat_chunk=None):
+    """Predicted boundary index of every ray [n_lat, n_pz, n_angle] (int64,
+    0..n_amp): the network output for every grid IC and its step fit
+    (step_fit_boundary at the surviving level 1), all on the device
+    (bmx_da_features, the MLP forward, bmx_step_fit); only the indices come
+    back to the host."""
+    knob_sets = np.asarray(knob_sets)
+    n_lat = len(knob_sets)
+    if lat_chunk is None:
+        lat_chunk = max(1, (1 << 21) // spec.n_ic)
+    dev = net.dev
+    geo = torch.from_numpy(geometry(spec).astype(np.float32)).to(dev)
+    kd = torch.from_numpy(knob_sets.astype(np.float32)).to(dev)
+    n_ray_lat = spec.n_pz * spec.n_angle
+    kb = torch.empty(n_lat * n_ray_lat, dtype=torch.int32, device=dev)
+    X = torch.empty(min(lat_chunk, n_lat) * spec.n_ic, N_KNOBS + 3, dtype=torch.bfloat16, device=dev)
+    torch.cuda.synchronize(dev)
+    for a in range(0, n_lat, lat_chunk):
+        b = min(n_lat, a + lat_chunk)
+        xb = X[:(b - a) * spec.n_ic]
+        rc = net.lib.bmx_da_features(net.ctx.h, b - a, spec.n_ic, N_KNOBS, 3,
+                                     C.c_void_p(kd.data_ptr() + a * N_KNOBS * 4), KNOB_RANGE, _p(geo), _p(xb))
+        net._check(rc, "bmx_da_features")
+        y = net.forward(xb)
+        rc = net.lib.bmx_step_fit(net.ctx.h, (b - a) * n_ray_lat, spec.n_amp, _p(y), 1.0,
+                                  C.c_void_p(kb.data_ptr() + a * n_ray_lat * 4))
+        net._check(rc, "bmx_step_fit")
+        net.sync()
+    return kb.cpu().numpy().astype(np.int64).reshape(n_lat, spec.n_pz, spec.n_angle)
+
+
 def _resolve_rays(ctx, knob_sets, spec, lo, hi, refine, turns_per_block, max_rounds):
     """FP64 rounds of surrogate_scan for one chunk of lattices on context ctx:
     the band [lo, hi] first, then gap refinement until every ray is resolved.
-    Returns (known survival [n_lat, n_pz, n_angle, n_amp] (-1 = not tracked),
+    Bookkeeping is per ray: bnd = the first tracked loss (n_amp if none) and
+    a_top = the highest tracked survivor below it (-1 if none); the ray is
+    resolved when no untracked amplitude lies between them.
+    Returns (bnd [n_lat, n_pz, n_angle], number of tracked ICs,
     rounds [(n_ics, seconds, pet)])."""
     import time
     from . import tracker
     n_lat = len(knob_sets)
     N = spec.n_turns
-    shape = (n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
-    known = np.full(shape, -1, dtype=np.int64)
+    A = spec.n_amp
+    n_ray_lat = spec.n_pz * spec.n_angle
+    R = n_lat * n_ray_lat
     lat = upload_variants(ctx, knob_sets, spec)
-    off = spec.offsets()
-    kk = np.arange(spec.n_amp)
+    off = spec.offsets().reshape(n_ray_lat, A, 3)
+    bnd = np.full(R, A, dtype=np.int64)
+    a_top = np.full(R, -1, dtype=np.int64)
+    tracked = np.zeros((R, A), dtype=bool)
+    alive_r, alive_k = [], []  # every tracked survivor
     rounds = []
-    next_mask = (kk >= lo[..., None]) & (kk <= hi[..., None])
+    # round 1: the band of every ray
+    lo, hi = lo.reshape(-1), hi.reshape(-1)
+    cnt = np.maximum(hi - lo + 1, 0)
+    ray = np.repeat(np.arange(R), cnt)
+    amp = lo[ray] + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
     for _ in range(max_rounds):
-        idx = np.argwhere(next_mask & (known < 0))  # sorted by lattice
-        if len(idx) == 0:
+        keep = ~tracked[ray, amp]
+        ray, amp = ray[keep], amp[keep]
+        if len(ray) == 0:
             break
-        o = off[idx[:, 1], idx[:, 2], idx[:, 3]]
-        counts = np.bincount(idx[:, 0], minlength=n_lat)
-        per = []
-        for part in np.split(o, np.cumsum(counts)[:-1]):
-            c = np.zeros((6, len(part)))
-            c[0], c[2], c[5] = part[:, 0], part[:, 1], part[:, 2]
-            per.append(c)
+        tracked[ray, amp] = True
+        o = off[ray % n_ray_lat, amp]
+        counts = np.bincount(ray // n_ray_lat, minlength=n_lat)
+        c = np.zeros((6, len(ray)))
+        c[0], c[2], c[5] = o[:, 0], o[:, 1], o[:, 2]
+        per = np.split(c, np.cumsum(counts)[:-1], axis=1)
         tr0 = time.perf_counter()
         surv, st = tracker.track_ics(lat, per, N, turns_per_block, return_stats=True)
-        rounds.append((len(idx), time.perf_counter() - tr0, st["pet"]))
-        known[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]] = np.concatenate(surv)
-        # resolution per ray: b = first tracked loss (n_amp if none), a = the
-        # highest tracked survivor below it (-1 if none).  All amplitudes in
-        # (a, b) are untracked; the ray is resolved when none are left.  The
-        # next round tracks up to `refine` evenly spaced points of (a, b).
-        tracked = known >= 0
-        lostm = tracked & (known < N)
-        has_lost = lostm.any(-1)
-        bnd = np.where(has_lost, np.argmax(lostm, -1), spec.n_amp)
-        alive_below = tracked & (known >= N) & (kk < bnd[..., None])
-        a_top = np.where(alive_below.any(-1), spec.n_amp - 1 - np.argmax(alive_below[..., ::-1], -1), -1)
-        if refine <= 0:  # the whole gap in one round
-            next_mask = (kk > a_top[..., None]) & (kk < bnd[..., None])
-            continue
+        rounds.append((len(ray), time.perf_counter() - tr0, st["pet"]))
+        surv = np.concatenate(surv)
+        lost = surv < N
+        np.minimum.at(bnd, ray[lost], amp[lost])
+        alive_r.append(ray[~lost])
+        alive_k.append(amp[~lost])
+        ar, ak = np.concatenate(alive_r), np.concatenate(alive_k)
+        below = ak < bnd[ar]
+        a_top[:] = -1
+        np.maximum.at(a_top, ar[below], ak[below])
         gap = bnd - a_top - 1
+        open_r = np.nonzero(gap > 0)[0]
+        if len(open_r) == 0:
+            break
+        g = gap[open_r]
+        if refine <= 0:  # the whole gap in one round
+            ray = np.repeat(open_r, g)
+            amp = a_top[ray] + 1 + (np.arange(g.sum()) - np.repeat(np.cumsum(g) - g, g))
+            continue
+        # up to `refine` evenly spaced points of each open gap (a_top, bnd)
         j = np.arange(1, refine + 1)
-        pts = a_top[..., None] + np.ceil((gap[..., None] + 1) * j / (refine + 1)).astype(np.int64)
-        pts = np.where((pts > a_top[..., None]) & (pts < bnd[..., None]), pts, -1)
-        sel = np.zeros(shape, dtype=bool)
-        li, pi, ai, ji = np.nonzero(pts >= 0)
-        sel[li, pi, ai, pts[li, pi, ai, ji]] = True
-        next_mask = sel
+        pts = a_top[open_r, None] + np.ceil((g[:, None] + 1) * j / (refine + 1)).astype(np.int64)
+        ok = (pts > a_top[open_r, None]) & (pts < bnd[open_r, None])
+        ray = np.broadcast_to(open_r[:, None], pts.shape)[ok]
+        amp = pts[ok]
+        u = np.unique(ray * A + amp)
+        ray, amp = u // A, u % A
     lat.close()
-    return known, rounds
+    return bnd.reshape(n_lat, spec.n_pz, spec.n_angle), int(tracked.sum()), rounds
 
 
 def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_rounds=64, refine=6, workers=1):
@@ -539,8 +585,7 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
     knob_sets = np.asarray(knob_sets)
     n_lat = len(knob_sets)
     N = spec.n_turns
-    pred = predict_knobs(net, knob_sets, spec)
-    kb = step_fit_boundary(pred, 1.0)
+    kb = predict_boundaries(net, knob_sets, spec)
     amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
     # band in index space: amplitudes within +-band of the boundary amplitude
     a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
@@ -574,12 +619,10 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
             c.close()
     if errors:
         raise errors[0]
-    known = np.concatenate([r[0] for r in results], 0)
-    rounds = [x for r in results for x in r[1]]
-    n_tracked = int((known >= 0).sum())
     # surviving prefix: untracked amplitudes below the first tracked loss count as alive
-    lostm = (known >= 0) & (known < N)
-    runs = np.where(lostm.any(-1), np.argmax(lostm, -1), spec.n_amp)
+    runs = np.concatenate([r[0] for r in results], 0)
+    n_tracked = sum(r[1] for r in results)
+    rounds = [x for r in results for x in r[2]]
     area = np.array([area_from_runs(runs[v, spec.n_pz // 2], spec) for v in range(n_lat)])
     t1 = time.perf_counter()
     timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb)

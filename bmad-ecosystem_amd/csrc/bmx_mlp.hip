@@ -227,6 +227,49 @@ __global__ __launch_bounds__(256) void da_features_kernel(int64_t n_rows, int n_
   for (int g = 0; g < n_geo; g++) x[n_knob + g] = (__bf16)geo[(int64_t)i * n_geo + g];
 }
 
+// Least-squares step fit of each ray's predicted survival profile: the k in
+// 0..K minimising sum_{j<k} (p_j - alive)^2 + sum_{j>=k} (p_j - mean_{j>=k} p)^2
+// (a step from the surviving level to a free lower level), ties to the
+// larger k.  One lane per ray, sums in FP64; a workgroup stages its RAYS
+// profiles through LDS (coalesced loads, rows padded by one word against
+// bank conflicts).
+template <int RAYS>
+__global__ __launch_bounds__(RAYS) void step_fit_kernel(int n_rays, int K, const float* __restrict__ pred, float alive,
+                                                        int32_t* __restrict__ out) {
+  extern __shared__ float tile[];
+  int r0 = blockIdx.x * RAYS;
+  int nr = min(RAYS, n_rays - r0);
+  int ld = K + 1;
+  for (int e = threadIdx.x; e < nr * K; e += RAYS) tile[(e / K) * ld + e % K] = pred[(int64_t)r0 * K + e];
+  __syncthreads();
+  if ((int)threadIdx.x >= nr) return;
+  const float* p = tile + threadIdx.x * ld;
+  double S = 0.0, Q = 0.0;
+  for (int j = 0; j < K; j++) {
+    double v = p[j];
+    S += v;
+    Q += v * v;
+  }
+  double pre = 0.0, ps = 0.0, pq = 0.0, best = __builtin_inf();
+  int kbest = 0;
+  for (int k = 0; k <= K; k++) {
+    int cnt = K - k;
+    double sk = S - ps, qk = Q - pq;
+    double cost = pre + (cnt > 0 ? qk - sk * sk / cnt : 0.0);
+    if (cost <= best) {
+      best = cost;
+      kbest = k;
+    }
+    if (k < K) {
+      double v = p[k], d = v - (double)alive;
+      pre += d * d;
+      ps += v;
+      pq += v * v;
+    }
+  }
+  out[r0 + threadIdx.x] = kbest;
+}
+
 }  // namespace
 
 // defined in bmx_track.hip (hidden visibility)
@@ -364,6 +407,26 @@ int32_t bmx_da_features(bmx_ctx* ctx, int32_t n_lat, int32_t n_ic, int32_t n_kno
   int64_t rows = (int64_t)n_lat * n_ic;
   hipLaunchKernelGGL(da_features_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, bmx_ctx_stream(ctx), rows,
                      n_ic, n_knob, n_geo, knobs, 1.0f / knob_range, geo, (__bf16*)X_bf16);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+}
+
+int32_t bmx_step_fit(bmx_ctx* ctx, int32_t n_rays, int32_t n_amp, const float* pred, float alive_level,
+                     int32_t* k_out) {
+  if (!ctx || !pred || !k_out || n_rays <= 0 || n_amp <= 0 || n_amp > 2048)
+    return bmx_ctx_set_error(ctx, "bmx_step_fit: invalid arguments");
+  // rays per workgroup so that the staged profiles stay within 33 KiB of LDS
+  hipStream_t st = bmx_ctx_stream(ctx);
+  size_t row = (size_t)(n_amp + 1) * sizeof(float);
+  if (n_amp <= 128)
+    hipLaunchKernelGGL(step_fit_kernel<64>, dim3((unsigned)((n_rays + 63) / 64)), dim3(64), 64 * row, st, n_rays,
+                       n_amp, pred, alive_level, k_out);
+  else if (n_amp <= 256)
+    hipLaunchKernelGGL(step_fit_kernel<32>, dim3((unsigned)((n_rays + 31) / 32)), dim3(32), 32 * row, st, n_rays,
+                       n_amp, pred, alive_level, k_out);
+  else
+    hipLaunchKernelGGL(step_fit_kernel<4>, dim3((unsigned)((n_rays + 3) / 4)), dim3(4), 4 * row, st, n_rays, n_amp,
+                       pred, alive_level, k_out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
 }

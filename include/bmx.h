@@ -326,6 +326,14 @@ int32_t bmx_mse_grad(bmx_ctx *ctx, int32_t n, const float *pred, const float *ta
 int32_t bmx_da_features(bmx_ctx *ctx, int32_t n_lat, int32_t n_ic, int32_t n_knob, int32_t n_geo,
                         const float *knobs, float knob_range, const float *geo, void *X_bf16);
 
+/* Boundary index of each of n_rays predicted survival profiles pred[ray, n_amp]
+ * (device pointers): the k in 0..n_amp minimising
+ * sum_{j<k} (p_j - alive_level)^2 + sum_{j>=k} (p_j - mean_{j>=k} p)^2, ties to
+ * the larger k (n_amp: the whole ray survives).  n_amp <= 2048.  No reference
+ * counterpart (the DA surrogate's screening step). */
+int32_t bmx_step_fit(bmx_ctx *ctx, int32_t n_rays, int32_t n_amp, const float *pred, float alive_level,
+                     int32_t *k_out);
+
 /* Wait for the work queued on the context's stream. */
 int32_t bmx_ctx_sync(bmx_ctx *ctx);
 

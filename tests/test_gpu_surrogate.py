@@ -197,3 +197,42 @@ def test_da_features_match_host_rows(ctx):
     got = X.float().cpu().numpy().astype(np.float64)
     # the host rows are float32 knobs / 0.2; the device scales float32 knobs by 1 / 0.2
     assert np.max(np.abs(got - want)) <= 2.0 ** -8 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("K", [1, 7, 128, 300])
+def test_step_fit_matches_host(ctx, K):
+    """bmx_step_fit picks, for every ray, a boundary whose least-squares step
+    cost equals the host step_fit_boundary's minimum (the sums are FP64 in
+    both, accumulated in different orders, so a near-tie may resolve to a
+    different k of equal cost), and the same k where the minimum is clear."""
+    rng = np.random.default_rng(K)
+    n = 3000
+    # noisy steps at random positions, some flat profiles (ties), some all-survive / all-lost
+    kt = rng.integers(0, K + 1, n)
+    p = np.where(np.arange(K)[None, :] < kt[:, None], 1.0, 0.2) + 0.05 * rng.standard_normal((n, K))
+    p[:50] = 1.0
+    p[50:100] = 0.0
+    p = p.astype(np.float32)
+    want = surrogate.step_fit_boundary(p, 1.0)
+    tp = torch.from_numpy(p).to(DEV)
+    out = torch.full((n,), -1, dtype=torch.int32, device=DEV)
+    torch.cuda.synchronize()
+    assert ctx.lib.bmx_step_fit(ctx.h, n, K, _p(tp), 1.0, _p(out)) == 0
+    assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
+    got = out.cpu().numpy().astype(np.int64)
+    assert np.all((got >= 0) & (got <= K))
+
+    def cost(k):
+        q = p.astype(np.float64)
+        c = np.empty(n)
+        for i in range(n):
+            a, b = q[i, :k[i]], q[i, k[i]:]
+            c[i] = np.sum((a - 1.0) ** 2) + (np.sum((b - b.mean()) ** 2) if len(b) else 0.0)
+        return c
+
+    cg, cw = cost(got), cost(want)
+    assert np.all(cg <= cw + 1e-9 * (1 + np.abs(cw)))
+    same = got == want
+    print(f"K={K}: identical k on {same.mean():.4f} of rays", flush=True)
+    assert same.mean() > 0.99
+    assert np.all(got[:50] == K) and np.all(want[:50] == K)

@@ -28,12 +28,12 @@ def _run(monkeypatch, truth, pred_shift, islands=None, refine=0):
             alive = False
         return N if alive else 7
 
-    # predicted survival: alive below truth + shift
+    # predicted survival: alive below truth + shift; boundaries from its step fit
     def fake_predict(net, knob_sets, spec_):
         out = np.zeros((n_lat, spec.n_pz, spec.n_angle, spec.n_amp), dtype=np.float32)
         kb = np.clip(truth + pred_shift, 0, spec.n_amp)
         out[...] = np.where(amp_k[None, None, None, :] < kb[..., None], 1.0, 0.0)
-        return out
+        return surrogate.step_fit_boundary(out, 1.0)
 
     tracked = []
 
@@ -51,7 +51,7 @@ def _run(monkeypatch, truth, pred_shift, islands=None, refine=0):
             res.append(s)
         return (res, dict(pet=0.0)) if return_stats else res
 
-    monkeypatch.setattr(surrogate, "predict_knobs", fake_predict)
+    monkeypatch.setattr(surrogate, "predict_boundaries", fake_predict)
     monkeypatch.setattr(surrogate, "upload_variants", lambda ctx, k, s: _FakeLat())
     from bmx import tracker
     monkeypatch.setattr(tracker, "track_ics", fake_track_ics)
