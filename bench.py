@@ -69,10 +69,12 @@ def da_scan(ctx, n_train=512, n_test=512, epochs=8):
     [-0.2 %, +0.2 %], 512 turns.  The network (4 x 256 SiLU, BF16 MFMA) is
     trained first on a plain scan of n_train settings; that one-time cost and
     the number of scanned lattices that repays it are reported beside the
-    rates."""
+    rates.  The screened scan resolves the lattices in two chunks on two
+    streams, so that the latency-bound refinement rounds of one overlap the
+    other's work."""
     from bmx import surrogate
     spec = surrogate.ScanSpec(**DA_SPEC)
-    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs)
+    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs, workers=2)
     keep = ("plain_lattices_per_hour", "surrogate_lattices_per_hour", "speedup", "da_area_err_mean",
             "da_area_err_max", "frac_within_2pct", "rays_identical", "ics_tracked_plain", "ics_tracked_surrogate",
             "pet_plain", "pet_surrogate", "plain_s", "surrogate_s", "train_scan_s", "train_s", "break_even_lattices",
