@@ -94,6 +94,14 @@ class Lattice:
         _check(ctx.h, rc, "bmx_lattice_upload")
         self.h = h
 
+    def cells(self):
+        """(ops per repeating cell, runs of it) tracked by the lattice's
+        runtime-compiled cell kernels; (0, 0) when it runs on the op
+        interpreter alone (bmx_lattice_cells)."""
+        n, k = C.c_int32(0), C.c_int64(0)
+        _check(self.ctx.h, self.ctx.lib.bmx_lattice_cells(self.h, C.byref(n), C.byref(k)), "bmx_lattice_cells")
+        return n.value, k.value
+
     def close(self):
         if self.h:
             self.ctx.lib.bmx_lattice_destroy(self.h)

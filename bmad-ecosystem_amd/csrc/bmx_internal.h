@@ -52,6 +52,8 @@ enum : int32_t {
   // offset_particle unset of a pure transverse offset, x += c[SHIFT_C],
   // y += c[SHIFT_C + 1] (the element's OP_OFFSET_UNSET, folded)
   F_SHIFT_OUT = 1 << 19,
+  // first op of a run of the lattice's repeating cell (runtime-compiled cell kernels)
+  F_CELL = 1 << 20,
 };
 
 // aperture constants of an op carrying F_END (6 slots, see aperture())
@@ -88,6 +90,7 @@ struct LatticeConsts {
   double two_max_ap; // 2 * max_aperture_limit
   int32_t aperture_limit_on;
   int32_t species;
+  const int32_t* cell_prv;  // device: last F_CELL op at or before each op (-1: none); cell kernels only
 };
 
 }  // namespace bmx

@@ -228,6 +228,14 @@ int32_t bmx_lattice_upload(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele_
                            const bmx_ref *ref, const bmx_config *cfg, bmx_lattice **out);
 void bmx_lattice_destroy(bmx_lattice *lat);
 
+/* The lattice's runtime-compiled cell kernels: *cell_len = ops in its
+ * repeating cell and *n_cells = runs of that cell the kernels track straight
+ * through (both 0 when the lattice runs on the built-in op interpreter alone:
+ * no cell covers half of its ops, BMX_JIT=0 in the environment, or the
+ * runtime compiler failed).  Results are the same either way.  No reference
+ * counterpart. */
+int32_t bmx_lattice_cells(const bmx_lattice *lat, int32_t *cell_len, int64_t *n_cells);
+
 /* Track a bunch through elements ix_start+1 .. ix_end of variant 0 with
  * track_bunch wrap-around semantics (ix_end <= ix_start wraps through the
  * end of the lattice), repeated n_turns times.  direction must be +1.
