@@ -5,9 +5,10 @@
 // (a fixed sequence of element kinds with different strengths), so the op
 // program is mostly one window of op types and flags, repeated.  For such a
 // lattice this file finds that window (the cell), generates jit_cell(): the
-// cell's ops as calls of run_op with the type and flags as compile-time
-// constants (so every dispatch test folds away and the compiler schedules
-// across op boundaries), and compiles bmx_ops.h + jit_cell + bmx_kernels.h
+// cell's ops as calls of run_op with each op's type, flags and count as
+// compile-time constants (so every dispatch test folds away, kick orders and
+// slice loops are unrolled, and the compiler schedules across op
+// boundaries), and compiles bmx_ops.h + jit_cell + bmx_kernels.h
 // with hipRTC for the device.  The walk then tracks every run of the cell
 // through jit_cell and the remaining ops through run_op.  Both evaluate the
 // same op functions in the same order.
@@ -38,8 +39,11 @@ namespace {
 // header texts, generated from the sources by the build (bmx_jit_src.inc)
 #include "bmx_jit_src.inc"
 
+// type (8 bits), flags (24 bits) and n (32 bits): what the cell kernel's
+// code is specialised on
 uint64_t op_sig(const DevOp& op) {
-  return ((uint64_t)(uint32_t)op.type << 32) | (uint32_t)(op.flags & ~F_CELL);
+  return ((uint64_t)(uint32_t)op.n << 32) | ((uint64_t)((uint32_t)op.flags & ~(uint32_t)F_CELL & 0xffffffu) << 8) |
+         (uint32_t)(op.type & 0xff);
 }
 
 bool ele_start(const std::vector<DevOp>& prog, size_t k) { return k == 0 || prog[k].ie != prog[k - 1].ie; }
@@ -116,7 +120,8 @@ std::string cell_source(const std::vector<DevOp>& prog, size_t rep, int len) {
   for (int i = 0; i < len; i++) {
     const DevOp& op = prog[rep + i];
     s += "  run_op(L, o[" + std::to_string(i) + "], " + std::to_string(op.type) + ", " +
-         std::to_string(op.flags & ~F_CELL) + ", lc);\n  loss(o[" + std::to_string(i) + "]);\n";
+         std::to_string(op.flags & ~F_CELL) + ", " + std::to_string(op.n) + ", lc);\n  loss(o[" +
+         std::to_string(i) + "]);\n";
   }
   s += "}\n";
   s += kSrcKernelsH;
@@ -165,7 +170,7 @@ bool plan_cells(std::vector<DevOp>& prog, CellPlan* plan) {
   plan->n_cells = starts.size();
   for (int i = 0; i < best_len; i++) {
     const DevOp& op = prog[best_rep + i];
-    plan->key += std::to_string(op.type) + ":" + std::to_string(op.flags & ~F_CELL) + ";";
+    plan->key += std::to_string(op.type) + ":" + std::to_string(op.flags & ~F_CELL) + ":" + std::to_string(op.n) + ";";
   }
   return true;
 }

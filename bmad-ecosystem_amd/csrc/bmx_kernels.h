@@ -109,8 +109,9 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
             }
 #endif
             int type = __builtin_amdgcn_readfirstlane(op.type);
+            int n = __builtin_amdgcn_readfirstlane(op.n);
 #pragma unroll
-            for (int j = 0; j < NP; j++) run_op(P.L[j], op, type, flags, lc);
+            for (int j = 0; j < NP; j++) run_op(P.L[j], op, type, flags, n, lc);
             loss_check(op);
             k++;
           }

@@ -199,14 +199,14 @@ __device__ __forceinline__ void kick_horner(const double* c, int nh, int lo, dou
   }
 }
 
-__device__ __forceinline__ void kick_eval(const DevOp& op, double x, double y, double& kr, double& ki) {
-  kick_horner(op.c, __builtin_amdgcn_readfirstlane(op.flags) & 0xff, __builtin_amdgcn_readfirstlane(op.n), x, y, kr,
-              ki);
+// nh = flags & 0xff, lo = n of the op (wave-uniform; constants in a cell kernel)
+__device__ __forceinline__ void kick_eval(const DevOp& op, int nh, int lo, double x, double y, double& kr, double& ki) {
+  kick_horner(op.c, nh, lo, x, y, kr, ki);
 }
 
-__device__ __forceinline__ void op_kick(Lane& L, const DevOp& op) {
+__device__ __forceinline__ void op_kick(Lane& L, const DevOp& op, int nh, int lo) {
   double kr, ki;
-  kick_eval(op, L.x, L.y, kr, ki);
+  kick_eval(op, nh, lo, L.x, L.y, kr, ki);
   L.px -= kr;
   L.py += ki;
 }
@@ -241,9 +241,7 @@ __device__ __forceinline__ void kd_single(Lane& L, double len, int n_step, doubl
   }
 }
 
-__device__ __forceinline__ void op_kd_slices(Lane& L, const DevOp& op) {
-  int n_step = __builtin_amdgcn_readfirstlane(op.n);
-  int fl = __builtin_amdgcn_readfirstlane(op.flags);
+__device__ __forceinline__ void op_kd_slices(Lane& L, const DevOp& op, int n_step, int fl) {
   int nh = fl & 0xff, lo = (fl >> 8) & 0xff;
   double len = op.c[0];
   if (nh == 0 && lo == 2) {
@@ -270,9 +268,9 @@ __device__ __forceinline__ void op_kd_slices(Lane& L, const DevOp& op) {
 }
 
 // apply_multipole_kicks of track_a_bend (track_a_bend.f90:338-471): kick scaled by (1 + g x).
-__device__ __forceinline__ void op_bend_kick(Lane& L, const DevOp& op) {
+__device__ __forceinline__ void op_bend_kick(Lane& L, const DevOp& op, int nh, int lo) {
   double kr, ki;
-  kick_eval(op, L.x, L.y, kr, ki);
+  kick_eval(op, nh, lo, L.x, L.y, kr, ki);
   double f = 1.0 + op.c[13] * L.x;
   L.px -= f * kr;
   L.py += f * ki;
@@ -631,12 +629,12 @@ __device__ void op_bend_k1(Lane& L, const DevOp& op, const LatticeConsts& lc) {
 
 // hwang_bend_edge_kick (bmad/modules/fringe_mod.f90:240-418), time_dir = +1.
 // c: 0 gt, 1 gt2, 2 gs2, 3 k1_tane, 4 fg_factor, 5 g_tot, 6 tan_e
-__device__ __forceinline__ void op_hwang(Lane& L, const DevOp& op) {
+__device__ __forceinline__ void op_hwang(Lane& L, const DevOp& op, int flags) {
   double gt = op.c[0], gt2 = op.c[1], gs2 = op.c[2], k1t = op.c[3], fg = op.c[4], g_tot = op.c[5], te = op.c[6];
   double ef = L.irp;
   double v0 = L.x, v1 = L.px, v2 = L.y, v3 = L.py;
   double dx, dpx, dy, dpy, dz;
-  if (__builtin_amdgcn_readfirstlane(op.flags) & F_HWANG_ENTER) {
+  if (flags & F_HWANG_ENTER) {
     dx = (-gt2 * v0 * v0 + gs2 * v2 * v2) * ef / 2.0;
     dpx = (gt * g_tot * (1.0 + 2.0 * te * te) * v2 * v2 / 2.0 + gt2 * (v0 * v1 - v2 * v3) + k1t * (v0 * v0 - v2 * v2)) * ef;
     dy = gt2 * v0 * v2 * ef;
@@ -678,7 +676,7 @@ __device__ __forceinline__ void op_tilt(Lane& L, const DevOp& op) {
 
 // offset_particle for a straight element, set (offset_particle.f90:170-200, 255-282)
 // and unset (:410-470); c: 0..2 offsets, 3 L/2, 4..12 W (row-major).
-__device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set) {
+__device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set, int flags) {
   double rel_p = 1.0 + L.pz;
   double r0, r1, r2;
   if (set) {
@@ -691,7 +689,7 @@ __device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set) {
     r2 = 0.0 + op.c[3];  // r(3) - ds_center with ds_center = -L/2
   }
   double s_pos;  // s_body (set) or s_lab (unset)
-  if (__builtin_amdgcn_readfirstlane(op.flags) & F_OFFSET_PITCH) {
+  if (flags & F_OFFSET_PITCH) {
     const double* w = &op.c[4];
     double n0 = w[0] * r0 + w[1] * r1 + w[2] * r2;
     double n1 = w[3] * r0 + w[4] * r1 + w[5] * r2;
@@ -742,9 +740,8 @@ __device__ __forceinline__ void energy_kick(Lane& L, double dE, const LatticeCon
 // track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140), relative time:
 // particle_rf_time = -z/(beta c) (bmad/code/particle_rf_time.f90:89).
 // c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f); n = n_slice
-__device__ void op_rfcavity(Lane& L, const DevOp& op, const LatticeConsts& lc) {
+__device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, const LatticeConsts& lc) {
   double len = op.c[0], volt = op.c[1], freq = op.c[3], half_t = op.c[4];
-  int n_slice = __builtin_amdgcn_readfirstlane(op.n);
   double time = -L.z / (L.beta * D_C_LIGHT);
   if (freq != 0.0 && !(time >= -half_t && time < half_t)) time = d_modulo2(time, half_t);
   double phase = d_modulo2(D_TWOPI * (op.c[2] - time * freq), D_PI);
@@ -874,31 +871,31 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
 // into a switch.  The tests are grouped so that an op passes about half of
 // them.
 #define TB(t) (1u << (t))
-__device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, const LatticeConsts& lc) {
+__device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, int n, const LatticeConsts& lc) {
   unsigned tb = (unsigned)__builtin_amdgcn_readfirstlane((int)(1u << type));
   if (tb & (TB(OP_DRIFT) | TB(OP_KICK) | TB(OP_KD_SLICES))) {
     if (tb & TB(OP_DRIFT)) op_drift(L, op.c[0]);
-    if (tb & TB(OP_KICK)) op_kick(L, op);
-    if (tb & TB(OP_KD_SLICES)) op_kd_slices(L, op);
+    if (tb & TB(OP_KICK)) op_kick(L, op, flags & 0xff, n);
+    if (tb & TB(OP_KD_SLICES)) op_kd_slices(L, op, n, flags);
   }
   if (tb & (TB(OP_QUAD) | TB(OP_BEND) | TB(OP_SBEND_FUSED))) {
     if (tb & TB(OP_QUAD)) op_quad(L, op, lc);
     if (tb & (TB(OP_BEND) | TB(OP_SBEND_FUSED))) op_sbend(L, op, flags, (tb & TB(OP_SBEND_FUSED)) != 0, lc);
   }
   if (tb & (TB(OP_HWANG) | TB(OP_RFCAVITY))) {
-    if (tb & TB(OP_HWANG)) op_hwang(L, op);
-    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, lc);
+    if (tb & TB(OP_HWANG)) op_hwang(L, op, flags);
+    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, n, lc);
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN);
   if (tb & RARE) {
     if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
     if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
-    if (tb & TB(OP_BEND_KICK)) op_bend_kick(L, op);
+    if (tb & TB(OP_BEND_KICK)) op_bend_kick(L, op, flags & 0xff, n);
     if (tb & TB(OP_LINEAR_EDGE)) op_linear_edge(L, op);
     if (tb & TB(OP_TILT)) op_tilt(L, op);
-    if (tb & TB(OP_OFFSET_SET)) op_offset(L, op, true);
-    if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false);
+    if (tb & TB(OP_OFFSET_SET)) op_offset(L, op, true, flags);
+    if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false, flags);
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
     if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
   }
