@@ -14,7 +14,8 @@
 // same op functions in the same order.
 //
 // Compiled modules are kept per (device, cell) for the life of the process.
-// BMX_JIT=0 in the environment turns the cell kernels off.
+// BMX_JIT=0 in the environment turns the cell kernels off; BMX_JIT_FLAGS adds
+// compiler options (e.g. -DBMX_WAVES_PER_EU=3).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -177,7 +178,8 @@ bool plan_cells(std::vector<DevOp>& prog, CellPlan* plan) {
 
 const JitKernels* jit_kernels(int device, const std::vector<DevOp>& prog, const CellPlan& plan, std::string* err) {
   std::lock_guard<std::mutex> lock(g_mu);
-  auto key = std::make_pair(device, plan.key);
+  const char* fl = std::getenv("BMX_JIT_FLAGS");
+  auto key = std::make_pair(device, plan.key + "|" + (fl ? fl : ""));
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return &it->second.k;
   hipDeviceProp_t prop;
@@ -195,9 +197,17 @@ const JitKernels* jit_kernels(int device, const std::vector<DevOp>& prog, const 
     return nullptr;
   }
   std::string arch_opt = "--offload-arch=" + arch;
-  const char* opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops"};
-  hiprtcResult r = hiprtcCompileProgram(p, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  std::vector<std::string> extra;  // BMX_JIT_FLAGS: further compiler options, space separated
+  if (const char* e = std::getenv("BMX_JIT_FLAGS")) {
+    std::string f(e), w;
+    for (char ch : f + " ") {
+      if (ch == ' ') { if (!w.empty()) extra.push_back(w); w.clear(); } else w += ch;
+    }
+  }
+  std::vector<const char*> opts = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                                   "-packed-fp32-ops"};
+  for (auto& x : extra) opts.push_back(x.c_str());
+  hiprtcResult r = hiprtcCompileProgram(p, (int)opts.size(), opts.data());
   if (r != HIPRTC_SUCCESS) {
     size_t ls = 0;
     hiprtcGetProgramLogSize(p, &ls);
