@@ -133,6 +133,14 @@ interface
     type(c_ptr), value :: lat
   end subroutine
 
+  function bmx_lattice_cells(lat, cell_len, n_cells) bind(C, name = 'bmx_lattice_cells') result(rc)
+    import
+    type(c_ptr), value :: lat
+    integer(c_int32_t), intent(out) :: cell_len
+    integer(c_int64_t), intent(out) :: n_cells
+    integer(c_int32_t) :: rc
+  end function
+
   function bmx_track_bunch(ctx, lat, ix_start, ix_end, n_turns, direction, p, stats) &
                            bind(C, name = 'bmx_track_bunch') result(rc)
     import
