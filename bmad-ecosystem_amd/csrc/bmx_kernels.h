@@ -85,7 +85,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
         auto loss_check = [&](const DevOp& op) {
 #pragma unroll
           for (int j = 0; j < NP; j++) {
-            if (P.L[j].state != BMX_ALIVE) {
+            if (__builtin_expect(P.L[j].state != BMX_ALIVE, 0)) {
               if (P.live[j]) {
                 P.live[j] = false;
                 P.ie_lost[j] = op.ie;

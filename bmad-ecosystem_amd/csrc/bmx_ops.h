@@ -88,6 +88,11 @@ __device__ __forceinline__ double poly(const cdouble* c, double x) {
 // exec, on every op whether the block runs or not).
 __device__ __forceinline__ void uniform_block() { asm volatile(""); }
 
+// The rare paths (a lane off an op's common path, a loss, an exact-form
+// fallback) are marked unlikely, so that their code is laid out after the
+// hot code of the element loop and the loop stays compact in the
+// instruction cache.
+
 // true on every lane of the wave when b holds on any active lane
 __device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
@@ -136,7 +141,7 @@ __device__ __forceinline__ void drift_c(Lane& L, double len, double d0, double d
   double u = pxr * pxr + pyr * pyr;
   double S = u * (0.5 + u * (0.375 + u * (d0 + u * (d1 + u * d2))));
   bool wide = u >= 1e-4;
-  if (wave_any(wide)) {
+  if (__builtin_expect(wave_any(wide), 0)) {
     bool move = true;
     if (wide) {
       if (u >= 1.0) {
@@ -325,7 +330,7 @@ __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeC
   double kx = -k1, ky = k1;     // quad_mat2_calc(-k1, ...) for x, quad_mat2_calc(k1, ...) for y
   double cx, sx, cy, sy;
   double w = k1 * len * len;
-  if (fabs(w) <= 1.0) {
+  if (__builtin_expect(fabs(w) <= 1.0, 1)) {
     quad_cs_pair(w, len, cy, sy, cx, sx);
   } else {
     double sqrt_k = sqrt(fabs(k1));
@@ -564,7 +569,7 @@ __device__ __forceinline__ void op_bend(Lane& L, const DevOp& op, const LatticeC
   bool common = !paraxial & (p2 > 0.25) & (p2 < 4.0) & (fabs(sphi) <= 0.05) & (fabs(angle) <= 0.15) & (r >= 0.25) &
                 (r <= 4.0) & (fabs(xn) <= lc.max_aperture_limit) & (L_v < -1e-100) & (L_c2 < 1e200) &
                 (fabs(L_u) <= -0.2 * L_v) & (fabs(half_p) <= 0.1) & (fabs(e) <= 0.2);
-  if (wave_any(!common)) {
+  if (__builtin_expect(wave_any(!common), 0)) {
     L = bend_general(L, op.c, lc.p0c, lc.mc2sq, lc.max_aperture_limit, lc.two_max_ap);
     return;
   }
@@ -769,7 +774,7 @@ __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
   double rel_p = 1.0 + L.pz;
   double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
   bool bad = (fabs(L.x) > mal) | (fabs(L.y) > mal) | (rel_p < 0.0) | (f > 0.0);
-  if (!bad) return false;
+  if (__builtin_expect(!bad, 1)) return false;
   if (L.x > mal) { L.state = BMX_LOST_POS_X; return true; }
   if (-L.x > mal) { L.state = BMX_LOST_NEG_X; return true; }
   if (L.y > mal) { L.state = BMX_LOST_POS_Y; return true; }
@@ -789,7 +794,7 @@ __device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptic
     // screen with the reciprocal widths: it is within a few ulp of the
     // reference's r, so r_fast < 1 - 1e-13 decides r < 1 exactly
     double fx = xp * c[2], fy = yp * c[3];
-    if (fx * fx + fy * fy < 1.0 - 1e-13) return;
+    if (__builtin_expect(fx * fx + fy * fy < 1.0 - 1e-13, 1)) return;
     double ax = xp / c[4], ay = yp / c[5];
     double r = ax * ax + ay * ay;
     if (r < 1.0) return;
@@ -855,7 +860,7 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
     }
   }
   bool check = (L.state == BMX_ALIVE) & !inside;
-  if (wave_any(check)) {
+  if (__builtin_expect(wave_any(check), 0)) {
     if (check && !too_large(L, lc) && (flags & F_END_APERTURE)) aperture(L, c, flags & F_END_ELLIPTICAL);
   }
 }
