@@ -54,6 +54,11 @@ enum : int32_t {
   F_SHIFT_OUT = 1 << 19,
   // first op of a run of the lattice's repeating cell (runtime-compiled cell kernels)
   F_CELL = 1 << 20,
+  // with F_END_APERTURE: the aperture lies inside the max_aperture_limit box
+  // (its screen implies orbit_too_large's position limits), and (elliptical)
+  // it is centred on the axis
+  F_END_INBOX = 1 << 21,
+  F_END_CENTERED = 1 << 22,
 };
 
 // aperture constants of an op carrying F_END (6 slots, see aperture())

@@ -101,7 +101,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
             const DevOp& op = sops[k];
             int flags = __builtin_amdgcn_readfirstlane(op.flags);
 #ifdef BMX_CELL_LEN
-            if ((flags & F_CELL) && k + BMX_CELL_LEN <= m) {
+            if (__builtin_expect((flags & F_CELL) && k + BMX_CELL_LEN <= m, 1)) {
               static_assert(NP == 1, "cell kernels carry one particle per lane");
               jit_cell(P.L[0], sops + k, lc, loss_check);
               k += BMX_CELL_LEN;

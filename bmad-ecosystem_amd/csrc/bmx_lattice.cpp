@@ -593,7 +593,20 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       last = &prog.back();
     }
     last->flags |= end_flags;
-    if (exit_ap) ap_consts(*last, END_C);
+    if (exit_ap) {
+      ap_consts(*last, END_C);
+      const double* c = last->c + END_C;
+      double mal = cfg.max_aperture_limit * (1.0 - 1e-12);
+      bool inbox, centered = false;
+      if (ap_flags & F_APERTURE_ELLIPTICAL) {  // |x - cx| < wx on the screen
+        inbox = std::fabs(c[0]) + c[4] < mal && std::fabs(c[1]) + c[5] < mal;
+        centered = c[0] == 0.0 && c[1] == 0.0;
+      } else {  // x1 <= x <= x2, y1 <= y <= y2 on the screen when all four limits are set
+        inbox = c[0] != 0.0 && c[1] != 0.0 && c[2] != 0.0 && c[3] != 0.0 && std::fabs(c[0]) < mal &&
+                std::fabs(c[1]) < mal && std::fabs(c[2]) < mal && std::fabs(c[3]) < mal;
+      }
+      last->flags |= (inbox ? F_END_INBOX : 0) | (centered ? F_END_CENTERED : 0);
+    }
   }
   ele_op_begin.push_back((int32_t)prog.size());
   (void)kCLight;

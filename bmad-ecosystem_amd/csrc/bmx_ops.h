@@ -849,10 +849,20 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
   double mal = lc.max_aperture_limit;
   double rel_p = 1.0 + L.pz;
   double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
-  bool inside = (fabs(L.x) <= mal) & (fabs(L.y) <= mal) & (rel_p >= 0.0) & (f <= 0.0);
+  bool inside = (rel_p >= 0.0) & (f <= 0.0);
+  // F_END_INBOX: the aperture lies inside the max_aperture_limit box, so its
+  // screen implies |x|, |y| <= max_aperture_limit
+  if (!(flags & F_END_INBOX)) inside &= (fabs(L.x) <= mal) & (fabs(L.y) <= mal);
   if (flags & F_END_APERTURE) {
     if (flags & F_END_ELLIPTICAL) {
-      double fx = (L.x - c[0]) * c[2], fy = (L.y - c[1]) * c[3];
+      double fx, fy;
+      if (flags & F_END_CENTERED) {  // cx = cy = 0: x - 0 = x exactly
+        fx = L.x * c[2];
+        fy = L.y * c[3];
+      } else {
+        fx = (L.x - c[0]) * c[2];
+        fy = (L.y - c[1]) * c[3];
+      }
       inside &= fx * fx + fy * fy < 1.0 - 1e-13;
     } else {
       inside &= (c[0] == 0.0 || L.x >= c[0]) & (c[1] == 0.0 || L.x <= c[1]) & (c[2] == 0.0 || L.y >= c[2]) &
