@@ -262,7 +262,8 @@ int used_consts(const DevOp& op) {
   switch (op.type) {
     case OP_ELE_END: return 0;
     case OP_DRIFT: return 1;
-    case OP_QUAD: case OP_TILT: return 2;
+    case OP_QUAD: return 3;
+    case OP_TILT: return 2;
     case OP_KICK: return 2 * (nh + 1);
     case OP_KD_SLICES: return 1 + 2 * (nh + 1);
     case OP_RFCAVITY: return 5;
@@ -419,6 +420,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           DevOp op = make_op(OP_QUAD, ie);
           op.c[0] = step_len;
           op.c[1] = b1 / e.l;  // k1 * rel_p
+          op.c[2] = op.c[1] * step_len * step_len;
           prog.push_back(op);
           emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
         }

@@ -123,9 +123,8 @@ __device__ __forceinline__ double d_modulo2(double x, double amp) {
 // track_a_drift (bmad/low_level/track_a_drift.f90:25-116), forward, with reference motion.
 // With u = (px^2 + py^2)/(1+pz)^2 and S = 1/sqrt(1-u) - 1:
 //   x += L px_rel (1 + S),  z += L (beta/beta0 - 1 - S)   [sqrt_one(-u)/ps = -S].
-// For u < 1e-4 (angles below 10 mrad) S is summed as its binomial series to
-// u^5 (truncation < 3e-25), which needs no division or square root; larger
-// angles take the exact form.
+// For u < 1e-4 (angles below 10 mrad) S is summed as its binomial series,
+// which needs no division or square root; larger angles take the exact form.
 __device__ __forceinline__ void drift_apply(Lane& L, double len, double pxr, double pyr, double S) {
   double lps = len * (1.0 + S);
   L.x += lps * pxr;
@@ -133,18 +132,22 @@ __device__ __forceinline__ void drift_apply(Lane& L, double len, double pxr, dou
   L.z += len * (L.bb - S);
 }
 
-// The series is evaluated for every lane; only a wave holding a lane at
-// u >= 1e-4 takes the exact form (and the lost_pz test, u >= 1, which leaves
-// the particle where it is) for those lanes.
+// Every lane sums the series to u^3, exact to 3e-25 for u < 1e-6 (angles
+// below 1 mrad, the usual case).  Only a wave holding a lane at u >= 1e-6
+// takes, for those lanes, the series to u^5 (truncation < 3e-25 for
+// u < 1e-4), the exact form, or the lost_pz test (u >= 1, which leaves the
+// particle where it is).
 __device__ __forceinline__ void drift_c(Lane& L, double len, double d0, double d1, double d2) {
   double pxr = L.px * L.irp, pyr = L.py * L.irp;
   double u = pxr * pxr + pyr * pyr;
-  double S = u * (0.5 + u * (0.375 + u * (d0 + u * (d1 + u * d2))));
-  bool wide = u >= 1e-4;
+  double S = u * (0.5 + u * (0.375 + u * d0));
+  bool wide = u >= 1e-6;
   if (__builtin_expect(wave_any(wide), 0)) {
     bool move = true;
     if (wide) {
-      if (u >= 1.0) {
+      if (u < 1e-4) {
+        S = u * (0.5 + u * (0.375 + u * (d0 + u * (d1 + u * d2))));
+      } else if (u >= 1.0) {
         L.state = BMX_LOST_PZ;
         move = false;
       } else {
@@ -323,13 +326,15 @@ __device__ __forceinline__ void quad_cs_pair(double w, double len, double& cp, d
   sm = (es - w * os) * len;
 }
 
+// c: 0 step length L, 1 k1 (1+pz) = b1 / L, 2 k1 (1+pz) L^2
 __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeConsts& lc) {
   double len = op.c[0];
   double rel_p = 1.0 + L.pz;
-  double k1 = op.c[1] * L.irp;  // charge_dir b1 / (L rel_p)
-  double kx = -k1, ky = k1;     // quad_mat2_calc(-k1, ...) for x, quad_mat2_calc(k1, ...) for y
+  double irp = L.irp;
+  double k1 = op.c[1] * irp;  // charge_dir b1 / (L rel_p)
+  double kx = -k1, ky = k1;   // quad_mat2_calc(-k1, ...) for x, quad_mat2_calc(k1, ...) for y
   double cx, sx, cy, sy;
-  double w = k1 * len * len;
+  double w = op.c[2] * irp;   // k1 L^2
   if (__builtin_expect(fabs(w) <= 1.0, 1)) {
     quad_cs_pair(w, len, cy, sy, cx, sx);
   } else {
@@ -344,15 +349,21 @@ __device__ __forceinline__ void op_quad(Lane& L, const DevOp& op, const LatticeC
     if (k1 > 0.0) { cx = cs; sx = sn * isk; cy = ch; sy = sh * isk; }  // x focusing
     else          { cx = ch; sx = sh * isk; cy = cs; sy = sn * isk; }  // y focusing
   }
-  double irp = L.irp, irp2 = irp * irp;
-  double zx0 = kx * (-cx * sx + len) / 4.0, zx1 = -kx * sx * sx * irp / 2.0, zx2 = -(cx * sx + len) * irp2 / 4.0;
-  double zy0 = ky * (-cy * sy + len) / 4.0, zy1 = -ky * sy * sy * irp / 2.0, zy2 = -(cy * sy + len) * irp2 / 4.0;
+  // z: per plane the reference's three terms c1 x^2 + c2 x px + c3 px^2 with
+  // c1 = k (L - c s)/4, c2 = -k s^2 / (2 (1+pz)), c3 = -(L + c s) / (4 (1+pz)^2)
+  // (track_a_quadrupole.f90:82-112), written with p = px / (1+pz) as
+  // [x (k (L - c s) x - 2 k s^2 p) - (L + c s) p^2] / 4
   double x = L.x, px = L.px, y = L.y, py = L.py;
-  L.z += zx0 * x * x + zx1 * x * px + zx2 * px * px + zy0 * y * y + zy1 * y * py + zy2 * py * py;
-  L.x = cx * x + (sx * irp) * px;
-  L.px = (kx * sx * rel_p) * x + cx * px;
-  L.y = cy * y + (sy * irp) * py;
-  L.py = (ky * sy * rel_p) * y + cy * py;
+  double pxr = px * irp, pyr = py * irp;
+  double ax = cx * sx, ay = cy * sy;
+  double kxs = kx * sx, kys = ky * sy;
+  double zx = x * (kx * (len - ax) * x - 2.0 * kxs * sx * pxr) - (len + ax) * pxr * pxr;
+  double zy = y * (ky * (len - ay) * y - 2.0 * kys * sy * pyr) - (len + ay) * pyr * pyr;
+  L.z += 0.25 * (zx + zy);
+  L.x = cx * x + sx * pxr;
+  L.px = (kxs * rel_p) * x + cx * px;
+  L.y = cy * y + sy * pyr;
+  L.py = (kys * rel_p) * y + cy * py;
   L.z += low_energy_dz(L, lc, len);
 }
 

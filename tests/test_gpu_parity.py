@@ -143,6 +143,11 @@ def test_c1_ring_with_rf_one_turn(ctx):
     coords = beam.matched_gaussian(2048, tw, 10e-9, 0.1e-9, 1e-3, 1e-3, seed=3)
     g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1)
     r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, n_threads=8)
+    # the oracle's own spread between two FP-legal evaluation orders (plain and
+    # FMA-contracted) on the same particles: the floor of this metric
+    f = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, n_threads=8, fma=True)
+    floor = coord_errors(f, r, r["state"] == abi.ALIVE, _path(eles, 1))
+    print("oracle plain vs fma:", {k: f"{v:.2e}" for k, v in floor.items()}, flush=True)
     _compare(g, r, _path(eles, 1))
 
 
