@@ -94,7 +94,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 # every entry point declared in include/bmx.h
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
-    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_lattice_cells", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
+    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
     "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
     "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_step_fit", "bmx_ctx_sync",
@@ -126,6 +126,9 @@ def load_library(path=None):
                                        C.POINTER(Ref), C.POINTER(Config), C.POINTER(C.c_void_p)]
     lib.bmx_lattice_destroy.restype = None
     lib.bmx_lattice_destroy.argtypes = [C.c_void_p]
+    lib.bmx_lattice_check.restype = C.c_int32
+    lib.bmx_lattice_check.argtypes = [C.POINTER(EleRec), C.c_int32, C.POINTER(Ref), C.POINTER(Config),
+                                      C.POINTER(C.c_int64), C.c_char_p, C.c_int32]
     lib.bmx_lattice_cells.restype = C.c_int32
     lib.bmx_lattice_cells.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
     lib.bmx_track_bunch.restype = C.c_int32

@@ -67,6 +67,19 @@ class Context:
         return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit)
 
 
+def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3):
+    """(ok, n_ops, reason): whether the device path accepts this element list,
+    decided on the host with no device (bmx_lattice_check)."""
+    lib = abi.load_library()
+    arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
+    ref = abi.Ref(species, 0)
+    cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit)
+    n_ops = C.c_int64(0)
+    msg = C.create_string_buffer(512)
+    rc = lib.bmx_lattice_check(arr, len(arr), C.byref(ref), C.byref(cfg), C.byref(n_ops), msg, len(msg))
+    return rc == 0, n_ops.value, msg.value.decode()
+
+
 class Lattice:
     """A compiled, device-resident element program (bmx_lattice_upload)."""
 

@@ -24,7 +24,7 @@ enum OpType : int32_t {
   OP_BEND_KICK = 7,      // multipole kick inside a bend, scaled by (1 + g x)
   OP_HWANG = 8,          // Hwang bend edge kick (basic_bend / hard_edge_only)
   OP_LINEAR_EDGE = 9,    // linear bend edge kick
-  OP_HARD_MULT_EDGE = 10,// hard multipole edge kick of quadrupole / sextupole
+  OP_HARD_MULT_EDGE = 10,// hard multipole edge kick of quadrupole / sextupole, n = 1 / 2
   OP_OFFSET_SET = 11,    // lab -> body frame (offsets, pitches, tilt)
   OP_OFFSET_UNSET = 12,  // body -> lab frame
   OP_TOO_LARGE = 13,     // orbit_too_large check inside an element (early return)
@@ -39,8 +39,9 @@ enum OpType : int32_t {
 // op flags
 enum : int32_t {
   F_APERTURE_ELLIPTICAL = 1,
-  F_HWANG_ENTER = 2,      // OP_HWANG / OP_HARD_MULT_EDGE at the first track edge
+  F_HWANG_ENTER = 2,      // OP_HWANG at the first track edge
   F_OFFSET_PITCH = 4,     // OP_OFFSET_*: general rotation, else pure translation
+  F_OFFSET_AFFINE = 8,    // OP_OFFSET_*: bend frame change, r = W (x, y, 0) + t, t at c[0..2]
   F_OFFSET_MOVE = 32,     // OP_OFFSET_*: has a translation/pitch part
   F_EDGE_EXIT = 64,       // OP_SBEND_FUSED: exit edge present (F_HWANG_ENTER: entrance edge present)
   // the element ends after this op's body: orbit_too_large, then (with

@@ -251,6 +251,128 @@ void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool se
   if (set && tilt) prog.push_back(t);
 }
 
+// Frame changes of offset_particle for a bend (bmad/code/offset_particle.f90:
+// 208-235 set, :409-440 unset).  Every step there is a rotation S and a shift
+// applied to the position r and (the rotation alone) to the orientation w, so
+// the whole chain from (x, y, 0) is r = W (x, y, 0) + t with W the final w.
+struct Affine {
+  double w[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double t[3] = {0, 0, 0};
+  void rotate(const double S[3][3]) {  // r -> S r, w -> S w
+    double w2[3][3], t2[3];
+    for (int i = 0; i < 3; i++) {
+      t2[i] = S[i][0] * t[0] + S[i][1] * t[1] + S[i][2] * t[2];
+      for (int j = 0; j < 3; j++) w2[i][j] = S[i][0] * w[0][j] + S[i][1] * w[1][j] + S[i][2] * w[2][j];
+    }
+    std::memcpy(w, w2, sizeof(w));
+    std::memcpy(t, t2, sizeof(t));
+  }
+};
+
+void mat3_unit(double m[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m[i][j] = (i == j) ? 1.0 : 0.0;
+}
+
+// rotate_mat (sim_utils/math/rotation_3d_mod.f90), left multiplication:
+// m = R m with R the rotation by `angle` about the y (axis 1) or z (axis 2) axis
+void rotate_left(double m[3][3], int axis, double angle) {
+  if (angle == 0) return;
+  double ca = std::cos(angle), sa = std::sin(angle);
+  int a = 0, b = (axis == 1) ? 2 : 1;  // rows mixed: (x, z) for y, (x, y) for z
+  double sgn = (axis == 1) ? -1.0 : 1.0;  // y: x' = ca x + sa z; z: x' = ca x - sa y
+  for (int j = 0; j < 3; j++) {
+    double u = m[a][j], v = m[b][j];
+    m[a][j] = ca * u - sgn * sa * v;
+    m[b][j] = sgn * sa * u + ca * v;
+  }
+}
+
+// bend_shift (bmad/geometry/bend_shift.f90): the frame delta_s further along
+// a reference arc of curvature g, rolled by ref_tilt when has_tilt
+void bend_shift(Affine& p, double g, double delta_s, bool has_tilt, double ref_tilt) {
+  double angle = delta_s * g;
+  if (angle == 0) {
+    p.t[2] -= delta_s;
+    return;
+  }
+  double tlt = has_tilt ? ref_tilt : 0.0;
+  double S[3][3];
+  mat3_unit(S);
+  if (tlt != 0) {
+    rotate_left(S, 2, -ref_tilt);
+    rotate_left(S, 1, angle);
+    rotate_left(S, 2, ref_tilt);
+  } else {
+    rotate_left(S, 1, angle);
+  }
+  double sh = std::sin(0.5 * angle);
+  double L[3] = {-2.0 * sh * sh / g, 0.0, -std::sin(angle) / g};  // cos_one(angle) = -2 sin^2(angle/2)
+  if (has_tilt && ref_tilt != 0) {
+    double c = std::cos(ref_tilt), s = std::sin(ref_tilt), l0 = L[0], l1 = L[1];
+    L[0] = c * l0 - s * l1;
+    L[1] = s * l0 + c * l1;
+  }
+  p.rotate(S);
+  for (int i = 0; i < 3; i++) p.t[i] += L[i];
+}
+
+// floor_angles_to_w_mat (bmad/geometry/floor_angles_to_w_mat.f90), psi = 0
+void pitch_w(double theta, double phi, bool inverse, double w[3][3]) {
+  double st = std::sin(theta), ct = std::cos(theta), sp = std::sin(phi), cp = std::cos(phi);
+  double m[3][3] = {{ct, -st * sp, st * cp}, {0.0, cp, sp}, {-st, -ct * sp, ct * cp}};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) w[i][j] = inverse ? m[j][i] : m[i][j];
+}
+
+// offset_particle of a misaligned bend (g != 0 or ref_tilt != 0), direction
+// and orientation +1, drift to the edge: one OP_OFFSET_* with F_OFFSET_AFFINE,
+// c[0..2] = t, c[3] = L/2, c[4..12] = W row-major.
+void emit_bend_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set) {
+  double g = e.g, L_half = 0.5 * e.l, rt = e.ref_tilt;
+  double ws[3][3], wt[3][3];
+  Affine p;
+  if (set) {
+    bend_shift(p, g, L_half, true, rt);  // ds_center = L/2 (s_pos0 = 0)
+    p.t[0] -= e.x_offset;
+    p.t[1] -= e.y_offset;
+    p.t[2] -= e.z_offset;
+    pitch_w(e.x_pitch, e.y_pitch, true, ws);  // ele_misalignment_L_S_calc, transposed
+    p.rotate(ws);
+    if (rt != 0) {
+      bend_shift(p, g, -L_half, true, rt);
+      mat3_unit(wt);
+      rotate_left(wt, 2, -rt);  // w_mat_for_tilt(-ref_tilt)
+      p.rotate(wt);
+      bend_shift(p, g, L_half, false, 0.0);
+    }
+    bend_shift(p, g, -L_half, false, 0.0);  // drift_to upstream: r(3) += 0
+  } else {
+    bend_shift(p, g, -L_half, false, 0.0);  // ds_center = -L/2 (s_pos0 = L)
+    if (rt != 0) {
+      bend_shift(p, g, -L_half, false, 0.0);
+      mat3_unit(wt);
+      rotate_left(wt, 2, rt);  // w_mat_for_tilt(ref_tilt)
+      p.rotate(wt);
+      bend_shift(p, g, L_half, true, rt);
+    }
+    pitch_w(e.x_pitch, e.y_pitch, false, ws);
+    p.rotate(ws);
+    p.t[0] += e.x_offset;
+    p.t[1] += e.y_offset;
+    p.t[2] += e.z_offset;
+    bend_shift(p, g, L_half, true, rt);  // drift_to downstream
+    p.t[2] += e.l;
+  }
+  DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
+  op.flags |= F_OFFSET_MOVE | F_OFFSET_PITCH | F_OFFSET_AFFINE;
+  for (int i = 0; i < 3; i++) op.c[i] = p.t[i];
+  op.c[3] = L_half;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) op.c[4 + 3 * i + j] = p.w[i][j];
+  prog.push_back(op);
+}
+
 }  // namespace
 
 namespace {
@@ -263,7 +385,7 @@ int used_consts(const DevOp& op) {
     case OP_ELE_END: return 0;
     case OP_DRIFT: return 1;
     case OP_QUAD: return 3;
-    case OP_TILT: return 2;
+    case OP_TILT: case OP_HARD_MULT_EDGE: return 2;
     case OP_KICK: return 2 * (nh + 1);
     case OP_KD_SLICES: return 1 + 2 * (nh + 1);
     case OP_RFCAVITY: return 5;
@@ -358,6 +480,22 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       if (key == BMX_KEY_MULTIPOLE || key == BMX_KEY_AB_MULTIPOLE) key = BMX_KEY_MARKER;
       else if (key != BMX_KEY_SBEND) key = BMX_KEY_DRIFT;
     }
+    // hard_multipole_edge_kick of a quadrupole (n = 1, b = k1) or sextupole
+    // (n = 2, b = k2) at one track edge, where fringe_type and fringe_at put
+    // one (fringe_mod.f90:714-721, fringe_here.f90:15-35); the soft
+    // quadrupole edge kick has no strength here (fq1 = fq2 = 0)
+    auto hard_edge = [&](int n, double b, bool enter) {
+      if (e.fringe_type != BMX_FRINGE_HARD_EDGE_ONLY && e.fringe_type != BMX_FRINGE_FULL) return;
+      bool here = enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
+                        : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS);
+      if (!here) return;
+      DevOp op = make_op(OP_HARD_MULT_EDGE, ie);
+      op.n = n;
+      double c0 = b / (4.0 * (n + 2));
+      op.c[0] = enter ? -c0 : c0;
+      op.c[1] = (double)(n + 3) / (double)(n + 1);
+      prog.push_back(op);
+    };
     std::string e2;
     switch (key) {
       case BMX_KEY_MARKER:
@@ -404,8 +542,6 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       }
       case BMX_KEY_QUADRUPOLE: {
         // track_a_quadrupole.f90:17-148
-        if (e.fringe_type != BMX_FRINGE_NONE && e.fringe_at != BMX_NO_END)
-          return fail("quadrupole fringe kicks are not supported on the device path");
         AB ab;
         double b1;
         if (!magnetic_ab(e, true, &ab, &b1, &e2)) return fail(e2.c_str());
@@ -415,6 +551,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         double step_len = e.l / (n_step > 0 ? n_step : 1);
         double r_step = 1.0 / (n_step > 0 ? n_step : 1);
         emit_offset(prog, ie, e, true, true);
+        hard_edge(1, e.k1, true);
         emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
         for (int i = 1; i <= n_step; i++) {
           DevOp op = make_op(OP_QUAD, ie);
@@ -424,6 +561,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           prog.push_back(op);
           emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
         }
+        hard_edge(1, e.k1, false);
         emit_offset(prog, ie, e, false, true);
         break;
       }
@@ -431,14 +569,12 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       case BMX_KEY_VKICKER: case BMX_KEY_KICKER: case BMX_KEY_INSTRUMENT: case BMX_KEY_MONITOR:
       case BMX_KEY_ECOLLIMATOR: case BMX_KEY_RCOLLIMATOR: case BMX_KEY_PIPE: {
         // track_a_thick_multipole.f90:18-133
-        if (key == BMX_KEY_SEXTUPOLE && e.fringe_type != BMX_FRINGE_NONE && e.fringe_at != BMX_NO_END &&
-            (e.fringe_type == BMX_FRINGE_FULL || e.fringe_type == BMX_FRINGE_HARD_EDGE_ONLY))
-          return fail("sextupole hard-edge fringe is not supported on the device path");
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
         int n_step = (ab.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
         double step_len = e.l / n_step, r_step = 1.0 / n_step;
         emit_offset(prog, ie, e, true, true);
+        if (key == BMX_KEY_SEXTUPOLE) hard_edge(2, e.k2, true);
         int lo = 0;
         while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
         if (ab.ix_max >= 0 && step_len != 0 && ab.ix_max - lo < 6) {
@@ -460,12 +596,29 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
             emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
           }
         }
+        if (key == BMX_KEY_SEXTUPOLE) hard_edge(2, e.k2, false);
         emit_offset(prog, ie, e, false, true);
         break;
       }
       case BMX_KEY_SBEND: {
         // track_a_bend.f90:18-333
-        if (is_misaligned(e) || e.ref_tilt != 0) return fail("sbend misalignment / ref_tilt is not supported on the device path");
+        // offset_particle (has_misalign: offsets, pitches or ref_tilt); a bend
+        // ignores the element tilt (it has ref_tilt instead)
+        bool mis = is_misaligned(e), has_mis = mis || e.ref_tilt != 0;
+        auto bend_offset = [&](bool set) {
+          if (!has_mis) return;
+          if (!mis) {  // tilt_coords(+-ref_tilt) alone (offset_particle.f90:249-251, :472-474)
+            DevOp t = make_op(OP_TILT, ie);
+            t.c[0] = std::cos(set ? e.ref_tilt : -e.ref_tilt);
+            t.c[1] = std::sin(set ? e.ref_tilt : -e.ref_tilt);
+            prog.push_back(t);
+          } else if (e.g == 0 && e.ref_tilt == 0) {
+            emit_offset(prog, ie, e, set, false);  // straight-element frame change
+          } else {
+            emit_bend_offset(prog, ie, e, set);
+          }
+        };
+        bend_offset(true);
         if (e.fringe_type == BMX_FRINGE_FULL || e.fringe_type == BMX_FRINGE_SAD_FULL || e.fringe_type == BMX_FRINGE_SOFT_EDGE_ONLY)
           return fail("sbend fringe_type full / sad_full / soft_edge_only is not supported on the device path");
         auto emit_edge = [&](bool enter) {
@@ -543,6 +696,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         prog.push_back(make_op(OP_TOO_LARGE, ie));
         emit_edge(false);
         fuse_sbend(prog, body_begin);
+        bend_offset(false);
         break;
       }
       case BMX_KEY_RFCAVITY: {

@@ -674,6 +674,38 @@ __device__ __forceinline__ void op_hwang(Lane& L, const DevOp& op, int flags) {
   L.z += dz;
 }
 
+// hard_multipole_edge_kick (bmad/modules/fringe_mod.f90:691-864) of a
+// quadrupole (n = 1, b = k1) or sextupole (n = 2, b = k2) in the element frame
+// (a = 0, charge_dir = 1).  The reference's complex expressions with
+// cab = c0 / (1+pz) real, poly_n1 = z^n = (ar, ai), poly = z^(n+1) = (br, bi),
+// z = x + i y, written out: with xny = x - i cn y (for fx) and y + i cn x
+// (for fy), Re(cab w) = cab Re(w), and the derivatives of the reference's
+// dpoly and dxny.  The second derivatives only feed its matrix.
+// c: 0 c0 = -+ b / (4 (n+2)) (negative at the first track edge), 1 cn = (n+3)/(n+1)
+__device__ __forceinline__ void op_hard_mult_edge(Lane& L, const DevOp& op, int n) {
+  double x = L.x, y = L.y, px = L.px, py = L.py;
+  double c = op.c[0] * L.irp, cn = op.c[1];
+  double ar = x, ai = y;
+  if (n == 2) {
+    ar = x * x - y * y;
+    ai = x * y + y * x;
+  }
+  double br = ar * x - ai * y, bi = ar * y + ai * x;
+  double m = (double)(n + 1);
+  double fx = c * (br * x + cn * (bi * y));
+  double dfx_dx = c * (m * (ar * x + cn * (ai * y)) + br);
+  double dfx_dy = c * (cn * bi - m * (ai * x - cn * (ar * y)));
+  double fy = c * (br * y - cn * (bi * x));
+  double dfy_dx = c * (m * (ar * y - cn * (ai * x)) - cn * bi);
+  double dfy_dy = c * (br - m * (cn * (ar * x) + ai * y));
+  double denom = (1.0 - dfx_dx) * (1.0 - dfy_dy) - dfx_dy * dfy_dx;
+  L.x = x - fx;
+  L.px = px + ((1.0 - dfy_dy - denom) * px + dfy_dx * py) / denom;
+  L.y = y - fy;
+  L.py = py + (dfx_dy * px + (1.0 - dfx_dx - denom) * py) / denom;
+  L.z += (L.px * fx + L.py * fy) * L.irp;
+}
+
 // linear_bend_edge_kick (fringe_mod.f90:129-190): c0 = ht_x, c1 = ht_y
 __device__ __forceinline__ void op_linear_edge(Lane& L, const DevOp& op) {
   L.px += op.c[0] * L.x;
@@ -694,6 +726,28 @@ __device__ __forceinline__ void op_tilt(Lane& L, const DevOp& op) {
 // and unset (:410-470); c: 0..2 offsets, 3 L/2, 4..12 W (row-major).
 __device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set, int flags) {
   double rel_p = 1.0 + L.pz;
+  if (flags & F_OFFSET_AFFINE) {
+    // a misaligned bend (offset_particle.f90:208-235, :409-440): its chain of
+    // arc shifts, offsets and rotations composed on the host, acting on
+    // (x, y, 0) and the momentum (px, py, pz_s) with pz_s = sqrt((1+pz)^2 - px^2 - py^2)
+    const double* w = &op.c[4];
+    double pzz = rel_p * rel_p - L.px * L.px - L.py * L.py;
+    if (pzz <= 0.0) { L.state = BMX_LOST_PZ; return; }
+    double x = L.x, y = L.y;
+    double r0 = w[0] * x + w[1] * y + op.c[0];
+    double r1 = w[3] * x + w[4] * y + op.c[1];
+    double r2 = w[6] * x + w[7] * y + op.c[2];
+    double p2 = sqrt(pzz);
+    double q0 = w[0] * L.px + w[1] * L.py + w[2] * p2;
+    double q1 = w[3] * L.px + w[4] * L.py + w[5] * p2;
+    L.px = q0;
+    L.py = q1;
+    L.x = r0;
+    L.y = r1;
+    double ds = (set ? 0.0 : 2.0 * op.c[3]) - r2;
+    if (fabs(ds) > 1e-14) op_drift_noref(L, ds);
+    return;
+  }
   double r0, r1, r2;
   if (set) {
     r0 = L.x - op.c[0];
@@ -913,7 +967,8 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, n, lc);
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
-                            TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN);
+                            TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |
+                            TB(OP_HARD_MULT_EDGE);
   if (tb & RARE) {
     if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
     if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
@@ -924,6 +979,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false, flags);
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
     if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
+    if (tb & TB(OP_HARD_MULT_EDGE)) op_hard_mult_edge(L, op, n);
   }
   if (flags & F_SHIFT_OUT) {  // offset_particle unset, pure translation (op_offset without pitch)
     uniform_block();

@@ -228,6 +228,27 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
   return 0;
 }
 
+int32_t bmx_lattice_check(const bmx_ele_rec* eles, int32_t n_ele, const bmx_ref* ref, const bmx_config* cfg,
+                          int64_t* n_ops, char* msg, int32_t msg_len) {
+  if (msg && msg_len > 0) msg[0] = '\0';
+  std::string err;
+  int32_t rc = 0;
+  std::vector<DevOp> prog;
+  if (!eles || !ref || n_ele <= 0) {
+    err = "invalid arguments";
+    rc = -1;
+  } else {
+    bmx_config c = cfg ? *cfg : bmx_config{1, 0, 1e3};
+    std::vector<int32_t> eb;
+    LatticeConsts lc;
+    std::memset(&lc, 0, sizeof(lc));
+    if (!compile_variant(eles, n_ele, *ref, c, prog, eb, &lc, &err)) rc = -1;
+  }
+  if (n_ops) *n_ops = rc == 0 ? (int64_t)prog.size() : 0;
+  if (rc != 0 && msg && msg_len > 0) std::snprintf(msg, (size_t)msg_len, "%s", err.c_str());
+  return rc;
+}
+
 int32_t bmx_lattice_cells(const bmx_lattice* lat, int32_t* cell_len, int64_t* n_cells) {
   if (!lat || !cell_len || !n_cells) return -1;
   *cell_len = lat->jit ? lat->cell_len : 0;

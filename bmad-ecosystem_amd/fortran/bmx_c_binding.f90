@@ -133,6 +133,19 @@ interface
     type(c_ptr), value :: lat
   end subroutine
 
+  function bmx_lattice_check(eles, n_ele, ref, cfg, n_ops, msg, msg_len) &
+                             bind(C, name = 'bmx_lattice_check') result(rc)
+    import
+    type(bmx_ele_rec) :: eles(*)
+    integer(c_int32_t), value :: n_ele
+    type(bmx_ref) :: ref
+    type(bmx_config) :: cfg
+    integer(c_int64_t), intent(out) :: n_ops
+    character(kind=c_char) :: msg(*)
+    integer(c_int32_t), value :: msg_len
+    integer(c_int32_t) :: rc
+  end function
+
   function bmx_lattice_cells(lat, cell_len, n_cells) bind(C, name = 'bmx_lattice_cells') result(rc)
     import
     type(c_ptr), value :: lat

@@ -228,6 +228,15 @@ int32_t bmx_lattice_upload(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele_
                            const bmx_ref *ref, const bmx_config *cfg, bmx_lattice **out);
 void bmx_lattice_destroy(bmx_lattice *lat);
 
+/* Host-only check of one element list against the device path, with no
+ * device, context or allocation on it: 0 when bmx_lattice_upload would
+ * accept it (*n_ops: the ops it compiles to), else -1 with the reason (the
+ * element and the unsupported feature) in msg.  n_ops and msg may be NULL.
+ * No reference counterpart: it answers ahead of time what the upload would
+ * reject. */
+int32_t bmx_lattice_check(const bmx_ele_rec *eles, int32_t n_ele, const bmx_ref *ref, const bmx_config *cfg,
+                          int64_t *n_ops, char *msg, int32_t msg_len);
+
 /* The lattice's runtime-compiled cell kernels: *cell_len = ops in its
  * repeating cell and *n_cells = runs of that cell the kernels track straight
  * through (both 0 when the lattice runs on the built-in op interpreter alone:

@@ -90,6 +90,21 @@ ELEMENT_CASES = {
     "rfcavity": [lattice.ele("rfcavity", l=1.0, voltage=1e6, rf_frequency=500e6, phi0=0.1, num_steps=10)],
     "rfcavity_offset_tilt": [lattice.ele("rfcavity", l=0.5, voltage=5e5, rf_frequency=500e6, phi0=0.0, num_steps=5,
                                          x_offset=1e-3, tilt=0.1, a={2: 0.02})],
+    "quad_hard_edge": [lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, fringe_type=abi.FRINGE_HARD_EDGE_ONLY)],
+    "quad_full_fringe_entrance_tilt": [lattice.ele("quadrupole", l=0.4, k1=-1.2, num_steps=3, tilt=0.2, x_offset=1e-3,
+                                                   fringe_type=abi.FRINGE_FULL, fringe_at=abi.ENTRANCE_END,
+                                                   b={2: 0.5})],
+    "quad_soft_edge_only": [lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, fringe_type=abi.FRINGE_SOFT_EDGE_ONLY)],
+    "sext_hard_edge_exit": [lattice.ele("sextupole", l=0.2, k2=-20.0, num_steps=4, fringe_type=abi.FRINGE_FULL,
+                                        fringe_at=abi.EXIT_END, y_offset=2e-4)],
+    "sext_hard_edge_both": [lattice.ele("sextupole", l=0.3, k2=30.0, num_steps=2,
+                                        fringe_type=abi.FRINGE_HARD_EDGE_ONLY)],
+    "sbend_offset_pitch": [lattice.ele("sbend", l=2.0, angle=0.05, num_steps=1, x_offset=1e-3, y_offset=-5e-4,
+                                       z_offset=2e-3, x_pitch=2e-4, y_pitch=-3e-4)],
+    "sbend_ref_tilt": [lattice.ele("sbend", l=1.5, angle=0.03, ref_tilt=0.4, num_steps=1)],
+    "sbend_ref_tilt_offset_k1": [lattice.ele("sbend", l=1.0, g=0.02, k1=0.2, ref_tilt=-0.3, x_offset=5e-4,
+                                             y_pitch=1e-4, e1=0.01, e2=0.02, num_steps=4)],
+    "sbend_zero_g_offset": [lattice.ele("sbend", l=0.8, g=0.0, x_offset=1e-3, x_pitch=1e-4, num_steps=1)],
     "rcollimator": [lattice.ele("rcollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "ecollimator": [lattice.ele("ecollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "aperture_entrance": [lattice.ele("drift", l=0.5, x_limit=3e-3, y_limit=3e-3, aperture_at=abi.ENTRANCE_END,

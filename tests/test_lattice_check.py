@@ -1,0 +1,106 @@
+"""bmx_lattice_check on the CPU: the host-side compile of an element list
+for the device path accepts what the device tracks and names what it
+rejects, with no device present.  The accepted set of the reference's
+tracking_method_test elements is pinned, so a change in device coverage shows
+up here."""
+import json
+import os
+
+import pytest
+
+from bmx import abi, lattice, tracker
+from helpers import SPECIES, ele_from_spec
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _check(eles, species=abi.ELECTRON):
+    eles = lattice.set_reference([lattice.EleRec_copy(e) for e in eles], species, e_tot=1e9)
+    return tracker.check_lattice(eles, species)
+
+
+def _n_ops(e):
+    ok, n, msg = _check([e])
+    assert ok, msg
+    return n
+
+
+def test_plain_elements_compile():
+    ok, n, msg = _check([lattice.ele("drift", l=1.0), lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1)])
+    assert ok and msg == "" and n >= 2
+
+
+@pytest.mark.parametrize("fringe_at,extra", [(abi.BOTH_ENDS, 2), (abi.ENTRANCE_END, 1), (abi.EXIT_END, 1),
+                                             (abi.NO_END, 0)])
+@pytest.mark.parametrize("fringe_type", [abi.FRINGE_HARD_EDGE_ONLY, abi.FRINGE_FULL])
+def test_quad_and_sextupole_hard_edge_ops(fringe_type, fringe_at, extra):
+    """hard_multipole_edge_kick (fringe_mod.f90:714-721) adds one op per edge
+    that fringe_at selects, for fringe_type hard_edge_only and full."""
+    for key, kw in (("quadrupole", dict(k1=0.8)), ("sextupole", dict(k2=5.0))):
+        base = _n_ops(lattice.ele(key, l=0.4, num_steps=2, **kw))
+        n = _n_ops(lattice.ele(key, l=0.4, num_steps=2, fringe_type=fringe_type, fringe_at=fringe_at, **kw))
+        assert n == base + extra, (key, n, base)
+
+
+def test_quad_soft_edge_only_is_a_no_op():
+    """soft_quadrupole_edge_kick has no strength without fq1/fq2
+    (fringe_mod.f90:605-607): no ops."""
+    base = _n_ops(lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1))
+    assert _n_ops(lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1,
+                              fringe_type=abi.FRINGE_SOFT_EDGE_ONLY)) == base
+
+
+def test_octupole_fringe_is_ignored():
+    """apply_element_edge_kick has no magnetic edge for octupoles."""
+    base = _n_ops(lattice.ele("octupole", l=0.3, k3=50.0, num_steps=2))
+    assert _n_ops(lattice.ele("octupole", l=0.3, k3=50.0, num_steps=2, fringe_type=abi.FRINGE_FULL)) == base
+
+
+@pytest.mark.parametrize("kw,extra", [(dict(ref_tilt=0.3), 1), (dict(x_offset=1e-3), 2),
+                                      (dict(x_offset=1e-3, ref_tilt=0.3, y_pitch=1e-4), 2)])
+def test_misaligned_bend_adds_frame_changes(kw, extra):
+    """offset_particle set and unset around the bend: two tilt_coords for
+    ref_tilt alone (the element end folds into the second), two frame changes
+    with offsets or pitches (the element end stays an op of its own)."""
+    base = _n_ops(lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1))
+    assert _n_ops(lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, **kw)) == base + extra
+
+
+@pytest.mark.parametrize("e,needle", [
+    (lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, a_elec={1: 1e4}), "electric multipoles"),
+    (lattice.ele("drift", l=1.0, orientation=-1), "orientation -1"),
+    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_FULL), "fringe_type full"),
+    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_SAD_FULL), "fringe_type full"),
+    (lattice.ele("rfcavity", l=1.0, voltage=1e6, rf_frequency=5e8, num_steps=10, coupler_strength=0.1), "coupler"),
+])
+def test_rejections_name_the_feature(e, needle):
+    ok, n, msg = _check([lattice.ele("drift", l=0.5), e])
+    assert not ok and n == 0
+    assert msg.startswith("element 1 ") and needle in msg, msg
+
+
+def test_invalid_arguments():
+    lib = abi.load_library()
+    import ctypes as C
+    msg = C.create_string_buffer(64)
+    ref = abi.Ref(abi.ELECTRON, 0)
+    assert lib.bmx_lattice_check(None, 0, C.byref(ref), None, None, msg, len(msg)) == -1
+    assert msg.value == b"invalid arguments"
+
+
+def test_golden_elements_the_device_accepts():
+    """Of the reference's tracking_method_test elements, the device path takes
+    every one without electric multipoles."""
+    tm = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
+    sp = SPECIES[tm["particle"]]
+    accepted, rejected = set(), {}
+    for name, spec in tm["elements"].items():
+        ok, _, msg = tracker.check_lattice([ele_from_spec(spec, sp, e_tot=tm["e_tot"])], sp)
+        if ok:
+            accepted.add(name)
+        else:
+            rejected[name] = msg
+    assert all("electric multipoles" in m for m in rejected.values()), rejected
+    assert sorted(rejected) == ["QUADRUPOLE1", "QUADRUPOLE2", "QUADRUPOLE3", "QUADRUPOLE4", "QUADRUPOLE5", "SBEND3",
+                                "SBEND7"]
+    assert len(accepted) == len(tm["elements"]) - 7
