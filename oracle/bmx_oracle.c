@@ -1260,7 +1260,8 @@ static void linear_bend_edge_kick(const bmx_ele_rec *ele, int ref_species, int p
 }
 
 /* bend_edge_kick: fringe_mod.f90:39-127 (fringe types hard_edge_only,
- * basic_bend, linear_edge, none; full/sad types rejected at setup) */
+ * basic_bend, linear_edge, none; the entry points refuse full / sad_full /
+ * soft_edge_only) */
 static void bend_edge_kick(const bmx_ele_rec *ele, int ref_species, int particle_at, o_coord *orb) {
   if (!fringe_here(ele, orb, particle_at)) return;
   if (particle_at == O_FIRST_EDGE) hard_multipole_edge_kick(ele, ref_species, particle_at, orb);
@@ -2090,6 +2091,23 @@ static void init_coord(o_coord *orb, const double *vec, const bmx_ele_rec *ele, 
 /* exported entry points (ctypes from tests/ and bench.py)                   */
 /* ======================================================================== */
 
+/* Elements this restatement does not model: an sbend edge of fringe_type
+ * full (exact_bend_edge_kick, fringe_mod.f90:1465-1686), sad_full or
+ * soft_edge_only (sad_soft_bend_edge_kick, :892-996).  The entry points
+ * refuse them (-2) rather than track past the missing edge map. */
+static int unmodelled(const bmx_ele_rec *ele) {
+  if (ele->key != BMX_KEY_SBEND || ele->fringe_at == BMX_NO_END) return 0;
+  return ele->fringe_type == BMX_FRINGE_FULL || ele->fringe_type == BMX_FRINGE_SAD_FULL ||
+         ele->fringe_type == BMX_FRINGE_SOFT_EDGE_ONLY;
+}
+
+static int any_unmodelled(const bmx_ele_rec *eles, int n_ele) {
+  int ie;
+  for (ie = 0; ie < n_ele; ie++)
+    if (unmodelled(&eles[ie])) return 1;
+  return 0;
+}
+
 /* One particle through one element (the tracking_method_test harness,
  * regression_tests/tracking_method_test/tracking_method_test.f90:188-199).
  * io: in = vec[6]; out = vec[6], t, beta, state.  Returns 0 or <0 on error. */
@@ -2098,6 +2116,7 @@ ORACLE_API int oracle_track1(const bmx_ele_rec *ele, int ref_species, int specie
                              int *state_out, double t0_override) {
   o_coord a, b;
   int err, i;
+  if (unmodelled(ele)) return -2;
   init_coord(&a, vec_in, ele, species, direction, direction == 1 ? 0 : 1);
   if (t0_override == t0_override && t0_override != 0) a.t = t0_override; /* NaN => keep */
   err = track1(&a, ele, ref_species, &b);
@@ -2161,6 +2180,7 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
   int64_t i;
   int turn, ie;
   (void)n_threads;
+  if (any_unmodelled(eles, n_ele)) return -2;
   c_multi_init(); /* lazily built table: build it before the parallel region */
   for (turn = 0; turn < n_turns; turn++) {
     for (ie = 0; ie < n_ele; ie++) {
@@ -2205,6 +2225,7 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
                                            double *beta, int *state, int *ix_lost, int *turn_lost, int n_threads) {
   int64_t i;
   (void)n_threads;
+  if (any_unmodelled(eles, n_ele)) return -2;
   c_multi_init(); /* lazily built table: build it before the parallel region */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64) num_threads(n_threads > 0 ? n_threads : 1)
@@ -2251,6 +2272,7 @@ ORACLE_API int oracle_jacobian(const bmx_ele_rec *eles, int n_ele, int ref_speci
                                double beta, double *out, double *J) {
   o_coord a, b;
   int turn, ie, i, j;
+  if (any_unmodelled(eles, n_ele)) return -2;
   for (i = 0; i < 6; i++) {
     a.vec[i] = real(vec[i]);
     a.vec[i].d[i] = 1.0;

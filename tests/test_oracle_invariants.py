@@ -169,3 +169,17 @@ def test_thin_cavity_energy_gain():
     E1 = E0 + dE
     pz1 = math.sqrt(E1 * E1 - M * M) / P0C - 1
     assert abs(out[5] - pz1) < 1e-14
+
+
+@pytest.mark.parametrize("ft", [abi.FRINGE_FULL, abi.FRINGE_SAD_FULL, abi.FRINGE_SOFT_EDGE_ONLY])
+def test_oracle_refuses_unmodelled_bend_edges(ft):
+    """exact_bend_edge_kick and sad_soft_bend_edge_kick are not restated:
+    the oracle refuses such a bend instead of tracking past its edges; with
+    fringe_at = no_end nothing is missing and it tracks."""
+    e = lattice.set_reference([lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=ft)], abi.ELECTRON,
+                              e_tot=1e9)
+    with pytest.raises(RuntimeError):
+        oracle.track_bunch(e, abi.ELECTRON, 1, np.zeros((6, 4)))
+    e[0].fringe_at = abi.NO_END
+    r = oracle.track_bunch(e, abi.ELECTRON, 1, np.zeros((6, 4)))
+    assert np.all(r["state"] == abi.ALIVE)
