@@ -34,6 +34,8 @@ enum OpType : int32_t {
   OP_TILT = 17,          // tilt_coords rotation, c[0] = cos, c[1] = sin
   OP_KD_SLICES = 18,     // whole kick-drift body of a thick multipole: half kick, n x (drift, kick), last half
   OP_SBEND_FUSED = 19,   // hard edge (e = 0, no fringe integral), orbit_too_large, exact body, orbit_too_large, hard edge
+  OP_EKICK = 20,         // thin electric ab multipole kick (1/beta) with its pz, beta, z update
+  OP_EFRINGE = 21,       // electric longitudinal fringe: energy kick of the multipole potential
 };
 
 // op flags

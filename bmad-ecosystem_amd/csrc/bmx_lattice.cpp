@@ -36,6 +36,15 @@ double species_mass(int s) {
   return 0;
 }
 
+// charge_of (particle_species_mod.f90), in units of e
+double species_charge(int s) {
+  switch (s) {
+    case BMX_ELECTRON: case BMX_ANTIPROTON: case BMX_MUON: return -1;
+    case BMX_POSITRON: case BMX_PROTON: case BMX_ANTIMUON: return 1;
+  }
+  return 0;
+}
+
 int nint(double x) { return (int)(x < 0 ? -std::floor(-x + 0.5) : std::floor(x + 0.5)); }
 
 bool has_hkick_attr(int key) {
@@ -52,7 +61,7 @@ bool has_hkick_attr(int key) {
 // false, magnetic, include_kicks) returns them; *b1 receives the pulled
 // quadrupole term when b1 != nullptr.
 struct AB {
-  double a[BMX_N_POLE], b[BMX_N_POLE];
+  double a[BMX_N_POLE + 1], b[BMX_N_POLE + 1];  // + 1: the electric potential runs one order higher
   int ix_max;
 };
 
@@ -146,6 +155,27 @@ bool magnetic_ab(const bmx_ele_rec& e, bool include_kicks, AB* out, double* b1, 
     r.b[1] = 0;
     if (r.ix_max <= 1) r.ix_max = max_nonzero(r.a, r.b, 1);
   }
+  return true;
+}
+
+// multipole_ele_to_ab for the electric poles (multipole_ele_to_ab.f90:31-235,
+// pole_type electric, no kicks, no tilt): a_elec, b_elec, scaled by
+// r0_elec^-n when scale_multipoles and r0_elec is neither 0 nor 1.  Thin
+// multipole elements track kt kicks of their magnetic poles alone.
+bool electric_ab(const bmx_ele_rec& e, AB* out, std::string* err) {
+  AB& r = *out;
+  std::memset(&r, 0, sizeof(r));
+  r.ix_max = -1;
+  if (!e.is_on || e.key == BMX_KEY_PIPE || !e.has_elec_poles) return true;
+  if (e.key == BMX_KEY_MULTIPOLE || e.key == BMX_KEY_AB_MULTIPOLE) return true;
+  for (int n = 0; n < BMX_N_POLE; n++) { r.a[n] = e.a_elec[n]; r.b[n] = e.b_elec[n]; }
+  if (max_nonzero(r.a, r.b, BMX_N_POLE - 1) < 0) return true;
+  if (e.field_master) { *err = "field_master multipoles are not supported"; return false; }
+  if (e.scale_multipoles && e.r0_elec != 0 && e.r0_elec != 1) {
+    double f = e.r0_elec;
+    for (int n = 0; n < BMX_N_POLE; n++) { f /= e.r0_elec; r.a[n] *= f; r.b[n] *= f; }
+  }
+  r.ix_max = max_nonzero(r.a, r.b, BMX_N_POLE - 1);
   return true;
 }
 
@@ -386,7 +416,7 @@ int used_consts(const DevOp& op) {
     case OP_DRIFT: return 1;
     case OP_QUAD: return 3;
     case OP_TILT: case OP_HARD_MULT_EDGE: return 2;
-    case OP_KICK: return 2 * (nh + 1);
+    case OP_KICK: case OP_EKICK: case OP_EFRINGE: return 2 * (nh + 1);
     case OP_KD_SLICES: return 1 + 2 * (nh + 1);
     case OP_RFCAVITY: return 5;
     default: return 14;
@@ -433,10 +463,16 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     if (e.p0c != p0c || e.p0c_start != p0c || e.e_tot != e_tot)
       return fail("reference energy changes along the lattice (only rfcavity-style lattices are supported)");
     if (e.orientation != 1) return fail("orientation -1 is not supported on the device path");
-    if (e.has_elec_poles) {
-      bool any = false;
-      for (int n = 0; n < BMX_N_POLE; n++) any = any || e.a_elec[n] != 0 || e.b_elec[n] != 0;
-      if (any) return fail("electric multipoles are not supported on the device path");
+    AB ae;  // electric multipoles
+    {
+      std::string ee;
+      if (!electric_ab(e, &ae, &ee)) return fail(ee.c_str());
+      int lo_e = 0;
+      while (lo_e <= ae.ix_max && ae.a[lo_e] == 0 && ae.b[lo_e] == 0) lo_e++;
+      if (ae.ix_max >= 0 && e.key == BMX_KEY_SBEND && e.is_on)
+        return fail("electric multipoles in a bend are not supported on the device path");
+      if (ae.ix_max >= 0 && ae.ix_max - lo_e >= 7)
+        return fail("electric multipoles spanning more than 7 orders are not supported on the device path");
     }
     if (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 || e.pz_aperture_width2 != 0)
       return fail("collimator phase-space apertures are not supported on the device path");
@@ -496,6 +532,38 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       op.c[1] = (double)(n + 3) / (double)(n + 1);
       prog.push_back(op);
     };
+    // ab_multipole_kicks of the electric poles (multipole_mod.f90:82-174,
+    // ab_multipole_kick's electric branch :338-345): the ab kick of a f and
+    // -b f with f = charge / (beta p0c) x scale, as kick coefficients
+    // scale charge / p0c (-b, a); the device divides by beta and updates pz,
+    // beta and z (OP_EKICK)
+    double charge = species_charge(ref.species);
+    auto emit_ekicks = [&](double scale) {
+      if (ae.ix_max < 0) return;
+      AB t = ae;
+      double f = scale * charge / p0c;
+      for (int n = 0; n <= ae.ix_max; n++) { t.b[n] = -f * ae.b[n]; t.a[n] = f * ae.a[n]; }
+      emit_kicks(prog, ie, t, 1.0, OP_EKICK);
+    };
+    // electric_longitudinal_fringe (apply_element_edge_kick.f90:283-329) at
+    // one edge, where fringe_at puts an edge kick: the energy kick
+    // -ff phi = Re sum_i C_i z^(i+1), C_i = ff (b_i - i a_i) / (i+1),
+    // ff = at_sign charge (at_sign +1 entering, -1 leaving) (OP_EFRINGE)
+    auto emit_efringe = [&](bool enter) {
+      if (e.l == 0 || ae.ix_max < 0) return;
+      bool here = enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
+                        : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS);
+      if (!here) return;
+      double ff = (enter ? 1.0 : -1.0) * charge;
+      AB t;
+      std::memset(&t, 0, sizeof(t));
+      for (int i = 0; i <= ae.ix_max; i++) {
+        t.b[i + 1] = ff * ae.b[i] / (i + 1);
+        t.a[i + 1] = -ff * ae.a[i] / (i + 1);
+      }
+      t.ix_max = ae.ix_max + 1;
+      emit_kicks(prog, ie, t, 1.0, OP_EFRINGE);
+    };
     std::string e2;
     switch (key) {
       case BMX_KEY_MARKER:
@@ -546,13 +614,15 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         double b1;
         if (!magnetic_ab(e, true, &ab, &b1, &e2)) return fail(e2.c_str());
         int n_step = 1;
-        if (ab.ix_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
+        if (ab.ix_max > -1 || ae.ix_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
         if (e.l == 0) n_step = 0;
         double step_len = e.l / (n_step > 0 ? n_step : 1);
         double r_step = 1.0 / (n_step > 0 ? n_step : 1);
         emit_offset(prog, ie, e, true, true);
         hard_edge(1, e.k1, true);
+        emit_efringe(true);
         emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
+        emit_ekicks(step_len / 2);
         for (int i = 1; i <= n_step; i++) {
           DevOp op = make_op(OP_QUAD, ie);
           op.c[0] = step_len;
@@ -560,7 +630,9 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           op.c[2] = op.c[1] * step_len * step_len;
           prog.push_back(op);
           emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
+          emit_ekicks((i == n_step) ? step_len / 2 : step_len);
         }
+        emit_efringe(false);
         hard_edge(1, e.k1, false);
         emit_offset(prog, ie, e, false, true);
         break;
@@ -571,13 +643,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         // track_a_thick_multipole.f90:18-133
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
-        int n_step = (ab.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
+        int n_step = (ab.ix_max == -1 && ae.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
         double step_len = e.l / n_step, r_step = 1.0 / n_step;
         emit_offset(prog, ie, e, true, true);
         if (key == BMX_KEY_SEXTUPOLE) hard_edge(2, e.k2, true);
+        emit_efringe(true);
         int lo = 0;
         while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
-        if (ab.ix_max >= 0 && step_len != 0 && ab.ix_max - lo < 6) {
+        if (ab.ix_max >= 0 && step_len != 0 && ab.ix_max - lo < 6 && ae.ix_max < 0) {
           // one op for the body; the half kicks are the full-step kick x 0.5,
           // which is exact (the reference scales by r_step / 2)
           DevOp op = make_op(OP_KD_SLICES, ie);
@@ -591,11 +664,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           prog.push_back(op);
         } else {
           emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
+          emit_ekicks(step_len / 2);
           for (int i = 1; i <= n_step; i++) {
             if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
             emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
+            emit_ekicks((i == n_step) ? step_len / 2 : step_len);
           }
         }
+        emit_efringe(false);
         if (key == BMX_KEY_SEXTUPOLE) hard_edge(2, e.k2, false);
         emit_offset(prog, ie, e, false, true);
         break;
@@ -708,6 +784,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
         emit_offset(prog, ie, e, true, true);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
+        emit_ekicks(0.5 * e.l);
         int n_slice = std::max(1, nint(std::fabs(e.l) / (e.l / (e.num_steps > 0 ? e.num_steps : 1))));
         if (e.l == 0) n_slice = 1;
         DevOp op = make_op(OP_RFCAVITY, ie);
@@ -718,6 +795,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         op.c[3] = e.rf_frequency;
         op.c[4] = (e.rf_frequency != 0) ? 0.5 / e.rf_frequency : 0.0;
         prog.push_back(op);
+        emit_ekicks(0.5 * e.l);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
         emit_offset(prog, ie, e, false, true);
         break;

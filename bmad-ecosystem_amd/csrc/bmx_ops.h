@@ -807,6 +807,43 @@ __device__ __forceinline__ void energy_kick(Lane& L, double dE, const LatticeCon
   refresh_invariants(L, lc);
 }
 
+// ab_multipole_kicks of electric multipoles (bmad/modules/multipole_mod.f90:
+// 82-174, with ab_multipole_kick's electric branch :338-345): the ab kick of
+// coefficients proportional to 1/beta (the rest folded into the op), then
+// the momentum change it implies: (1+pz)^2 - px^2 - py^2 is kept, beta follows
+// the new pz, and z scales with beta.  Coefficients as op_kick.
+__device__ __forceinline__ void op_ekick(Lane& L, const DevOp& op, int nh, int lo, const LatticeConsts& lc) {
+  double kr, ki;
+  kick_horner(op.c, nh, lo, L.x, L.y, kr, ki);
+  double ib = 1.0 / L.beta;
+  double kx = -kr * ib, ky = ki * ib;
+  double px0 = L.px, py0 = L.py, pz0 = L.pz;
+  L.px = px0 + kx;
+  L.py = py0 + ky;
+  double rel0 = 1.0 + pz0;
+  double alpha = (kx * (2.0 * px0 + kx) + ky * (2.0 * py0 + ky)) / (rel0 * rel0);
+  if (alpha < -1.0) {
+    L.state = BMX_LOST_PZ;
+    return;
+  }
+  L.pz = pz0 + rel0 * d_sqrt_one(alpha);
+  double rel = 1.0 + L.pz, mc = lc.mc2 / lc.p0c;
+  double beta_new = rel / sqrt(rel * rel + mc * mc);
+  L.z = L.z * beta_new / L.beta;
+  L.beta = beta_new;
+  refresh_invariants(L, lc);
+}
+
+// electric_longitudinal_fringe (bmad/low_level/apply_element_edge_kick.f90:
+// 283-329), bmad_standard field: apply_energy_kick(-ff phi) with phi the
+// scalar potential of the electric multipoles.  The op holds the
+// coefficients C of -ff phi = Re sum C_n (x + i y)^n, in op_kick's layout.
+__device__ __forceinline__ void op_efringe(Lane& L, const DevOp& op, int nh, int lo, const LatticeConsts& lc) {
+  double kr, ki;
+  kick_horner(op.c, nh, lo, L.x, L.y, kr, ki);
+  energy_kick(L, kr, lc);
+}
+
 // track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140), relative time:
 // particle_rf_time = -z/(beta c) (bmad/code/particle_rf_time.f90:89).
 // c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f); n = n_slice
@@ -968,7 +1005,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |
-                            TB(OP_HARD_MULT_EDGE);
+                            TB(OP_HARD_MULT_EDGE) | TB(OP_EKICK) | TB(OP_EFRINGE);
   if (tb & RARE) {
     if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
     if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
@@ -980,6 +1017,8 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
     if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
     if (tb & TB(OP_HARD_MULT_EDGE)) op_hard_mult_edge(L, op, n);
+    if (tb & TB(OP_EKICK)) op_ekick(L, op, flags & 0xff, n, lc);
+    if (tb & TB(OP_EFRINGE)) op_efringe(L, op, flags & 0xff, n, lc);
   }
   if (flags & F_SHIFT_OUT) {  // offset_particle unset, pure translation (op_offset without pitch)
     uniform_block();

@@ -105,6 +105,15 @@ ELEMENT_CASES = {
     "sbend_ref_tilt_offset_k1": [lattice.ele("sbend", l=1.0, g=0.02, k1=0.2, ref_tilt=-0.3, x_offset=5e-4,
                                              y_pitch=1e-4, e1=0.01, e2=0.02, num_steps=4)],
     "sbend_zero_g_offset": [lattice.ele("sbend", l=0.8, g=0.0, x_offset=1e-3, x_pitch=1e-4, num_steps=1)],
+    "quad_electric": [lattice.ele("quadrupole", l=0.6, k1=0.5, num_steps=4, a_elec={1: 2e4}, b_elec={0: -2e3},
+                                  tilt=0.3, x_offset=1e-3)],
+    "quad_electric_hard_edge": [lattice.ele("quadrupole", l=0.6, k1=-1.0, num_steps=3, b_elec={1: 5e4, 2: 1e6},
+                                            r0_elec=0.05, fringe_type=abi.FRINGE_FULL)],
+    "sext_electric_entrance": [lattice.ele("sextupole", l=0.3, k2=10.0, num_steps=3, b_elec={2: 2e6}, a_elec={0: 3e3},
+                                           fringe_at=abi.ENTRANCE_END)],
+    "instrument_electric": [lattice.ele("instrument", l=0.4, b_elec={1: 3e4})],
+    "rfcavity_electric": [lattice.ele("rfcavity", l=0.5, voltage=5e5, rf_frequency=500e6, phi0=0.05, num_steps=5,
+                                      a_elec={1: 1e4})],
     "rcollimator": [lattice.ele("rcollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "ecollimator": [lattice.ele("ecollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "aperture_entrance": [lattice.ele("drift", l=0.5, x_limit=3e-3, y_limit=3e-3, aperture_at=abi.ENTRANCE_END,

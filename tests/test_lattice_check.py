@@ -67,7 +67,8 @@ def test_misaligned_bend_adds_frame_changes(kw, extra):
 
 
 @pytest.mark.parametrize("e,needle", [
-    (lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, a_elec={1: 1e4}), "electric multipoles"),
+    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, a_elec={1: 1e4}), "electric multipoles in a bend"),
+    (lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, b_elec={0: 1.0, 7: 1.0}), "more than 7 orders"),
     (lattice.ele("drift", l=1.0, orientation=-1), "orientation -1"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_FULL), "fringe_type full"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_SAD_FULL), "fringe_type full"),
@@ -88,9 +89,29 @@ def test_invalid_arguments():
     assert msg.value == b"invalid arguments"
 
 
+@pytest.mark.parametrize("fringe_at,n_fringe", [(abi.BOTH_ENDS, 2), (abi.ENTRANCE_END, 1), (abi.NO_END, 0)])
+def test_electric_multipole_ops(fringe_at, n_fringe):
+    """A quadrupole with electric multipoles: an electric kick after every
+    magnetic kick (n_step + 1 of them, ab_multipole_kicks) and an electric
+    longitudinal fringe at each edge fringe_at selects, whatever fringe_type
+    (has_elec_poles makes the edge kicks run)."""
+    kw = dict(l=0.6, k1=0.5, num_steps=4, fringe_at=fringe_at)
+    base = _n_ops(lattice.ele("quadrupole", b={2: 1.0}, **kw))
+    n = _n_ops(lattice.ele("quadrupole", b={2: 1.0}, a_elec={1: 2e4}, b_elec={0: -2e3}, **kw))
+    assert n == base + 5 + n_fringe
+
+
+def test_electric_poles_of_thin_multipoles_are_ignored():
+    """track1_bmad tracks multipole / ab_multipole elements with the kt kicks
+    of their magnetic poles alone."""
+    for key in ("multipole", "ab_multipole"):
+        kw = dict(knl={2: 1.0}) if key == "multipole" else dict(b={2: 1.0})
+        assert _n_ops(lattice.ele(key, a_elec={1: 1e4}, **kw)) == _n_ops(lattice.ele(key, **kw))
+
+
 def test_golden_elements_the_device_accepts():
     """Of the reference's tracking_method_test elements, the device path takes
-    every one without electric multipoles."""
+    every one but the bends with electric multipoles."""
     tm = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
     sp = SPECIES[tm["particle"]]
     accepted, rejected = set(), {}
@@ -100,7 +121,6 @@ def test_golden_elements_the_device_accepts():
             accepted.add(name)
         else:
             rejected[name] = msg
-    assert all("electric multipoles" in m for m in rejected.values()), rejected
-    assert sorted(rejected) == ["QUADRUPOLE1", "QUADRUPOLE2", "QUADRUPOLE3", "QUADRUPOLE4", "QUADRUPOLE5", "SBEND3",
-                                "SBEND7"]
-    assert len(accepted) == len(tm["elements"]) - 7
+    assert all("electric multipoles in a bend" in m for m in rejected.values()), rejected
+    assert sorted(rejected) == ["SBEND3", "SBEND7"]
+    assert len(accepted) == len(tm["elements"]) - 2
