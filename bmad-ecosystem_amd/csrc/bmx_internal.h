@@ -36,6 +36,7 @@ enum OpType : int32_t {
   OP_SBEND_FUSED = 19,   // hard edge (e = 0, no fringe integral), orbit_too_large, exact body, orbit_too_large, hard edge
   OP_EKICK = 20,         // thin electric ab multipole kick (1/beta) with its pz, beta, z update
   OP_EFRINGE = 21,       // electric longitudinal fringe: energy kick of the multipole potential
+  OP_BEND_EKICK = 22,    // electric multipole kick inside a bend, scaled by (1 + g x) / (ps beta)
 };
 
 // op flags

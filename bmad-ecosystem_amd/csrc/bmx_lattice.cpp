@@ -192,7 +192,8 @@ DevOp make_op(int type, int ie) {
 // for orders n_lo..n_lo+k; the kick is dpx = -Re K, dpy = Im K with
 // K = sum c_n (x + i y)^n  (ab_multipole_kick, bmad/modules/multipole_mod.f90:314-401).
 void emit_kicks(std::vector<DevOp>& prog, int ie, const AB& ab, double scale, int type, double g = 0) {
-  const int cap = (type == OP_BEND_KICK) ? 6 : 7;  // OP_BEND_KICK keeps g in c[13]
+  const bool bend = (type == OP_BEND_KICK || type == OP_BEND_EKICK);
+  const int cap = bend ? 6 : 7;  // bend kicks keep g in c[13]
   if (ab.ix_max < 0) return;
   int lo = 0;
   while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
@@ -205,7 +206,7 @@ void emit_kicks(std::vector<DevOp>& prog, int ie, const AB& ab, double scale, in
       op.c[2 * (n - lo)] = scale * ab.b[n];
       op.c[2 * (n - lo) + 1] = scale * ab.a[n];
     }
-    if (type == OP_BEND_KICK) op.c[13] = g;
+    if (bend) op.c[13] = g;
     prog.push_back(op);
     lo = hi + 1;
     while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
@@ -417,6 +418,7 @@ int used_consts(const DevOp& op) {
     case OP_QUAD: return 3;
     case OP_TILT: case OP_HARD_MULT_EDGE: return 2;
     case OP_KICK: case OP_EKICK: case OP_EFRINGE: return 2 * (nh + 1);
+    case OP_BEND_KICK: case OP_BEND_EKICK: return 14;
     case OP_KD_SLICES: return 1 + 2 * (nh + 1);
     case OP_RFCAVITY: return 5;
     default: return 14;
@@ -469,10 +471,8 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       if (!electric_ab(e, &ae, &ee)) return fail(ee.c_str());
       int lo_e = 0;
       while (lo_e <= ae.ix_max && ae.a[lo_e] == 0 && ae.b[lo_e] == 0) lo_e++;
-      if (ae.ix_max >= 0 && e.key == BMX_KEY_SBEND && e.is_on)
-        return fail("electric multipoles in a bend are not supported on the device path");
-      if (ae.ix_max >= 0 && ae.ix_max - lo_e >= 7)
-        return fail("electric multipoles spanning more than 7 orders are not supported on the device path");
+      if (ae.ix_max >= 0 && ae.ix_max - lo_e >= ((e.key == BMX_KEY_SBEND) ? 6 : 7))
+        return fail("electric multipoles spanning more than 7 orders (6 in a bend) are not supported on the device path");
     }
     if (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 || e.pz_aperture_width2 != 0)
       return fail("collimator phase-space apertures are not supported on the device path");
@@ -728,6 +728,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         };
         size_t body_begin = prog.size();
         emit_edge(true);
+        emit_efringe(true);
         prog.push_back(make_op(OP_TOO_LARGE, ie));
         AB ab;
         double b1;
@@ -740,11 +741,22 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         if (!e.is_on) { dg = -g; g_tot = 0; b1 = 0; }
         int ix_mag_max = ab.ix_max;
         int n_step = 1;
-        if (ix_mag_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
+        if (ix_mag_max > -1 || ae.ix_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
         double r_step = 1.0 / n_step, step_len = e.l / n_step, angle = g * step_len;
         AB abk = ab;  // kicks run over orders 0..ix_mag_max as computed before dg/b1 were pulled out
         abk.ix_max = ix_mag_max;
-        if (ix_mag_max > -1) emit_kicks(prog, ie, abk, 0.5 * r_step, OP_BEND_KICK, g);
+        // bend_apply_multipole_kicks: the magnetic kick, then the electric one
+        // from the fields at the same x, y (coefficients step_len coef charge / p0c (b, -a))
+        auto bend_kicks = [&](double coef) {
+          if (ix_mag_max > -1) emit_kicks(prog, ie, abk, coef * r_step, OP_BEND_KICK, g);
+          if (ae.ix_max > -1) {
+            AB t = ae;
+            double f = step_len * coef * charge / p0c;
+            for (int n = 0; n <= ae.ix_max; n++) { t.b[n] = f * ae.b[n]; t.a[n] = -f * ae.a[n]; }
+            emit_kicks(prog, ie, t, 1.0, OP_BEND_EKICK, g);
+          }
+        };
+        bend_kicks(0.5);
         for (int n = 1; n <= n_step; n++) {
           if (b1 != 0) {
             DevOp op = make_op(OP_BEND_K1, ie);
@@ -767,9 +779,10 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
             op.c[8] = sincc_a; op.c[9] = lc->beta0; op.c[10] = dg; op.c[13] = step_len / lc->beta0;
             prog.push_back(op);
           }
-          if (ix_mag_max > -1) emit_kicks(prog, ie, abk, (n == n_step ? 0.5 : 1.0) * r_step, OP_BEND_KICK, g);
+          bend_kicks(n == n_step ? 0.5 : 1.0);
         }
         prog.push_back(make_op(OP_TOO_LARGE, ie));
+        emit_efringe(false);
         emit_edge(false);
         fuse_sbend(prog, body_begin);
         bend_offset(false);

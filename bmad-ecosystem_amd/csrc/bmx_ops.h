@@ -844,6 +844,31 @@ __device__ __forceinline__ void op_efringe(Lane& L, const DevOp& op, int nh, int
   energy_kick(L, kr, lc);
 }
 
+// bend_apply_multipole_kicks, electric part (bmad/low_level/track_a_bend.f90:
+// 338-471, bmad_standard fields): the kick of the field E = (Re K, -Im K),
+// K = sum c_n (x + i y)^n, scaled by (1 + g x) / (ps beta) with
+// ps = sqrt((1+pz)^2 - px^2 - py^2) / (1+pz) (step length, coefficient and
+// charge / p0c folded into c), then pz, beta and z as the reference updates
+// them.  c: kick coefficients as op_kick, 13 g.
+__device__ __forceinline__ void op_bend_ekick(Lane& L, const DevOp& op, int nh, int lo, const LatticeConsts& lc) {
+  double kr, ki;
+  kick_horner(op.c, nh, lo, L.x, L.y, kr, ki);
+  double px0 = L.px, py0 = L.py, rel0 = 1.0 + L.pz;
+  double ps2 = rel0 * rel0 - px0 * px0 - py0 * py0;
+  double ps = sqrt(ps2) / rel0;
+  double f = (1.0 + op.c[13] * L.x) / (ps * L.beta);
+  double kx = f * kr, ky = -f * ki;
+  double alpha = (kx * (2.0 * px0 + kx) + ky * (2.0 * py0 + ky)) / (rel0 * rel0);
+  L.px = px0 + kx;
+  L.py = py0 + ky;
+  L.pz = L.pz + rel0 * d_sqrt_one(alpha);
+  double mc = lc.mc2 / lc.p0c;
+  double beta_new = (1.0 + L.pz) / sqrt(ps2 + L.px * L.px + L.py * L.py + mc * mc);
+  L.z = L.z * beta_new / L.beta;
+  L.beta = beta_new;
+  refresh_invariants(L, lc);
+}
+
 // track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140), relative time:
 // particle_rf_time = -z/(beta c) (bmad/code/particle_rf_time.f90:89).
 // c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f); n = n_slice
@@ -1005,7 +1030,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |
-                            TB(OP_HARD_MULT_EDGE) | TB(OP_EKICK) | TB(OP_EFRINGE);
+                            TB(OP_HARD_MULT_EDGE) | TB(OP_EKICK) | TB(OP_EFRINGE) | TB(OP_BEND_EKICK);
   if (tb & RARE) {
     if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
     if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
@@ -1019,6 +1044,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_HARD_MULT_EDGE)) op_hard_mult_edge(L, op, n);
     if (tb & TB(OP_EKICK)) op_ekick(L, op, flags & 0xff, n, lc);
     if (tb & TB(OP_EFRINGE)) op_efringe(L, op, flags & 0xff, n, lc);
+    if (tb & TB(OP_BEND_EKICK)) op_bend_ekick(L, op, flags & 0xff, n, lc);
   }
   if (flags & F_SHIFT_OUT) {  // offset_particle unset, pure translation (op_offset without pitch)
     uniform_block();

@@ -114,6 +114,10 @@ ELEMENT_CASES = {
     "instrument_electric": [lattice.ele("instrument", l=0.4, b_elec={1: 3e4})],
     "rfcavity_electric": [lattice.ele("rfcavity", l=0.5, voltage=5e5, rf_frequency=500e6, phi0=0.05, num_steps=5,
                                       a_elec={1: 1e4})],
+    "sbend_electric_edges": [lattice.ele("sbend", l=0.6, g=0.01, e1=0.1, e2=0.2, num_steps=4, a_elec={1: 2e4},
+                                         b_elec={0: -2e3}, r0_elec=3.0, x_offset=5e-4, ref_tilt=0.1)],
+    "sbend_k1_electric": [lattice.ele("sbend", l=0.6, g=0.02, k1=0.3, num_steps=5, b_elec={1: 3e4, 2: 2e5},
+                                      fringe_at=abi.EXIT_END)],
     "rcollimator": [lattice.ele("rcollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "ecollimator": [lattice.ele("ecollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "aperture_entrance": [lattice.ele("drift", l=0.5, x_limit=3e-3, y_limit=3e-3, aperture_at=abi.ENTRANCE_END,

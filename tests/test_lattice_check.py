@@ -67,7 +67,7 @@ def test_misaligned_bend_adds_frame_changes(kw, extra):
 
 
 @pytest.mark.parametrize("e,needle", [
-    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, a_elec={1: 1e4}), "electric multipoles in a bend"),
+    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, b_elec={0: 1.0, 6: 1.0}), "6 in a bend"),
     (lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, b_elec={0: 1.0, 7: 1.0}), "more than 7 orders"),
     (lattice.ele("drift", l=1.0, orientation=-1), "orientation -1"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_FULL), "fringe_type full"),
@@ -110,8 +110,8 @@ def test_electric_poles_of_thin_multipoles_are_ignored():
 
 
 def test_golden_elements_the_device_accepts():
-    """Of the reference's tracking_method_test elements, the device path takes
-    every one but the bends with electric multipoles."""
+    """The device path takes every one of the reference's tracking_method_test
+    elements."""
     tm = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
     sp = SPECIES[tm["particle"]]
     accepted, rejected = set(), {}
@@ -121,6 +121,5 @@ def test_golden_elements_the_device_accepts():
             accepted.add(name)
         else:
             rejected[name] = msg
-    assert all("electric multipoles in a bend" in m for m in rejected.values()), rejected
-    assert sorted(rejected) == ["SBEND3", "SBEND7"]
-    assert len(accepted) == len(tm["elements"]) - 2
+    assert not rejected, rejected
+    assert len(accepted) == len(tm["elements"]) == 22
