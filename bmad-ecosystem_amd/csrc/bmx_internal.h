@@ -47,6 +47,14 @@ enum : int32_t {
   F_OFFSET_AFFINE = 8,    // OP_OFFSET_*: bend frame change, r = W (x, y, 0) + t, t at c[0..2]
   F_OFFSET_MOVE = 32,     // OP_OFFSET_*: has a translation/pitch part
   F_EDGE_EXIT = 64,       // OP_SBEND_FUSED: exit edge present (F_HWANG_ENTER: entrance edge present)
+  F_COUPLER_ENTER = 1 << 8,  // OP_RFCAVITY: coupler kick at the entrance (c[5..7])
+  F_COUPLER_EXIT = 1 << 9,   // OP_RFCAVITY: coupler kick at the exit
+  // collimator phase-space limits (px, py, z, pz width2 / center pairs):
+  // OP_APERTURE_IN with F_APERTURE_PS holds them at c[6..13]; an OP_ELE_END
+  // with F_END_PS at c[0..7], its aperture constants then at c[8..13];
+  // F_PS_ELLIPTICAL: (px, py) and (z, pz) are ellipses (ecollimator)
+  F_PS_ELLIPTICAL = 1 << 10,
+  F_APERTURE_PS = 1 << 11,
   // the element ends after this op's body: orbit_too_large, then (with
   // F_END_APERTURE) the exit aperture with its constants at c[END_C ..]
   F_END = 1 << 16,
@@ -63,6 +71,7 @@ enum : int32_t {
   // it is centred on the axis
   F_END_INBOX = 1 << 21,
   F_END_CENTERED = 1 << 22,
+  F_END_PS = 1 << 23,
 };
 
 // aperture constants of an op carrying F_END (6 slots, see aperture())

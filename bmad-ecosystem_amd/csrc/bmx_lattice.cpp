@@ -252,7 +252,10 @@ bool is_misaligned(const bmx_ele_rec& e) {
 // offset_particle set/unset for a straight element (bmad/code/offset_particle.f90:150-330):
 // OP_OFFSET_* c[0..2] = x_off, y_off, z_off; c[3] = L/2; c[4..12] = W row-major
 // (W^-1 when setting); the element tilt follows (set) or precedes (unset) as OP_TILT.
-void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set, bool with_tilt) {
+// half_hv: offset_particle's half hkick / vkick (set_hvkicks, :300-304 and
+// :393-397), a dipole kick between the frame change and the tilt.
+void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set, bool with_tilt,
+                 bool half_hv = false) {
   bool mis = is_misaligned(e);
   bool tilt = with_tilt && e.tilt != 0;
   DevOp t = make_op(OP_TILT, ie);
@@ -260,7 +263,12 @@ void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool se
     t.c[0] = std::cos(set ? e.tilt : -e.tilt);
     t.c[1] = std::sin(set ? e.tilt : -e.tilt);
   }
+  DevOp hv = make_op(OP_KICK, ie);  // px += hkick / 2, py += vkick / 2: K = -hkick / 2 + i vkick / 2
+  hv.c[0] = -0.5 * e.hkick;
+  hv.c[1] = 0.5 * e.vkick;
+  bool kick = half_hv && (e.hkick != 0 || e.vkick != 0);
   if (!set && tilt) prog.push_back(t);
+  if (!set && kick) prog.push_back(hv);
   if (mis) {
     DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
     op.flags |= F_OFFSET_MOVE;
@@ -279,6 +287,7 @@ void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool se
     for (int i = 0; i < 9; i++) op.c[4 + i] = w[i];
     prog.push_back(op);
   }
+  if (set && kick) prog.push_back(hv);
   if (set && tilt) prog.push_back(t);
 }
 
@@ -420,7 +429,7 @@ int used_consts(const DevOp& op) {
     case OP_KICK: case OP_EKICK: case OP_EFRINGE: return 2 * (nh + 1);
     case OP_BEND_KICK: case OP_BEND_EKICK: return 14;
     case OP_KD_SLICES: return 1 + 2 * (nh + 1);
-    case OP_RFCAVITY: return 5;
+    case OP_RFCAVITY: return (op.flags & (F_COUPLER_ENTER | F_COUPLER_EXIT)) ? 8 : 5;
     default: return 14;
   }
 }
@@ -474,8 +483,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       if (ae.ix_max >= 0 && ae.ix_max - lo_e >= ((e.key == BMX_KEY_SBEND) ? 6 : 7))
         return fail("electric multipoles spanning more than 7 orders (6 in a bend) are not supported on the device path");
     }
-    if (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 || e.pz_aperture_width2 != 0)
-      return fail("collimator phase-space apertures are not supported on the device path");
+    // collimator phase-space limits (check_aperture_limit.f90, ecollimator /
+    // rcollimator only; other elements ignore these fields)
+    bool is_coll = e.key == BMX_KEY_ECOLLIMATOR || e.key == BMX_KEY_RCOLLIMATOR;
+    bool ps_set = is_coll && (e.px_aperture_width2 != 0 || e.py_aperture_width2 != 0 || e.z_aperture_width2 != 0 ||
+                              e.pz_aperture_width2 != 0);
+    const double ps_c[8] = {e.px_aperture_width2, e.px_aperture_center, e.py_aperture_width2, e.py_aperture_center,
+                            e.z_aperture_width2,  e.z_aperture_center,  e.pz_aperture_width2, e.pz_aperture_center};
+    int ps_flags = ps_set ? (e.key == BMX_KEY_ECOLLIMATOR ? F_PS_ELLIPTICAL : 0) : 0;
     if (e.exact_multipoles != 0 && e.exact_multipoles != 1) return fail("exact_multipoles must be off");
     if (e.num_steps < 1 && e.l != 0) return fail("num_steps must be >= 1");
 
@@ -504,10 +519,15 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         c[0] = x1; c[1] = x2; c[2] = y1; c[3] = y2;
       }
     };
-    if (cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_ENTRANCE_END || e.aperture_at == BMX_BOTH_ENDS)) {
+    if (cfg.aperture_limit_on && (ap_set || ps_set) &&
+        (e.aperture_at == BMX_ENTRANCE_END || e.aperture_at == BMX_BOTH_ENDS)) {
       DevOp op = make_op(OP_APERTURE_IN, ie);
       op.flags = ap_flags;
       ap_consts(op, 0);
+      if (ps_set) {
+        op.flags |= F_APERTURE_PS | ps_flags;
+        for (int i = 0; i < 8; i++) op.c[6 + i] = ps_c[i];
+      }
       prog.push_back(op);
     }
 
@@ -789,13 +809,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         break;
       }
       case BMX_KEY_RFCAVITY: {
-        // track_a_rfcavity.f90:17-146 (relative time, standing wave, no coupler kick)
-        if (e.coupler_strength != 0) return fail("rfcavity coupler kicks are not supported on the device path");
-        if (e.hkick != 0 || e.vkick != 0) return fail("rfcavity hkick/vkick are not supported on the device path");
+        // track_a_rfcavity.f90:17-146 (relative time, standing wave).  Its
+        // offset_particle calls keep the default set_hvkicks, so hkick and
+        // vkick act twice: as half kicks there and in the ab kicks
+        // (multipole_ele_to_ab with include_kicks, :60-66)
         if (e.rf_frequency == 0 && e.voltage != 0) return fail("rfcavity with zero rf_frequency");
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
-        emit_offset(prog, ie, e, true, true);
+        emit_offset(prog, ie, e, true, true, true);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
         emit_ekicks(0.5 * e.l);
         int n_slice = std::max(1, nint(std::fabs(e.l) / (e.l / (e.num_steps > 0 ? e.num_steps : 1))));
@@ -807,10 +828,21 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         op.c[2] = e.phi0 + e.phi0_multipass;
         op.c[3] = e.rf_frequency;
         op.c[4] = (e.rf_frequency != 0) ? 0.5 / e.rf_frequency : 0.0;
+        if (e.coupler_strength != 0) {  // rf_coupler_kick.f90:41-68
+          bool both = e.coupler_at == BMX_BOTH_ENDS;
+          if (e.coupler_at == BMX_ENTRANCE_END || both) op.flags |= F_COUPLER_ENTER;
+          if (e.coupler_at == BMX_EXIT_END || both) op.flags |= F_COUPLER_EXIT;
+          double grad = (e.l == 0) ? 0.0 : e.voltage / e.l;
+          double dp = grad * e.coupler_strength;
+          op.c[5] = dp * std::cos(2 * kPi * e.coupler_angle);
+          op.c[6] = dp * std::sin(2 * kPi * e.coupler_angle);
+          if (both) { op.c[5] /= 2; op.c[6] /= 2; }
+          op.c[7] = 2 * kPi * e.coupler_phase;
+        }
         prog.push_back(op);
         emit_ekicks(0.5 * e.l);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
-        emit_offset(prog, ie, e, false, true);
+        emit_offset(prog, ie, e, false, true, true);
         break;
       }
       default:
@@ -821,7 +853,8 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     // last op when that op leaves c[END_C ..] free, else an OP_ELE_END.  A
     // pure-translation offset_particle unset before it folds into the op
     // before that (F_SHIFT_OUT).
-    bool exit_ap = cfg.aperture_limit_on && ap_set && (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS);
+    bool exit_ap = cfg.aperture_limit_on && (ap_set || ps_set) &&
+                   (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS);
     int end_flags = F_END | (exit_ap ? (F_END_APERTURE | ((ap_flags & F_APERTURE_ELLIPTICAL) ? F_END_ELLIPTICAL : 0)) : 0);
     size_t first = (size_t)ele_op_begin.back();
     if (prog.size() >= first + 2) {
@@ -833,6 +866,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         body.c[SHIFT_C + 1] = u.c[1];
         prog.pop_back();
       }
+    }
+    if (exit_ap && ps_set) {  // an element end of its own: phase-space limits at c[0..7], aperture at c[8..13]
+      DevOp op = make_op(OP_ELE_END, ie);
+      op.flags = end_flags | F_END_PS | ps_flags;
+      for (int i = 0; i < 8; i++) op.c[i] = ps_c[i];
+      ap_consts(op, 8);
+      prog.push_back(op);
+      continue;
     }
     DevOp* last = (prog.size() > first) ? &prog.back() : nullptr;
     if (!last || used_consts(*last) > END_C) {

@@ -871,13 +871,27 @@ __device__ __forceinline__ void op_bend_ekick(Lane& L, const DevOp& op, int nh, 
 
 // track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140), relative time:
 // particle_rf_time = -z/(beta c) (bmad/code/particle_rf_time.f90:89).
-// c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f); n = n_slice
-__device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, const LatticeConsts& lc) {
+// c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f),
+// 5, 6 coupler dp_x, dp_y, 7 2 pi coupler_phase; n = n_slice
+__device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, const LatticeConsts& lc) {
   double len = op.c[0], volt = op.c[1], freq = op.c[3], half_t = op.c[4];
   double time = -L.z / (L.beta * D_C_LIGHT);
   if (freq != 0.0 && !(time >= -half_t && time < half_t)) time = d_modulo2(time, half_t);
   double phase = d_modulo2(D_TWOPI * (op.c[2] - time * freq), D_PI);
   double dl = len / n_slice;
+  // rf_coupler_kick (bmad/low_level/rf_coupler_kick.f90:20-106) at one edge
+  auto coupler = [&]() {
+    double ph = 0.5 * D_PI - phase + op.c[7];
+    double dpx = op.c[5], dpy = op.c[6];
+    L.px += dpx * cos(ph) / lc.p0c;
+    L.py += dpy * cos(ph) / lc.p0c;
+    double f2 = sin(ph) * D_TWOPI * freq / D_C_LIGHT;
+    energy_kick(L, (dpx * L.x + dpy * L.y) * f2, lc);
+  };
+  if (flags & F_COUPLER_ENTER) {
+    coupler();
+    if (L.state != BMX_ALIVE) return;
+  }
   for (int i = 0; i <= n_slice; i++) {
     double dE = volt * sin(phase) / n_slice;
     if (i == 0 || i == n_slice) dE = dE / 2.0;
@@ -891,6 +905,7 @@ __device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, const Lattice
       phase = phase + D_TWOPI * freq * (L.z - z0) / (D_C_LIGHT * L.beta);
     }
   }
+  if (flags & F_COUPLER_EXIT) coupler();
 }
 
 // orbit_too_large (bmad/low_level/orbit_too_large.f90:26-120).  A straight-
@@ -915,7 +930,48 @@ __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
 // aperture part of check_aperture_limit (bmad/low_level/check_aperture_limit.f90:257-309)
 // c, rectangular: 0 x1_lim, 1 x2_lim, 2 y1_lim, 3 y2_lim;
 // elliptical: 0 cx, 1 cy, 2 1/wx, 3 1/wy, 4 wx, 5 wy
-__device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptical) {
+// The phase-space limits of a collimator (check_aperture_limit.f90, the
+// ecollimator / rcollimator branch before the x, y limits; check_this_limit):
+// ps = px, py, z, pz (width2, center) pairs; an ecollimator takes each of
+// (px, py) and (z, pz) as an ellipse when both widths are set.  uf: the
+// unstable factor the x, y limits are compared with.
+__device__ void ps_limits(Lane& L, const double* ps, bool ell, double& uf) {
+  auto one = [&](double v, double w2, double c0) {
+    if (w2 == 0.0) return;
+    if (v > c0 - w2 && v < c0 + w2) return;
+    L.state = BMX_LOST;
+    double d1 = (c0 - w2) - v, d2 = v - (c0 + w2);
+    uf = d1 > d2 ? d1 : d2;
+  };
+  auto pair = [&](double u, double v, const double* q) {
+    if (!ell || q[0] == 0.0 || q[2] == 0.0) {
+      one(u, q[0], q[1]);
+      one(v, q[2], q[3]);
+      return;
+    }
+    double a = (u - q[1]) / q[0], b = (v - q[3]) / q[2];
+    double r = a * a + b * b;
+    if (r > 1.0) {
+      L.state = BMX_LOST;
+      uf = sqrt(r) - 1.0;
+    }
+  };
+  pair(L.px, L.py, ps);
+  pair(L.z, L.pz, ps + 4);
+}
+
+// true when ps_limits would leave the particle alone (the same tests)
+__device__ __forceinline__ bool ps_inside(const Lane& L, const double* ps, bool ell) {
+  auto one = [&](double v, double w2, double c0) { return w2 == 0.0 || (v > c0 - w2 && v < c0 + w2); };
+  auto pair = [&](double u, double v, const double* q) {
+    if (!ell || q[0] == 0.0 || q[2] == 0.0) return one(u, q[0], q[1]) && one(v, q[2], q[3]);
+    double a = (u - q[1]) / q[0], b = (v - q[3]) / q[2];
+    return !(a * a + b * b > 1.0);
+  };
+  return pair(L.px, L.py, ps) && pair(L.z, L.pz, ps + 4);
+}
+
+__device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptical, double uf = 0.0) {
   if (elliptical) {
     double xp = L.x - c[0], yp = L.y - c[1];
     // screen with the reciprocal widths: it is within a few ulp of the
@@ -929,7 +985,7 @@ __device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptic
     else L.state = (yp > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
     return;
   }
-  double x1 = c[0], x2 = c[1], y1 = c[2], y2 = c[3], uf = 0.0, f;
+  double x1 = c[0], x2 = c[1], y1 = c[2], y2 = c[3], f;
   if (x1 != 0.0 && L.x < x1) { f = fabs((L.x - x1) / (x2 - x1)); if (f > uf) { L.state = BMX_LOST_NEG_X; uf = f; } }
   if (x2 != 0.0 && L.x > x2) { f = fabs((L.x - x2) / (x2 - x1)); if (f > uf) { L.state = BMX_LOST_POS_X; uf = f; } }
   if (y1 != 0.0 && L.y < y1) { f = fabs((L.y - y1) / (y2 - y1)); if (f > uf) { L.state = BMX_LOST_NEG_Y; uf = f; } }
@@ -971,8 +1027,10 @@ __device__ __forceinline__ void op_sbend(Lane& L, const DevOp& op, int flags, bo
 // wave; only when some lane fails it do the reference's ordered tests run.
 // The elliptical screen is the one of aperture(); a NaN coordinate fails
 // every screen and takes the ordered tests, as in the reference.
-// c: the aperture constants (as aperture()).
-__device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, const LatticeConsts& lc) {
+// c: the aperture constants (as aperture()); ps: collimator phase-space
+// limits (F_END_PS).
+__device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, const LatticeConsts& lc,
+                                        const double* ps) {
   double mal = lc.max_aperture_limit;
   double rel_p = 1.0 + L.pz;
   double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
@@ -996,9 +1054,14 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
                 (c[3] == 0.0 || L.y <= c[3]);
     }
   }
+  if (flags & F_END_PS) inside &= ps_inside(L, ps, flags & F_PS_ELLIPTICAL);
   bool check = (L.state == BMX_ALIVE) & !inside;
   if (__builtin_expect(wave_any(check), 0)) {
-    if (check && !too_large(L, lc) && (flags & F_END_APERTURE)) aperture(L, c, flags & F_END_ELLIPTICAL);
+    if (check && !too_large(L, lc) && (flags & F_END_APERTURE)) {
+      double uf = 0.0;
+      if (flags & F_END_PS) ps_limits(L, ps, flags & F_PS_ELLIPTICAL, uf);
+      aperture(L, c, flags & F_END_ELLIPTICAL, uf);
+    }
   }
 }
 
@@ -1026,7 +1089,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   }
   if (tb & (TB(OP_HWANG) | TB(OP_RFCAVITY))) {
     if (tb & TB(OP_HWANG)) op_hwang(L, op, flags);
-    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, n, lc);
+    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, n, flags, lc);
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |
@@ -1040,7 +1103,11 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_OFFSET_SET)) op_offset(L, op, true, flags);
     if (tb & TB(OP_OFFSET_UNSET)) op_offset(L, op, false, flags);
     if (tb & TB(OP_TOO_LARGE)) too_large(L, lc);
-    if (tb & TB(OP_APERTURE_IN)) aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL);
+    if (tb & TB(OP_APERTURE_IN)) {
+      double uf = 0.0;
+      if (flags & F_APERTURE_PS) ps_limits(L, op.c + 6, flags & F_PS_ELLIPTICAL, uf);
+      aperture(L, op.c, flags & F_APERTURE_ELLIPTICAL, uf);
+    }
     if (tb & TB(OP_HARD_MULT_EDGE)) op_hard_mult_edge(L, op, n);
     if (tb & TB(OP_EKICK)) op_ekick(L, op, flags & 0xff, n, lc);
     if (tb & TB(OP_EFRINGE)) op_efringe(L, op, flags & 0xff, n, lc);
@@ -1059,6 +1126,6 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
       }
     }
   }
-  if (flags & F_END) ele_end(L, op.c + END_C, flags, lc);
+  if (flags & F_END) ele_end(L, op.c + ((flags & F_END_PS) ? 8 : END_C), flags, lc, op.c);
 }
 #undef TB

@@ -118,6 +118,21 @@ ELEMENT_CASES = {
                                          b_elec={0: -2e3}, r0_elec=3.0, x_offset=5e-4, ref_tilt=0.1)],
     "sbend_k1_electric": [lattice.ele("sbend", l=0.6, g=0.02, k1=0.3, num_steps=5, b_elec={1: 3e4, 2: 2e5},
                                       fringe_at=abi.EXIT_END)],
+    "rfcavity_hvkick_tilt_offset": [lattice.ele("rfcavity", l=0.8, voltage=4e5, rf_frequency=500e6, phi0=0.02,
+                                                num_steps=4, hkick=1e-4, vkick=-2e-4, tilt=0.2, x_offset=1e-3)],
+    "rfcavity_coupler_both": [lattice.ele("rfcavity", l=1.0, voltage=1e6, rf_frequency=500e6, phi0=0.1, num_steps=5,
+                                          coupler_strength=0.05, coupler_angle=0.1, coupler_phase=0.2,
+                                          coupler_at=abi.BOTH_ENDS)],
+    "rfcavity_coupler_entrance": [lattice.ele("rfcavity", l=0.6, voltage=5e5, rf_frequency=1.3e9, phi0=-0.05,
+                                              num_steps=3, coupler_strength=0.1, coupler_angle=0.3,
+                                              coupler_at=abi.ENTRANCE_END)],
+    "rcollimator_phase_space": [lattice.ele("rcollimator", l=0.5, x_limit=3e-3, y_limit=2e-3,
+                                            px_aperture_width2=3e-4, py_aperture_width2=4e-4, pz_aperture_width2=3e-3,
+                                            z_aperture_width2=4e-3, z_aperture_center=1e-4)],
+    "ecollimator_phase_space_both_ends": [lattice.ele("ecollimator", l=0.4, x_limit=4e-3, y_limit=3e-3,
+                                                      aperture_at=abi.BOTH_ENDS, px_aperture_width2=4e-4,
+                                                      py_aperture_width2=5e-4, px_aperture_center=2e-5,
+                                                      pz_aperture_width2=3e-3)],
     "rcollimator": [lattice.ele("rcollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "ecollimator": [lattice.ele("ecollimator", l=0.5, x_limit=3e-3, y_limit=2e-3)],
     "aperture_entrance": [lattice.ele("drift", l=0.5, x_limit=3e-3, y_limit=3e-3, aperture_at=abi.ENTRANCE_END,

@@ -72,7 +72,6 @@ def test_misaligned_bend_adds_frame_changes(kw, extra):
     (lattice.ele("drift", l=1.0, orientation=-1), "orientation -1"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_FULL), "fringe_type full"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_SAD_FULL), "fringe_type full"),
-    (lattice.ele("rfcavity", l=1.0, voltage=1e6, rf_frequency=5e8, num_steps=10, coupler_strength=0.1), "coupler"),
 ])
 def test_rejections_name_the_feature(e, needle):
     ok, n, msg = _check([lattice.ele("drift", l=0.5), e])
@@ -123,3 +122,16 @@ def test_golden_elements_the_device_accepts():
             rejected[name] = msg
     assert not rejected, rejected
     assert len(accepted) == len(tm["elements"]) == 22
+
+
+def test_phase_space_limits_only_on_collimators():
+    """check_aperture_limit reads the phase-space limits of ecollimator and
+    rcollimator alone: on a drift they change nothing; a collimator with
+    them gets its own element-end op (and an entrance check where
+    aperture_at asks for one), even with no x, y limits."""
+    ps = dict(px_aperture_width2=1e-3, pz_aperture_width2=2e-3)
+    assert _n_ops(lattice.ele("drift", l=1.0, **ps)) == _n_ops(lattice.ele("drift", l=1.0))
+    for key in ("rcollimator", "ecollimator"):
+        assert _n_ops(lattice.ele(key, l=0.5, **ps)) == _n_ops(lattice.ele(key, l=0.5)) + 1
+        both = _n_ops(lattice.ele(key, l=0.5, aperture_at=abi.BOTH_ENDS, **ps))
+        assert both == _n_ops(lattice.ele(key, l=0.5)) + 2
