@@ -62,26 +62,34 @@ DA_SPEC = dict(n_turns=512, n_angle=64, n_amp=128, n_pz=5, amp_max_x=0.01, amp_r
                pz_max=0.002, n_cells=50)
 
 
-def da_scan(ctx, n_train=512, n_test=512, epochs=8):
+def da_scan(ctx, n_train=2048, n_test=512, epochs=8):
     """DA lattices/hour, surrogate-screened vs plain FP64 grid tracking, on
     unseen sextupole-knob settings of a 50-cell (400-element) ring: polar grid
     64 angles x 128 amplitudes (0..10 mm, y at half the x amplitude) x 5 pz in
     [-0.2 %, +0.2 %], 512 turns.  The network (4 x 256 SiLU, BF16 MFMA) is
     trained first on a plain scan of n_train settings; that one-time cost and
     the number of scanned lattices that repays it are reported beside the
-    rates.  The screened scan resolves the lattices in two chunks on two
-    streams, so that the latency-bound refinement rounds of one overlap the
-    other's work."""
+    rates.  The screened scan runs in bracket mode: FP64 tracking of the
+    predicted boundary and the amplitude below it on every ray, one more
+    round of probes (4 up / 3 down) or bracket fill where that does not pin
+    the first loss, then the rays still open take the boundary of their
+    resolved neighbours in angle.  It resolves the lattices in two chunks on
+    two streams, so that the rounds of one overlap the other's work."""
     from bmx import surrogate
     spec = surrogate.ScanSpec(**DA_SPEC)
-    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs, workers=2)
+    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs, workers=2, window=(1, 0),
+                                probe_up=4, probe_down=3, max_rounds=2)
     keep = ("plain_lattices_per_hour", "surrogate_lattices_per_hour", "speedup", "da_area_err_mean",
-            "da_area_err_max", "frac_within_2pct", "rays_identical", "ics_tracked_plain", "ics_tracked_surrogate",
+            "da_area_err_max", "frac_within_2pct", "rays_identical", "rays_open",
+            "da_area_err_all_slices_frac_within_2pct", "da_area_err_all_slices_abs_max_rel_mean_area",
+            "ics_tracked_plain", "ics_tracked_surrogate",
             "pet_plain", "pet_surrogate", "plain_s", "surrogate_s", "train_scan_s", "train_s", "break_even_lattices",
             "pred_boundary_within_band", "mlp_forward_tflops", "mlp_train_step_tflops", "n_train", "n_test")
     out = {k: r[k] for k in keep}
+    out["rounds"] = r["rounds"]
     out["workload"] = ("50-cell ring (400 elements), 32 sextupole-family knobs U(+-20%), grid 64 x 128 x 5 "
-                       "(40,960 ICs/lattice), 512 turns, FP64; band +-10% of the predicted boundary re-tracked")
+                       "(40,960 ICs/lattice), 512 turns, FP64; DA area = on-momentum slice; bracket mode: "
+                       "2 FP64 rounds around the predicted boundary, open rays from resolved neighbours")
     return out
 
 

@@ -492,22 +492,26 @@ def predict_boundaries(net, knob_sets, spec, lat_chunk=None):
     return kb.cpu().numpy().astype(np.int64).reshape(n_lat, spec.n_pz, spec.n_angle)
 
 
-def _resolve_rays(ctx, knob_sets, spec, lo, hi, refine, turns_per_block, max_rounds):
-    """FP64 rounds of surrogate_scan for one chunk of lattices on context ctx:
-    the band [lo, hi] first, then gap refinement until every ray is resolved.
-    Bookkeeping is per ray: bnd = the first tracked loss (n_amp if none) and
-    a_top = the highest tracked survivor below it (-1 if none); the ray is
-    resolved when no untracked amplitude lies between them.
-    Returns (bnd [n_lat, n_pz, n_angle], number of tracked ICs,
+def _resolve_rays(lat, n_lat, spec, lo, hi, refine, turns_per_block, max_rounds, up=0, down=0):
+    """FP64 rounds of surrogate_scan for one chunk of n_lat lattices,
+    uploaded as the variants of lat (closed on return): the window [lo, hi]
+    of every ray first, then further rounds until every
+    ray is resolved or max_rounds have run.  Bookkeeping is per ray:
+    bnd = the first tracked loss (n_amp if none) and a_top = the highest
+    tracked survivor below it (-1 if none); the ray is resolved when no
+    untracked amplitude lies between them.  Later rounds: with up = down = 0,
+    up to `refine` evenly spaced points of each open gap (refine = 0: the
+    whole gap); with up, down > 0 (bracket mode), the `up` amplitudes above
+    a_top of a ray with no loss yet, the `down` amplitudes below bnd of a ray
+    with no survivor yet, and the whole gap of a ray holding both.
+    Returns (bnd, a_top [n_lat, n_pz, n_angle], number of tracked ICs,
     rounds [(n_ics, seconds, pet)])."""
     import time
     from . import tracker
-    n_lat = len(knob_sets)
     N = spec.n_turns
     A = spec.n_amp
     n_ray_lat = spec.n_pz * spec.n_angle
     R = n_lat * n_ray_lat
-    lat = upload_variants(ctx, knob_sets, spec)
     off = spec.offsets().reshape(n_ray_lat, A, 3)
     bnd = np.full(R, A, dtype=np.int64)
     a_top = np.full(R, -1, dtype=np.int64)
@@ -519,11 +523,14 @@ def _resolve_rays(ctx, knob_sets, spec, lo, hi, refine, turns_per_block, max_rou
     cnt = np.maximum(hi - lo + 1, 0)
     ray = np.repeat(np.arange(R), cnt)
     amp = lo[ray] + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
-    for _ in range(max_rounds):
+    for rnd in range(max_rounds):
         keep = ~tracked[ray, amp]
         ray, amp = ray[keep], amp[keep]
         if len(ray) == 0:
             break
+        # the launch takes the ICs lattice by lattice: order them by ray
+        order = np.argsort(ray * A + amp, kind="stable")
+        ray, amp = ray[order], amp[order]
         tracked[ray, amp] = True
         o = off[ray % n_ray_lat, amp]
         counts = np.bincount(ray // n_ray_lat, minlength=n_lat)
@@ -544,9 +551,26 @@ def _resolve_rays(ctx, knob_sets, spec, lo, hi, refine, turns_per_block, max_rou
         np.maximum.at(a_top, ar[below], ak[below])
         gap = bnd - a_top - 1
         open_r = np.nonzero(gap > 0)[0]
-        if len(open_r) == 0:
+        if len(open_r) == 0 or rnd == max_rounds - 1:
             break
         g = gap[open_r]
+        if up > 0 or down > 0:  # bracket mode
+            no_loss, no_surv = bnd[open_r] == A, a_top[open_r] == -1
+            rr, aa = [], []
+            o = open_r[no_loss & ~no_surv]
+            pts = a_top[o, None] + 1 + np.arange(up)[None, :]
+            rr.append(np.broadcast_to(o[:, None], pts.shape)[pts < A])
+            aa.append(pts[pts < A])
+            o = open_r[no_surv & ~no_loss]
+            pts = bnd[o, None] - 1 - np.arange(down)[None, :]
+            rr.append(np.broadcast_to(o[:, None], pts.shape)[pts >= 0])
+            aa.append(pts[pts >= 0])
+            both = ~no_surv & ~no_loss
+            o, gb = open_r[both], g[both]
+            rr.append(np.repeat(o, gb))
+            aa.append(np.repeat(a_top[o], gb) + 1 + (np.arange(gb.sum()) - np.repeat(np.cumsum(gb) - gb, gb)))
+            ray, amp = np.concatenate(rr).astype(np.int64), np.concatenate(aa).astype(np.int64)
+            continue
         if refine <= 0:  # the whole gap in one round
             ray = np.repeat(open_r, g)
             amp = a_top[ray] + 1 + (np.arange(g.sum()) - np.repeat(np.cumsum(g) - g, g))
@@ -560,72 +584,129 @@ def _resolve_rays(ctx, knob_sets, spec, lo, hi, refine, turns_per_block, max_rou
         u = np.unique(ray * A + amp)
         ray, amp = u // A, u % A
     lat.close()
-    return bnd.reshape(n_lat, spec.n_pz, spec.n_angle), int(tracked.sum()), rounds
+    shape = (n_lat, spec.n_pz, spec.n_angle)
+    return bnd.reshape(shape), a_top.reshape(shape), int(tracked.sum()), rounds
 
 
-def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_rounds=64, refine=6, workers=1):
+def neighbour_estimate(bnd, a_top, kb, reach=3):
+    """Boundary of every ray [..., n_angle]: bnd where the ray is resolved
+    (a_top = bnd - 1), else the mean boundary of the resolved rays up to
+    `reach` angles away on either side (weights 1/distance; the angles span
+    0..pi, so the two ends are not neighbours), or the prediction kb where
+    none is resolved, rounded and clipped into the ray's bracket
+    [a_top + 1, bnd]."""
+    res = a_top == bnd - 1
+    b = np.where(res, bnd, 0).astype(np.float64)
+    w = res.astype(np.float64)
+    num = np.zeros_like(b)
+    den = np.zeros_like(b)
+    for d in range(1, reach + 1):
+        num[..., d:] += b[..., :-d] / d  # neighbour d angles below
+        den[..., d:] += w[..., :-d] / d
+        num[..., :-d] += b[..., d:] / d  # neighbour d angles above
+        den[..., :-d] += w[..., d:] / d
+    est = np.where(den > 0, num / np.maximum(den, 1e-300), kb)
+    est = np.clip(np.rint(est).astype(np.int64), a_top + 1, bnd)
+    return np.where(res, bnd, est)
+
+
+def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_rounds=64, refine=6, workers=1,
+                   window=None, probe_up=4, probe_down=3):
     """Surrogate-screened DA scan.  Every ray (lattice, pz, angle) gets a
-    predicted boundary index from the network (step_fit_boundary of its
-    predicted profile); the initial conditions whose amplitude lies within
-    +-band of the predicted boundary amplitude are re-tracked in FP64
+    predicted boundary index kb from the network (step_fit_boundary of its
+    predicted profile).
+
+    Exact mode (window None): the initial conditions whose amplitude lies
+    within +-band of the predicted boundary amplitude are re-tracked in FP64
     (bmx_track_ics, one launch for all rays of a chunk).  Where the re-tracked
     band does not pin the first loss -- untracked amplitudes remain between
     the highest tracked survivor and the first tracked loss (or the grid
     edge) -- the next round tracks up to `refine` evenly spaced points of the
-    gap (refine = 0: the whole gap in one round), until every ray is resolved: plain FP64 tracking is the fallback and the judge of
-    every boundary.  workers > 1 splits the lattices into that
-    many chunks resolved concurrently, each on its own context (HIP stream),
-    so the latency-bound late rounds of one chunk overlap the others' work.
+    gap (refine = 0: the whole gap in one round), until every ray is
+    resolved: plain FP64 tracking is the fallback and the judge of every
+    boundary.
+
+    Bracket mode (window = (a, b)): round 1 tracks amplitudes kb-a .. kb+b of
+    every ray; each later round probes probe_up amplitudes above the top
+    survivor of a ray with no loss yet, probe_down below the first loss of a
+    ray with no survivor yet, and the whole gap of a bracketed ray; after
+    max_rounds a ray still open takes neighbour_estimate().  Every round is
+    as long as its longest survivor (n_turns), so a bounded number of rounds
+    bounds the scan time; the rays left open are a few per cent, scattered,
+    and their neighbours in angle carry the boundary.
+
+    workers > 1 splits the lattices into that many chunks resolved
+    concurrently, each on its own context (HIP stream), so the
+    latency-bound late rounds of one chunk overlap the others' work; their
+    lattice uploads (host work) run while the network predicts.
     Returns (runs [n_lat, n_pz, n_angle], area [n_lat], timing dict
-    (total / predict / track seconds, rounds), n_tracked)."""
+    (total / predict / track seconds, rounds, kb, n_open), n_tracked)."""
     import threading
     import time
     from . import tracker
     t0 = time.perf_counter()
     knob_sets = np.asarray(knob_sets)
     n_lat = len(knob_sets)
-    N = spec.n_turns
-    kb = predict_boundaries(net, knob_sets, spec)
-    amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
-    # band in index space: amplitudes within +-band of the boundary amplitude
-    a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
-    lo = np.searchsorted(amps, a_b * (1 - band), side="left")
-    hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
-    lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
-    hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
-    t_pred = time.perf_counter()
     workers = max(1, min(workers, n_lat))
     bounds = np.linspace(0, n_lat, workers + 1).astype(int)
     results = [None] * workers
     errors = []
+    box = {}
+    ready = threading.Event()
+    ctxs = [ctx] if workers == 1 else [tracker.Context(ctx.device) for _ in range(workers)]
 
-    def work(w, c):
+    def work(w):
         try:
             a, b = bounds[w], bounds[w + 1]
-            results[w] = _resolve_rays(c, knob_sets[a:b], spec, lo[a:b], hi[a:b], refine, turns_per_block, max_rounds)
+            lat = upload_variants(ctxs[w], knob_sets[a:b], spec)
+            ready.wait()
+            if "lo" not in box:  # the prediction failed
+                lat.close()
+                return
+            results[w] = _resolve_rays(lat, b - a, spec, box["lo"][a:b], box["hi"][a:b], refine, turns_per_block,
+                                       max_rounds, box["up"], box["down"])
         except Exception as e:  # re-raised in the caller
             errors.append(e)
 
-    if workers == 1:
-        work(0, ctx)
-    else:
-        ctxs = [tracker.Context(ctx.device) for _ in range(workers)]
-        th = [threading.Thread(target=work, args=(w, ctxs[w])) for w in range(workers)]
-        for t in th:
-            t.start()
+    th = [threading.Thread(target=work, args=(w,)) for w in range(workers)] if workers > 1 else []
+    for t in th:
+        t.start()
+    try:
+        kb = predict_boundaries(net, knob_sets, spec)
+        amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
+        # band in index space: amplitudes within +-band of the boundary amplitude
+        a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
+        if window is None:
+            lo = np.searchsorted(amps, a_b * (1 - band), side="left")
+            hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
+            lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
+            hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
+            box.update(lo=lo, hi=hi, up=0, down=0)
+        else:
+            box.update(lo=np.clip(kb - window[0], 0, spec.n_amp - 1), hi=np.clip(kb + window[1], 0, spec.n_amp - 1),
+                       up=max(1, probe_up), down=max(1, probe_down))
+        t_pred = time.perf_counter()
+    finally:
+        ready.set()
+        if workers == 1:
+            work(0)
         for t in th:
             t.join()
-        for c in ctxs:
-            c.close()
+        if workers > 1:
+            for c in ctxs:
+                c.close()
     if errors:
         raise errors[0]
     # surviving prefix: untracked amplitudes below the first tracked loss count as alive
-    runs = np.concatenate([r[0] for r in results], 0)
-    n_tracked = sum(r[1] for r in results)
-    rounds = [x for r in results for x in r[2]]
+    bnd = np.concatenate([r[0] for r in results], 0)
+    a_top = np.concatenate([r[1] for r in results], 0)
+    n_tracked = sum(r[2] for r in results)
+    rounds = [x for r in results for x in r[3]]
+    runs = neighbour_estimate(bnd, a_top, kb)
+    n_open = int(np.sum(a_top != bnd - 1))
     area = np.array([area_from_runs(runs[v, spec.n_pz // 2], spec) for v in range(n_lat)])
     t1 = time.perf_counter()
-    timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb)
+    timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb, n_open=n_open)
     return runs, area, timing, n_tracked
 
 
@@ -661,31 +742,48 @@ def inference_throughput(net, n=1 << 18, reps=10):
 
 
 def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed=20261205, band=0.10,
-                  gamma=TARGET_GAMMA, workers=1, refine=6):
+                  gamma=TARGET_GAMMA, workers=1, refine=6, window=None, probe_up=4, probe_down=3, max_rounds=64,
+                  scan_chunk=512):
     """Surrogate-screened vs plain DA scans of n_test unseen knob settings:
-    trains a fresh 4 x 256 network on a plain scan of n_train settings, then
-    scans the test settings both ways.  Returns a dict of timings,
-    lattices/hour (lattice upload + tracking for plain; upload + inference +
-    band search + FP64 re-tracking for the surrogate), FP64-tracked IC counts
-    and the DA-area error statistics."""
+    trains a fresh 4 x 256 network on a plain scan of n_train settings (in
+    chunks of scan_chunk lattices), then scans the test settings both ways
+    (surrogate_scan's exact mode, or its bracket mode with window, probe_up,
+    probe_down and max_rounds).  Returns a dict of timings, lattices/hour
+    (lattice upload + tracking for plain; upload + inference + FP64
+    re-tracking for the surrogate), FP64-tracked IC counts, the rays left to
+    the neighbour estimate and the DA-area error statistics: of the on-momentum
+    slice (the DA area) and of every pz slice."""
     import time
     rng = np.random.default_rng(seed)
     k_train = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_train, N_KNOBS))
     k_test = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_test, N_KNOBS))
-    s_tr, _, t_gen = plain_scan(ctx, k_train, spec)
+    Y, t_gen = [], dict(total=0.0, track=0.0)
+    for a in range(0, n_train, scan_chunk):
+        s_tr, _, t = plain_scan(ctx, k_train[a:a + scan_chunk], spec)
+        Y.append(target(s_tr.reshape(-1), spec.n_turns, gamma))
+        t_gen["total"] += t["total"]
+        t_gen["track"] += t["track"]
+        del s_tr
+    Y = np.concatenate(Y)
     t0 = time.perf_counter()
     net = MLP(ctx, [N_KNOBS + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
     X = features(k_train, spec)
-    Y = target(s_tr.reshape(-1), spec.n_turns, gamma)
-    del s_tr
     losses = train(net, X, Y, epochs=epochs, batch=batch)
     del X, Y
     t_train = time.perf_counter() - t0
     s_te, a_plain, t_plain = plain_scan(ctx, k_test, spec)
     plain_runs = np.cumprod(s_te >= spec.n_turns, axis=-1).sum(-1)
     del s_te
-    runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band, workers=workers, refine=refine)
+    runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band, workers=workers, refine=refine,
+                                                   window=window, probe_up=probe_up, probe_down=probe_down,
+                                                   max_rounds=max_rounds)
     err = np.abs(a_sur - a_plain) / np.maximum(a_plain, 1e-300)
+    # every pz slice: relative area error where the slice has area, absolute (in units of the mean
+    # on-momentum area) everywhere
+    a_all_p = np.array([[area_from_runs(plain_runs[v, p], spec) for p in range(spec.n_pz)] for v in range(n_test)])
+    a_all_s = np.array([[area_from_runs(runs[v, p], spec) for p in range(spec.n_pz)] for v in range(n_test)])
+    has = a_all_p > 0
+    err_all = np.abs(a_all_s - a_all_p)[has] / a_all_p[has]
     plain_s = t_plain["total"]
     sur_s = t_sur["total"]
     per_lat_gain = plain_s / n_test - sur_s / n_test
@@ -698,7 +796,12 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
         "speedup": plain_s / sur_s,
         "ics_tracked_plain": n_test * spec.n_ic, "ics_tracked_surrogate": int(n_tracked),
         "rays_identical": float(np.mean(runs == plain_runs)),
+        "rays_open": t_sur["n_open"] / runs.size,
+        "da_area_err_all_slices_frac_within_2pct": float(np.mean(err_all <= 0.02)),
+        "da_area_err_all_slices_abs_max_rel_mean_area": float(np.abs(a_all_s - a_all_p).max() / np.mean(a_plain)),
         "pred_boundary_err_mean": float(np.mean(np.abs(t_sur["kb"] - plain_runs))),
+        "pred_boundary_exact": float(np.mean(t_sur["kb"] == plain_runs)),
+        "pred_boundary_err_q90": float(np.quantile(np.abs(t_sur["kb"] - plain_runs), 0.9)),
         "pred_boundary_within_band": float(np.mean(np.abs(t_sur["kb"] - plain_runs) <= np.maximum(1, band * plain_runs))),
         "da_area_err_mean": float(err.mean()), "da_area_err_max": float(err.max()),
         "frac_within_2pct": float(np.mean(err <= 0.02)),

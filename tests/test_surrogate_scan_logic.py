@@ -16,7 +16,7 @@ class _FakeLat:
         pass
 
 
-def _run(monkeypatch, truth, pred_shift, islands=None, refine=0):
+def _run(monkeypatch, truth, pred_shift, islands=None, refine=0, **kw):
     spec = surrogate.ScanSpec(n_turns=100, n_angle=6, n_amp=40, n_pz=3)
     n_lat = truth.shape[0]
     N = spec.n_turns
@@ -55,7 +55,10 @@ def _run(monkeypatch, truth, pred_shift, islands=None, refine=0):
     monkeypatch.setattr(surrogate, "upload_variants", lambda ctx, k, s: _FakeLat())
     from bmx import tracker
     monkeypatch.setattr(tracker, "track_ics", fake_track_ics)
-    runs, area, _, n_tracked = surrogate.surrogate_scan(None, None, np.zeros((n_lat, 32)), spec, refine=refine)
+    runs, area, timing, n_tracked = surrogate.surrogate_scan(None, None, np.zeros((n_lat, 32)), spec, refine=refine,
+                                                            **kw)
+    if kw:
+        return runs, n_tracked, spec, timing
     return runs, n_tracked, spec
 
 
@@ -119,3 +122,72 @@ def test_step_fit_boundary():
                   [0.2, 0.1, 0.1, 0, 0, 0, 0, 0],      # lost from the start
                   [1, 1, 0.2, 1, 1, 0.1, 0.1, 0.1]])   # one mispredicted point does not move it
     assert list(surrogate.step_fit_boundary(p, 1.0)) == [5, 8, 0, 5]
+
+
+@pytest.mark.parametrize("shift", [0, 1, -1, 3, -2])
+def test_bracket_mode_resolves_near_predictions(monkeypatch, shift):
+    """Bracket mode, window kb-1..kb, probes 4 up / 3 down, two rounds: a
+    prediction within reach of the second round's probes is resolved exactly,
+    with a handful of tracked ICs per ray."""
+    rng = np.random.default_rng(abs(shift) + 11)
+    truth = rng.integers(5, 35, size=(4, 3, 6))
+    runs, n_tracked, spec, timing = _run(monkeypatch, truth, shift, window=(1, 0), probe_up=4, probe_down=3,
+                                         max_rounds=2)
+    assert np.array_equal(runs, truth)
+    assert timing["n_open"] == 0
+    assert n_tracked <= truth.size * (2 + 4)
+    assert len(timing["rounds"]) == (1 if shift == 0 else 2)
+
+
+def test_bracket_mode_bounds_rounds_and_estimates_open_rays(monkeypatch):
+    """A prediction far off leaves rays open after max_rounds: no third round
+    runs, and each open ray's boundary lies inside the bracket the tracked
+    ICs prove (above the top survivor, at or below the first loss)."""
+    truth = np.full((2, 3, 6), 25)
+    runs, n_tracked, spec, timing = _run(monkeypatch, truth, -15, window=(1, 0), probe_up=4, probe_down=3,
+                                         max_rounds=2)
+    assert len(timing["rounds"]) == 2
+    assert timing["n_open"] == truth.size
+    # tracked: kb-1, kb (both alive) then 4 above: the top survivor is kb + 4 = 14, no loss seen
+    assert np.all(runs >= 15) and np.all(runs <= spec.n_amp)
+
+
+def test_neighbour_estimate():
+    """An open ray takes the 1/distance-weighted mean of the resolved rays
+    within 3 angles (not across the ends of the 0..pi fan), clipped into its
+    bracket; resolved rays keep their boundary; an open ray with no resolved
+    neighbour keeps the prediction."""
+    bnd = np.array([[20, 22, 40, 24, 26, 28]])
+    a_top = np.array([[19, 21, 10, 23, 25, 27]])  # ray 2 open: survivor at 10, first loss at 40
+    kb = np.full_like(bnd, 5)
+    est = surrogate.neighbour_estimate(bnd, a_top, kb)
+    # distances from angle 2: angles 1, 3 at 1; 0, 4 at 2; 5 at 3
+    num = 22 + 24 + (20 + 26) / 2 + 28 / 3
+    den = 1 + 1 + 1 + 1 / 3
+    assert est[0, 2] == int(np.rint(num / den))
+    # the end ray 0 sees angles 1..3 only
+    a_e, b_e = a_top.copy(), bnd.copy()
+    a_e[0, 0], b_e[0, 0] = 5, 40
+    assert surrogate.neighbour_estimate(b_e, a_e, kb)[0, 0] == int(np.rint((22 + 24 / 3) / (1 + 1 / 3)))
+    # no resolved neighbour: the prediction, clipped into the bracket
+    a_n = np.array([[5, 5, 5, 5, 5, 5]])
+    assert np.array_equal(surrogate.neighbour_estimate(bnd, a_n, kb), np.full_like(bnd, 6))
+    assert np.array_equal(np.delete(est[0], 2), np.delete(bnd[0], 2))
+    # clipped into the bracket
+    a_top2 = a_top.copy()
+    a_top2[0, 2] = 30
+    assert surrogate.neighbour_estimate(bnd, a_top2, kb)[0, 2] == 31
+
+
+def test_bracket_mode_mixed_misses_across_lattices(monkeypatch):
+    """Predictions off by different amounts ray by ray, some above and some
+    below the truth, on several lattices: the second round mixes probes
+    above, probes below and bracket fills of different lattices in one
+    launch, and every IC must be tracked on its own lattice."""
+    rng = np.random.default_rng(5)
+    truth = rng.integers(8, 32, size=(5, 3, 6))
+    shift = rng.choice([-4, -3, -2, -1, 0, 1, 2, 3], size=truth.shape)  # within reach of 4 up / 3 down
+    runs, n_tracked, spec, timing = _run(monkeypatch, truth, shift, window=(1, 0), probe_up=4, probe_down=3,
+                                         max_rounds=2)
+    assert timing["n_open"] == 0
+    assert np.array_equal(runs, truth)
