@@ -51,3 +51,18 @@ def test_surrogate_da_scan_end_to_end(ctx):
     _, area_u, _, n_u = surrogate.surrogate_scan(ctx, untrained, k_test, spec)
     print(f"untrained network: tracked {n_u} ICs", flush=True)
     assert n_tr < n_u
+
+    # bracket mode on two streams: two FP64 rounds, the open rays from their neighbours.  Every
+    # boundary it reports lies in the bracket its tracked ICs prove, the resolved rays agree with
+    # the plain scan wherever survival is monotonic, and it tracks fewer ICs than the exact mode.
+    runs_b, area_b, t_b, n_b = surrogate.surrogate_scan(ctx, net, k_test, spec, workers=2, window=(1, 0),
+                                                        probe_up=4, probe_down=3, max_rounds=2)
+    err_b = np.abs(area_b - area_plain) / area_plain
+    print(f"bracket mode: tracked {n_b} ICs in {len(t_b['rounds'])} rounds, {t_b['n_open']} rays open; "
+          f"DA-area error mean {err_b.mean():.4f} max {err_b.max():.4f}; identical {np.mean(runs_b == plain_runs):.3f}",
+          flush=True)
+    assert len(t_b["rounds"]) <= 2 * 2  # two rounds per stream
+    assert n_b < n_tr
+    assert t_b["n_open"] <= 0.25 * runs_b.size
+    assert np.mean(runs_b == plain_runs) >= 0.7
+    assert err_b.mean() <= 0.02
