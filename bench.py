@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate-vs-plain DA scan comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy (reference for HBM byte counters) and exit")
+    ap.add_argument("--da-only", action="store_true",
+                    help="only run the surrogate-vs-plain DA scan comparison and print its record")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); particle tracking "
@@ -121,6 +123,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": src.numel() * 8, "copy_calibration_bytes_written":
                           dst.numel() * 8, "launches": 3}), flush=True)
+        return
+
+    if args.da_only:
+        print(json.dumps({"da_scan": da_scan(tracker.Context(0))}), flush=True)
         return
 
     eles = lattice.ring_c1(n_cells=500, n_rf=8)

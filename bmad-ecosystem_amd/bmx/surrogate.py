@@ -361,34 +361,40 @@ def area_from_runs(runs, spec):
     return da.polygon_area(a * np.cos(th), a * spec.amp_ratio_y * np.sin(th))
 
 
-def train(net, X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, lr_final=2e-4):
+def train(net, X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, lr_final=2e-4, loss_every=16):
     """Minibatch Adam on (X, Y) (numpy float32) with a cosine-decayed learning
-    rate; every epoch is one shuffled pass.  Returns the per-epoch mean losses."""
+    rate; every epoch is one shuffled pass.  Each epoch gathers its shuffled
+    copy of the data on the device once; its steps then run back to back on
+    the network's stream, which the host joins every `loss_every` steps (and
+    at the end of the epoch) to read the loss.  Returns the per-epoch means
+    of those sampled losses."""
     dev = net.dev
     Xd = torch.from_numpy(np.ascontiguousarray(X)).to(dev)
     Yd = torch.from_numpy(np.ascontiguousarray(Y)).to(dev)
     g = torch.Generator(device="cpu").manual_seed(seed)
     n = Xd.shape[0]
-    steps = epochs * max(1, n // batch)
+    nb = n // batch
+    steps = epochs * max(1, nb)
     step = 0
     losses = []
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g).to(dev)
-        acc = torch.zeros(1, device=dev)
-        nb = 0
-        for b in range(0, n - batch + 1, batch):
-            idx = perm[b:b + batch]
-            xb = Xd.index_select(0, idx).contiguous()
-            yb = Yd.index_select(0, idx).contiguous()
-            torch.cuda.synchronize(dev)
+        Xe = Xd.index_select(0, perm)
+        Ye = Yd.index_select(0, perm)
+        torch.cuda.synchronize(dev)  # the gathered epoch is complete before the network's stream reads it
+        acc, cnt = 0.0, 0
+        for k in range(nb):
             frac = step / max(steps - 1, 1)
             cur = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac))
-            loss = net.train_step(xb, yb, float(cur))
-            net.sync()
-            acc += loss
-            nb += 1
+            loss = net.train_step(Xe[k * batch:(k + 1) * batch], Ye[k * batch:(k + 1) * batch], float(cur))
             step += 1
-        losses.append(float(acc.item()) / max(nb, 1))
+            if k % loss_every == loss_every - 1 or k == nb - 1:
+                net.sync()
+                acc += float(loss.item())
+                cnt += 1
+        net.sync()  # every step has read Xe, Ye before they go
+        del Xe, Ye
+        losses.append(acc / max(cnt, 1))
     return losses
 
 

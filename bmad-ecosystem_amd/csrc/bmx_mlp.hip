@@ -9,6 +9,7 @@
 // v_mfma_f32_16x16x32_bf16 (A: 16 x 32, B: 32 x 16, C: 16 x 16 FP32).
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -149,20 +150,54 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int ks
   epilogue_store(v, (int64_t)m * ldc + n, n, epi, C, Cb, bias, aux);
 }
 
-// column sums of a row-major fp32 matrix: out[n] = sum_m X[m * ld + n]
-__global__ __launch_bounds__(256) void colsum_kernel(int M, int N, const float* __restrict__ X, int ld,
-                                                    float* __restrict__ out) {
-  int n = blockIdx.x;
+// Column sums of a row-major fp32 matrix, out[n] = sum_m X[m * ld + n] (the
+// bias gradients), in two passes with a fixed summation order.  Pass 1 gives
+// one partial per (row block, column): for N >= 32 a workgroup sums
+// COLSUM_ROWS rows of 256 consecutive columns, lane n reading column n (each
+// row a coalesced load); for narrow matrices a workgroup sums COLSUM_ROWS1
+// rows of one column.  Pass 2 adds each column's partials.
+constexpr int COLSUM_ROWS = 16;
+constexpr int COLSUM_ROWS1 = 1024;
+
+__global__ __launch_bounds__(256) void colsum_partial_kernel(int M, int N, const float* __restrict__ X, int ld,
+                                                            float* __restrict__ P) {
+  int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  int m0 = blockIdx.y * COLSUM_ROWS, m1 = min(M, m0 + COLSUM_ROWS);
   float s = 0.f;
-  for (int m = threadIdx.x; m < M; m += 256) s += X[(int64_t)m * ld + n];
-  __shared__ float red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  for (int m = m0; m < m1; m++) s += X[(int64_t)m * ld + n];
+  P[(int64_t)blockIdx.y * N + n] = s;
+}
+
+// narrow pass 1: workgroup (b, n) sums rows [b COLSUM_ROWS1, (b+1) COLSUM_ROWS1) of column n
+__global__ __launch_bounds__(256) void colsum_partial1_kernel(int M, int N, const float* __restrict__ X, int ld,
+                                                             float* __restrict__ P) {
+  int n = blockIdx.y;
+  int m0 = blockIdx.x * COLSUM_ROWS1;
+  float s = 0.f;
+  for (int k = 0; k < COLSUM_ROWS1 / 256; k++) {
+    int m = m0 + k * 256 + (int)threadIdx.x;
+    if (m < M) s += X[(int64_t)m * ld + n];
   }
-  if (threadIdx.x == 0) out[n] = red[0];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) P[(int64_t)blockIdx.x * N + n] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// one wave per column: lane j adds the partials j, j + 64, ..., then a fixed
+// shuffle tree over the wave (the column index is uniform over a wave, so a
+// wave leaves or stays whole)
+__global__ __launch_bounds__(256) void colsum_final_kernel(int nb, int N, const float* __restrict__ P,
+                                                          float* __restrict__ out) {
+  int n = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  int j = (int)(threadIdx.x & 63);
+  if (n >= N) return;
+  float s = 0.f;
+  for (int b = j; b < nb; b += 64) s += P[(int64_t)b * N + n];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (j == 0) out[n] = s;
 }
 
 // Adam (Kingma & Ba 2015) on fp32 master weights, refreshing the bf16 copy.
@@ -276,13 +311,14 @@ __global__ __launch_bounds__(RAYS) void step_fit_kernel(int n_rays, int K, const
 extern "C" hipStream_t bmx_ctx_stream(const bmx_ctx* ctx);
 extern "C" int32_t bmx_ctx_set_error(bmx_ctx* ctx, const char* msg);
 
-// per-context split-K workspace, grown on demand, released by bmx_ctx_destroy
+// per-context workspaces (slot 0: split-K partial products, slot 1:
+// column-sum partials), grown on demand, released by bmx_ctx_destroy
 static std::mutex g_ws_mu;
-static std::unordered_map<const bmx_ctx*, std::pair<float*, size_t>> g_ws;
+static std::unordered_map<const bmx_ctx*, std::array<std::pair<float*, size_t>, 2>> g_ws;
 
-static float* splitk_workspace(const bmx_ctx* ctx, size_t floats) {
+static float* splitk_workspace(const bmx_ctx* ctx, size_t floats, int slot = 0) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  auto& e = g_ws[ctx];
+  auto& e = g_ws[ctx][slot];
   if (e.second < floats) {
     if (e.first) (void)hipFree(e.first);
     e.first = nullptr;
@@ -312,7 +348,8 @@ extern "C" __attribute__((visibility("hidden"))) void bmx_mlp_release(const bmx_
   std::lock_guard<std::mutex> lk(g_ws_mu);
   auto it = g_ws.find(ctx);
   if (it == g_ws.end()) return;
-  if (it->second.first) (void)hipFree(it->second.first);
+  for (auto& e : it->second)
+    if (e.first) (void)hipFree(e.first);
   g_ws.erase(it);
 }
 
@@ -366,8 +403,20 @@ int32_t bmx_gemm_bf16(bmx_ctx* ctx, int32_t trans_a, int32_t trans_b, int32_t M,
 }
 
 int32_t bmx_colsum_f32(bmx_ctx* ctx, int32_t M, int32_t N, const float* X, int32_t ld, float* out) {
-  if (!ctx || !X || !out || M <= 0 || N <= 0) return bmx_ctx_set_error(ctx, "bmx_colsum_f32: invalid arguments");
-  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)N), dim3(256), 0, bmx_ctx_stream(ctx), M, N, X, ld, out);
+  if (!ctx || !X || !out || M <= 0 || N <= 0 || ld < N)
+    return bmx_ctx_set_error(ctx, "bmx_colsum_f32: invalid arguments");
+  hipStream_t s = bmx_ctx_stream(ctx);
+  bool wide = N >= 32;
+  int rows = wide ? COLSUM_ROWS : COLSUM_ROWS1;
+  int nb = (M + rows - 1) / rows;
+  float* P = splitk_workspace(ctx, (size_t)nb * N, 1);
+  if (!P) return bmx_ctx_set_error(ctx, "bmx_colsum_f32: workspace allocation failed");
+  if (wide)
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)nb), dim3(256), 0, s, M, N,
+                       X, ld, P);
+  else
+    hipLaunchKernelGGL(colsum_partial1_kernel, dim3((unsigned)nb, (unsigned)N), dim3(256), 0, s, M, N, X, ld, P);
+  hipLaunchKernelGGL(colsum_final_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s, nb, N, P, out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
 }

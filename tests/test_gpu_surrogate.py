@@ -98,6 +98,17 @@ def test_colsum_mse_adam(ctx):
     assert ctx.lib.bmx_colsum_f32(ctx.h, 3000, 77, _p(tX), 77, _p(out)) == 0
     assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
     assert np.allclose(out.cpu().numpy(), X.astype(np.float64).sum(0), rtol=1e-5, atol=1e-4)
+    # the two-pass path at the network's shapes (16384 x 256), with more than 256 columns and a row
+    # stride wider than the matrix, and the narrow path (one column; five of a wider stride)
+    for M, N, ld in [(16384, 256, 256), (1000, 300, 320), (5000, 1, 1), (2000, 5, 7)]:
+        Xw = rng.standard_normal((M, ld)).astype(np.float32)
+        tXw = torch.from_numpy(Xw).to(DEV)
+        outw = torch.zeros(N, dtype=torch.float32, device=DEV)
+        torch.cuda.synchronize()
+        assert ctx.lib.bmx_colsum_f32(ctx.h, M, N, _p(tXw), ld, _p(outw)) == 0
+        assert ctx.lib.bmx_ctx_sync(ctx.h) == 0
+        want = Xw[:, :N].astype(np.float64).sum(0)
+        assert np.allclose(outw.cpu().numpy(), want, rtol=1e-5, atol=2e-4 * np.sqrt(M / 3000)), (M, N, ld)
 
     n = 5000
     p = rng.standard_normal(n).astype(np.float32)
