@@ -99,6 +99,44 @@ def periodic_twiss(eles):
     return tuple(out) + tuple(tunes)
 
 
+def twiss_from_mat2(m):
+    """(beta, alpha, phi) of one plane from its 2x2 one-turn block, as
+    twiss_from_mat2 (bmad/low_level/twiss_from_mat2.f90:19-60): the block
+    normalised by its determinant, cos phi = trace / 2, phi in [0, 2 pi).
+    Raises ValueError for an unstable or non-symplectic block."""
+    m = np.asarray(m, dtype=np.float64)
+    det = m[0, 0] * m[1, 1] - m[0, 1] * m[1, 0]
+    m = m / det
+    t_cos = (m[0, 0] + m[1, 1]) / 2.0
+    if abs(t_cos) >= 1.0:
+        raise ValueError("unstable matrix")
+    t_sin = math.copysign(1.0, m[0, 1]) * math.sqrt(1.0 - t_cos * t_cos)
+    phi = math.atan2(t_sin, t_cos)
+    if phi < 0:
+        phi += 2 * math.pi
+    alpha = (m[0, 0] - m[1, 1]) / (2.0 * t_sin)
+    radical = -(1 + alpha * alpha) * m[0, 1] / m[1, 0]
+    if radical <= 0:
+        raise ValueError("non-symplectic matrix")
+    return math.sqrt(radical), alpha, phi
+
+
+def twiss_from_tracking(lat, orbit=None, d_orb=None, variant=0):
+    """Periodic Twiss (beta_x, alpha_x, beta_y, alpha_y, nu_x, nu_y) at the
+    lattice start from the one-turn matrix the device forms by tracking about
+    orbit (tracker.one_turn_matrix), its x and y blocks taken as uncoupled, as
+    twiss_from_tracking (bmad/code/twiss_from_tracking.f90) feeds
+    twiss_at_start.  Also returns the matrix."""
+    from . import tracker
+    m, status = tracker.one_turn_matrix(lat, orbit, d_orb)
+    if status[variant] != 0:
+        raise ValueError("a probe particle was lost")
+    M = m[variant]
+    bx, ax, px = twiss_from_mat2(M[0:2, 0:2])
+    by, ay, py = twiss_from_mat2(M[2:4, 2:4])
+    return (bx, ax, by, ay, px / (2 * math.pi), py / (2 * math.pi)), M
+
+
 def matched_gaussian(n, twiss, eps_x, eps_y, sigma_z, sigma_pz, seed):
     """(6, n) float64 array of a Twiss-matched 6-D Gaussian bunch."""
     bx, ax, by, ay = twiss[:4]

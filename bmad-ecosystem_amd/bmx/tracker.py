@@ -273,6 +273,24 @@ def closed_orbit(lat, n_dim=4, guess=None):
     return orb, status
 
 
+def one_turn_matrix(lat, orbit=None, d_orb=None):
+    """One-turn 6x6 matrix of every variant by tracking about orbit [n_var, 6]
+    (zeros when None), central differences with steps d_orb[6] (default
+    1e-5 each) (bmx_one_turn_matrix).  Returns (mat6 [n_var, 6, 6], status)."""
+    nv = lat.n_variants
+    orb = np.ascontiguousarray(np.zeros((nv, 6)) if orbit is None else np.asarray(orbit, np.float64).reshape(nv, 6))
+    d = None if d_orb is None else np.ascontiguousarray(np.asarray(d_orb, np.float64).reshape(6))
+    m = np.zeros((nv, 36))
+    status = np.zeros(nv, dtype=np.int32)
+    st = abi.Stats()
+    dp = None if d is None else d.ctypes.data_as(C.POINTER(C.c_double))
+    rc = lat.ctx.lib.bmx_one_turn_matrix(lat.ctx.h, lat.h, orb.ctypes.data_as(C.POINTER(C.c_double)), dp,
+                                         m.ctypes.data_as(C.POINTER(C.c_double)),
+                                         status.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(st))
+    _check(lat.ctx.h, rc, "bmx_one_turn_matrix")
+    return m.reshape(nv, 6, 6), status
+
+
 def track_ics(lat, ics_per_variant, n_turns, turns_per_block=0, return_stats=False):
     """Survival turns of explicit initial conditions, one [6, n_v] host array
     per variant (bmx_track_ics).  Returns a list of int32 arrays."""

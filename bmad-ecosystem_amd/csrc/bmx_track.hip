@@ -842,4 +842,56 @@ int32_t bmx_closed_orbit(bmx_ctx* ctx, const bmx_lattice* lat, int32_t n_dim, do
   return 0;
 }
 
+int32_t bmx_one_turn_matrix(bmx_ctx* ctx, const bmx_lattice* lat, const double* orbit, const double* d_orb,
+                            double* mat6, int32_t* status, bmx_stats* out) {
+  if (!ctx || !lat || !orbit || !mat6 || !status) return set_err(ctx, "invalid arguments");
+  double d[6];
+  for (int k = 0; k < 6; k++) {
+    d[k] = d_orb ? d_orb[k] : 1e-5;  // bmad_com%d_orb
+    if (!(d[k] > 0)) return set_err(ctx, "d_orb must be positive");
+  }
+  int nv = lat->n_variants;
+  const int np = 12;  // orbit +- d[k] e_k
+  int64_t n = (int64_t)nv * np;
+  std::vector<int64_t> off(nv + 1);
+  for (int v = 0; v <= nv; v++) off[v] = (int64_t)v * np;
+  std::vector<double> c(6 * n), beta(n);
+  std::vector<int32_t> state(n), ixl(n), tl(n);
+  double mc2 = lat->lc.mc2, p0c = lat->lc.p0c;
+  for (int v = 0; v < nv; v++)
+    for (int j = 0; j < np; j++) {
+      int64_t k = off[v] + j;
+      for (int q = 0; q < 6; q++) {
+        double x = orbit[v * 6 + q];
+        if (q == j / 2) x += (j % 2 == 0) ? d[q] : -d[q];
+        c[(size_t)q * n + k] = x;
+      }
+      double pz = c[(size_t)5 * n + k], pc = p0c * (1.0 + pz);
+      beta[k] = pc / std::sqrt(pc * pc + mc2 * mc2);
+      state[k] = BMX_ALIVE;
+    }
+  bmx_particles p;
+  std::memset(&p, 0, sizeof(p));
+  p.x = &c[0]; p.px = &c[(size_t)n]; p.y = &c[(size_t)2 * n]; p.py = &c[(size_t)3 * n];
+  p.z = &c[(size_t)4 * n]; p.pz = &c[(size_t)5 * n];
+  p.beta = beta.data(); p.state = state.data(); p.ix_lost = ixl.data(); p.turn_lost = tl.data();
+  p.n = n;
+  p.on_device = 0;
+  bmx_stats st;
+  if (bmx_track_multi(ctx, lat, off.data(), 1, &p, &st) != 0) return -1;
+  if (out) *out = st;
+  for (int v = 0; v < nv; v++) {
+    bool lost = false;
+    for (int j = 0; j < np; j++) lost |= state[off[v] + j] != BMX_ALIVE;
+    status[v] = lost ? 2 : 0;
+    for (int i = 0; i < 6; i++)
+      for (int q = 0; q < 6; q++) {
+        int64_t kp = off[v] + 2 * q, km = kp + 1;
+        mat6[(size_t)v * 36 + i * 6 + q] =
+            lost ? 0.0 : (c[(size_t)i * n + kp] - c[(size_t)i * n + km]) / (2.0 * d[q]);
+      }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -240,6 +240,18 @@ interface
     type(bmx_stats) :: stats
     integer(c_int32_t) :: rc
   end function
+
+  function bmx_one_turn_matrix(ctx, lat, orbit, d_orb, mat6, status, stats) &
+                               bind(C, name = 'bmx_one_turn_matrix') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    real(c_double), intent(in) :: orbit(*)
+    type(c_ptr), value :: d_orb
+    real(c_double), intent(out) :: mat6(*)   ! (6, 6, n_variants), each matrix the transpose of mat6(i, j)
+    integer(c_int32_t), intent(out) :: status(*)
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
 end interface
 
 contains

@@ -307,6 +307,16 @@ int32_t bmx_bunch_moments(bmx_ctx *ctx, const bmx_particles *p, double *moments)
 int32_t bmx_closed_orbit(bmx_ctx *ctx, const bmx_lattice *lat, int32_t n_dim, double *orbit, int32_t *status,
                          bmx_stats *out);
 
+/* One-turn transfer matrix of every variant by tracking, the matrix step of
+ * twiss_from_tracking (bmad/code/twiss_from_tracking.f90:51-70): central
+ * differences of one turn about orbit (n_variants x 6) with steps d_orb[6]
+ * (NULL: 1e-5 each, bmad_com%d_orb): mat6 is n_variants x 36, row-major,
+ * mat6[v][i][j] = d out_i / d in_j.  status[v]: 0, or 2 when a probe was
+ * lost (its matrix is zeros).  Cavities act as uploaded; the reference turns
+ * them off for its Twiss (upload the lattice without rf for that). */
+int32_t bmx_one_turn_matrix(bmx_ctx *ctx, const bmx_lattice *lat, const double *orbit, const double *d_orb,
+                            double *mat6, int32_t *status, bmx_stats *out);
+
 /* ------------------------------------------------------------------------
  * Dynamic-aperture surrogate kernels (no counterpart in the reference; they
  * serve the learned screening of lattice variants).  Device pointers; work

@@ -241,6 +241,56 @@ def test_one_turn_jacobian_matches_symplectic_oracle(ctx, ring):
         assert symp <= 1e-6
 
 
+def test_one_turn_matrix_and_twiss_from_tracking(ctx):
+    """bmx_one_turn_matrix (central differences of device tracking, steps
+    d_orb = 1e-5 as twiss_from_tracking takes them), for three ring variants
+    of different length in one call: against the oracle differenced with the
+    same 12 probes (the same matrix to rounding), and against the oracle's
+    exact one-turn Jacobian to the truncation of a central difference at that
+    step.  Then the periodic Twiss from the device matrix (twiss_from_mat2 of
+    each plane): equal to rounding to the Twiss of the oracle's same-probe
+    matrix, and within the central difference's truncation (5e-6 on this
+    ring, from the sextupoles) of the exact one, which the linear-optics
+    model of the cell reproduces to 1e-14."""
+    vs = lattice.da_variants(n_var=3, min_cells=6, max_cells=14)
+    offs = [0]
+    for v in vs:
+        offs.append(offs[-1] + len(v))
+    lat = ctx.lattice([e for v in vs for e in v], abi.ELECTRON, variant_offsets=offs)
+    m, status = tracker.one_turn_matrix(lat)
+    assert np.all(status == 0)
+    d = 1e-5
+    probes = np.zeros((6, 12))
+    for q in range(6):
+        probes[q, 2 * q], probes[q, 2 * q + 1] = d, -d
+    for v, eles in enumerate(vs):
+        _, J = oracle.jacobian(eles, abi.ELECTRON, 1, np.zeros(6))
+        r = oracle.track_bunch(eles, abi.ELECTRON, 1, probes)
+        o = np.array([r[c] for c in ("x", "px", "y", "py", "z", "pz")])
+        m_fd = (o[:, 0::2] - o[:, 1::2]) / (2 * d)
+        scale = np.max(np.abs(J))
+        err_fd = np.max(np.abs(m[v] - m_fd)) / scale
+        err = np.max(np.abs(m[v] - J)) / scale
+        print(f"one-turn matrix variant {v}: vs oracle same probes {err_fd:.2e}, vs exact Jacobian {err:.2e}, "
+              f"symplectic error {oracle.symp_error(m[v]):.2e}", flush=True)
+        assert err_fd <= 1e-9
+        assert err <= 1e-6
+        assert oracle.symp_error(m[v]) <= 1e-5
+    lat.close()
+    eles = lattice.fodo_ring_c0()
+    lat = ctx.lattice(eles, abi.ELECTRON)
+    tw, M = beam.twiss_from_tracking(lat)
+    lin = beam.periodic_twiss(eles)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, probes)
+    o = np.array([r[c] for c in ("x", "px", "y", "py", "z", "pz")])
+    m_fd = (o[:, 0::2] - o[:, 1::2]) / (2 * d)
+    tw_fd = [*beam.twiss_from_mat2(m_fd[0:2, 0:2])[:2], *beam.twiss_from_mat2(m_fd[2:4, 2:4])[:2]]
+    print("twiss from tracking:", np.round(tw, 8), "linear model:", np.round(lin, 8), flush=True)
+    np.testing.assert_allclose(tw[:4], tw_fd, rtol=1e-9)
+    np.testing.assert_allclose(tw, lin, rtol=1e-5)
+    lat.close()
+
+
 def test_host_buffer_path(ctx):
     eles = lattice.fodo_ring_c0(n_cells=5)
     coords = _coords(1000, 4, [1e-3, 1e-4, 1e-3, 1e-4, 1e-3, 1e-3])
