@@ -657,43 +657,52 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
     bounds = np.linspace(0, n_lat, workers + 1).astype(int)
     results = [None] * workers
     errors = []
-    box = {}
-    ready = threading.Event()
+    boxes = [{} for _ in range(workers)]
+    ready = [threading.Event() for _ in range(workers)]
     ctxs = [ctx] if workers == 1 else [tracker.Context(ctx.device) for _ in range(workers)]
+    amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
+
+    def window_of(kb):
+        if window is None:  # band in index space: amplitudes within +-band of the boundary amplitude
+            a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
+            lo = np.searchsorted(amps, a_b * (1 - band), side="left")
+            hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
+            lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
+            hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
+            return dict(lo=lo, hi=hi, up=0, down=0)
+        return dict(lo=np.clip(kb - window[0], 0, spec.n_amp - 1), hi=np.clip(kb + window[1], 0, spec.n_amp - 1),
+                    up=max(1, probe_up), down=max(1, probe_down))
 
     def work(w):
         try:
             a, b = bounds[w], bounds[w + 1]
             lat = upload_variants(ctxs[w], knob_sets[a:b], spec)
-            ready.wait()
-            if "lo" not in box:  # the prediction failed
+            ready[w].wait()
+            bx = boxes[w]
+            if "lo" not in bx:  # the prediction failed
                 lat.close()
                 return
-            results[w] = _resolve_rays(lat, b - a, spec, box["lo"][a:b], box["hi"][a:b], refine, turns_per_block,
-                                       max_rounds, box["up"], box["down"])
+            results[w] = _resolve_rays(lat, b - a, spec, bx["lo"], bx["hi"], refine, turns_per_block, max_rounds,
+                                       bx["up"], bx["down"])
         except Exception as e:  # re-raised in the caller
             errors.append(e)
 
+    # the network predicts chunk by chunk, and each chunk's FP64 rounds start (on its own stream) while
+    # the next chunk is predicted
     th = [threading.Thread(target=work, args=(w,)) for w in range(workers)] if workers > 1 else []
     for t in th:
         t.start()
+    kbs = []
     try:
-        kb = predict_boundaries(net, knob_sets, spec)
-        amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
-        # band in index space: amplitudes within +-band of the boundary amplitude
-        a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
-        if window is None:
-            lo = np.searchsorted(amps, a_b * (1 - band), side="left")
-            hi = np.searchsorted(amps, a_b * (1 + band), side="right") - 1
-            lo = np.clip(np.minimum(lo, kb - 1), 0, spec.n_amp - 1)
-            hi = np.clip(np.maximum(hi, kb), 0, spec.n_amp - 1)
-            box.update(lo=lo, hi=hi, up=0, down=0)
-        else:
-            box.update(lo=np.clip(kb - window[0], 0, spec.n_amp - 1), hi=np.clip(kb + window[1], 0, spec.n_amp - 1),
-                       up=max(1, probe_up), down=max(1, probe_down))
+        for w in range(workers):
+            kb_w = predict_boundaries(net, knob_sets[bounds[w]:bounds[w + 1]], spec)
+            kbs.append(kb_w)
+            boxes[w].update(window_of(kb_w))
+            ready[w].set()
         t_pred = time.perf_counter()
     finally:
-        ready.set()
+        for e in ready:
+            e.set()
         if workers == 1:
             work(0)
         for t in th:
@@ -703,6 +712,7 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
                 c.close()
     if errors:
         raise errors[0]
+    kb = np.concatenate(kbs, 0)
     # surviving prefix: untracked amplitudes below the first tracked loss count as alive
     bnd = np.concatenate([r[0] for r in results], 0)
     a_top = np.concatenate([r[1] for r in results], 0)
