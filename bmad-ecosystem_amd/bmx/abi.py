@@ -62,7 +62,7 @@ class EleRec(C.Structure):
 
 
 class Ref(C.Structure):
-    _fields_ = [("species", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("species", C.c_int32), ("tracked_species", C.c_int32)]
 
 
 class Config(C.Structure):

@@ -63,16 +63,17 @@ class Context:
         sc = abi.Schedule(int(turns_per_block), int(graph_depth), 0.0)
         _check(self.h, self.lib.bmx_ctx_set_schedule(self.h, C.byref(sc)), "bmx_ctx_set_schedule")
 
-    def lattice(self, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3):
-        return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit)
+    def lattice(self, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3,
+                tracked_species=None):
+        return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit, tracked_species)
 
 
-def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3):
+def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3, tracked_species=None):
     """(ok, n_ops, reason): whether the device path accepts this element list,
     decided on the host with no device (bmx_lattice_check)."""
     lib = abi.load_library()
     arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
-    ref = abi.Ref(species, 0)
+    ref = abi.Ref(species, tracked_species or 0)
     cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit)
     n_ops = C.c_int64(0)
     msg = C.create_string_buffer(512)
@@ -81,16 +82,20 @@ def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3)
 
 
 class Lattice:
-    """A compiled, device-resident element program (bmx_lattice_upload)."""
+    """A compiled, device-resident element program (bmx_lattice_upload).
+    tracked_species: the bunch species when it is the antiparticle of the
+    reference species (its kicks and voltages change sign)."""
 
-    def __init__(self, ctx, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3):
+    def __init__(self, ctx, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3,
+                 tracked_species=None):
         self.ctx = ctx
         self.species = species
+        self.tracked_species = tracked_species or species
         arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
         self.n_ele_total = len(arr)
         self.p0c = arr[0].p0c
         self.e_tot = arr[0].e_tot
-        ref = abi.Ref(species, 0)
+        ref = abi.Ref(species, tracked_species or 0)
         cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit)
         if variant_offsets is None:
             offs_p = None

@@ -7,7 +7,8 @@
 // track_a_rfcavity (bmad/low_level/track_a_rfcavity.f90:17-80).
 //
 // Scope of the device path: forward tracking (direction = +1, time_dir = +1),
-// element orientation +1, magnetic multipoles, relative-time RF.  Anything
+// element orientation +1 and -1 (sbend +1), the reference species or its
+// antiparticle, relative-time RF.  Anything
 // else is rejected with an error message naming the element; the CPU side
 // never silently substitutes another path.
 #include <algorithm>
@@ -253,9 +254,12 @@ bool is_misaligned(const bmx_ele_rec& e) {
 // OP_OFFSET_* c[0..2] = x_off, y_off, z_off; c[3] = L/2; c[4..12] = W row-major
 // (W^-1 when setting); the element tilt follows (set) or precedes (unset) as OP_TILT.
 // half_hv: offset_particle's half hkick / vkick (set_hvkicks, :300-304 and
-// :393-397), a dipole kick between the frame change and the tilt.
+// :393-397), a dipole kick between the frame change and the tilt, scaled by
+// charge_dir = orientation x rel_tracking_charge_to_mass.  Orientation -1
+// takes the affine form of emit_reversed_offset.
+void emit_reversed_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set);
 void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set, bool with_tilt,
-                 bool half_hv = false) {
+                 bool half_hv = false, double charge_dir = 1.0) {
   bool mis = is_misaligned(e);
   bool tilt = with_tilt && e.tilt != 0;
   DevOp t = make_op(OP_TILT, ie);
@@ -264,12 +268,14 @@ void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool se
     t.c[1] = std::sin(set ? e.tilt : -e.tilt);
   }
   DevOp hv = make_op(OP_KICK, ie);  // px += hkick / 2, py += vkick / 2: K = -hkick / 2 + i vkick / 2
-  hv.c[0] = -0.5 * e.hkick;
-  hv.c[1] = 0.5 * e.vkick;
+  hv.c[0] = -0.5 * charge_dir * e.hkick;
+  hv.c[1] = 0.5 * charge_dir * e.vkick;
   bool kick = half_hv && (e.hkick != 0 || e.vkick != 0);
   if (!set && tilt) prog.push_back(t);
   if (!set && kick) prog.push_back(hv);
-  if (mis) {
+  if (mis && e.orientation == -1) {
+    emit_reversed_offset(prog, ie, e, set);
+  } else if (mis) {
     DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
     op.flags |= F_OFFSET_MOVE;
     if (e.x_pitch != 0 || e.y_pitch != 0) op.flags |= F_OFFSET_PITCH;
@@ -365,6 +371,42 @@ void pitch_w(double theta, double phi, bool inverse, double w[3][3]) {
     for (int j = 0; j < 3; j++) w[i][j] = inverse ? m[j][i] : m[i][j];
 }
 
+// offset_particle of a straight element with orientation -1 and direction +1
+// (offset_particle.f90 with sign_z_vel = -1): the particle enters the body
+// frame at s = L with the z component of its momentum pointing backwards, and
+// the drift to the edge runs against s when setting.  Set: r = W^-1 (x - x_off,
+// y - y_off, L/2 - z_off) + (0, 0, L/2), drift against s over L - r(3).
+// Unset: r = W (x, y, -L/2) + (x_off, y_off, z_off + L/2), s -> L - r(3),
+// drift over L - s.  As an OP_OFFSET_* with F_OFFSET_AFFINE: rows 0 and 1 of
+// W with the momentum's z column negated, row 2 holding L - r(3), so the op's
+// noref drift over (0 or L) - row 2 is either reference drift (a drift of
+// length d against s is the noref drift of length -d).
+void emit_reversed_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set) {
+  double len = e.l, lh = 0.5 * e.l, w[3][3], v[3], t[3];
+  pitch_w(e.x_pitch, e.y_pitch, set, w);
+  if (set) { v[0] = -e.x_offset; v[1] = -e.y_offset; v[2] = lh - e.z_offset; }
+  else { v[0] = 0.0; v[1] = 0.0; v[2] = -lh; }
+  for (int i = 0; i < 3; i++) t[i] = w[i][0] * v[0] + w[i][1] * v[1] + w[i][2] * v[2];
+  if (set) {
+    t[2] += lh;
+  } else {
+    t[0] += e.x_offset;
+    t[1] += e.y_offset;
+    t[2] += e.z_offset + lh;
+  }
+  DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
+  op.flags |= F_OFFSET_MOVE | F_OFFSET_PITCH | F_OFFSET_AFFINE;
+  op.c[0] = t[0];
+  op.c[1] = t[1];
+  op.c[2] = len - t[2];
+  op.c[3] = lh;
+  double* c = op.c + 4;
+  c[0] = w[0][0]; c[1] = w[0][1]; c[2] = -w[0][2];
+  c[3] = w[1][0]; c[4] = w[1][1]; c[5] = -w[1][2];
+  c[6] = -w[2][0]; c[7] = -w[2][1]; c[8] = -w[2][2];
+  prog.push_back(op);
+}
+
 // offset_particle of a misaligned bend (g != 0 or ref_tilt != 0), direction
 // and orientation +1, drift to the edge: one OP_OFFSET_* with F_OFFSET_AFFINE,
 // c[0..2] = t, c[3] = L/2, c[4..12] = W row-major.
@@ -452,6 +494,15 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
   lc->mc2 = mc2;
   lc->beta0 = p0c / e_tot;
   lc->species = ref.species;
+  // the tracked species: the reference species or its antiparticle, whose
+  // rel_tracking_charge_to_mass (rel_tracking_charge_to_mass.f90:15-35) is the
+  // charge ratio; the device's mass, beta and energy constants stay valid
+  int tracked = ref.tracked_species ? ref.tracked_species : ref.species;
+  if (species_mass(tracked) != mc2 || species_charge(tracked) == 0) {
+    *err = "the tracked species must be the reference species or its antiparticle";
+    return false;
+  }
+  const double rel_charge = species_charge(tracked) / species_charge(ref.species);
   lc->mc2sq = mc2 * mc2;
   lc->p0c_sq = p0c * p0c;
   {
@@ -473,7 +524,22 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     ele_op_begin.push_back((int32_t)prog.size());
     if (e.p0c != p0c || e.p0c_start != p0c || e.e_tot != e_tot)
       return fail("reference energy changes along the lattice (only rfcavity-style lattices are supported)");
-    if (e.orientation != 1) return fail("orientation -1 is not supported on the device path");
+    if (e.orientation != 1 && e.orientation != -1) return fail("orientation must be +1 or -1");
+    if (e.orientation == -1 && e.key == BMX_KEY_SBEND)
+      return fail("sbend with orientation -1 is not supported on the device path");
+    // charge_dir of the magnetic fields: orientation x rel_tracking_charge_to_mass
+    // (direction +1); a reversed element meets its exit end at the first track
+    // edge (physical_ele_end.f90), which fringe_here and the aperture and
+    // coupler switches test
+    const bool rev = e.orientation == -1;
+    const double cdir = e.orientation * rel_charge;
+    auto at_end = [&](int where, bool first_edge) {
+      int phys = (first_edge != rev) ? BMX_ENTRANCE_END : BMX_EXIT_END;
+      return where == phys || where == BMX_BOTH_ENDS;
+    };
+    auto scale_ab = [&](AB& t, double f) {
+      for (int n = 0; n < BMX_N_POLE + 1; n++) { t.a[n] *= f; t.b[n] *= f; }
+    };
     AB ae;  // electric multipoles
     {
       std::string ee;
@@ -519,8 +585,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         c[0] = x1; c[1] = x2; c[2] = y1; c[3] = y2;
       }
     };
-    if (cfg.aperture_limit_on && (ap_set || ps_set) &&
-        (e.aperture_at == BMX_ENTRANCE_END || e.aperture_at == BMX_BOTH_ENDS)) {
+    if (cfg.aperture_limit_on && (ap_set || ps_set) && at_end(e.aperture_at, true)) {
       DevOp op = make_op(OP_APERTURE_IN, ie);
       op.flags = ap_flags;
       ap_consts(op, 0);
@@ -542,12 +607,10 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     // quadrupole edge kick has no strength here (fq1 = fq2 = 0)
     auto hard_edge = [&](int n, double b, bool enter) {
       if (e.fringe_type != BMX_FRINGE_HARD_EDGE_ONLY && e.fringe_type != BMX_FRINGE_FULL) return;
-      bool here = enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
-                        : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS);
-      if (!here) return;
+      if (!at_end(e.fringe_at, enter)) return;
       DevOp op = make_op(OP_HARD_MULT_EDGE, ie);
       op.n = n;
-      double c0 = b / (4.0 * (n + 2));
+      double c0 = cdir * b / (4.0 * (n + 2));
       op.c[0] = enter ? -c0 : c0;
       op.c[1] = (double)(n + 3) / (double)(n + 1);
       prog.push_back(op);
@@ -557,7 +620,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     // -b f with f = charge / (beta p0c) x scale, as kick coefficients
     // scale charge / p0c (-b, a); the device divides by beta and updates pz,
     // beta and z (OP_EKICK)
-    double charge = species_charge(ref.species);
+    double charge = species_charge(tracked);
     auto emit_ekicks = [&](double scale) {
       if (ae.ix_max < 0) return;
       AB t = ae;
@@ -571,9 +634,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     // ff = at_sign charge (at_sign +1 entering, -1 leaving) (OP_EFRINGE)
     auto emit_efringe = [&](bool enter) {
       if (e.l == 0 || ae.ix_max < 0) return;
-      bool here = enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
-                        : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS);
-      if (!here) return;
+      if (!at_end(e.fringe_at, enter)) return;
       double ff = (enter ? 1.0 : -1.0) * charge;
       AB t;
       std::memset(&t, 0, sizeof(t));
@@ -618,12 +679,13 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         if (key == BMX_KEY_MULTIPOLE && e.has_mag_poles && e.a[0] != 0) {
           DevOp op = make_op(OP_KICK_REF0, ie);
           double t = e.b[0] + e.tilt;
-          op.c[0] = e.a[0] * std::cos(t);
-          op.c[1] = e.a[0] * std::sin(t);
+          op.c[0] = cdir * e.a[0] * std::cos(t);
+          op.c[1] = cdir * e.a[0] * std::sin(t);
           prog.push_back(op);
           ab.a[0] = ab.b[0] = 0;
           ab.ix_max = max_nonzero(ab.a, ab.b, BMX_N_POLE - 1);
         }
+        scale_ab(ab, cdir);
         emit_kicks(prog, ie, ab, 1.0, OP_KICK);
         if (mis) emit_offset(prog, ie, e, false, false);
         break;
@@ -633,6 +695,8 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         AB ab;
         double b1;
         if (!magnetic_ab(e, true, &ab, &b1, &e2)) return fail(e2.c_str());
+        scale_ab(ab, cdir);
+        b1 *= cdir;
         int n_step = 1;
         if (ab.ix_max > -1 || ae.ix_max > -1) n_step = std::max(nint(std::fabs(e.l) / (e.l / e.num_steps)), 1);
         if (e.l == 0) n_step = 0;
@@ -663,6 +727,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         // track_a_thick_multipole.f90:18-133
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
+        scale_ab(ab, cdir);
         int n_step = (ab.ix_max == -1 && ae.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
         double step_len = e.l / n_step, r_step = 1.0 / n_step;
         emit_offset(prog, ie, e, true, true);
@@ -722,7 +787,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
                       (enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
                              : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS));
           if (!here) return;
-          double g_tot = e.is_on ? (e.g + e.dg) : 0.0, k1 = e.is_on ? e.k1 : 0.0;
+          double g_tot = e.is_on ? (e.g + e.dg) * cdir : 0.0, k1 = e.is_on ? e.k1 * cdir : 0.0;
           double ang = enter ? e.e1 : e.e2;
           double fint_gap = enter ? e.fint * e.hgap : e.fintx * e.hgapx;
           if (e.fringe_type == BMX_FRINGE_HARD_EDGE_ONLY) fint_gap = 0;
@@ -753,11 +818,12 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         AB ab;
         double b1;
         if (!magnetic_ab(e, true, &ab, &b1, &e2)) return fail(e2.c_str());
+        b1 *= cdir;
         if (std::fabs(b1) < 1e-10) { ab.b[1] = b1; b1 = 0; }
         double g = e.g, dg;
         if (e.l == 0) dg = 0;
         else { dg = ab.b[0] / e.l; ab.b[0] = 0; }
-        double g_tot = g + dg;
+        double g_tot = (g + dg) * cdir;
         if (!e.is_on) { dg = -g; g_tot = 0; b1 = 0; }
         int ix_mag_max = ab.ix_max;
         int n_step = 1;
@@ -765,6 +831,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         double r_step = 1.0 / n_step, step_len = e.l / n_step, angle = g * step_len;
         AB abk = ab;  // kicks run over orders 0..ix_mag_max as computed before dg/b1 were pulled out
         abk.ix_max = ix_mag_max;
+        scale_ab(abk, cdir);  // c_dir / charge_of(ref) of apply_multipole_kicks
         // bend_apply_multipole_kicks: the magnetic kick, then the electric one
         // from the fields at the same x, y (coefficients step_len coef charge / p0c (b, -a))
         auto bend_kicks = [&](double coef) {
@@ -816,7 +883,8 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         if (e.rf_frequency == 0 && e.voltage != 0) return fail("rfcavity with zero rf_frequency");
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
-        emit_offset(prog, ie, e, true, true, true);
+        scale_ab(ab, cdir);
+        emit_offset(prog, ie, e, true, true, true, cdir);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
         emit_ekicks(0.5 * e.l);
         int n_slice = std::max(1, nint(std::fabs(e.l) / (e.l / (e.num_steps > 0 ? e.num_steps : 1))));
@@ -824,14 +892,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         DevOp op = make_op(OP_RFCAVITY, ie);
         op.n = n_slice;
         op.c[0] = e.l;
-        op.c[1] = e.is_on ? e.voltage : 0.0;  // e_accel_field * charge_dir (charge_dir = 1)
+        op.c[1] = (e.is_on ? e.voltage : 0.0) * cdir;  // e_accel_field * charge_dir
         op.c[2] = e.phi0 + e.phi0_multipass;
         op.c[3] = e.rf_frequency;
         op.c[4] = (e.rf_frequency != 0) ? 0.5 / e.rf_frequency : 0.0;
         if (e.coupler_strength != 0) {  // rf_coupler_kick.f90:41-68
           bool both = e.coupler_at == BMX_BOTH_ENDS;
-          if (e.coupler_at == BMX_ENTRANCE_END || both) op.flags |= F_COUPLER_ENTER;
-          if (e.coupler_at == BMX_EXIT_END || both) op.flags |= F_COUPLER_EXIT;
+          if (at_end(e.coupler_at, true)) op.flags |= F_COUPLER_ENTER;
+          if (at_end(e.coupler_at, false)) op.flags |= F_COUPLER_EXIT;
           double grad = (e.l == 0) ? 0.0 : e.voltage / e.l;
           double dp = grad * e.coupler_strength;
           op.c[5] = dp * std::cos(2 * kPi * e.coupler_angle);
@@ -842,7 +910,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         prog.push_back(op);
         emit_ekicks(0.5 * e.l);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
-        emit_offset(prog, ie, e, false, true, true);
+        emit_offset(prog, ie, e, false, true, true, cdir);
         break;
       }
       default:
@@ -853,8 +921,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     // last op when that op leaves c[END_C ..] free, else an OP_ELE_END.  A
     // pure-translation offset_particle unset before it folds into the op
     // before that (F_SHIFT_OUT).
-    bool exit_ap = cfg.aperture_limit_on && (ap_set || ps_set) &&
-                   (e.aperture_at == BMX_EXIT_END || e.aperture_at == BMX_BOTH_ENDS);
+    bool exit_ap = cfg.aperture_limit_on && (ap_set || ps_set) && at_end(e.aperture_at, false);
     int end_flags = F_END | (exit_ap ? (F_END_APERTURE | ((ap_flags & F_APERTURE_ELLIPTICAL) ? F_END_ELLIPTICAL : 0)) : 0);
     size_t first = (size_t)ele_op_begin.back();
     if (prog.size() >= first + 2) {

@@ -53,7 +53,7 @@ type, bind(C) :: bmx_ele_rec
 end type
 
 type, bind(C) :: bmx_ref
-  integer(c_int32_t) :: species = bmx_electron, reserved = 0
+  integer(c_int32_t) :: species = bmx_electron, tracked_species = 0
 end type
 
 type, bind(C) :: bmx_config

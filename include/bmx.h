@@ -138,10 +138,13 @@ typedef struct bmx_ele_rec {
   double a_elec[BMX_N_POLE], b_elec[BMX_N_POLE]; /* electric An/Bn */
 } bmx_ele_rec;
 
-/* Reference particle of the branch (lat%param%particle). */
+/* Reference particle of the branch (lat%param%particle) and the species the
+ * compiled lattice tracks (bunch%species): 0 = the reference species, else
+ * the reference species or its antiparticle (rel_tracking_charge_to_mass,
+ * bmad/code/rel_tracking_charge_to_mass.f90:15-35, is then -1). */
 typedef struct bmx_ref {
-  int32_t species;  /* BMX_ELECTRON, ... */
-  int32_t reserved;
+  int32_t species;          /* BMX_ELECTRON, ... */
+  int32_t tracked_species;  /* 0 or a BMX_* species of the same mass */
 } bmx_ref;
 
 /* Global switches (subset of bmad_com, bmad/modules/bmad_struct.f90:2296-2343). */

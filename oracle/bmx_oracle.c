@@ -85,6 +85,12 @@ ORACLE_API void oracle_set_com(int aperture_limit_on, int absolute_time_tracking
   o_com.max_aperture_limit = max_aperture_limit;
 }
 
+/* Species of the particles the multi-particle entry points track (bunch%species);
+ * 0 = the reference species passed to them. */
+static int o_tracked_species = 0;
+ORACLE_API void oracle_set_tracked_species(int species) { o_tracked_species = species; }
+#define O_TRACKED(ref) (o_tracked_species ? o_tracked_species : (ref))
+
 /* ------------------------------------------------------------------------ */
 /* species: sim_utils/interfaces/particle_species_mod.f90:1505-1680         */
 /* ------------------------------------------------------------------------ */
@@ -2193,7 +2199,7 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
         if (state[i] != BMX_ALIVE) continue;
         a.vec[0] = PDEP(x[i]); a.vec[1] = PDEP(px[i]); a.vec[2] = PDEP(y[i]); a.vec[3] = PDEP(py[i]);
         a.vec[4] = PDEP(z[i]); a.vec[5] = PDEP(pz[i]);
-        a.species = ref_species;
+        a.species = O_TRACKED(ref_species);
         a.direction = 1;
         a.time_dir = 1;
         a.state = BMX_ALIVE;
@@ -2236,7 +2242,7 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
     if (state[i] != BMX_ALIVE) continue;
     a.vec[0] = PDEP(x[i]); a.vec[1] = PDEP(px[i]); a.vec[2] = PDEP(y[i]); a.vec[3] = PDEP(py[i]);
     a.vec[4] = PDEP(z[i]); a.vec[5] = PDEP(pz[i]);
-    a.species = ref_species;
+    a.species = O_TRACKED(ref_species);
     a.direction = 1;
     a.time_dir = 1;
     a.state = BMX_ALIVE;
@@ -2277,7 +2283,7 @@ ORACLE_API int oracle_jacobian(const bmx_ele_rec *eles, int n_ele, int ref_speci
     a.vec[i] = real(vec[i]);
     a.vec[i].d[i] = 1.0;
   }
-  a.species = ref_species;
+  a.species = O_TRACKED(ref_species);
   a.direction = 1;
   a.time_dir = 1;
   a.state = BMX_ALIVE;

@@ -30,8 +30,8 @@ def _coords(n, seed, scale):
     return c
 
 
-def _gpu_track(ctx, eles, species, coords, n_turns):
-    lat = ctx.lattice(eles, species)
+def _gpu_track(ctx, eles, species, coords, n_turns, tracked_species=None):
+    lat = ctx.lattice(eles, species, tracked_species=tracked_species)
     b = tracker.Bunch(coords, lat.p0c, abi.MASS[species], e_tot=lat.e_tot)
     tracker.track_bunch(lat, b, n_turns=n_turns)
     out = b.numpy()
@@ -147,6 +147,27 @@ def test_element_parity(ctx, name, e_tot):
     coords = _coords(4096, 11, SCALE)
     g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1)
     r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords)
+    _compare(g, r, _path(eles, 1))
+
+
+@pytest.mark.parametrize("name", sorted(ELEMENT_CASES))
+@pytest.mark.parametrize("variant", ["reversed", "reversed_anti", "anti"])
+def test_element_parity_reversed_and_antiparticle(ctx, name, variant):
+    """Every element case with orientation -1 (sbends keep +1: the device
+    path refuses reversed bends), tracked as the reference species or as its
+    antiparticle (a positron through an electron lattice), against the
+    oracle's offset_particle / fringe_here / charge_dir logic for those."""
+    eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=1e9)
+    if variant.startswith("reversed"):
+        if all(e.key == abi.KEY_SBEND for e in eles):
+            pytest.skip("sbend orientation -1 is refused by the device path")
+        for e in eles:
+            if e.key != abi.KEY_SBEND:
+                e.orientation = -1
+    tracked = abi.POSITRON if variant.endswith("anti") else None
+    coords = _coords(4096, 13, SCALE)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=tracked)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, tracked_species=tracked or 0)
     _compare(g, r, _path(eles, 1))
 
 

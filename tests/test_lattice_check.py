@@ -14,9 +14,9 @@ from helpers import SPECIES, ele_from_spec
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def _check(eles, species=abi.ELECTRON):
+def _check(eles, species=abi.ELECTRON, tracked_species=None):
     eles = lattice.set_reference([lattice.EleRec_copy(e) for e in eles], species, e_tot=1e9)
-    return tracker.check_lattice(eles, species)
+    return tracker.check_lattice(eles, species, tracked_species=tracked_species)
 
 
 def _n_ops(e):
@@ -69,7 +69,8 @@ def test_misaligned_bend_adds_frame_changes(kw, extra):
 @pytest.mark.parametrize("e,needle", [
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, b_elec={0: 1.0, 6: 1.0}), "6 in a bend"),
     (lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, b_elec={0: 1.0, 7: 1.0}), "more than 7 orders"),
-    (lattice.ele("drift", l=1.0, orientation=-1), "orientation -1"),
+    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, orientation=-1), "sbend with orientation -1"),
+    (lattice.ele("drift", l=1.0, orientation=2), "orientation must be +1 or -1"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_FULL), "fringe_type full"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_SAD_FULL), "fringe_type full"),
 ])
@@ -135,3 +136,57 @@ def test_phase_space_limits_only_on_collimators():
         assert _n_ops(lattice.ele(key, l=0.5, **ps)) == _n_ops(lattice.ele(key, l=0.5)) + 1
         both = _n_ops(lattice.ele(key, l=0.5, aperture_at=abi.BOTH_ENDS, **ps))
         assert both == _n_ops(lattice.ele(key, l=0.5)) + 2
+
+
+def test_tracked_species_must_be_the_antiparticle():
+    """A lattice compiles for the reference species or its antiparticle; a
+    species of another mass is refused before any element."""
+    eles = [lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1)]
+    for sp, tr in ((abi.ELECTRON, abi.POSITRON), (abi.POSITRON, abi.ELECTRON), (abi.PROTON, abi.ANTIPROTON),
+                   (abi.ELECTRON, abi.ELECTRON)):
+        ok, n, msg = _check(eles, sp, tr)
+        assert ok and n == _check(eles, sp)[1], msg
+    ok, n, msg = _check(eles, abi.ELECTRON, abi.PROTON)
+    assert not ok and n == 0 and "antiparticle" in msg, msg
+
+
+@pytest.mark.parametrize("key,kw", [("quadrupole", dict(k1=0.8)), ("sextupole", dict(k2=5.0)),
+                                    ("rfcavity", dict(voltage=1e6, rf_frequency=5e8)),
+                                    ("multipole", dict(knl={2: 1.0}, l=0.0))])
+def test_reversed_elements_compile_to_as_many_ops(key, kw):
+    """Orientation -1 changes constants and which physical end an edge is,
+    not the op sequence: a misaligned reversed element's frame changes are
+    one affine op each, like the forward element's offsets."""
+    kw = dict(dict(l=0.4, num_steps=2), **kw)
+    mis = dict(x_offset=1e-3, y_pitch=2e-4, z_offset=1e-3, tilt=0.1)
+    for extra in ({}, mis, dict(mis, fringe_type=abi.FRINGE_FULL, fringe_at=abi.BOTH_ENDS)):
+        if key in ("rfcavity", "multipole") and "fringe_type" in extra:
+            continue
+        assert _n_ops(lattice.ele(key, orientation=-1, **kw, **extra)) == _n_ops(lattice.ele(key, **kw, **extra))
+
+
+def test_reversed_element_edges_follow_the_physical_end():
+    """fringe_at = entrance_end on a reversed quadrupole: the hard edge is
+    still one op (met at the second track edge); an aperture at the entrance
+    end of a reversed drift is checked at its second track edge, where the
+    element end op carries it."""
+    q = dict(l=0.4, k1=0.8, num_steps=2, fringe_type=abi.FRINGE_FULL, fringe_at=abi.ENTRANCE_END)
+    assert _n_ops(lattice.ele("quadrupole", orientation=-1, **q)) == _n_ops(lattice.ele("quadrupole", **q))
+    ap = dict(l=1.0, x1_limit=0.01, x2_limit=0.01, y1_limit=0.01, y2_limit=0.01, aperture_at=abi.ENTRANCE_END)
+    fwd, rev = _n_ops(lattice.ele("drift", **ap)), _n_ops(lattice.ele("drift", orientation=-1, **ap))
+    assert fwd == rev + 1  # forward: an entrance aperture op; reversed: folded into the drift's end
+
+
+def test_golden_elements_accepted_reversed_for_antiparticles():
+    """The -Anti_O variant of tracking_method_test: every element except sbend
+    and rfcavity reversed and tracked as the antiparticle."""
+    tm = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
+    sp = SPECIES[tm["particle"]]
+    anti = abi.POSITRON + abi.ELECTRON - sp
+    for name, spec in tm["elements"].items():
+        e = ele_from_spec(spec, sp, e_tot=tm["e_tot"])
+        flip = spec["key"] not in ("sbend", "rfcavity")
+        if flip:
+            e.orientation = -1
+        ok, _, msg = tracker.check_lattice([e], sp, tracked_species=anti if flip else None)
+        assert ok, (name, msg)
