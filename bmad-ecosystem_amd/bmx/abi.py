@@ -66,7 +66,7 @@ class Ref(C.Structure):
 
 
 class Config(C.Structure):
-    _fields_ = [("aperture_limit_on", C.c_int32), ("reserved", C.c_int32), ("max_aperture_limit", C.c_double)]
+    _fields_ = [("aperture_limit_on", C.c_int32), ("direction", C.c_int32), ("max_aperture_limit", C.c_double)]
 
 
 class Particles(C.Structure):

@@ -68,13 +68,14 @@ class Context:
         return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit, tracked_species)
 
 
-def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3, tracked_species=None):
-    """(ok, n_ops, reason): whether the device path accepts this element list,
-    decided on the host with no device (bmx_lattice_check)."""
+def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3, tracked_species=None, direction=1):
+    """(ok, n_ops, reason): whether the device path accepts this element list
+    for tracking in `direction`, decided on the host with no device
+    (bmx_lattice_check)."""
     lib = abi.load_library()
     arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
     ref = abi.Ref(species, tracked_species or 0)
-    cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit)
+    cfg = abi.Config(1 if aperture_limit_on else 0, int(direction), max_aperture_limit)
     n_ops = C.c_int64(0)
     msg = C.create_string_buffer(512)
     rc = lib.bmx_lattice_check(arr, len(arr), C.byref(ref), C.byref(cfg), C.byref(n_ops), msg, len(msg))
@@ -174,7 +175,9 @@ class Bunch:
 
 
 def track_bunch(lat, bunch, n_turns=1, ix_start=0, ix_end=0, direction=1, sync=True):
-    """n_turns passes of track_bunch(lat, bunch, ele_start = ix_start, ele_end = ix_end).
+    """n_turns passes of track_bunch(lat, bunch, ele_start = ix_start, ele_end = ix_end,
+    direction).  direction = -1 tracks backward, elements ix_start down to
+    ix_end + 1 (the lattice's backward program, compiled on first use).
     Returns bmx_stats as a dict."""
     if sync:
         torch.cuda.synchronize(bunch.device)

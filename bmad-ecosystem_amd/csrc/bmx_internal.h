@@ -37,6 +37,8 @@ enum OpType : int32_t {
   OP_EKICK = 20,         // thin electric ab multipole kick (1/beta) with its pz, beta, z update
   OP_EFRINGE = 21,       // electric longitudinal fringe: energy kick of the multipole potential
   OP_BEND_EKICK = 22,    // electric multipole kick inside a bend, scaled by (1 + g x) / (ps beta)
+  OP_DRIFT_BACK = 23,    // exact drift tracked backward (direction = -1), c[0] = L, c[1] = beta_ref
+  OP_ZBETA = 24,         // z -= c[0] beta (the quadrupole's backward 2 c beta delta_ref_time per step)
 };
 
 // op flags

@@ -6,7 +6,8 @@
 // Hwang edge map (bmad/modules/fringe_mod.f90:240-418) and the RF set-up of
 // track_a_rfcavity (bmad/low_level/track_a_rfcavity.f90:17-80).
 //
-// Scope of the device path: forward tracking (direction = +1, time_dir = +1),
+// Scope of the device path: forward and backward tracking (direction = +1 or
+// -1, time_dir = +1; backward without rf cavities or misaligned bends),
 // element orientation +1 and -1 (sbend +1), the reference species or its
 // antiparticle, relative-time RF.  Anything
 // else is rejected with an error message naming the element; the CPU side
@@ -255,10 +256,11 @@ bool is_misaligned(const bmx_ele_rec& e) {
 // (W^-1 when setting); the element tilt follows (set) or precedes (unset) as OP_TILT.
 // half_hv: offset_particle's half hkick / vkick (set_hvkicks, :300-304 and
 // :393-397), a dipole kick between the frame change and the tilt, scaled by
-// charge_dir = orientation x rel_tracking_charge_to_mass.  Orientation -1
-// takes the affine form of emit_reversed_offset.
+// charge_dir = orientation x direction x rel_tracking_charge_to_mass.  rev:
+// sign_z_vel = orientation x direction is -1, the affine form of
+// emit_reversed_offset.
 void emit_reversed_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set);
-void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set, bool with_tilt,
+void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool rev, bool set, bool with_tilt,
                  bool half_hv = false, double charge_dir = 1.0) {
   bool mis = is_misaligned(e);
   bool tilt = with_tilt && e.tilt != 0;
@@ -273,7 +275,7 @@ void emit_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool se
   bool kick = half_hv && (e.hkick != 0 || e.vkick != 0);
   if (!set && tilt) prog.push_back(t);
   if (!set && kick) prog.push_back(hv);
-  if (mis && e.orientation == -1) {
+  if (mis && rev) {
     emit_reversed_offset(prog, ie, e, set);
   } else if (mis) {
     DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
@@ -371,8 +373,9 @@ void pitch_w(double theta, double phi, bool inverse, double w[3][3]) {
     for (int j = 0; j < 3; j++) w[i][j] = inverse ? m[j][i] : m[i][j];
 }
 
-// offset_particle of a straight element with orientation -1 and direction +1
-// (offset_particle.f90 with sign_z_vel = -1): the particle enters the body
+// offset_particle of a straight element with sign_z_vel = orientation x
+// direction = -1 (offset_particle.f90; both orientation -1 forward and
+// orientation +1 backward give the same frame change): the particle enters the body
 // frame at s = L with the z component of its momentum pointing backwards, and
 // the drift to the edge runs against s when setting.  Set: r = W^-1 (x - x_off,
 // y - y_off, L/2 - z_off) + (0, 0, L/2), drift against s over L - r(3).
@@ -407,15 +410,19 @@ void emit_reversed_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e
   prog.push_back(op);
 }
 
-// offset_particle of a misaligned bend (g != 0 or ref_tilt != 0), direction
-// and orientation +1, drift to the edge: one OP_OFFSET_* with F_OFFSET_AFFINE,
-// c[0..2] = t, c[3] = L/2, c[4..12] = W row-major.
-void emit_bend_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set) {
-  double g = e.g, L_half = 0.5 * e.l, rt = e.ref_tilt;
+// offset_particle of a misaligned bend (g != 0 or ref_tilt != 0), orientation
+// +1, drift to the edge: one OP_OFFSET_* with F_OFFSET_AFFINE, c[0..2] = t,
+// c[3] = L/2, c[4..12] = W row-major.  rev: direction -1 (sign_z_vel = -1):
+// the arc shifts to the centre and to the edge change sign, the momentum's z
+// column is negated and row 2 is rewritten so that the op's noref drift over
+// (0 or L) - row 2 is the reference's drift against s (set: over the chain's
+// r(3); unset: over r(3) as well, the edge being s = 0).
+void emit_bend_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bool set, bool rev) {
+  double g = e.g, L_half = 0.5 * e.l, rt = e.ref_tilt, sz = rev ? -1.0 : 1.0;
   double ws[3][3], wt[3][3];
   Affine p;
   if (set) {
-    bend_shift(p, g, L_half, true, rt);  // ds_center = L/2 (s_pos0 = 0)
+    bend_shift(p, g, sz * L_half, true, rt);  // orientation x ds_center, ds_center = L/2 - s_pos0
     p.t[0] -= e.x_offset;
     p.t[1] -= e.y_offset;
     p.t[2] -= e.z_offset;
@@ -428,9 +435,9 @@ void emit_bend_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bo
       p.rotate(wt);
       bend_shift(p, g, L_half, false, 0.0);
     }
-    bend_shift(p, g, -L_half, false, 0.0);  // drift_to upstream: r(3) += 0
+    bend_shift(p, g, -sz * L_half, false, 0.0);  // drift_to upstream: r(3) += (1 - sign_z_vel) L/2, folded below
   } else {
-    bend_shift(p, g, -L_half, false, 0.0);  // ds_center = -L/2 (s_pos0 = L)
+    bend_shift(p, g, -sz * L_half, false, 0.0);  // ds_center = L/2 - s_pos0
     if (rt != 0) {
       bend_shift(p, g, -L_half, false, 0.0);
       mat3_unit(wt);
@@ -443,8 +450,8 @@ void emit_bend_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bo
     p.t[0] += e.x_offset;
     p.t[1] += e.y_offset;
     p.t[2] += e.z_offset;
-    bend_shift(p, g, L_half, true, rt);  // drift_to downstream
-    p.t[2] += e.l;
+    bend_shift(p, g, sz * L_half, true, rt);  // drift_to downstream
+    if (!rev) p.t[2] += e.l;                  // r(3) += (1 + sign_z_vel) L/2
   }
   DevOp op = make_op(set ? OP_OFFSET_SET : OP_OFFSET_UNSET, ie);
   op.flags |= F_OFFSET_MOVE | F_OFFSET_PITCH | F_OFFSET_AFFINE;
@@ -452,6 +459,12 @@ void emit_bend_offset(std::vector<DevOp>& prog, int ie, const bmx_ele_rec& e, bo
   op.c[3] = L_half;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) op.c[4 + 3 * i + j] = p.w[i][j];
+  if (rev) {
+    op.c[4 + 2] = -op.c[4 + 2];  // p_z = -sqrt(...)
+    op.c[4 + 5] = -op.c[4 + 5];
+    for (int j = 0; j < 3; j++) op.c[4 + 6 + j] = -op.c[4 + 6 + j];
+    op.c[2] = (set ? 0.0 : e.l) - p.t[2];
+  }
   prog.push_back(op);
 }
 
@@ -465,7 +478,8 @@ int used_consts(const DevOp& op) {
   int nh = op.flags & 0xff;
   switch (op.type) {
     case OP_ELE_END: return 0;
-    case OP_DRIFT: return 1;
+    case OP_DRIFT: case OP_ZBETA: return 1;
+    case OP_DRIFT_BACK: return 2;
     case OP_QUAD: return 3;
     case OP_TILT: case OP_HARD_MULT_EDGE: return 2;
     case OP_KICK: case OP_EKICK: case OP_EFRINGE: return 2 * (nh + 1);
@@ -503,6 +517,15 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     return false;
   }
   const double rel_charge = species_charge(tracked) / species_charge(ref.species);
+  // tracking direction (bmx_config.direction): backward programs run the
+  // elements last to first (track_bunch, beam_mod.f90:131-147), each element
+  // with direction = -1 in its maps
+  if (cfg.direction != 0 && cfg.direction != 1 && cfg.direction != -1) {
+    *err = "direction must be +1 or -1";
+    return false;
+  }
+  const int dir = (cfg.direction == -1) ? -1 : 1;
+  const double beta_ref_drift = p0c / std::sqrt(p0c * p0c + mc2 * mc2);  // track_a_drift's backward beta_ref
   lc->mc2sq = mc2 * mc2;
   lc->p0c_sq = p0c * p0c;
   {
@@ -514,7 +537,8 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     lc->lez_me2 = me * me;
   }
 
-  for (int ie = 0; ie < n_ele; ie++) {
+  for (int k = 0; k < n_ele; k++) {
+    const int ie = (dir == 1) ? k : n_ele - 1 - k;
     const bmx_ele_rec& e = eles[ie];
     auto fail = [&](const char* what) {
       std::snprintf(buf, sizeof(buf), "element %d (key %d): %s", ie, e.key, what);
@@ -527,12 +551,20 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     if (e.orientation != 1 && e.orientation != -1) return fail("orientation must be +1 or -1");
     if (e.orientation == -1 && e.key == BMX_KEY_SBEND)
       return fail("sbend with orientation -1 is not supported on the device path");
-    // charge_dir of the magnetic fields: orientation x rel_tracking_charge_to_mass
-    // (direction +1); a reversed element meets its exit end at the first track
-    // edge (physical_ele_end.f90), which fringe_here and the aperture and
-    // coupler switches test
-    const bool rev = e.orientation == -1;
-    const double cdir = e.orientation * rel_charge;
+    // charge_dir of the magnetic fields: orientation x direction x
+    // rel_tracking_charge_to_mass; with orientation x direction = -1 the
+    // particle meets the element's exit end at its first track edge
+    // (physical_ele_end.f90), which fringe_here and the aperture and coupler
+    // switches test
+    const bool rev = e.orientation * dir == -1;
+    const double cdir = e.orientation * dir * rel_charge;
+    // a drift of the body: track_a_drift with include_ref_motion, forward or backward
+    auto emit_drift = [&](double len) {
+      DevOp op = make_op(dir == 1 ? OP_DRIFT : OP_DRIFT_BACK, ie);
+      op.c[0] = len;
+      if (dir == -1) op.c[1] = beta_ref_drift;
+      prog.push_back(op);
+    };
     auto at_end = [&](int where, bool first_edge) {
       int phys = (first_edge != rev) ? BMX_ENTRANCE_END : BMX_EXIT_END;
       return where == phys || where == BMX_BOTH_ENDS;
@@ -650,7 +682,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       case BMX_KEY_MARKER:
         break;
       case BMX_KEY_DRIFT: {
-        if (e.l != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = e.l; prog.push_back(op); }
+        if (e.l != 0) emit_drift(e.l);
         break;
       }
       case BMX_KEY_MULTIPOLE: case BMX_KEY_AB_MULTIPOLE: {
@@ -675,7 +707,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           }
         }
         bool mis = is_misaligned(e);
-        if (mis) emit_offset(prog, ie, e, true, false);
+        if (mis) emit_offset(prog, ie, e, rev, true, false);
         if (key == BMX_KEY_MULTIPOLE && e.has_mag_poles && e.a[0] != 0) {
           DevOp op = make_op(OP_KICK_REF0, ie);
           double t = e.b[0] + e.tilt;
@@ -687,7 +719,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         }
         scale_ab(ab, cdir);
         emit_kicks(prog, ie, ab, 1.0, OP_KICK);
-        if (mis) emit_offset(prog, ie, e, false, false);
+        if (mis) emit_offset(prog, ie, e, rev, false, false);
         break;
       }
       case BMX_KEY_QUADRUPOLE: {
@@ -702,7 +734,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         if (e.l == 0) n_step = 0;
         double step_len = e.l / (n_step > 0 ? n_step : 1);
         double r_step = 1.0 / (n_step > 0 ? n_step : 1);
-        emit_offset(prog, ie, e, true, true);
+        emit_offset(prog, ie, e, rev, true, true);
         hard_edge(1, e.k1, true);
         emit_efringe(true);
         emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
@@ -713,12 +745,17 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           op.c[1] = b1 / e.l;  // k1 * rel_p
           op.c[2] = op.c[1] * step_len * step_len;
           prog.push_back(op);
+          if (dir == -1) {  // track_a_quadrupole.f90: z -= 2 c beta delta_ref_time, every step
+            DevOp zb = make_op(OP_ZBETA, ie);
+            zb.c[0] = 2.0 * kCLight * e.delta_ref_time;
+            prog.push_back(zb);
+          }
           emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
           emit_ekicks((i == n_step) ? step_len / 2 : step_len);
         }
         emit_efringe(false);
         hard_edge(1, e.k1, false);
-        emit_offset(prog, ie, e, false, true);
+        emit_offset(prog, ie, e, rev, false, true);
         break;
       }
       case BMX_KEY_SEXTUPOLE: case BMX_KEY_OCTUPOLE: case BMX_KEY_THICK_MULTIPOLE: case BMX_KEY_HKICKER:
@@ -730,12 +767,12 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         scale_ab(ab, cdir);
         int n_step = (ab.ix_max == -1 && ae.ix_max == -1) ? 1 : std::max(nint(std::fabs(e.l / (e.l / e.num_steps))), 1);
         double step_len = e.l / n_step, r_step = 1.0 / n_step;
-        emit_offset(prog, ie, e, true, true);
+        emit_offset(prog, ie, e, rev, true, true);
         if (key == BMX_KEY_SEXTUPOLE) hard_edge(2, e.k2, true);
         emit_efringe(true);
         int lo = 0;
         while (lo <= ab.ix_max && ab.a[lo] == 0 && ab.b[lo] == 0) lo++;
-        if (ab.ix_max >= 0 && step_len != 0 && ab.ix_max - lo < 6 && ae.ix_max < 0) {
+        if (ab.ix_max >= 0 && step_len != 0 && ab.ix_max - lo < 6 && ae.ix_max < 0 && dir == 1) {
           // one op for the body; the half kicks are the full-step kick x 0.5,
           // which is exact (the reference scales by r_step / 2)
           DevOp op = make_op(OP_KD_SLICES, ie);
@@ -751,14 +788,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
           emit_kicks(prog, ie, ab, r_step / 2, OP_KICK);
           emit_ekicks(step_len / 2);
           for (int i = 1; i <= n_step; i++) {
-            if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
+            if (step_len != 0) emit_drift(step_len);
             emit_kicks(prog, ie, ab, (i == n_step) ? r_step / 2 : r_step, OP_KICK);
             emit_ekicks((i == n_step) ? step_len / 2 : step_len);
           }
         }
         emit_efringe(false);
         if (key == BMX_KEY_SEXTUPOLE) hard_edge(2, e.k2, false);
-        emit_offset(prog, ie, e, false, true);
+        emit_offset(prog, ie, e, rev, false, true);
         break;
       }
       case BMX_KEY_SBEND: {
@@ -774,26 +811,24 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
             t.c[1] = std::sin(set ? e.ref_tilt : -e.ref_tilt);
             prog.push_back(t);
           } else if (e.g == 0 && e.ref_tilt == 0) {
-            emit_offset(prog, ie, e, set, false);  // straight-element frame change
+            emit_offset(prog, ie, e, rev, set, false);  // straight-element frame change
           } else {
-            emit_bend_offset(prog, ie, e, set);
+            emit_bend_offset(prog, ie, e, set, rev);
           }
         };
         bend_offset(true);
         if (e.fringe_type == BMX_FRINGE_FULL || e.fringe_type == BMX_FRINGE_SAD_FULL || e.fringe_type == BMX_FRINGE_SOFT_EDGE_ONLY)
           return fail("sbend fringe_type full / sad_full / soft_edge_only is not supported on the device path");
         auto emit_edge = [&](bool enter) {
-          bool here = e.fringe_type != BMX_FRINGE_NONE &&
-                      (enter ? (e.fringe_at == BMX_ENTRANCE_END || e.fringe_at == BMX_BOTH_ENDS)
-                             : (e.fringe_at == BMX_EXIT_END || e.fringe_at == BMX_BOTH_ENDS));
-          if (!here) return;
+          if (e.fringe_type == BMX_FRINGE_NONE || !at_end(e.fringe_at, enter)) return;
+          bool phys_in = enter != rev;  // e1, fint, hgap at the physical entrance end
           double g_tot = e.is_on ? (e.g + e.dg) * cdir : 0.0, k1 = e.is_on ? e.k1 * cdir : 0.0;
-          double ang = enter ? e.e1 : e.e2;
-          double fint_gap = enter ? e.fint * e.hgap : e.fintx * e.hgapx;
+          double ang = phys_in ? e.e1 : e.e2;
+          double fint_gap = phys_in ? e.fint * e.hgap : e.fintx * e.hgapx;
           if (e.fringe_type == BMX_FRINGE_HARD_EDGE_ONLY) fint_gap = 0;
           double ce = std::cos(ang), se = std::sin(ang), te = se / ce, sec = 1 / ce;
           if (e.fringe_type == BMX_FRINGE_LINEAR_EDGE) {
-            double fint = enter ? e.fint : e.fintx, hgap = enter ? e.hgap : e.hgapx;
+            double fint = phys_in ? e.fint : e.fintx, hgap = phys_in ? e.hgap : e.hgapx;
             DevOp op = make_op(OP_LINEAR_EDGE, ie);
             op.c[0] = g_tot * te;
             op.c[1] = (fint == 0) ? -op.c[0] : -g_tot * std::tan(ang - 2 * fint * g_tot * hgap * (1 + se * se) / ce);
@@ -849,9 +884,10 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
             DevOp op = make_op(OP_BEND_K1, ie);
             double k1 = b1 / e.l;
             op.c[0] = step_len; op.c[1] = g; op.c[2] = g_tot; op.c[3] = k1;
+            op.c[4] = dir * e.orientation;  // the sign of z0 (direction x orientation)
             prog.push_back(op);
           } else if ((g == 0 && dg == 0) || step_len == 0) {
-            if (step_len != 0) { DevOp op = make_op(OP_DRIFT, ie); op.c[0] = step_len; prog.push_back(op); }
+            if (step_len != 0) emit_drift(step_len);
           } else {
             DevOp op = make_op(OP_BEND, ie);
             double sinc_a = (std::fabs(angle) < 1e-8) ? 1.0 : std::sin(angle) / angle;
@@ -881,10 +917,11 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         // vkick act twice: as half kicks there and in the ab kicks
         // (multipole_ele_to_ab with include_kicks, :60-66)
         if (e.rf_frequency == 0 && e.voltage != 0) return fail("rfcavity with zero rf_frequency");
+        if (dir == -1) return fail("rfcavity in backward tracking (absolute-time rf) is not supported on the device path");
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
         scale_ab(ab, cdir);
-        emit_offset(prog, ie, e, true, true, true, cdir);
+        emit_offset(prog, ie, e, rev, true, true, true, cdir);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
         emit_ekicks(0.5 * e.l);
         int n_slice = std::max(1, nint(std::fabs(e.l) / (e.l / (e.num_steps > 0 ? e.num_steps : 1))));
@@ -910,7 +947,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         prog.push_back(op);
         emit_ekicks(0.5 * e.l);
         emit_kicks(prog, ie, ab, 0.5, OP_KICK);
-        emit_offset(prog, ie, e, false, true, true, cdir);
+        emit_offset(prog, ie, e, rev, false, true, true, cdir);
         break;
       }
       default:

@@ -166,6 +166,19 @@ __device__ __forceinline__ void op_drift(Lane& L, double len) {
   drift_c(L, len, sg(d[0]), sg(d[1]), sg(d[2]));
 }
 
+// track_a_drift with direction = -1 (bmad/low_level/track_a_drift.f90:25-116,
+// backward branch): x and y as forward, dz = L ((-beta / beta_ref - 1) - (1 / ps - 1))
+__device__ __forceinline__ void op_drift_back(Lane& L, const DevOp& op) {
+  double len = op.c[0];
+  double pxr = L.px * L.irp, pyr = L.py * L.irp;
+  double pxy2 = pxr * pxr + pyr * pyr;
+  if (pxy2 >= 1.0) { L.state = BMX_LOST_PZ; return; }
+  double ps = sqrt(1.0 - pxy2);
+  L.x += len * pxr / ps;
+  L.y += len * pyr / ps;
+  L.z += len * ((-L.beta / op.c[1] - 1.0) - (1.0 / ps - 1.0));
+}
+
 // drift with include_ref_motion = false (offset_particle's drift to the edge)
 __device__ __forceinline__ void op_drift_noref(Lane& L, double len) {
   double pxr = L.px * L.irp, pyr = L.py * L.irp;
@@ -638,7 +651,7 @@ __device__ void op_bend_k1(Lane& L, const DevOp& op, const LatticeConsts& lc) {
   L.px = tau_x * om_x * om_x * rel_p * s_x * r0 + c_x * r1;
   L.y = c_y * r2 + s_y * r3 / rel_p;
   L.py = tau_y * om_y * om_y * rel_p * s_y * r2 + c_y * r3;
-  L.z = r4 + z0 + z1 * r0 + z2 * r1 + z11 * r0 * r0 + z12 * r0 * r1 + z22 * r1 * r1 + z33 * r2 * r2 + z34 * r2 * r3 +
+  L.z = r4 + op.c[4] * z0 + z1 * r0 + z2 * r1 + z11 * r0 * r0 + z12 * r0 * r1 + z22 * r1 * r1 + z33 * r2 * r2 + z34 * r2 * r3 +
         z44 * r3 * r3;
   L.z += low_energy_dz(L, lc, length);
 }
@@ -1093,7 +1106,8 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |
-                            TB(OP_HARD_MULT_EDGE) | TB(OP_EKICK) | TB(OP_EFRINGE) | TB(OP_BEND_EKICK);
+                            TB(OP_HARD_MULT_EDGE) | TB(OP_EKICK) | TB(OP_EFRINGE) | TB(OP_BEND_EKICK) |
+                            TB(OP_DRIFT_BACK) | TB(OP_ZBETA);
   if (tb & RARE) {
     if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
     if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
@@ -1112,6 +1126,8 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_EKICK)) op_ekick(L, op, flags & 0xff, n, lc);
     if (tb & TB(OP_EFRINGE)) op_efringe(L, op, flags & 0xff, n, lc);
     if (tb & TB(OP_BEND_EKICK)) op_bend_ekick(L, op, flags & 0xff, n, lc);
+    if (tb & TB(OP_DRIFT_BACK)) op_drift_back(L, op);
+    if (tb & TB(OP_ZBETA)) L.z -= op.c[0] * L.beta;
   }
   if (flags & F_SHIFT_OUT) {  // offset_particle unset, pure translation (op_offset without pitch)
     uniform_block();

@@ -67,6 +67,14 @@ struct bmx_lattice {
   int64_t n_cells;
   int64_t n_ops;
   LatticeConsts lc;
+  // backward tracking of variant 0 (bmx_track_bunch with direction = -1):
+  // its elements, compiled on first use into a program of their own
+  std::vector<bmx_ele_rec> eles0;
+  bmx_ref ref;
+  bmx_config cfg;
+  DevOp* d_ops_bwd = nullptr;
+  std::vector<int32_t> eb_bwd;
+  LatticeConsts lc_bwd;
 };
 
 // Launch a tracking kernel: the lattice's runtime-compiled cell kernel when
@@ -224,6 +232,9 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
       std::call_once(once, [&] { std::fprintf(stderr, "bmx: cell kernels unavailable, using the interpreter: %s\n", jerr.c_str()); });
     }
   }
+  lat->eles0.assign(eles + offs[0], eles + offs[1]);
+  lat->ref = *ref;
+  lat->cfg = c;
   *out = lat;
   return 0;
 }
@@ -263,6 +274,7 @@ void bmx_lattice_destroy(bmx_lattice* lat) {
   hipFree(lat->d_var_op_begin);
   hipFree(lat->d_var_n_ele);
   if (lat->d_cell_prv) hipFree(lat->d_cell_prv);
+  if (lat->d_ops_bwd) hipFree(lat->d_ops_bwd);
   delete lat;
 }
 
@@ -337,17 +349,50 @@ extern "C" {
 int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
                         int32_t direction, bmx_particles* p, bmx_stats* out) {
   if (!ctx || !lat || !p) return set_err(ctx, "invalid arguments");
-  if (direction != 1) return set_err(ctx, "only direction = +1 is supported on the device path");
+  if (direction != 1 && direction != -1) return set_err(ctx, "direction must be +1 or -1");
   if (n_turns < 0) return set_err(ctx, "n_turns < 0");
   if (!particles_ok(p)) return set_err(ctx, "particle arrays x..pz, beta, state are required");
   int n_ele = lat->var_n_ele[0];
   if (ix_start < 0 || ix_start > n_ele || ix_end < 0 || ix_end > n_ele) return set_err(ctx, "ix_start/ix_end out of range");
-  const std::vector<int32_t>& eb = lat->ele_op_begin[0];
-  // Bmad element numbering: elements ix_start+1 .. ix_end are array entries ix_start .. ix_end-1.
-  int lo1, hi1, lo2 = 0, hi2 = 0;
-  if (ix_end > ix_start) { lo1 = eb[ix_start]; hi1 = eb[ix_end]; }
-  else { lo1 = eb[ix_start]; hi1 = eb[n_ele]; lo2 = eb[0]; hi2 = eb[ix_end]; }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (direction == -1 && !lat->d_ops_bwd) {  // compile the backward program of variant 0 once
+    bmx_lattice* lw = const_cast<bmx_lattice*>(lat);
+    bmx_config c = lw->cfg;
+    c.direction = -1;
+    std::vector<DevOp> prog;
+    std::vector<int32_t> eb;
+    LatticeConsts lc;
+    std::memset(&lc, 0, sizeof(lc));
+    std::string err;
+    if (!compile_variant(lw->eles0.data(), n_ele, lw->ref, c, prog, eb, &lc, &err))
+      return set_err(ctx, "backward tracking: " + err);
+    lc.max_aperture_limit = c.max_aperture_limit;
+    lc.two_max_ap = 2.0 * c.max_aperture_limit;
+    lc.aperture_limit_on = c.aperture_limit_on;
+    DevOp* d = nullptr;
+    if (hipMalloc(&d, prog.size() * sizeof(DevOp)) != hipSuccess) return set_err(ctx, "hipMalloc of the backward program failed");
+    if (hipMemcpy(d, prog.data(), prog.size() * sizeof(DevOp), hipMemcpyHostToDevice) != hipSuccess) {
+      hipFree(d);
+      return set_err(ctx, "upload of the backward program failed");
+    }
+    lw->d_ops_bwd = d;
+    lw->eb_bwd = eb;
+    lw->lc_bwd = lc;
+  }
+  const bool bwd = direction == -1;
+  const std::vector<int32_t>& eb = bwd ? lat->eb_bwd : lat->ele_op_begin[0];
+  int lo1, hi1, lo2 = 0, hi2 = 0;
+  if (!bwd) {
+    // Bmad element numbering: elements ix_start+1 .. ix_end are array entries ix_start .. ix_end-1.
+    if (ix_end > ix_start) { lo1 = eb[ix_start]; hi1 = eb[ix_end]; }
+    else { lo1 = eb[ix_start]; hi1 = eb[n_ele]; lo2 = eb[0]; hi2 = eb[ix_end]; }
+  } else {
+    // backward (beam_mod.f90:131-147): elements ix_start down to ix_end+1,
+    // wrapping through 1 and n_ele when ix_start <= ix_end; the backward
+    // program holds array entry j at position n_ele - 1 - j
+    if (ix_start > ix_end) { lo1 = eb[n_ele - ix_start]; hi1 = eb[n_ele - ix_end]; }
+    else { lo1 = eb[n_ele - ix_start]; hi1 = eb[n_ele]; lo2 = eb[0]; hi2 = eb[n_ele - ix_end]; }
+  }
   int64_t n = p->n;
   if (out) std::memset(out, 0, sizeof(*out));
   if (n <= 0 || n_turns == 0) return 0;
@@ -356,8 +401,12 @@ int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, 
   const int block = 256;
   int64_t grid = (n + block * BMX_NP - 1) / (block * BMX_NP);
   (void)hipEventRecord(ctx->ev0, ctx->stream);
-  launch_track(lat->jit ? lat->jit->bunch : nullptr, bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block),
-               ctx->stream, lat->d_ops, lo1, hi1, lo2, hi2, lat->lc, n_turns, st.s, n, st.d_lost);
+  if (bwd)  // the backward program runs on the op interpreter
+    launch_track(nullptr, bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block), ctx->stream, lat->d_ops_bwd, lo1,
+                 hi1, lo2, hi2, lat->lc_bwd, n_turns, st.s, n, st.d_lost);
+  else
+    launch_track(lat->jit ? lat->jit->bunch : nullptr, bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block),
+                 ctx->stream, lat->d_ops, lo1, hi1, lo2, hi2, lat->lc, n_turns, st.s, n, st.d_lost);
   hipError_t le = hipGetLastError();
   (void)hipEventRecord(ctx->ev1, ctx->stream);
   return finish_particles(ctx, p, st, le, out);

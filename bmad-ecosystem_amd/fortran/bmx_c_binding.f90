@@ -57,7 +57,7 @@ type, bind(C) :: bmx_ref
 end type
 
 type, bind(C) :: bmx_config
-  integer(c_int32_t) :: aperture_limit_on = 1, reserved = 0
+  integer(c_int32_t) :: aperture_limit_on = 1, direction = 0
   real(c_double) :: max_aperture_limit = 1d3
 end type
 

@@ -150,7 +150,7 @@ typedef struct bmx_ref {
 /* Global switches (subset of bmad_com, bmad/modules/bmad_struct.f90:2296-2343). */
 typedef struct bmx_config {
   int32_t aperture_limit_on;     /* default 1 */
-  int32_t reserved;
+  int32_t direction;             /* bmx_lattice_check: 0 or +1 forward, -1 backward tracking */
   double max_aperture_limit;     /* default 1e3 m */
 } bmx_config;
 
@@ -250,8 +250,12 @@ int32_t bmx_lattice_cells(const bmx_lattice *lat, int32_t *cell_len, int64_t *n_
 
 /* Track a bunch through elements ix_start+1 .. ix_end of variant 0 with
  * track_bunch wrap-around semantics (ix_end <= ix_start wraps through the
- * end of the lattice), repeated n_turns times.  direction must be +1.
- * Equivalent to n_turns calls of track_bunch(lat, bunch, ele_start, ele_end). */
+ * end of the lattice), repeated n_turns times.  direction = -1 tracks
+ * backward through elements ix_start down to ix_end+1 (wrapping when
+ * ix_start <= ix_end), beam_mod.f90:131-147, with the lattice's backward
+ * program, compiled on the first such call (rf cavities and misaligned bends
+ * are refused there).  Equivalent to n_turns calls of track_bunch(lat, bunch,
+ * ele_start, ele_end, direction) (bmad/multiparticle/beam_mod.f90:103-170). */
 int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
                         int32_t n_turns, int32_t direction, bmx_particles *p, bmx_stats *out);
 

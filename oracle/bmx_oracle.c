@@ -90,6 +90,10 @@ ORACLE_API void oracle_set_com(int aperture_limit_on, int absolute_time_tracking
 static int o_tracked_species = 0;
 ORACLE_API void oracle_set_tracked_species(int species) { o_tracked_species = species; }
 #define O_TRACKED(ref) (o_tracked_species ? o_tracked_species : (ref))
+/* Direction of oracle_track_bunch: -1 tracks every element last to first with
+ * direction = -1 (track_bunch, bmad/multiparticle/beam_mod.f90:131-147). */
+static int o_direction = 1;
+ORACLE_API void oracle_set_direction(int direction) { o_direction = (direction == -1) ? -1 : 1; }
 
 /* ------------------------------------------------------------------------ */
 /* species: sim_utils/interfaces/particle_species_mod.f90:1505-1680         */
@@ -2189,8 +2193,10 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
   if (any_unmodelled(eles, n_ele)) return -2;
   c_multi_init(); /* lazily built table: build it before the parallel region */
   for (turn = 0; turn < n_turns; turn++) {
-    for (ie = 0; ie < n_ele; ie++) {
-      const bmx_ele_rec *ele = &eles[ie];
+    for (int k = 0; k < n_ele; k++) {
+      const bmx_ele_rec *ele;
+      ie = (o_direction == 1) ? k : n_ele - 1 - k;
+      ele = &eles[ie];
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(n_threads > 0 ? n_threads : 1)
 #endif
@@ -2200,7 +2206,7 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
         a.vec[0] = PDEP(x[i]); a.vec[1] = PDEP(px[i]); a.vec[2] = PDEP(y[i]); a.vec[3] = PDEP(py[i]);
         a.vec[4] = PDEP(z[i]); a.vec[5] = PDEP(pz[i]);
         a.species = O_TRACKED(ref_species);
-        a.direction = 1;
+        a.direction = o_direction;
         a.time_dir = 1;
         a.state = BMX_ALIVE;
         a.p0c = ele->p0c_start;

@@ -36,6 +36,8 @@ def _setup(lib):
     lib.oracle_set_com.argtypes = [C.c_int, C.c_int, C.c_double]
     lib.oracle_set_tracked_species.restype = None
     lib.oracle_set_tracked_species.argtypes = [C.c_int]
+    lib.oracle_set_direction.restype = None
+    lib.oracle_set_direction.argtypes = [C.c_int]
     lib.oracle_hwang_edge.restype = None
     lib.oracle_hwang_edge.argtypes = [C.POINTER(abi.EleRec), C.c_int, C.c_int, _DP, C.c_int, C.c_int]
     lib.oracle_check_aperture.restype = C.c_int
@@ -118,11 +120,12 @@ def beta_of(pz, p0c, mc2):
 
 
 def track_bunch(eles, species, n_turns, coords, beta=None, n_threads=1, particle_outer=False, counting=False,
-                fma=False, tracked_species=0):
+                fma=False, tracked_species=0, direction=1):
     """Track SoA coords (dict or (6, n) array) through the element array for n_turns.
     Returns dict with x..pz, beta, state, ix_lost, turn_lost (copies).
     fma=True uses the FMA-contracted build (a second FP evaluation order).
-    tracked_species: the particles' species when not the reference species."""
+    tracked_species: the particles' species when not the reference species.
+    direction = -1: every turn runs the elements last to first, backward."""
     L = lib_count() if counting else (lib_fma() if fma else lib())
     arr = eles if isinstance(eles, C.Array) else abi_array(eles)
     c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)
@@ -137,12 +140,16 @@ def track_bunch(eles, species, n_turns, coords, beta=None, n_threads=1, particle
     turn_lost = np.full(n, -1, dtype=np.int32)
     fn = L.oracle_track_particle_outer if particle_outer else L.oracle_track_bunch
     ptrs = [x.ctypes.data_as(_DP) for x in cols + [beta]]
+    if direction != 1 and particle_outer:
+        raise ValueError("the particle-outer loop tracks forward only")
     L.oracle_set_tracked_species(tracked_species or 0)
+    L.oracle_set_direction(direction)
     try:
         rc = fn(arr, len(arr), species, n_turns, n, *ptrs, state.ctypes.data_as(_IP), ix_lost.ctypes.data_as(_IP),
                 turn_lost.ctypes.data_as(_IP), n_threads)
     finally:
         L.oracle_set_tracked_species(0)
+        L.oracle_set_direction(1)
     if rc != 0:
         raise RuntimeError(f"oracle tracking returned {rc}")
     names = ["x", "px", "y", "py", "z", "pz"]

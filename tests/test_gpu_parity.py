@@ -30,10 +30,10 @@ def _coords(n, seed, scale):
     return c
 
 
-def _gpu_track(ctx, eles, species, coords, n_turns, tracked_species=None):
+def _gpu_track(ctx, eles, species, coords, n_turns, tracked_species=None, direction=1, ix_start=0, ix_end=0):
     lat = ctx.lattice(eles, species, tracked_species=tracked_species)
     b = tracker.Bunch(coords, lat.p0c, abi.MASS[species], e_tot=lat.e_tot)
-    tracker.track_bunch(lat, b, n_turns=n_turns)
+    tracker.track_bunch(lat, b, n_turns=n_turns, ix_start=ix_start, ix_end=ix_end, direction=direction)
     out = b.numpy()
     lat.close()
     return out
@@ -169,6 +169,57 @@ def test_element_parity_reversed_and_antiparticle(ctx, name, variant):
     g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=tracked)
     r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, tracked_species=tracked or 0)
     _compare(g, r, _path(eles, 1))
+
+
+@pytest.mark.parametrize("name", sorted(ELEMENT_CASES))
+@pytest.mark.parametrize("variant", ["backward", "backward_reversed_anti"])
+def test_element_parity_backward(ctx, name, variant):
+    """Every element case tracked backward (direction = -1): the backward
+    drift, the quadrupole's 2 c beta delta_ref_time, the k1 bend's z0 sign,
+    physical ends and the sign_z_vel = -1 frame changes, against the oracle's
+    backward loop; as the reference species (orientation +1) and as the
+    antiparticle through reversed elements."""
+    eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=1e9)
+    if any(e.key == abi.KEY_RFCAVITY for e in eles):
+        pytest.skip("backward rf needs absolute time, which the device path refuses")
+    tracked = None
+    if variant == "backward_reversed_anti":
+        if all(e.key == abi.KEY_SBEND for e in eles):
+            pytest.skip("sbend orientation -1 is refused by the device path")
+        for e in eles:
+            if e.key != abi.KEY_SBEND:
+                e.orientation = -1
+        tracked = abi.POSITRON
+    coords = _coords(4096, 17, SCALE)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=tracked, direction=-1)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, tracked_species=tracked or 0, direction=-1)
+    _compare(g, r, _path(eles, 1))
+
+
+def test_ring_backward_turns_and_element_ranges(ctx):
+    """A positron bunch circulating backward through an electron FODO ring
+    (20 cells, misaligned and tilted elements absent): 3 full turns, then
+    track_bunch's two backward ranges (ix_start > ix_end, and the wrap
+    through element 1 and the last element), loss elements included."""
+    eles = lattice.fodo_ring_c0(n_cells=20)
+    coords, _ = beam.c0_bunch(eles[:7], n=4096, seed=23)
+    coords[0] *= 5.0  # large enough for some losses on the 1 cm aperture
+    coords[2] *= 5.0
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 3, tracked_species=abi.POSITRON, direction=-1)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 3, coords, n_threads=8, tracked_species=abi.POSITRON, direction=-1)
+    n_lost = int(np.sum(r["state"] != abi.ALIVE))
+    print("backward ring: lost", n_lost, "of", coords.shape[1], flush=True)
+    _compare(g, r, _path(eles, 3))
+    n = len(eles)
+    for ix_start, ix_end in ((n - 5, 17), (30, 60)):
+        g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=abi.POSITRON, direction=-1,
+                       ix_start=ix_start, ix_end=ix_end)
+        idx = list(range(ix_end, ix_start)) if ix_start > ix_end else list(range(ix_end, n)) + list(range(ix_start))
+        r = oracle.track_bunch([eles[i] for i in idx], abi.ELECTRON, 1, coords, n_threads=8,
+                               tracked_species=abi.POSITRON, direction=-1)
+        lost = r["state"] != abi.ALIVE
+        r["ix_lost"] = np.where(lost, np.array(idx)[np.where(lost, r["ix_lost"], 0)], -1)
+        _compare(g, r, sum(eles[i].l for i in idx))
 
 
 def test_zero_particle_and_reference_orbit(ctx):

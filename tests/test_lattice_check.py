@@ -14,9 +14,9 @@ from helpers import SPECIES, ele_from_spec
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def _check(eles, species=abi.ELECTRON, tracked_species=None):
+def _check(eles, species=abi.ELECTRON, tracked_species=None, direction=1):
     eles = lattice.set_reference([lattice.EleRec_copy(e) for e in eles], species, e_tot=1e9)
-    return tracker.check_lattice(eles, species, tracked_species=tracked_species)
+    return tracker.check_lattice(eles, species, tracked_species=tracked_species, direction=direction)
 
 
 def _n_ops(e):
@@ -190,3 +190,40 @@ def test_golden_elements_accepted_reversed_for_antiparticles():
             e.orientation = -1
         ok, _, msg = tracker.check_lattice([e], sp, tracked_species=anti if flip else None)
         assert ok, (name, msg)
+
+
+def test_golden_elements_accepted_backward():
+    """The -Anti_D and -Anti_OD variants (direction = -1): every element but
+    the rf cavity, whose backward phase needs absolute time, compiles for
+    backward tracking."""
+    tm = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
+    sp = SPECIES[tm["particle"]]
+    for orient in (1, -1):
+        for name, spec in tm["elements"].items():
+            e = ele_from_spec(spec, sp, e_tot=tm["e_tot"])
+            if spec["key"] not in ("sbend", "rfcavity"):
+                e.orientation = orient
+            anti = abi.POSITRON + abi.ELECTRON - sp if -e.orientation == -1 else None
+            ok, _, msg = tracker.check_lattice([e], sp, tracked_species=anti, direction=-1)
+            assert ok == (spec["key"] != "rfcavity"), (name, msg)
+            if not ok:
+                assert "rfcavity in backward tracking" in msg
+
+
+def test_backward_programs():
+    """Backward quadrupoles carry track_a_quadrupole's -2 c beta delta_ref_time
+    after every body step; thick multipoles run their drifts and kicks as
+    separate ops; a direction other than +1 or -1 is refused."""
+    q = lattice.ele("quadrupole", l=0.6, k1=0.5, num_steps=3, b={2: 1.0})
+    assert _n_ops_dir(q, -1) == _n_ops_dir(q, 1) + 3
+    sx = lattice.ele("sextupole", l=0.4, k2=5.0, num_steps=4)
+    assert _n_ops_dir(sx, -1) == 4 + 5  # 4 drifts and 5 kicks (the element end folds into the last kick)
+    assert _n_ops_dir(sx, 1) == 1
+    ok, _, msg = _check([lattice.ele("drift", l=1.0)], direction=2)
+    assert not ok and "direction must be" in msg
+
+
+def _n_ops_dir(e, direction):
+    ok, n, msg = _check([e], direction=direction)
+    assert ok, msg
+    return n

@@ -147,3 +147,26 @@ def test_aperture_states(ename, kind, sign):
     assert states == [int(v) for v in gs]
     for u, g in zip(ufs, gu):
         assert printed_match(u, g, 3) or abs(u - g) <= 1e-8, (ufs, gu)
+
+
+@pytest.mark.parametrize("row", [r for r in TM["rows"] if r["variant"] in ("-Anti_D", "-Anti_OD")],
+                         ids=lambda r: r["name"] + r["variant"])
+def test_backward_bunch_tracking_matches_the_golden_rows(row):
+    """The oracle's backward multi-particle loop (track_bunch with direction
+    = -1, the device's parity reference for backward tracking) reproduces the
+    -Anti_D / -Anti_OD rows of tracking_method_test, tracked with relative
+    time (absolute time changes only rf phases; rf rows are left to track1)."""
+    spec = TM["elements"][row["name"]]
+    if spec["key"] == "rfcavity":
+        pytest.skip("backward rf needs absolute time")
+    ref_species = SPECIES[TM["particle"]]
+    e = ele_from_spec(spec, ref_species, e_tot=TM["e_tot"])
+    if row["variant"] == "-Anti_OD" and spec["key"] != "sbend":
+        e.orientation = -1
+    species = ref_species if -1 * e.orientation == 1 else abi.POSITRON + abi.ELECTRON - ref_species
+    coords = np.array(TM["start"], dtype=np.float64)[:, None]
+    out = oracle.track_bunch([e], ref_species, 1, coords, tracked_species=species, direction=-1)
+    assert out["state"][0] == abi.ALIVE
+    vec = [out[k][0] for k in ("x", "px", "y", "py", "z", "pz")]
+    bad = [(i, v, g) for i, (v, g) in enumerate(zip(vec, row["vec"])) if not printed_match(v, g, TM["printed_digits"])]
+    assert not bad, f"coordinates off the printed golden value: {bad}"
