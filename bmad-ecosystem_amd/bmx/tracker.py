@@ -207,13 +207,14 @@ def bunch_moments(ctx, bunch):
     return int(round(a[0])), a[1:7], sq
 
 
-def track_bunch_host(lat, coords, n_turns=1, beta=None):
-    """Same as track_bunch on host numpy arrays (the library stages them through HBM)."""
+def track_bunch_host(lat, coords, n_turns=1, beta=None, direction=1):
+    """Same as track_bunch over full turns on host numpy arrays (the library
+    stages them through HBM)."""
     c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)
     cols = [np.ascontiguousarray(c[i]) for i in range(6)]
     p, extra = _host_particles(lat, cols, beta)
     st = abi.Stats()
-    rc = lat.ctx.lib.bmx_track_bunch(lat.ctx.h, lat.h, 0, 0, int(n_turns), 1, C.byref(p), C.byref(st))
+    rc = lat.ctx.lib.bmx_track_bunch(lat.ctx.h, lat.h, 0, 0, int(n_turns), int(direction), C.byref(p), C.byref(st))
     _check(lat.ctx.h, rc, "bmx_track_bunch")
     out = {k: v for k, v in zip(Bunch.NAMES, cols)}
     out.update(extra)

@@ -370,6 +370,12 @@ def test_host_buffer_path(ctx):
     g = tracker.track_bunch_host(lat, coords, n_turns=2)
     r = oracle.track_bunch(eles, abi.ELECTRON, 2, coords)
     _compare(g, r, _path(eles, 2))
+    # the backward program from host buffers, on a lattice that already ran forward
+    lat_p = ctx.lattice(eles, abi.ELECTRON, tracked_species=abi.POSITRON)
+    tracker.track_bunch_host(lat_p, coords, n_turns=1)
+    g = tracker.track_bunch_host(lat_p, coords, n_turns=2, direction=-1)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 2, coords, tracked_species=abi.POSITRON, direction=-1)
+    _compare(g, r, _path(eles, 2))
 
 
 def test_grid_matches_oracle(ctx):
