@@ -93,6 +93,50 @@ def da_scan(ctx, n_train=2048, n_test=512, epochs=8):
     return out
 
 
+def error_study_grid(ctx, n_var):
+    """The batched-DA error study (n_var = 256 is the whole study; fewer is a
+    sample): the first n_var of the 256 seeded error variants (504..4000-element C1-style rings with 8 rf
+    cavities; quad k1 x (1 + 1e-3 g), sextupole k2 x (1 + 5e-3 g), 50 um
+    offsets), each scanned on the full polar grid of 64 angles x 128
+    amplitudes (0..30 sigma_x at eps_x = 10 nm, y at the same amplitude) x 5
+    pz in [-2 %, +2 %] for 1,024 turns, FP64.  One bmx_track_grid call per
+    pz slice (a progress line each) after the autotuner has picked the
+    turn-block schedule; lattices/hour = n_var / the summed wall time of those
+    calls (upload, compile and tuning excluded)."""
+    from bmx import beam, lattice, tracker
+    vs = lattice.da_variants(n_var=n_var)
+    offs = [0]
+    for v in vs:
+        offs.append(offs[-1] + len(v))
+    lat = ctx.lattice([e for v in vs for e in v], abi_species(), variant_offsets=offs)
+    tw = beam.periodic_twiss(lattice.ring_c1(n_cells=500, n_rf=0)[:8])
+    amp = 30.0 * (10e-9 * tw[0]) ** 0.5
+    n_turns, wall, pet, areas = 1024, 0.0, 0.0, []
+    sched = tracker.tune_grid(lat, n_turns, n_angle=64, n_amp=128, n_pz=5, amp_max_x=amp, amp_ratio_y=1.0,
+                              pz_min=-0.02, pz_max=0.02)
+    print(f"error study: schedule {sched}", file=sys.stderr, flush=True)
+    for pz in np.linspace(-0.02, 0.02, 5):
+        t0 = time.perf_counter()
+        surv, area, _, st = tracker.track_grid(lat, n_turns, n_angle=64, n_amp=128, n_pz=1, amp_max_x=amp,
+                                               amp_ratio_y=1.0, pz_min=pz, pz_max=pz, return_stats=True)
+        dt = time.perf_counter() - t0
+        wall += dt
+        pet += st["pet"]
+        areas.append(np.asarray(area).tolist())
+        print(f"error study: pz {pz:+.3f} done in {dt:.1f} s, survivors {float(np.mean(surv == n_turns)):.3f}",
+              file=sys.stderr, flush=True)
+    lat.close()
+    return {"workload": f"{n_var} of the 256 seeded error variants, grid 64 x 128 x 5 (0..{amp * 1e3:.2f} mm, "
+                        f"pz -2..2 %), 1024 turns, FP64",
+            "n_elements": [len(v) for v in vs], "lattices_per_hour": n_var / wall * 3600.0, "wall_s": wall,
+            "pet_per_s": pet / wall, "schedule": sched, "da_area_on_momentum_m2": areas[2]}
+
+
+def abi_species():
+    from bmx import abi
+    return abi.ELECTRON
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +150,8 @@ def main():
                     help="only run a plain 512 MiB device copy (reference for HBM byte counters) and exit")
     ap.add_argument("--da-only", action="store_true",
                     help="only run the surrogate-vs-plain DA scan comparison and print its record")
+    ap.add_argument("--error-study-sample", type=int, default=0, metavar="N_VAR",
+                    help="only run the batched-DA error study on its first N_VAR variants and print its record")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); particle tracking "
@@ -127,6 +173,9 @@ def main():
 
     if args.da_only:
         print(json.dumps({"da_scan": da_scan(tracker.Context(0))}), flush=True)
+        return
+    if args.error_study_sample > 0:
+        print(json.dumps({"error_study": error_study_grid(tracker.Context(0), args.error_study_sample)}), flush=True)
         return
 
     eles = lattice.ring_c1(n_cells=500, n_rf=8)
