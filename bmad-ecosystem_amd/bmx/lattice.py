@@ -172,8 +172,11 @@ def ring_c1(n_cells=500, n_rf=8, seed=20261203, species=abi.ELECTRON, e_tot=6e9,
     """Large ring from the C0 cell pattern: cell QF SF(thick) OC D B D QD SD(thick),
     8 elements per cell.  Thick sextupoles L=0.2 m with k2 L = +-1.5 at 4 kick-drift
     slices, thin octupoles k3L ~ U(-50, 50) (seeded), bends closing the ring
-    (angle 2 pi / n_cells), n_rf of the 1.0 m drifts replaced by 500 MHz, 1 MV
-    rfcavities at phi0 = 0 (stable above transition), elliptical aperture
+    (angle 2 pi / n_cells), n_rf of the 1.0 m drifts replaced by 1 MV
+    rfcavities at phi0 = 0 (stable above transition) whose frequency is the
+    harmonic of the revolution frequency nearest 500 MHz (h = 4503 on the
+    500-cell ring), so that the reference particle sees the same rf phase on
+    every turn in absolute time as in relative time; elliptical aperture
     x_ap x y_ap.  Optional per-element relative errors (arrays indexed by
     quad / sextupole count) and transverse offsets."""
     rng = np.random.default_rng(seed)
@@ -217,7 +220,24 @@ def ring_c1(n_cells=500, n_rf=8, seed=20261203, species=abi.ELECTRON, e_tot=6e9,
             ele("sextupole", l=0.2, k2=-7.5 * fs, num_steps=4, x_offset=os_[0], y_offset=os_[1], **ap),
         ]
         eles += cell
-    return set_reference(eles, species, e_tot=e_tot)
+    set_reference(eles, species, e_tot=e_tot)
+    return set_rf_harmonic(eles, 500e6)
+
+
+def revolution_time(eles):
+    """Reference time of one pass through the element list [s]."""
+    return eles[-1].ref_time - (eles[0].ref_time - eles[0].delta_ref_time)
+
+
+def set_rf_harmonic(eles, f_target):
+    """Set every rfcavity's frequency to the harmonic h / T of the
+    revolution time T nearest f_target (h = nint(f_target T))."""
+    t_rev = revolution_time(eles)
+    h = max(1, _nint(f_target * t_rev))
+    for e in eles:
+        if e.key == abi.KEY_RFCAVITY:
+            e.rf_frequency = h / t_rev
+    return eles
 
 
 def da_variants(n_var=256, seed=20261204, min_cells=63, max_cells=500, n_rf=8, offsets_scale=1.0):

@@ -496,7 +496,6 @@ int used_consts(const DevOp& op) {
 bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, const bmx_config& cfg,
                      std::vector<DevOp>& prog, std::vector<int32_t>& ele_op_begin, LatticeConsts* lc,
                      std::string* err) {
-  (void)cfg;
   char buf[256];
   if (n_ele <= 0) { *err = "empty lattice"; return false; }
   double p0c = eles[0].p0c, e_tot = eles[0].e_tot;

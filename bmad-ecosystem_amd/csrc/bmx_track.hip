@@ -68,13 +68,13 @@ struct bmx_lattice {
   int64_t n_ops;
   LatticeConsts lc;
   // backward tracking of variant 0 (bmx_track_bunch with direction = -1):
-  // its elements, compiled on first use into a program of their own
-  std::vector<bmx_ele_rec> eles0;
-  bmx_ref ref;
-  bmx_config cfg;
+  // its elements compiled at upload into a program of their own, so that no
+  // tracking call compiles or allocates (d_ops_bwd stays null, with the
+  // reason in bwd_err, when the backward path refuses the lattice)
   DevOp* d_ops_bwd = nullptr;
   std::vector<int32_t> eb_bwd;
   LatticeConsts lc_bwd;
+  std::string bwd_err;
 };
 
 // Launch a tracking kernel: the lattice's runtime-compiled cell kernel when
@@ -170,6 +170,7 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
   if (!ctx || !eles || !ref || !out || n_ele_total <= 0 || n_variants <= 0) return set_err(ctx, "invalid arguments");
   *out = nullptr;
   bmx_config c = cfg ? *cfg : bmx_config{1, 0, 1e3};
+  c.direction = 1;  // the forward program; the backward one of variant 0 is compiled below
   std::vector<int32_t> offs;
   if (variant_offsets) offs.assign(variant_offsets, variant_offsets + n_variants + 1);
   else if (n_variants == 1) offs = {0, n_ele_total};
@@ -209,7 +210,7 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
   if (hipMalloc(&lat->d_ops, prog.size() * sizeof(DevOp)) != hipSuccess ||
       hipMalloc(&lat->d_var_op_begin, lat->var_op_begin.size() * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&lat->d_var_n_ele, lat->var_n_ele.size() * sizeof(int32_t)) != hipSuccess) {
-    delete lat;
+    bmx_lattice_destroy(lat);
     return set_err(ctx, "hipMalloc of the element program failed");
   }
   HIP_TRY(ctx, hipMemcpy(lat->d_ops, prog.data(), prog.size() * sizeof(DevOp), hipMemcpyHostToDevice));
@@ -232,9 +233,32 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
       std::call_once(once, [&] { std::fprintf(stderr, "bmx: cell kernels unavailable, using the interpreter: %s\n", jerr.c_str()); });
     }
   }
-  lat->eles0.assign(eles + offs[0], eles + offs[1]);
-  lat->ref = *ref;
-  lat->cfg = c;
+  {  // backward program of variant 0: elements last to first, each with direction -1
+    bmx_config cb = c;
+    cb.direction = -1;
+    std::vector<DevOp> bprog;
+    LatticeConsts lb;
+    std::memset(&lb, 0, sizeof(lb));
+    std::string berr;
+    if (compile_variant(eles + offs[0], offs[1] - offs[0], *ref, cb, bprog, lat->eb_bwd, &lb, &berr)) {
+      lb.max_aperture_limit = cb.max_aperture_limit;
+      lb.two_max_ap = 2.0 * cb.max_aperture_limit;
+      lb.aperture_limit_on = cb.aperture_limit_on;
+      if (hipMalloc(&lat->d_ops_bwd, bprog.size() * sizeof(DevOp)) != hipSuccess) {
+        lat->d_ops_bwd = nullptr;
+        bmx_lattice_destroy(lat);
+        return set_err(ctx, "hipMalloc of the backward program failed");
+      }
+      if (hipMemcpy(lat->d_ops_bwd, bprog.data(), bprog.size() * sizeof(DevOp), hipMemcpyHostToDevice) != hipSuccess) {
+        bmx_lattice_destroy(lat);
+        return set_err(ctx, "upload of the backward program failed");
+      }
+      lat->lc_bwd = lb;
+    } else {
+      lat->eb_bwd.clear();
+      lat->bwd_err = berr;
+    }
+  }
   *out = lat;
   return 0;
 }
@@ -355,30 +379,7 @@ int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, 
   int n_ele = lat->var_n_ele[0];
   if (ix_start < 0 || ix_start > n_ele || ix_end < 0 || ix_end > n_ele) return set_err(ctx, "ix_start/ix_end out of range");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  if (direction == -1 && !lat->d_ops_bwd) {  // compile the backward program of variant 0 once
-    bmx_lattice* lw = const_cast<bmx_lattice*>(lat);
-    bmx_config c = lw->cfg;
-    c.direction = -1;
-    std::vector<DevOp> prog;
-    std::vector<int32_t> eb;
-    LatticeConsts lc;
-    std::memset(&lc, 0, sizeof(lc));
-    std::string err;
-    if (!compile_variant(lw->eles0.data(), n_ele, lw->ref, c, prog, eb, &lc, &err))
-      return set_err(ctx, "backward tracking: " + err);
-    lc.max_aperture_limit = c.max_aperture_limit;
-    lc.two_max_ap = 2.0 * c.max_aperture_limit;
-    lc.aperture_limit_on = c.aperture_limit_on;
-    DevOp* d = nullptr;
-    if (hipMalloc(&d, prog.size() * sizeof(DevOp)) != hipSuccess) return set_err(ctx, "hipMalloc of the backward program failed");
-    if (hipMemcpy(d, prog.data(), prog.size() * sizeof(DevOp), hipMemcpyHostToDevice) != hipSuccess) {
-      hipFree(d);
-      return set_err(ctx, "upload of the backward program failed");
-    }
-    lw->d_ops_bwd = d;
-    lw->eb_bwd = eb;
-    lw->lc_bwd = lc;
-  }
+  if (direction == -1 && !lat->d_ops_bwd) return set_err(ctx, "backward tracking: " + lat->bwd_err);
   const bool bwd = direction == -1;
   const std::vector<int32_t>& eb = bwd ? lat->eb_bwd : lat->ele_op_begin[0];
   int lo1, hi1, lo2 = 0, hi2 = 0;

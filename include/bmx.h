@@ -150,7 +150,8 @@ typedef struct bmx_ref {
 /* Global switches (subset of bmad_com, bmad/modules/bmad_struct.f90:2296-2343). */
 typedef struct bmx_config {
   int32_t aperture_limit_on;     /* default 1 */
-  int32_t direction;             /* bmx_lattice_check: 0 or +1 forward, -1 backward tracking */
+  int32_t direction;             /* bmx_lattice_check only: 0 or +1 forward, -1 backward tracking
+                                    (bmx_lattice_upload ignores it and compiles both directions) */
   double max_aperture_limit;     /* default 1e3 m */
 } bmx_config;
 
@@ -253,8 +254,8 @@ int32_t bmx_lattice_cells(const bmx_lattice *lat, int32_t *cell_len, int64_t *n_
  * end of the lattice), repeated n_turns times.  direction = -1 tracks
  * backward through elements ix_start down to ix_end+1 (wrapping when
  * ix_start <= ix_end), beam_mod.f90:131-147, with the lattice's backward
- * program, compiled on the first such call (rf cavities and misaligned bends
- * are refused there).  Equivalent to n_turns calls of track_bunch(lat, bunch,
+ * program, compiled at upload (a lattice the backward path refuses returns
+ * the reason here).  Equivalent to n_turns calls of track_bunch(lat, bunch,
  * ele_start, ele_end, direction) (bmad/multiparticle/beam_mod.f90:103-170). */
 int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
                         int32_t n_turns, int32_t direction, bmx_particles *p, bmx_stats *out);
