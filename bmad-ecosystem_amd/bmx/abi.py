@@ -7,6 +7,7 @@ import ctypes as C
 import os
 
 N_POLE = 22
+NOALLOC_MAX_PARTICLES = 65536  # BMX_NOALLOC_MAX_PARTICLES
 
 # particle states (bmad/modules/bmad_struct.f90:520-527)
 NOT_SET, PRE_BORN, ALIVE, LOST, LOST_NEG_X, LOST_POS_X, LOST_NEG_Y, LOST_POS_Y, LOST_Z, LOST_PZ = (
@@ -94,7 +95,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 # every entry point declared in include/bmx.h
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
-    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch", "bmx_track_multi", "bmx_track_grid",
+    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch",
+    "bmx_track_bunch_noalloc", "bmx_track_multi", "bmx_track_grid",
     "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_one_turn_matrix", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
     "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_step_fit", "bmx_ctx_sync",
@@ -134,6 +136,8 @@ def load_library(path=None):
     lib.bmx_track_bunch.restype = C.c_int32
     lib.bmx_track_bunch.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.POINTER(Particles), C.POINTER(Stats)]
+    lib.bmx_track_bunch_noalloc.restype = C.c_int32
+    lib.bmx_track_bunch_noalloc.argtypes = lib.bmx_track_bunch.argtypes
     lib.bmx_track_multi.restype = C.c_int32
     lib.bmx_track_multi.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.POINTER(Particles),
                                     C.POINTER(Stats)]

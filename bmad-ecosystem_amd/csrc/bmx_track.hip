@@ -50,7 +50,46 @@ struct bmx_ctx {
   hipEvent_t ev0, ev1;
   std::string err;
   bmx_schedule sched;  // turn-block schedule used when a call passes turns_per_block <= 0
+  void* scratch = nullptr;  // device address of this device's `scratch` array (bmx_track_bunch_noalloc)
+  std::string scratch_err;
 };
+
+// ---------------------------------------------------------------------------
+// Static scratch of bmx_track_bunch_noalloc: the lost-particle counter and
+// staging copies of host-resident particle arrays, in one file-scope device
+// array, so that the entry point allocates nothing.  Every GPU holds its own
+// copy until the process ends; each context looks its address up once, at
+// creation, with its device current.  Layout: the counter in the first 256
+// bytes, then NOALLOC_MAX_PARTICLES slots of 8 double arrays and 3 int32
+// arrays (x, px, y, py, z, pz, beta, one spare; state, ix_lost, turn_lost).
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_PARTICLES = BMX_NOALLOC_MAX_PARTICLES;
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_BYTES =
+    NOALLOC_HEAD + (size_t)NOALLOC_MAX_PARTICLES * (8 * sizeof(double) + 3 * sizeof(int32_t));
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device, filled once
+
+// The device address of `scratch` on `device` (which must be current); null
+// with the HIP error text in *err when the lookup fails.
+static void* scratch_address(int device, std::string* err) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
+__global__ void bmx_zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
 
 struct bmx_lattice {
   int device;
@@ -148,6 +187,7 @@ int32_t bmx_ctx_create(bmx_ctx** out, int32_t device, uint32_t flags) {
     delete c;
     return -5;
   }
+  c->scratch = scratch_address(device, &c->scratch_err);  // failure only disables bmx_track_bunch_noalloc
   *out = c;
   return 0;
 }
@@ -313,6 +353,7 @@ struct Staged {
   size_t by[10];
   std::vector<void*> owned;
   unsigned long long* d_lost = nullptr;
+  bool own_lost = true;  // d_lost was allocated by the call (not the static scratch)
 };
 
 static int stage_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st) {
@@ -354,7 +395,7 @@ static int finish_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st, hi
   float ms = 0;
   (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   for (void* q : st.owned) (void)hipFree(q);
-  (void)hipFree(st.d_lost);
+  if (st.own_lost) (void)hipFree(st.d_lost);
   if (le != hipSuccess) return set_err(ctx, std::string("kernel launch: ") + hipGetErrorString(le));
   if (se != hipSuccess) return set_err(ctx, std::string("stream sync: ") + hipGetErrorString(se));
   if (out) {
@@ -364,20 +405,59 @@ static int finish_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st, hi
   return 0;
 }
 
+// Device views of a bmx_particles in the static scratch: device arrays are
+// used in place, host arrays (at most NOALLOC_MAX_PARTICLES) are copied into
+// their scratch slots; the counter is cleared by a kernel.  No allocation.
+static int stage_particles_static(bmx_ctx* ctx, const bmx_particles* p, Staged& st) {
+  int64_t n = p->n;
+  unsigned char* base = (unsigned char*)ctx->scratch;
+  size_t bd = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
+  void* hp[10] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost};
+  size_t by[10] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi};
+  const size_t N = (size_t)NOALLOC_MAX_PARTICLES;
+  for (int k = 0; k < 10; k++) {
+    st.hp[k] = hp[k];
+    st.by[k] = by[k];
+    st.dp[k] = nullptr;
+    if (!hp[k]) continue;
+    if (p->on_device) { st.dp[k] = hp[k]; continue; }
+    size_t off = (k < 7) ? NOALLOC_HEAD + (size_t)k * N * sizeof(double)
+                         : NOALLOC_HEAD + 8 * N * sizeof(double) + (size_t)(k - 7) * N * sizeof(int32_t);
+    st.dp[k] = base + off;
+    if (hipMemcpyAsync(st.dp[k], hp[k], by[k], hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return set_err(ctx, "upload of the particle arrays into the static scratch failed");
+  }
+  st.d_lost = (unsigned long long*)base;
+  st.own_lost = false;
+  hipLaunchKernelGGL(bmx_zero_u64_kernel, dim3(1), dim3(1), 0, ctx->stream, st.d_lost);
+  st.s.x = (double*)st.dp[0]; st.s.px = (double*)st.dp[1]; st.s.y = (double*)st.dp[2]; st.s.py = (double*)st.dp[3];
+  st.s.z = (double*)st.dp[4]; st.s.pz = (double*)st.dp[5]; st.s.beta = (double*)st.dp[6];
+  st.s.state = (int32_t*)st.dp[7]; st.s.ix_lost = (int32_t*)st.dp[8]; st.s.turn_lost = (int32_t*)st.dp[9];
+  return 0;
+}
+
 static bool particles_ok(const bmx_particles* p) {
   return p->x && p->px && p->y && p->py && p->z && p->pz && p->beta && p->state;
 }
 
-extern "C" {
-
-int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
-                        int32_t direction, bmx_particles* p, bmx_stats* out) {
+// bmx_track_bunch and bmx_track_bunch_noalloc: one body, the particle
+// staging (allocated per call, or the static scratch) chosen by no_alloc.
+static int32_t track_bunch_common(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end,
+                                  int32_t n_turns, int32_t direction, bmx_particles* p, bmx_stats* out,
+                                  bool no_alloc) {
   if (!ctx || !lat || !p) return set_err(ctx, "invalid arguments");
   if (direction != 1 && direction != -1) return set_err(ctx, "direction must be +1 or -1");
   if (n_turns < 0) return set_err(ctx, "n_turns < 0");
   if (!particles_ok(p)) return set_err(ctx, "particle arrays x..pz, beta, state are required");
   int n_ele = lat->var_n_ele[0];
   if (ix_start < 0 || ix_start > n_ele || ix_end < 0 || ix_end > n_ele) return set_err(ctx, "ix_start/ix_end out of range");
+  if (no_alloc) {
+    if (!ctx->scratch) return set_err(ctx, "static scratch unavailable: " + ctx->scratch_err);
+    if (!p->on_device && p->n > NOALLOC_MAX_PARTICLES)
+      return set_err(ctx, "bmx_track_bunch_noalloc: " + std::to_string(p->n) +
+                              " host-resident particles exceed the static staging area of " +
+                              std::to_string(NOALLOC_MAX_PARTICLES) + " (pass device arrays, which have no limit)");
+  }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if (direction == -1 && !lat->d_ops_bwd) return set_err(ctx, "backward tracking: " + lat->bwd_err);
   const bool bwd = direction == -1;
@@ -398,7 +478,7 @@ int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, 
   if (out) std::memset(out, 0, sizeof(*out));
   if (n <= 0 || n_turns == 0) return 0;
   Staged st;
-  if (stage_particles(ctx, p, st) != 0) return -1;
+  if ((no_alloc ? stage_particles_static(ctx, p, st) : stage_particles(ctx, p, st)) != 0) return -1;
   const int block = 256;
   int64_t grid = (n + block * BMX_NP - 1) / (block * BMX_NP);
   (void)hipEventRecord(ctx->ev0, ctx->stream);
@@ -411,6 +491,18 @@ int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, 
   hipError_t le = hipGetLastError();
   (void)hipEventRecord(ctx->ev1, ctx->stream);
   return finish_particles(ctx, p, st, le, out);
+}
+
+extern "C" {
+
+int32_t bmx_track_bunch(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
+                        int32_t direction, bmx_particles* p, bmx_stats* out) {
+  return track_bunch_common(ctx, lat, ix_start, ix_end, n_turns, direction, p, out, false);
+}
+
+int32_t bmx_track_bunch_noalloc(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end,
+                                int32_t n_turns, int32_t direction, bmx_particles* p, bmx_stats* out) {
+  return track_bunch_common(ctx, lat, ix_start, ix_end, n_turns, direction, p, out, true);
 }
 
 int32_t bmx_track_multi(bmx_ctx* ctx, const bmx_lattice* lat, const int64_t* part_offsets, int32_t n_turns,

@@ -260,6 +260,20 @@ int32_t bmx_lattice_cells(const bmx_lattice *lat, int32_t *cell_len, int64_t *n_
 int32_t bmx_track_bunch(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
                         int32_t n_turns, int32_t direction, bmx_particles *p, bmx_stats *out);
 
+/* bmx_track_bunch that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight turn or serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as bmx_track_bunch.  The loss counter
+ * and, for host-resident arrays, their staging copies live in a static
+ * device array of 256 B + 65536 x 76 B (about 5.0 MB) that each GPU holds
+ * until the process ends: host-resident bunches of more than
+ * BMX_NOALLOC_MAX_PARTICLES particles are refused with an error, device-
+ * resident arrays are tracked in place at any size.  Calls that overlap on
+ * one GPU (threads or contexts) share that array and race: serialise them. */
+#define BMX_NOALLOC_MAX_PARTICLES 65536
+int32_t bmx_track_bunch_noalloc(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
+                                int32_t n_turns, int32_t direction, bmx_particles *p, bmx_stats *out);
+
 /* Track particles of several lattice variants in one launch: particles
  * [part_offsets[v], part_offsets[v+1]) go through variant v for n_turns whole
  * turns; part_offsets has n_variants + 1 entries from 0 to p->n.  Replaces one
