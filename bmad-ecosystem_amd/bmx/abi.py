@@ -67,12 +67,13 @@ class Ref(C.Structure):
 
 
 class Config(C.Structure):
-    _fields_ = [("aperture_limit_on", C.c_int32), ("direction", C.c_int32), ("max_aperture_limit", C.c_double)]
+    _fields_ = [("aperture_limit_on", C.c_int32), ("direction", C.c_int32), ("max_aperture_limit", C.c_double),
+                ("absolute_time_tracking", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Particles(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("x", "px", "y", "py", "z", "pz", "beta", "state", "ix_lost", "turn_lost")] + [
-        ("n", C.c_int64), ("on_device", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [(n, C.c_void_p) for n in ("x", "px", "y", "py", "z", "pz", "beta", "state", "ix_lost", "turn_lost",
+                                          "t")] + [("n", C.c_int64), ("on_device", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):

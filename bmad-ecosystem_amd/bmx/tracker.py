@@ -64,18 +64,21 @@ class Context:
         _check(self.h, self.lib.bmx_ctx_set_schedule(self.h, C.byref(sc)), "bmx_ctx_set_schedule")
 
     def lattice(self, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3,
-                tracked_species=None):
-        return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit, tracked_species)
+                tracked_species=None, absolute_time_tracking=False):
+        return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit, tracked_species,
+                       absolute_time_tracking)
 
 
-def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3, tracked_species=None, direction=1):
+def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3, tracked_species=None, direction=1,
+                  absolute_time_tracking=False):
     """(ok, n_ops, reason): whether the device path accepts this element list
     for tracking in `direction`, decided on the host with no device
     (bmx_lattice_check)."""
     lib = abi.load_library()
     arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
     ref = abi.Ref(species, tracked_species or 0)
-    cfg = abi.Config(1 if aperture_limit_on else 0, int(direction), max_aperture_limit)
+    cfg = abi.Config(1 if aperture_limit_on else 0, int(direction), max_aperture_limit,
+                     1 if absolute_time_tracking else 0, 0)
     n_ops = C.c_int64(0)
     msg = C.create_string_buffer(512)
     rc = lib.bmx_lattice_check(arr, len(arr), C.byref(ref), C.byref(cfg), C.byref(n_ops), msg, len(msg))
@@ -88,7 +91,7 @@ class Lattice:
     reference species (its kicks and voltages change sign)."""
 
     def __init__(self, ctx, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3,
-                 tracked_species=None):
+                 tracked_species=None, absolute_time_tracking=False):
         self.ctx = ctx
         self.species = species
         self.tracked_species = tracked_species or species
@@ -97,7 +100,8 @@ class Lattice:
         self.p0c = arr[0].p0c
         self.e_tot = arr[0].e_tot
         ref = abi.Ref(species, tracked_species or 0)
-        cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit)
+        cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit, 1 if absolute_time_tracking else 0, 0)
+        self.absolute_time_tracking = bool(absolute_time_tracking)
         if variant_offsets is None:
             offs_p = None
             n_var = 1
@@ -138,7 +142,9 @@ class Bunch:
 
     NAMES = ("x", "px", "y", "py", "z", "pz")
 
-    def __init__(self, coords, p0c, mc2, device="cuda:0", beta=None, e_tot=None):
+    def __init__(self, coords, p0c, mc2, device="cuda:0", beta=None, e_tot=None, t=None):
+        """t: particle times [s] (coord_struct%t), carried and updated by
+        track_bunch when given (absolute-time rf reads them)."""
         c = torch.as_tensor(np.asarray(coords, dtype=np.float64).reshape(6, -1))
         self.n = c.shape[1]
         self.device = torch.device(device)
@@ -154,6 +160,7 @@ class Bunch:
         self.state = torch.ones(self.n, dtype=torch.int32, device=self.device)
         self.ix_lost = torch.full((self.n,), -1, dtype=torch.int32, device=self.device)
         self.turn_lost = torch.full((self.n,), -1, dtype=torch.int32, device=self.device)
+        self.t = None if t is None else torch.as_tensor(np.asarray(t, dtype=np.float64).reshape(-1)).to(self.device)
 
     def particles_struct(self):
         p = abi.Particles()
@@ -163,6 +170,7 @@ class Bunch:
         p.state = self.state.data_ptr()
         p.ix_lost = self.ix_lost.data_ptr()
         p.turn_lost = self.turn_lost.data_ptr()
+        p.t = None if self.t is None else self.t.data_ptr()
         p.n = self.n
         p.on_device = 1
         return p
@@ -171,6 +179,8 @@ class Bunch:
         out = {k: t.cpu().numpy() for k, t in zip(self.NAMES, self.cols)}
         out.update(beta=self.beta.cpu().numpy(), state=self.state.cpu().numpy(), ix_lost=self.ix_lost.cpu().numpy(),
                    turn_lost=self.turn_lost.cpu().numpy())
+        if self.t is not None:
+            out["t"] = self.t.cpu().numpy()
         return out
 
 

@@ -51,6 +51,8 @@ enum : int32_t {
   F_EDGE_EXIT = 64,       // OP_SBEND_FUSED: exit edge present (F_HWANG_ENTER: entrance edge present)
   F_COUPLER_ENTER = 1 << 8,  // OP_RFCAVITY: coupler kick at the entrance (c[5..7])
   F_COUPLER_EXIT = 1 << 9,   // OP_RFCAVITY: coupler kick at the exit
+  F_RF_ABS = 1 << 12,        // OP_RFCAVITY: absolute-time phase (c[8] time offset, c[9] frac(f T_lattice))
+  F_RF_BACK = 1 << 13,       // OP_RFCAVITY: backward drifts between the kicks (c[10] beta_ref)
   // collimator phase-space limits (px, py, z, pz width2 / center pairs):
   // OP_APERTURE_IN with F_APERTURE_PS holds them at c[6..13]; an OP_ELE_END
   // with F_END_PS at c[0..7], its aperture constants then at c[8..13];
@@ -111,6 +113,31 @@ struct LatticeConsts {
   int32_t aperture_limit_on;
   int32_t species;
   const int32_t* cell_prv;  // device: last F_CELL op at or before each op (-1: none); cell kernels only
+  // Reference time (absolute-time rf and the particle time t).  Lattice:
+  // t_lat = the reference time of one pass through the lattice; rt = device
+  // array of the reference times of the element boundaries (n_ele + 1:
+  // rt[j] = exit of array element j - 1, rt[0] = entrance of element 0).
+  // Per call (set by the entry point on its copy): tt_sign = +1 forward, -1
+  // backward (the reference time runs against s backward); tt_d = the
+  // reference time of one pass of the tracked range minus t_lat; ix_first =
+  // the array index of the first element tracked; t_start, t_end = the
+  // reference times at the start edge of the first pass and the end edge of
+  // the last.  The wave-uniform time of a pass (TurnTime) is k t_lat + r with
+  // k = tt_sign (turn + wrapped) and r = tt_sign turn tt_d.
+  double t_lat;
+  const double* rt;
+  double tt_d, t_start, t_end;
+  int32_t tt_sign, ix_first;
+  int32_t abs_time;  // the program has absolute-time rf cavities
+  int32_t n_ele;
+};
+
+// Reference time of the current pass relative to the lattice frame, as
+// k t_lat + r (k whole lattice periods, kept apart so that absolute-time rf
+// phases stay exact over many turns: each cavity holds frac(f t_lat)).
+struct TurnTime {
+  int32_t k;
+  double r;
 };
 
 }  // namespace bmx

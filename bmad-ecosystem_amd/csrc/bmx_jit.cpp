@@ -111,17 +111,25 @@ bool jit_enabled() {
   return !(e && e[0] == '0');
 }
 
+// absolute-time rf anywhere in the program: the lanes carry their time offset
+bool has_abs_time(const std::vector<DevOp>& prog) {
+  for (const DevOp& op : prog)
+    if (op.type == OP_RFCAVITY && (op.flags & F_RF_ABS)) return true;
+  return false;
+}
+
 std::string cell_source(const std::vector<DevOp>& prog, size_t rep, int len) {
   std::string s;
   s += "#include \"bmx.h\"\n#include \"bmx_internal.h\"\nusing namespace bmx;\n";
   s += "#define BMX_CELL_LEN " + std::to_string(len) + "\n";
+  s += std::string("#define BMX_ABS_TIME ") + (has_abs_time(prog) ? "1" : "0") + "\n";
   s += kSrcOpsH;
   s += "\ntemplate <class Loss>\n__device__ __forceinline__ void jit_cell(Lane& L, const DevOp* o, "
-       "const LatticeConsts& lc, Loss loss) {\n";
+       "const LatticeConsts& lc, const TurnTime& tt, Loss loss) {\n";
   for (int i = 0; i < len; i++) {
     const DevOp& op = prog[rep + i];
     s += "  run_op(L, o[" + std::to_string(i) + "], " + std::to_string(op.type) + ", " +
-         std::to_string(op.flags & ~F_CELL) + ", " + std::to_string(op.n) + ", lc);\n  loss(o[" +
+         std::to_string(op.flags & ~F_CELL) + ", " + std::to_string(op.n) + ", lc, tt);\n  loss(o[" +
          std::to_string(i) + "]);\n";
   }
   s += "}\n";
@@ -179,7 +187,7 @@ bool plan_cells(std::vector<DevOp>& prog, CellPlan* plan) {
 const JitKernels* jit_kernels(int device, const std::vector<DevOp>& prog, const CellPlan& plan, std::string* err) {
   std::lock_guard<std::mutex> lock(g_mu);
   const char* fl = std::getenv("BMX_JIT_FLAGS");
-  auto key = std::make_pair(device, plan.key + "|" + (fl ? fl : ""));
+  auto key = std::make_pair(device, plan.key + "|" + (has_abs_time(prog) ? "abs|" : "") + (fl ? fl : ""));
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return &it->second.k;
   hipDeviceProp_t prop;

@@ -44,7 +44,20 @@ __device__ __forceinline__ void park(Lane& L, const LatticeConsts& lc) {
   L.beta = lc.beta0;
   L.irp = 1.0;
   L.bb = 0.0;
+#if BMX_ABS_TIME
+  L.dt = 0.0;
+#endif
   L.state = BMX_ALIVE;
+}
+
+// Reference time at the far edge, in the tracking direction, of array
+// element ie on pass `turn` of a bmx_track_bunch call (variant 0).
+__device__ __forceinline__ double edge_time(const LatticeConsts& lc, int ie, int turn) {
+  bool fwd = lc.tt_sign > 0;
+  double base = fwd ? lc.rt[ie + 1] : lc.rt[ie];
+  bool wrapped = fwd ? (ie < lc.ix_first) : (ie > lc.ix_first);
+  double k = (double)lc.tt_sign * ((double)turn + (wrapped ? 1.0 : 0.0));
+  return base + k * lc.t_lat + (double)(lc.tt_sign * turn) * lc.tt_d;
 }
 
 // Walk of `turns` turns over ops [lo1, hi1) then [lo2, hi2) for one lane's
@@ -61,12 +74,19 @@ __device__ __forceinline__ void park(Lane& L, const LatticeConsts& lc) {
 // through run_op.  A chunk ends before a cell that would cross its end
 // (lc.cell_prv[i]: the last cell start at or before op i); a cell that
 // crosses the end of the tracked range runs op by op.
+//
+// turn0: the pass number of the first turn (survival scans in turn blocks);
+// the reference time of each pass (TurnTime, absolute-time rf) follows from
+// it, the turn and whether the segment lies past the lattice end.
 template <int NP, class OnLoss>
 __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2,
-                                     int turns, const LatticeConsts& lc, DevOp* sops, OnLoss on_loss) {
+                                     int turns, int turn0, const LatticeConsts& lc, DevOp* sops, OnLoss on_loss) {
   for (int t = 0; t < turns; t++) {
     for (int seg = 0; seg < 2; seg++) {
       int lo = seg ? lo2 : lo1, hi = seg ? hi2 : hi1;
+      TurnTime tt;
+      tt.k = lc.tt_sign * (turn0 + t + seg);
+      tt.r = (double)(lc.tt_sign * (turn0 + t)) * lc.tt_d;
       for (int c = lo; c < hi;) {
         int m = min(OPS_CHUNK, hi - c);
 #ifdef BMX_CELL_LEN
@@ -103,7 +123,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
 #ifdef BMX_CELL_LEN
             if (__builtin_expect((flags & F_CELL) && k + BMX_CELL_LEN <= m, 1)) {
               static_assert(NP == 1, "cell kernels carry one particle per lane");
-              jit_cell(P.L[0], sops + k, lc, loss_check);
+              jit_cell(P.L[0], sops + k, lc, tt, loss_check);
               k += BMX_CELL_LEN;
               continue;
             }
@@ -111,7 +131,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
             int type = __builtin_amdgcn_readfirstlane(op.type);
             int n = __builtin_amdgcn_readfirstlane(op.n);
 #pragma unroll
-            for (int j = 0; j < NP; j++) run_op(P.L[j], op, type, flags, n, lc);
+            for (int j = 0; j < NP; j++) run_op(P.L[j], op, type, flags, n, lc, tt);
             loss_check(op);
             k++;
           }
@@ -128,6 +148,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
 struct SoA {
   double *x, *px, *y, *py, *z, *pz, *beta;
   int32_t *state, *ix_lost, *turn_lost;
+  double* t;  // particle time (bunch kernel, may be null)
 };
 
 // Waves per SIMD the register allocation targets (occupancy); the lane
@@ -139,7 +160,11 @@ struct SoA {
 // NP particles of a bunch per lane: particle j of lane `tid` in workgroup
 // `blk` is base + j * 256 + tid (coalesced per j).  Load, `n_turns` turns
 // over [lo1, hi1) then [lo2, hi2), store.  Particles already lost on entry are
-// left untouched; a particle lost on the way is stored at its loss.
+// left untouched; a particle lost on the way is stored at its loss.  With a
+// time array, a tracked particle's t is replaced by its offset from the
+// reference relation, dt = t - t_start + z / (beta c), for the length of the
+// walk (so that kernels without the lane's dt keep it in memory), and set to
+// the reference time of its last edge + dt - z / (beta c) at the end.
 template <int NP>
 __device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const DevOp* __restrict__ ops, int lo1,
                                             int hi1, int lo2, int hi2, const LatticeConsts& lc, int n_turns, SoA& p,
@@ -161,6 +186,13 @@ __device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const Dev
       L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
       refresh_invariants(L, lc);
       P.live[j] = !too_large(L, lc);
+      if (p.t && P.live[j]) {
+        double dt = p.t[i] - lc.t_start + L.z / (L.beta * D_C_LIGHT);
+        p.t[i] = dt;
+#if BMX_ABS_TIME
+        L.dt = dt;
+#endif
+      }
       if (!P.live[j]) {  // lost before the first element
         p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
         p.state[i] = L.state;
@@ -171,9 +203,10 @@ __device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const Dev
       }
     }
   }
-  walk<NP>(P, ops, lo1, hi1, lo2, hi2, n_turns, lc, sops, [&](int j, const Lane& L, int t, int ie) {
+  walk<NP>(P, ops, lo1, hi1, lo2, hi2, n_turns, 0, lc, sops, [&](int j, const Lane& L, int t, int ie) {
     int64_t i = idx[j];
     p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
+    if (p.t) p.t[i] = edge_time(lc, ie, t) + p.t[i] - L.z / (L.beta * D_C_LIGHT);
     p.state[i] = L.state;
     if (p.ix_lost) p.ix_lost[i] = ie;
     if (p.turn_lost) p.turn_lost[i] = t;
@@ -185,6 +218,7 @@ __device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const Dev
     int64_t i = idx[j];
     const Lane& L = P.L[j];
     p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
+    if (p.t) p.t[i] = lc.t_end + p.t[i] - L.z / (L.beta * D_C_LIGHT);
   }
 }
 
@@ -312,7 +346,7 @@ extern "C" __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_blo
     }
   }
   int lo = var_op_begin[v], hi = var_op_begin[v + 1];
-  walk<NP>(P, ops, lo, hi, 0, 0, turns, lc, sops,
+  walk<NP>(P, ops, lo, hi, 0, 0, turns, turn0, lc, sops,
            [&](int j, const Lane&, int t, int) { survival[base + ic[j]] = turn0 + t; });
   // live particle-element-turns of this lane
   unsigned long long my_pet = 0;

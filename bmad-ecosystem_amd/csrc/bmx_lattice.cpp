@@ -485,7 +485,9 @@ int used_consts(const DevOp& op) {
     case OP_KICK: case OP_EKICK: case OP_EFRINGE: return 2 * (nh + 1);
     case OP_BEND_KICK: case OP_BEND_EKICK: return 14;
     case OP_KD_SLICES: return 1 + 2 * (nh + 1);
-    case OP_RFCAVITY: return (op.flags & (F_COUPLER_ENTER | F_COUPLER_EXIT)) ? 8 : 5;
+    case OP_RFCAVITY:
+      if (op.flags & (F_RF_ABS | F_RF_BACK)) return 11;
+      return (op.flags & (F_COUPLER_ENTER | F_COUPLER_EXIT)) ? 8 : 5;
     default: return 14;
   }
 }
@@ -527,6 +529,19 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
   const double beta_ref_drift = p0c / std::sqrt(p0c * p0c + mc2 * mc2);  // track_a_drift's backward beta_ref
   lc->mc2sq = mc2 * mc2;
   lc->p0c_sq = p0c * p0c;
+  // reference time of one pass (the sum of delta_ref_time, in extended
+  // precision like the reference's quad-precision particle time) and the
+  // per-call time fields at their whole-turn forward values
+  long double t_lat_ld = 0.0L;
+  for (int k = 0; k < n_ele; k++) t_lat_ld += (long double)eles[k].delta_ref_time;
+  lc->t_lat = (double)t_lat_ld;
+  lc->abs_time = cfg.absolute_time_tracking ? 1 : 0;
+  lc->n_ele = n_ele;
+  lc->tt_sign = 1;
+  lc->tt_d = 0.0;
+  lc->ix_first = 0;
+  lc->t_start = lc->t_end = 0.0;
+  lc->rt = nullptr;
   {
     double b0 = lc->beta0, me = mc2 / e_tot;
     lc->lez_k = mc2 * b0 * b0;
@@ -911,12 +926,14 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         break;
       }
       case BMX_KEY_RFCAVITY: {
-        // track_a_rfcavity.f90:17-146 (relative time, standing wave).  Its
-        // offset_particle calls keep the default set_hvkicks, so hkick and
-        // vkick act twice: as half kicks there and in the ab kicks
-        // (multipole_ele_to_ab with include_kicks, :60-66)
+        // track_a_rfcavity.f90:17-146 (standing wave).  Its offset_particle
+        // calls keep the default set_hvkicks, so hkick and vkick act twice:
+        // as half kicks there and in the ab kicks (multipole_ele_to_ab with
+        // include_kicks, :60-66).  The rf phase runs on relative time, or on
+        // absolute time with bmx_config.absolute_time_tracking
+        // (particle_rf_time.f90:72-89); backward tracking drifts backward
+        // between the kicks.
         if (e.rf_frequency == 0 && e.voltage != 0) return fail("rfcavity with zero rf_frequency");
-        if (dir == -1) return fail("rfcavity in backward tracking (absolute-time rf) is not supported on the device path");
         AB ab;
         if (!magnetic_ab(e, true, &ab, nullptr, &e2)) return fail(e2.c_str());
         scale_ab(ab, cdir);
@@ -928,10 +945,31 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         DevOp op = make_op(OP_RFCAVITY, ie);
         op.n = n_slice;
         op.c[0] = e.l;
-        op.c[1] = (e.is_on ? e.voltage : 0.0) * cdir;  // e_accel_field * charge_dir
-        op.c[2] = e.phi0 + e.phi0_multipass;
+        // e_accel_field * charge_dir, charge_dir = rel_tracking_charge x orientation (no direction)
+        op.c[1] = (e.is_on ? e.voltage : 0.0) * cdir * dir;
         op.c[3] = e.rf_frequency;
         op.c[4] = (e.rf_frequency != 0) ? 0.5 / e.rf_frequency : 0.0;
+        if (cfg.absolute_time_tracking) {
+          // time = t - ref_time_start: the lattice-frame reference time of the
+          // first track edge less ref_time_start is 0 forward and
+          // delta_ref_time backward (the particle enters at the exit end);
+          // whole lattice periods enter as frac(f t_lat) cycles each
+          op.flags |= F_RF_ABS;
+          op.c[8] = (dir == 1) ? 0.0 : e.delta_ref_time;
+          long double ft = (long double)e.rf_frequency * t_lat_ld;
+          op.c[9] = (double)(ft - floorl(ft));
+          op.c[2] = e.phi0;
+          if (e.orientation * dir == -1) {  // the backward wave: phase0 - 2 pi f dt_ref (:86-88)
+            long double fd = (long double)e.rf_frequency * ((long double)e.l / (kCLight * (long double)lc->beta0));
+            op.c[2] = (double)((long double)e.phi0 - (fd - floorl(fd)));
+          }
+        } else {
+          op.c[2] = e.phi0 + e.phi0_multipass;
+        }
+        if (dir == -1) {
+          op.flags |= F_RF_BACK;
+          op.c[10] = beta_ref_drift;
+        }
         if (e.coupler_strength != 0) {  // rf_coupler_kick.f90:41-68
           bool both = e.coupler_at == BMX_BOTH_ENDS;
           if (at_end(e.coupler_at, true)) op.flags |= F_COUPLER_ENTER;

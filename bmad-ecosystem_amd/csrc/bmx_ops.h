@@ -13,10 +13,22 @@
 // ---------------------------------------------------------------------------
 // per-lane particle state
 // ---------------------------------------------------------------------------
+// BMX_ABS_TIME 0 (runtime-compiled cell kernels of a lattice without
+// absolute-time rf) leaves out the particle's time offset.
+#ifndef BMX_ABS_TIME
+#define BMX_ABS_TIME 1
+#endif
 struct Lane {
   double x, px, y, py, z, pz, beta;
   double irp;  // 1 / (1 + pz)
   double bb;   // beta/beta0 - 1 as track_a_drift forms it (track_a_drift.f90:68-69)
+#if BMX_ABS_TIME
+  // t - t_ref + z / (beta c): the particle time t less its value on the
+  // reference relation t = t_ref(s) - z / (beta c), which every map of the
+  // path conserves (apply_energy_kick scales z with beta); zero for a
+  // particle set up by init_coord (init_coord.f90:222)
+  double dt;
+#endif
   int state;
 };
 
@@ -168,16 +180,16 @@ __device__ __forceinline__ void op_drift(Lane& L, double len) {
 
 // track_a_drift with direction = -1 (bmad/low_level/track_a_drift.f90:25-116,
 // backward branch): x and y as forward, dz = L ((-beta / beta_ref - 1) - (1 / ps - 1))
-__device__ __forceinline__ void op_drift_back(Lane& L, const DevOp& op) {
-  double len = op.c[0];
+__device__ __forceinline__ void drift_back(Lane& L, double len, double beta_ref) {
   double pxr = L.px * L.irp, pyr = L.py * L.irp;
   double pxy2 = pxr * pxr + pyr * pyr;
   if (pxy2 >= 1.0) { L.state = BMX_LOST_PZ; return; }
   double ps = sqrt(1.0 - pxy2);
   L.x += len * pxr / ps;
   L.y += len * pyr / ps;
-  L.z += len * ((-L.beta / op.c[1] - 1.0) - (1.0 / ps - 1.0));
+  L.z += len * ((-L.beta / beta_ref - 1.0) - (1.0 / ps - 1.0));
 }
+__device__ __forceinline__ void op_drift_back(Lane& L, const DevOp& op) { drift_back(L, op.c[0], op.c[1]); }
 
 // drift with include_ref_motion = false (offset_particle's drift to the edge)
 __device__ __forceinline__ void op_drift_noref(Lane& L, double len) {
@@ -882,15 +894,31 @@ __device__ __forceinline__ void op_bend_ekick(Lane& L, const DevOp& op, int nh, 
   refresh_invariants(L, lc);
 }
 
-// track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140), relative time:
-// particle_rf_time = -z/(beta c) (bmad/code/particle_rf_time.f90:89).
-// c: 0 L, 1 voltage, 2 phi0 (+ phi0_multipass), 3 rf_frequency, 4 1/(2 f),
-// 5, 6 coupler dp_x, dp_y, 7 2 pi coupler_phase; n = n_slice
-__device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, const LatticeConsts& lc) {
+// track_a_rfcavity body (bmad/low_level/track_a_rfcavity.f90:60-140).
+// particle_rf_time (bmad/code/particle_rf_time.f90:32-120): relative time
+// -z/(beta c) (:89); absolute time (F_RF_ABS) t - ref_time_start of the
+// element (:72-74), formed as c[8] + (k t_lat + r) + dt - z/(beta c) with
+// c[8] the lattice-frame reference time of the first track edge less
+// ref_time_start (0 forward, delta_ref_time backward) and the whole lattice
+// periods k t_lat taken as k frac(f t_lat) cycles (c[9]).  F_RF_BACK: the
+// drifts between the kicks are backward drifts (c[10] beta_ref).
+// c: 0 L, 1 voltage x charge_dir, 2 phi0 [+ phi0_multipass (relative) or
+// - frac(f dt_ref) (absolute, orientation x direction = -1)], 3 rf_frequency,
+// 4 1/(2 f), 5, 6 coupler dp_x, dp_y, 7 2 pi coupler_phase; n = n_slice
+__device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, const LatticeConsts& lc,
+                            const TurnTime& tt) {
   double len = op.c[0], volt = op.c[1], freq = op.c[3], half_t = op.c[4];
   double time = -L.z / (L.beta * D_C_LIGHT);
+  double kcyc = 0.0;
+#if BMX_ABS_TIME
+  if (flags & F_RF_ABS) {
+    time = (op.c[8] + tt.r + L.dt) + time;
+    kcyc = (double)tt.k * op.c[9];
+    kcyc -= rint(kcyc);
+  }
+#endif
   if (freq != 0.0 && !(time >= -half_t && time < half_t)) time = d_modulo2(time, half_t);
-  double phase = d_modulo2(D_TWOPI * (op.c[2] - time * freq), D_PI);
+  double phase = d_modulo2(D_TWOPI * (op.c[2] - time * freq - kcyc), D_PI);
   double dl = len / n_slice;
   // rf_coupler_kick (bmad/low_level/rf_coupler_kick.f90:20-106) at one edge
   auto coupler = [&]() {
@@ -913,7 +941,11 @@ __device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, co
     if (L.pz == -1.0) { L.state = BMX_LOST_PZ; return; }
     if (i != n_slice && dl != 0.0) {
       double z0 = L.z;
-      op_drift(L, dl);
+      if (flags & F_RF_BACK) {
+        drift_back(L, dl, op.c[10]);
+      } else {
+        op_drift(L, dl);
+      }
       if (L.state != BMX_ALIVE) return;
       phase = phase + D_TWOPI * freq * (L.z - z0) / (D_C_LIGHT * L.beta);
     }
@@ -1089,7 +1121,8 @@ __device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, con
 // into a switch.  The tests are grouped so that an op passes about half of
 // them.
 #define TB(t) (1u << (t))
-__device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, int n, const LatticeConsts& lc) {
+__device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int flags, int n, const LatticeConsts& lc,
+                                       const TurnTime& tt) {
   unsigned tb = (unsigned)__builtin_amdgcn_readfirstlane((int)(1u << type));
   if (tb & (TB(OP_DRIFT) | TB(OP_KICK) | TB(OP_KD_SLICES))) {
     if (tb & TB(OP_DRIFT)) op_drift(L, op.c[0]);
@@ -1102,7 +1135,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   }
   if (tb & (TB(OP_HWANG) | TB(OP_RFCAVITY))) {
     if (tb & TB(OP_HWANG)) op_hwang(L, op, flags);
-    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, n, flags, lc);
+    if (tb & TB(OP_RFCAVITY)) op_rfcavity(L, op, n, flags, lc, tt);
   }
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |

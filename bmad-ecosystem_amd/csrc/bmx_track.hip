@@ -114,6 +114,9 @@ struct bmx_lattice {
   std::vector<int32_t> eb_bwd;
   LatticeConsts lc_bwd;
   std::string bwd_err;
+  // reference times of variant 0's element boundaries (LatticeConsts::rt), host and device
+  std::vector<double> rt;
+  double* d_rt = nullptr;
 };
 
 // Launch a tracking kernel: the lattice's runtime-compiled cell kernel when
@@ -273,6 +276,19 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
       std::call_once(once, [&] { std::fprintf(stderr, "bmx: cell kernels unavailable, using the interpreter: %s\n", jerr.c_str()); });
     }
   }
+  {  // reference times of variant 0's element boundaries (particle time t, loss edges)
+    const bmx_ele_rec* e0 = eles + offs[0];
+    int n0 = offs[1] - offs[0];
+    lat->rt.resize(n0 + 1);
+    lat->rt[0] = e0[0].ref_time - e0[0].delta_ref_time;
+    for (int j = 1; j <= n0; j++) lat->rt[j] = e0[j - 1].ref_time;
+    if (hipMalloc(&lat->d_rt, lat->rt.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(lat->d_rt, lat->rt.data(), lat->rt.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      bmx_lattice_destroy(lat);
+      return set_err(ctx, "upload of the reference times failed");
+    }
+    lat->lc.rt = lat->d_rt;
+  }
   {  // backward program of variant 0: elements last to first, each with direction -1
     bmx_config cb = c;
     cb.direction = -1;
@@ -293,6 +309,7 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
         bmx_lattice_destroy(lat);
         return set_err(ctx, "upload of the backward program failed");
       }
+      lb.rt = lat->d_rt;
       lat->lc_bwd = lb;
     } else {
       lat->eb_bwd.clear();
@@ -339,6 +356,7 @@ void bmx_lattice_destroy(bmx_lattice* lat) {
   hipFree(lat->d_var_n_ele);
   if (lat->d_cell_prv) hipFree(lat->d_cell_prv);
   if (lat->d_ops_bwd) hipFree(lat->d_ops_bwd);
+  if (lat->d_rt) hipFree(lat->d_rt);
   delete lat;
 }
 
@@ -348,9 +366,9 @@ void bmx_lattice_destroy(bmx_lattice* lat) {
 // call owns; results go back and the buffers are freed in finish_particles.
 struct Staged {
   SoA s;
-  void* dp[10];
-  void* hp[10];
-  size_t by[10];
+  void* dp[11];
+  void* hp[11];
+  size_t by[11];
   std::vector<void*> owned;
   unsigned long long* d_lost = nullptr;
   bool own_lost = true;  // d_lost was allocated by the call (not the static scratch)
@@ -359,10 +377,10 @@ struct Staged {
 static int stage_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st) {
   int64_t n = p->n;
   size_t bd = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
-  void* hp[10] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost};
-  size_t by[10] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi};
+  void* hp[11] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost, p->t};
+  size_t by[11] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi, bd};
   int rc = 0;
-  for (int k = 0; k < 10; k++) {
+  for (int k = 0; k < 11; k++) {
     st.hp[k] = hp[k];
     st.by[k] = by[k];
     st.dp[k] = nullptr;
@@ -382,6 +400,7 @@ static int stage_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st) {
   st.s.x = (double*)st.dp[0]; st.s.px = (double*)st.dp[1]; st.s.y = (double*)st.dp[2]; st.s.py = (double*)st.dp[3];
   st.s.z = (double*)st.dp[4]; st.s.pz = (double*)st.dp[5]; st.s.beta = (double*)st.dp[6];
   st.s.state = (int32_t*)st.dp[7]; st.s.ix_lost = (int32_t*)st.dp[8]; st.s.turn_lost = (int32_t*)st.dp[9];
+  st.s.t = (double*)st.dp[10];
   return 0;
 }
 
@@ -389,7 +408,7 @@ static int finish_particles(bmx_ctx* ctx, const bmx_particles* p, Staged& st, hi
   unsigned long long h_lost = 0;
   (void)hipMemcpyAsync(&h_lost, st.d_lost, sizeof(h_lost), hipMemcpyDeviceToHost, ctx->stream);
   if (!p->on_device)
-    for (int k = 0; k < 10; k++)
+    for (int k = 0; k < 11; k++)
       if (st.hp[k]) (void)hipMemcpyAsync(st.hp[k], st.dp[k], st.by[k], hipMemcpyDeviceToHost, ctx->stream);
   hipError_t se = hipStreamSynchronize(ctx->stream);
   float ms = 0;
@@ -412,17 +431,18 @@ static int stage_particles_static(bmx_ctx* ctx, const bmx_particles* p, Staged& 
   int64_t n = p->n;
   unsigned char* base = (unsigned char*)ctx->scratch;
   size_t bd = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
-  void* hp[10] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost};
-  size_t by[10] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi};
+  void* hp[11] = {p->x, p->px, p->y, p->py, p->z, p->pz, p->beta, p->state, p->ix_lost, p->turn_lost, p->t};
+  size_t by[11] = {bd, bd, bd, bd, bd, bd, bd, bi, bi, bi, bd};
   const size_t N = (size_t)NOALLOC_MAX_PARTICLES;
-  for (int k = 0; k < 10; k++) {
+  for (int k = 0; k < 11; k++) {
     st.hp[k] = hp[k];
     st.by[k] = by[k];
     st.dp[k] = nullptr;
     if (!hp[k]) continue;
     if (p->on_device) { st.dp[k] = hp[k]; continue; }
     size_t off = (k < 7) ? NOALLOC_HEAD + (size_t)k * N * sizeof(double)
-                         : NOALLOC_HEAD + 8 * N * sizeof(double) + (size_t)(k - 7) * N * sizeof(int32_t);
+                 : (k == 10) ? NOALLOC_HEAD + 7 * N * sizeof(double)  // t in the eighth double slot
+                             : NOALLOC_HEAD + 8 * N * sizeof(double) + (size_t)(k - 7) * N * sizeof(int32_t);
     st.dp[k] = base + off;
     if (hipMemcpyAsync(st.dp[k], hp[k], by[k], hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
       return set_err(ctx, "upload of the particle arrays into the static scratch failed");
@@ -433,6 +453,7 @@ static int stage_particles_static(bmx_ctx* ctx, const bmx_particles* p, Staged& 
   st.s.x = (double*)st.dp[0]; st.s.px = (double*)st.dp[1]; st.s.y = (double*)st.dp[2]; st.s.py = (double*)st.dp[3];
   st.s.z = (double*)st.dp[4]; st.s.pz = (double*)st.dp[5]; st.s.beta = (double*)st.dp[6];
   st.s.state = (int32_t*)st.dp[7]; st.s.ix_lost = (int32_t*)st.dp[8]; st.s.turn_lost = (int32_t*)st.dp[9];
+  st.s.t = (double*)st.dp[10];
   return 0;
 }
 
@@ -474,6 +495,21 @@ static int32_t track_bunch_common(bmx_ctx* ctx, const bmx_lattice* lat, int32_t 
     if (ix_start > ix_end) { lo1 = eb[n_ele - ix_start]; hi1 = eb[n_ele - ix_end]; }
     else { lo1 = eb[n_ele - ix_start]; hi1 = eb[n_ele]; lo2 = eb[0]; hi2 = eb[n_ele - ix_end]; }
   }
+  // reference time of the call (LatticeConsts::tt_*): the start edge, one
+  // pass of the range (whole turns: t_lat) and the end edge of the last pass
+  LatticeConsts lc = bwd ? lat->lc_bwd : lat->lc;
+  {
+    const std::vector<double>& rt = lat->rt;
+    double T = lc.t_lat, range;
+    if (!bwd) range = rt[ix_end] - rt[ix_start] + (ix_end <= ix_start ? T : 0.0);
+    else range = rt[ix_start] - rt[ix_end] + (ix_start <= ix_end ? T : 0.0);
+    if (ix_end == ix_start) range = T;
+    lc.tt_sign = bwd ? -1 : 1;
+    lc.tt_d = range - T;
+    lc.ix_first = bwd ? ix_start - 1 : ix_start;
+    lc.t_start = rt[ix_start];
+    lc.t_end = rt[ix_start] + (double)lc.tt_sign * (double)n_turns * range;
+  }
   int64_t n = p->n;
   if (out) std::memset(out, 0, sizeof(*out));
   if (n <= 0 || n_turns == 0) return 0;
@@ -484,10 +520,10 @@ static int32_t track_bunch_common(bmx_ctx* ctx, const bmx_lattice* lat, int32_t 
   (void)hipEventRecord(ctx->ev0, ctx->stream);
   if (bwd)  // the backward program runs on the op interpreter
     launch_track(nullptr, bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block), ctx->stream, lat->d_ops_bwd, lo1,
-                 hi1, lo2, hi2, lat->lc_bwd, n_turns, st.s, n, st.d_lost);
+                 hi1, lo2, hi2, lc, n_turns, st.s, n, st.d_lost);
   else
     launch_track(lat->jit ? lat->jit->bunch : nullptr, bmx_track_bunch_kernel, dim3((unsigned)grid), dim3(block),
-                 ctx->stream, lat->d_ops, lo1, hi1, lo2, hi2, lat->lc, n_turns, st.s, n, st.d_lost);
+                 ctx->stream, lat->d_ops, lo1, hi1, lo2, hi2, lc, n_turns, st.s, n, st.d_lost);
   hipError_t le = hipGetLastError();
   (void)hipEventRecord(ctx->ev1, ctx->stream);
   return finish_particles(ctx, p, st, le, out);
@@ -521,7 +557,9 @@ int32_t bmx_track_multi(bmx_ctx* ctx, const bmx_lattice* lat, const int64_t* par
   if (out) std::memset(out, 0, sizeof(*out));
   if (p->n <= 0 || n_turns == 0) return 0;
   Staged st;
-  if (stage_particles(ctx, p, st) != 0) return -1;
+  bmx_particles q = *p;
+  q.t = nullptr;  // the reference times are variant 0's: bmx_track_multi carries no particle time
+  if (stage_particles(ctx, &q, st) != 0) return -1;
   int64_t* d_off = nullptr;
   if (hipMalloc(&d_off, (nv + 1) * sizeof(int64_t)) != hipSuccess) {
     finish_particles(ctx, p, st, hipSuccess, nullptr);
@@ -867,6 +905,7 @@ int32_t bmx_bunch_moments(bmx_ctx* ctx, const bmx_particles* p, double* moments)
   bmx_particles q = *p;
   q.ix_lost = nullptr;
   q.turn_lost = nullptr;
+  q.t = nullptr;
   if (stage_particles(ctx, &q, st) != 0) return -1;
   const int blocks = 1024;
   double *d_part = nullptr, *d_out = nullptr;

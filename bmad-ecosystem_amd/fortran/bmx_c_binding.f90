@@ -59,11 +59,13 @@ end type
 type, bind(C) :: bmx_config
   integer(c_int32_t) :: aperture_limit_on = 1, direction = 0
   real(c_double) :: max_aperture_limit = 1d3
+  integer(c_int32_t) :: absolute_time_tracking = 0, reserved = 0
 end type
 
 type, bind(C) :: bmx_particles
   type(c_ptr) :: x = c_null_ptr, px = c_null_ptr, y = c_null_ptr, py = c_null_ptr, z = c_null_ptr, pz = c_null_ptr
   type(c_ptr) :: beta = c_null_ptr, state = c_null_ptr, ix_lost = c_null_ptr, turn_lost = c_null_ptr
+  type(c_ptr) :: t = c_null_ptr
   integer(c_int64_t) :: n = 0
   integer(c_int32_t) :: on_device = 0, reserved = 0
 end type

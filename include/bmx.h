@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define BMX_ABI_VERSION 2
+#define BMX_ABI_VERSION 3
 #define BMX_N_POLE 22 /* n_pole_maxx + 1, bmad/modules/bmad_struct.f90 */
 
 /* ---- particle state codes (bmad/modules/bmad_struct.f90:520-527) ---- */
@@ -153,12 +153,22 @@ typedef struct bmx_config {
   int32_t direction;             /* bmx_lattice_check only: 0 or +1 forward, -1 backward tracking
                                     (bmx_lattice_upload ignores it and compiles both directions) */
   double max_aperture_limit;     /* default 1e3 m */
+  int32_t absolute_time_tracking; /* bmad_com%absolute_time_tracking: rf phases from the particle
+                                     time t (particle_rf_time.f90:72-74) instead of -z/(beta c);
+                                     backward tracking of rf cavities needs it, as in the reference
+                                     harness (tracking_method_test.f90:170-172) */
+  int32_t reserved;
 } bmx_config;
 
 /*
  * Particle store, structure-of-arrays.  x..pz, beta and state are required;
- * ix_lost and turn_lost may be NULL.  beta is the particle velocity / c and is
- * carried exactly as coord_struct%beta (energy kicks update it).
+ * ix_lost, turn_lost and t may be NULL.  beta is the particle velocity / c and
+ * is carried exactly as coord_struct%beta (energy kicks update it).  t is the
+ * particle time coord_struct%t [s] in the lattice's reference-time frame
+ * (ref_time of the elements); bmx_track_bunch reads it (absolute-time rf
+ * phases) and writes it back, for a lost particle at the far edge of the
+ * element where it was lost.  It is carried as t - t_ref(s) + z/(beta c),
+ * which tracking conserves, so its precision is that of the double given.
  * on_device = 1: pointers are device (HBM) addresses; 0: host memory.
  */
 typedef struct bmx_particles {
@@ -167,6 +177,7 @@ typedef struct bmx_particles {
   int32_t *state;
   int32_t *ix_lost;    /* element index (0-based) where the particle was lost, -1 if alive */
   int32_t *turn_lost;  /* turn index where the particle was lost, -1 if alive */
+  double *t;           /* particle time [s] (coord_struct%t), may be NULL */
   int64_t n;
   int32_t on_device;
   int32_t reserved;

@@ -59,9 +59,18 @@ enum { O_FIRST_EDGE = 11, O_SECOND_EDGE = 12 };
 
 /* Particle record: the coord_struct fields this path reads or writes
  * (bmad/modules/bmad_struct.f90:546-575). */
+/* The particle time is real(qp) in the reference (bmad_struct.f90:549); the
+ * plain builds carry it in long double, the counting and dual-number builds
+ * in their scalar type. */
+#if defined(ORACLE_COUNT_FLOPS) || defined(ORACLE_DUAL)
+typedef real treal;
+#else
+typedef long double treal;
+#endif
+
 typedef struct o_coord {
   real vec[6];
-  real t;
+  treal t;
   real beta;
   real p0c;
   real s;
@@ -94,6 +103,11 @@ ORACLE_API void oracle_set_tracked_species(int species) { o_tracked_species = sp
  * direction = -1 (track_bunch, bmad/multiparticle/beam_mod.f90:131-147). */
 static int o_direction = 1;
 ORACLE_API void oracle_set_direction(int direction) { o_direction = (direction == -1) ? -1 : 1; }
+/* Particle times of oracle_track_bunch: NULL = each element starts from t = 0
+ * (relative-time tracking never reads t); else t[i] is carried from element
+ * to element and turn to turn, as coord_struct%t is (absolute-time rf). */
+static double *o_times = NULL;
+ORACLE_API void oracle_set_times(double *t) { o_times = t; }
 
 /* ------------------------------------------------------------------------ */
 /* species: sim_utils/interfaces/particle_species_mod.f90:1505-1680         */
@@ -176,6 +190,12 @@ static real modulo2(real x, real amp) {
 
 /* modulo2 evaluated in quad precision, as particle_rf_time does on its real(qp)
  * time (bmad/code/particle_rf_time.f90:116-118): exact when |x| < amp. */
+#if defined(ORACLE_COUNT_FLOPS) || defined(ORACLE_DUAL)
+static real floor_t(real x) { return floor(x); }
+#else
+static long double floor_t(long double x) { return floorl(x); }
+#endif
+
 static real modulo2_qp(real x, real amp) {
   long double xl, pl, m;
   if (x >= -amp && x < amp) return x;
@@ -1957,9 +1977,21 @@ static void track_a_rfcavity(o_coord *orbit, const bmx_ele_rec *ele, int ref_spe
   voltage = (ele->is_on ? (real)(ele->voltage) : (real)(0.0)) * charge_dir;
   /* particle_rf_time */
   /* absolute_time_ref_shift = true: time relative to the element's ref_time_start */
-  if (o_com.absolute_time_tracking) time = orbit->t - ((real)(ele->ref_time) - ele->delta_ref_time);
-  else time = -orbit->vec[4] / (orbit->beta * O_C_LIGHT);
-  if (ele->rf_frequency != 0) time = modulo2_qp(time, 0.5 / (real)(ele->rf_frequency));
+  if (o_com.absolute_time_tracking) {
+    /* t - ref_time_start and its modulo2 in the time's own precision */
+    treal tl = orbit->t - ((treal)(ele->ref_time) - (treal)(ele->delta_ref_time));
+    if (ele->rf_frequency != 0) {
+      treal amp = (treal)(0.5) / (treal)(ele->rf_frequency), per = 2 * amp;
+      if (!(tl >= -amp && tl < amp)) {
+        tl = tl - per * floor_t(tl / per);
+        if (tl >= amp) tl = tl - per;
+      }
+    }
+    time = (real)(tl);
+  } else {
+    time = -orbit->vec[4] / (orbit->beta * O_C_LIGHT);
+    if (ele->rf_frequency != 0) time = modulo2_qp(time, 0.5 / (real)(ele->rf_frequency));
+  }
   /* rf_ref_time_offset = 0 for a non-slave element */
   phase0 = O_TWOPI * (ele->phi0 - time * ele->rf_frequency);
   if (!o_com.absolute_time_tracking) phase0 = phase0 + O_TWOPI * ele->phi0_multipass;
@@ -2211,11 +2243,12 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
         a.state = BMX_ALIVE;
         a.p0c = ele->p0c_start;
         a.beta = PDEP(beta[i]);
-        a.t = 0;
+        a.t = o_times ? (treal)(o_times[i]) : (treal)(0.0);
         a.s = 0;
         a.phase1 = 0;
         a.location = O_UPSTREAM;
         track1(&a, ele, ref_species, &b);
+        if (o_times) o_times[i] = RV(b.t);
         x[i] = RV(b.vec[0]); px[i] = RV(b.vec[1]); y[i] = RV(b.vec[2]); py[i] = RV(b.vec[3]);
         z[i] = RV(b.vec[4]); pz[i] = RV(b.vec[5]);
         beta[i] = RV(b.beta);

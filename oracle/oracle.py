@@ -38,6 +38,8 @@ def _setup(lib):
     lib.oracle_set_tracked_species.argtypes = [C.c_int]
     lib.oracle_set_direction.restype = None
     lib.oracle_set_direction.argtypes = [C.c_int]
+    lib.oracle_set_times.restype = None
+    lib.oracle_set_times.argtypes = [_DP]
     lib.oracle_hwang_edge.restype = None
     lib.oracle_hwang_edge.argtypes = [C.POINTER(abi.EleRec), C.c_int, C.c_int, _DP, C.c_int, C.c_int]
     lib.oracle_check_aperture.restype = C.c_int
@@ -120,12 +122,14 @@ def beta_of(pz, p0c, mc2):
 
 
 def track_bunch(eles, species, n_turns, coords, beta=None, n_threads=1, particle_outer=False, counting=False,
-                fma=False, tracked_species=0, direction=1):
+                fma=False, tracked_species=0, direction=1, t=None, absolute_time=False):
     """Track SoA coords (dict or (6, n) array) through the element array for n_turns.
     Returns dict with x..pz, beta, state, ix_lost, turn_lost (copies).
     fma=True uses the FMA-contracted build (a second FP evaluation order).
     tracked_species: the particles' species when not the reference species.
-    direction = -1: every turn runs the elements last to first, backward."""
+    direction = -1: every turn runs the elements last to first, backward.
+    t: particle times carried through (returned as out["t"]); absolute_time:
+    rf phases from them (bmad_com%absolute_time_tracking)."""
     L = lib_count() if counting else (lib_fma() if fma else lib())
     arr = eles if isinstance(eles, C.Array) else abi_array(eles)
     c = np.array(coords, dtype=np.float64, copy=True).reshape(6, -1)
@@ -142,19 +146,34 @@ def track_bunch(eles, species, n_turns, coords, beta=None, n_threads=1, particle
     ptrs = [x.ctypes.data_as(_DP) for x in cols + [beta]]
     if direction != 1 and particle_outer:
         raise ValueError("the particle-outer loop tracks forward only")
+    if (t is not None or absolute_time) and particle_outer:
+        raise ValueError("the particle-outer loop carries no particle time")
+    tt = None
+    if t is not None:
+        tt = np.ascontiguousarray(np.array(t, dtype=np.float64, copy=True).reshape(n))
+    elif absolute_time:
+        raise ValueError("absolute-time tracking needs the particle times t")
     L.oracle_set_tracked_species(tracked_species or 0)
     L.oracle_set_direction(direction)
+    L.oracle_set_times(tt.ctypes.data_as(_DP) if tt is not None else None)
+    if absolute_time:
+        L.oracle_set_com(1, 1, 1e3)
     try:
         rc = fn(arr, len(arr), species, n_turns, n, *ptrs, state.ctypes.data_as(_IP), ix_lost.ctypes.data_as(_IP),
                 turn_lost.ctypes.data_as(_IP), n_threads)
     finally:
         L.oracle_set_tracked_species(0)
         L.oracle_set_direction(1)
+        L.oracle_set_times(None)
+        if absolute_time:
+            L.oracle_set_com(1, 0, 1e3)
     if rc != 0:
         raise RuntimeError(f"oracle tracking returned {rc}")
     names = ["x", "px", "y", "py", "z", "pz"]
     out = {k: v for k, v in zip(names, cols)}
     out.update(beta=beta, state=state, ix_lost=ix_lost, turn_lost=turn_lost)
+    if tt is not None:
+        out["t"] = tt
     return out
 
 

@@ -65,11 +65,17 @@ ELEMENTS = {
     # lines 69-70
     "RFCAVITY1": dict(key="rfcavity", l=4.5, rf_frequency=500e6, voltage=1e5, phi0=0.1, num_steps=10, x_offset=0.05,
                       y_pitch=0.001, tilt=0.1),
+    # lines 71-72: cavity_type = traveling_wave changes nothing in bmad_standard rfcavity tracking
+    # (track_a_rfcavity.f90 and e_accel_field.f90 never read it)
+    "RFCAVITY2": dict(key="rfcavity", l=4.5, rf_frequency=500e6, voltage=1e5, phi0=0.1, num_steps=10, x_offset=0.05,
+                      y_pitch=0.001, tilt=0.1, coupler_at="both_ends", coupler_strength=0.1),
     # lines 73-74 (zero length)
     "RFCAVITY3": dict(key="rfcavity", rf_frequency=500e6, voltage=1e5, phi0=0.1, num_steps=10, x_offset=0.05,
                       y_pitch=0.001, tilt=0.1, a={2: 0.02}),
     # line 84
     "SBEND3": dict(key="sbend", l=0.6, k1=0.3, k2=3, num_steps=10, x_offset=0.05, y_pitch=0.001, ref_tilt=0.1, **SB0),
+    # lines 88-89: an rbend, settled into an sbend by the parser rule below (rbend_as_sbend)
+    "RBEND6": None,
     # line 91
     "SBEND7": dict(key="sbend", l=0.6, g=0.01, e1=0.1, e2=0.2, num_steps=10, x_offset=0.05, y_pitch=0.001,
                    ref_tilt=0.1, **SB0),
@@ -81,6 +87,20 @@ ELEMENTS = {
     # line 130
     "VKICKER1": dict(key="vkicker", l=4.5, kick=0.005, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
 }
+
+def rbend_as_sbend(l_chord, g, e1, e2, **kw):
+    """An rbend given by chord length and g, as bmad/parsing/bmad_parser_mod.f90:5155-5191
+    settles it (fiducial_pt none): arc angle a = 2 asin(g l_chord / 2), arc length
+    l = l_chord a / (2 sin(a / 2)), angle = g l, e1 and e2 each + angle / 2, key sbend."""
+    a = 2.0 * math.asin(0.5 * l_chord * g)
+    l = l_chord * a / (2.0 * math.sin(0.5 * a))
+    angle = g * l
+    return dict(key="sbend", l=l, g=angle / l, e1=e1 + 0.5 * angle, e2=e2 + 0.5 * angle, **kw)
+
+
+# rb0 (line 80) has sb0's attributes; line 88-89: l = 0.6 is the chord length of an rbend
+ELEMENTS["RBEND6"] = rbend_as_sbend(0.6, 0.01, 0.1, 0.2, dg=-0.0099999, num_steps=10, x_offset=0.05,
+                                    y_pitch=0.001, ref_tilt=0.1, **SB0)
 
 VARIANTS = {"": (1, 1, False), "-Anti_O": (1, -1, False), "-Anti_D": (-1, 1, False), "-Anti_OD": (-1, -1, False),
             "-Abs_Time": (1, 1, True)}

@@ -10,11 +10,17 @@ SPECIES = {"positron": abi.POSITRON, "electron": abi.ELECTRON, "antimuon": abi.A
 NAMES = ("x", "px", "y", "py", "z", "pz")
 
 
+ENDS = {"entrance_end": abi.ENTRANCE_END, "exit_end": abi.EXIT_END, "both_ends": abi.BOTH_ENDS, "no_end": abi.NO_END}
+
+
 def ele_from_spec(spec, species, e_tot=None, p0c=None):
     spec = dict(spec)
     key = spec.pop("key")
     if "fringe_type" in spec and isinstance(spec["fringe_type"], str):
         spec["fringe_type"] = FRINGE[spec["fringe_type"]]
+    for k in ("coupler_at", "fringe_at", "aperture_at"):
+        if isinstance(spec.get(k), str):
+            spec[k] = ENDS[spec[k]]
     e = lattice.ele(key, **spec)
     lattice.set_reference([e], species, e_tot=e_tot, p0c=p0c)
     return e

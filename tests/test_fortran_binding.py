@@ -42,4 +42,4 @@ def test_fortran_binding_layout(tmp_path):
     for line in out[:6]:
         a, b = line.split()
         assert a == b, out
-    assert out[6].strip() == "2"
+    assert out[6].strip() == "3"

@@ -6,9 +6,11 @@ sbend and rfcavity, which keep both: tracking_method_test.f90:158-199), and
 the backward -Anti_D / -Anti_OD variants (direction = -1, the species flipped
 where direction x orientation = -1), whose element the device path accepts,
 tracked through bmx_track_bunch and compared with the values the reference
-printed (11 significant digits).  The backward rf rows are tracked with
-absolute time there and are refused by the device path.  Elements the
-device rejects are skipped with the reason bmx_lattice_check gives."""
+printed (11 significant digits).  The backward rows and the -Abs_Time rows
+are tracked with absolute time, as the reference harness does
+(tracking_method_test.f90:170-175), from init_coord's particle time at the
+start edge (init_coord.f90:222).  Elements the device rejects are skipped
+with the reason bmx_lattice_check gives."""
 import json
 import os
 
@@ -22,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TM = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
-ROWS = [r for r in TM["rows"] if r["variant"] in ("", "-Anti_O", "-Anti_D", "-Anti_OD")]
+ROWS = TM["rows"]
 
 
 @pytest.mark.parametrize("row", ROWS, ids=[r["name"] + r["variant"] for r in ROWS])
@@ -30,19 +32,27 @@ def test_device_tracking_method_row(ctx, row):
     sp = SPECIES[TM["particle"]]
     spec = TM["elements"][row["name"]]
     e = ele_from_spec(spec, sp, e_tot=TM["e_tot"])
-    direction, osign, _ = TM["variants"][row["variant"]]
+    direction, osign, abs_time = TM["variants"][row["variant"]]
+    abs_time = abs_time or direction == -1
     if spec["key"] not in ("sbend", "rfcavity"):
         e.orientation = osign
     # the harness flips the species where direction x orientation = -1
     tracked = abi.POSITRON + abi.ELECTRON - sp if direction * e.orientation == -1 else None
-    ok, _, msg = tracker.check_lattice([e], sp, tracked_species=tracked, direction=direction)
+    ok, _, msg = tracker.check_lattice([e], sp, tracked_species=tracked, direction=direction,
+                                       absolute_time_tracking=abs_time)
     if not ok:
         pytest.skip(f"device path rejects the element: {msg}")
     n = 64  # one wave of copies of the golden start vector
     coords = np.repeat(np.asarray(TM["start"], dtype=np.float64)[:, None], n, axis=1)
-    lat = ctx.lattice([e], sp, tracked_species=tracked)
-    b = tracker.Bunch(coords, lat.p0c, abi.MASS[sp], e_tot=lat.e_tot)
-    tracker.track_bunch(lat, b, n_turns=1, direction=direction)
+    lat = ctx.lattice([e], sp, tracked_species=tracked, absolute_time_tracking=abs_time)
+    # init_coord at the start edge: the entrance forward, the exit backward
+    pc = lat.p0c * (1.0 + coords[5])
+    beta = pc / np.sqrt(pc * pc + abi.MASS[sp] ** 2)
+    t_edge = e.ref_time - (e.delta_ref_time if direction == 1 else 0.0)
+    t0 = t_edge - coords[4] / (beta * abi.C_LIGHT)
+    b = tracker.Bunch(coords, lat.p0c, abi.MASS[sp], e_tot=lat.e_tot, t=t0 if abs_time else None)
+    ix = 0 if direction == 1 else 1
+    tracker.track_bunch(lat, b, n_turns=1, direction=direction, ix_start=ix, ix_end=0)
     out = b.numpy()
     lat.close()
     assert np.all(out["state"] == abi.ALIVE)

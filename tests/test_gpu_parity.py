@@ -30,9 +30,10 @@ def _coords(n, seed, scale):
     return c
 
 
-def _gpu_track(ctx, eles, species, coords, n_turns, tracked_species=None, direction=1, ix_start=0, ix_end=0):
-    lat = ctx.lattice(eles, species, tracked_species=tracked_species)
-    b = tracker.Bunch(coords, lat.p0c, abi.MASS[species], e_tot=lat.e_tot)
+def _gpu_track(ctx, eles, species, coords, n_turns, tracked_species=None, direction=1, ix_start=0, ix_end=0, t=None,
+               absolute_time=False):
+    lat = ctx.lattice(eles, species, tracked_species=tracked_species, absolute_time_tracking=absolute_time)
+    b = tracker.Bunch(coords, lat.p0c, abi.MASS[species], e_tot=lat.e_tot, t=t)
     tracker.track_bunch(lat, b, n_turns=n_turns, ix_start=ix_start, ix_end=ix_end, direction=direction)
     out = b.numpy()
     lat.close()
@@ -41,6 +42,35 @@ def _gpu_track(ctx, eles, species, coords, n_turns, tracked_species=None, direct
 
 def _path(eles, n_turns):
     return n_turns * sum(e.l for e in eles)
+
+
+def _ref_times(eles):
+    """Reference times of the element boundaries: [entrance of element 0, exit of each element]."""
+    return np.array([eles[0].ref_time - eles[0].delta_ref_time] + [e.ref_time for e in eles])
+
+
+def _init_times(eles, coords, species, edge, jitter=0.0, seed=5):
+    """init_coord's particle time at a boundary of the element list
+    (init_coord.f90:222: t = ref_time - z / (beta c)), plus a seeded offset
+    of up to `jitter` seconds per particle."""
+    p0c = eles[0].p0c
+    beta = oracle.beta_of(coords[5], p0c, abi.MASS[species])
+    beta = np.where(coords[5] == 0, p0c / eles[0].e_tot, beta)
+    t = _ref_times(eles)[edge] - coords[4] / (beta * abi.C_LIGHT)
+    if jitter:
+        t = t + jitter * np.random.default_rng(seed).uniform(-1.0, 1.0, coords.shape[1])
+    return t
+
+
+def _compare_times(g, r, tol=1e-12):
+    """Particle times of the surviving particles: relative to the largest |t|."""
+    alive = r["state"] == abi.ALIVE
+    if not alive.any():
+        return 0.0
+    err = float(np.max(np.abs(g["t"][alive] - r["t"][alive])) / np.max(np.abs(r["t"][alive])))
+    print(f"parity: t {err:.2e}", flush=True)
+    assert err <= tol, err
+    return err
 
 
 def _compare(g, r, path_length, tol=TOL):
@@ -180,8 +210,6 @@ def test_element_parity_backward(ctx, name, variant):
     backward loop; as the reference species (orientation +1) and as the
     antiparticle through reversed elements."""
     eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=1e9)
-    if any(e.key == abi.KEY_RFCAVITY for e in eles):
-        pytest.skip("backward rf needs absolute time, which the device path refuses")
     tracked = None
     if variant == "backward_reversed_anti":
         if all(e.key == abi.KEY_SBEND for e in eles):
@@ -194,6 +222,78 @@ def test_element_parity_backward(ctx, name, variant):
     g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=tracked, direction=-1)
     r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, tracked_species=tracked or 0, direction=-1)
     _compare(g, r, _path(eles, 1))
+
+
+@pytest.mark.parametrize("name", sorted(n for n in ELEMENT_CASES if n.startswith("rfcavity")))
+@pytest.mark.parametrize("direction", [1, -1])
+def test_rf_parity_absolute_time(ctx, name, direction):
+    """Every rf cavity case with absolute-time phases (particle_rf_time.f90:
+    72-74), forward and backward, from init_coord's times at the start edge
+    plus per-particle offsets of up to 0.3 ns (a sizeable fraction of an rf
+    period), against the oracle carrying coord_struct%t: coordinates, states
+    and the particle times."""
+    eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=1e9)
+    coords = _coords(4096, 19, SCALE)
+    edge = 0 if direction == 1 else len(eles)
+    t0 = _init_times(eles, coords, abi.ELECTRON, edge, jitter=3e-10)
+    ix = 0 if direction == 1 else len(eles)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, direction=direction, ix_start=ix, ix_end=ix, t=t0,
+                   absolute_time=True)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 1, coords, direction=direction, t=t0, absolute_time=True)
+    _compare(g, r, _path(eles, 1))
+    _compare_times(g, r)
+
+
+def _spread_tol(eles, n_turns, coords, **kw):
+    """Parity bar over several turns: 4 x the oracle's own spread between two
+    FP-legal evaluation orders (plain and FMA-contracted builds) on the same
+    particles, never below the one-turn bar."""
+    from helpers import coord_errors
+    a = oracle.track_bunch(eles, abi.ELECTRON, n_turns, coords, **kw)
+    b = oracle.track_bunch(eles, abi.ELECTRON, n_turns, coords, fma=True, **kw)
+    m = (a["state"] == abi.ALIVE) & (b["state"] == abi.ALIVE)
+    spread = max(coord_errors(a, b, m, _path(eles, n_turns)).values())
+    print(f"parity: oracle FP-order spread over {n_turns} turns {spread:.2e}", flush=True)
+    return max(TOL, 4.0 * spread)
+
+
+def test_ring_absolute_time_off_harmonic(ctx):
+    """A FODO ring with two 1 MV cavities whose 500 MHz is not a harmonic of
+    the revolution frequency: in absolute time the rf phase of the reference
+    particle slips from turn to turn (whole lattice periods enter the phase as
+    k frac(f T)), which relative time would not show.  One turn at the 1e-12
+    bar; five turns forward from init_coord's times, then three backward from
+    a mid-ring element, at 4 x the oracle's own FP-order spread (relative time
+    alongside, as the control), against the oracle carrying coord_struct%t."""
+    eles = lattice.fodo_ring_c0(n_cells=8)
+    for i in (10, 38):  # two 1.0 m drifts become cavities
+        e = eles[i]
+        eles[i] = lattice.ele("rfcavity", l=e.l, voltage=1e6, rf_frequency=500e6, phi0=0.02, num_steps=5,
+                              x_limit=0.01, y_limit=0.01, aperture_type=abi.APERTURE_ELLIPTICAL)
+    lattice.set_reference(eles, abi.ELECTRON, e_tot=1e9)
+    f_t = 500e6 * lattice.revolution_time(eles)
+    assert abs(f_t - round(f_t)) > 0.05, f_t  # off harmonic
+    coords, _ = beam.c0_bunch(eles[:7], n=4096, seed=23)
+    t0 = _init_times(eles, coords, abi.ELECTRON, 0)
+    for n_turns in (1, 5):
+        kw = dict(t=t0, absolute_time=True)
+        g = _gpu_track(ctx, eles, abi.ELECTRON, coords, n_turns, **kw)
+        r = oracle.track_bunch(eles, abi.ELECTRON, n_turns, coords, **kw)
+        _compare(g, r, _path(eles, n_turns), tol=TOL if n_turns == 1 else _spread_tol(eles, n_turns, coords, **kw))
+        _compare_times(g, r)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 5)
+    r_rel = oracle.track_bunch(eles, abi.ELECTRON, 5, coords)
+    _compare(g, r_rel, _path(eles, 5), tol=_spread_tol(eles, 5, coords))
+    assert np.max(np.abs(r_rel["pz"] - r["pz"])) > 1e-6  # the slip matters
+    # backward from the exit of element 20 (Bmad index), three turns
+    ix = 20
+    tb = _init_times(eles, coords, abi.ELECTRON, ix)
+    g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 3, direction=-1, ix_start=ix, ix_end=ix, t=tb, absolute_time=True)
+    rot = eles[ix:] + eles[:ix]  # the oracle's backward loop starts at the end of its list
+    r = oracle.track_bunch(rot, abi.ELECTRON, 3, coords, direction=-1, t=tb, absolute_time=True)
+    r["ix_lost"] = np.where(r["ix_lost"] >= 0, (r["ix_lost"] + ix) % len(eles), -1)
+    _compare(g, r, _path(eles, 3), tol=_spread_tol(rot, 3, coords, direction=-1, t=tb, absolute_time=True))
+    _compare_times(g, r)
 
 
 def test_ring_backward_turns_and_element_ranges(ctx):

@@ -122,7 +122,7 @@ def test_golden_elements_the_device_accepts():
         else:
             rejected[name] = msg
     assert not rejected, rejected
-    assert len(accepted) == len(tm["elements"]) == 22
+    assert len(accepted) == len(tm["elements"]) == 24
 
 
 def test_phase_space_limits_only_on_collimators():
@@ -193,9 +193,9 @@ def test_golden_elements_accepted_reversed_for_antiparticles():
 
 
 def test_golden_elements_accepted_backward():
-    """The -Anti_D and -Anti_OD variants (direction = -1): every element but
-    the rf cavity, whose backward phase needs absolute time, compiles for
-    backward tracking."""
+    """The -Anti_D and -Anti_OD variants (direction = -1): every element,
+    rf cavities included (their phase on relative or absolute time), compiles
+    for backward tracking."""
     tm = json.load(open(os.path.join(GOLD, "tracking_method_bmad_standard.json")))
     sp = SPECIES[tm["particle"]]
     for orient in (1, -1):
@@ -204,10 +204,10 @@ def test_golden_elements_accepted_backward():
             if spec["key"] not in ("sbend", "rfcavity"):
                 e.orientation = orient
             anti = abi.POSITRON + abi.ELECTRON - sp if -e.orientation == -1 else None
-            ok, _, msg = tracker.check_lattice([e], sp, tracked_species=anti, direction=-1)
-            assert ok == (spec["key"] != "rfcavity"), (name, msg)
-            if not ok:
-                assert "rfcavity in backward tracking" in msg
+            for abs_time in (False, True):
+                ok, _, msg = tracker.check_lattice([e], sp, tracked_species=anti, direction=-1,
+                                                   absolute_time_tracking=abs_time)
+                assert ok, (name, msg)
 
 
 def test_backward_programs():

@@ -47,7 +47,8 @@ def test_tracking_method_row(row):
 
 def test_tracking_method_row_count():
     # every element the device path supports appears in all four orientation/direction variants
-    assert len(TM["rows"]) == 90
+    # 24 elements x the 4 orientation/direction variants + the -Abs_Time rows of the 3 rf cavities
+    assert len(TM["rows"]) == 99
 
 
 FR = _load("fringe_test.json")
@@ -154,18 +155,21 @@ def test_aperture_states(ename, kind, sign):
 def test_backward_bunch_tracking_matches_the_golden_rows(row):
     """The oracle's backward multi-particle loop (track_bunch with direction
     = -1, the device's parity reference for backward tracking) reproduces the
-    -Anti_D / -Anti_OD rows of tracking_method_test, tracked with relative
-    time (absolute time changes only rf phases; rf rows are left to track1)."""
+    -Anti_D / -Anti_OD rows of tracking_method_test with absolute time, as
+    the harness tracks them (tracking_method_test.f90:170-172)."""
     spec = TM["elements"][row["name"]]
-    if spec["key"] == "rfcavity":
-        pytest.skip("backward rf needs absolute time")
     ref_species = SPECIES[TM["particle"]]
     e = ele_from_spec(spec, ref_species, e_tot=TM["e_tot"])
-    if row["variant"] == "-Anti_OD" and spec["key"] != "sbend":
+    if row["variant"] == "-Anti_OD" and spec["key"] not in ("sbend", "rfcavity"):
         e.orientation = -1
     species = ref_species if -1 * e.orientation == 1 else abi.POSITRON + abi.ELECTRON - ref_species
     coords = np.array(TM["start"], dtype=np.float64)[:, None]
-    out = oracle.track_bunch([e], ref_species, 1, coords, tracked_species=species, direction=-1)
+    # init_coord at the exit end (init_coord.f90:222), absolute time as the harness sets it
+    pc = e.p0c * (1.0 + coords[5])
+    beta = pc / np.sqrt(pc * pc + abi.MASS[species] ** 2)
+    t0 = e.ref_time - coords[4] / (beta * abi.C_LIGHT)
+    out = oracle.track_bunch([e], ref_species, 1, coords, tracked_species=species, direction=-1, t=t0,
+                             absolute_time=True)
     assert out["state"][0] == abi.ALIVE
     vec = [out[k][0] for k in ("x", "px", "y", "py", "z", "pz")]
     bad = [(i, v, g) for i, (v, g) in enumerate(zip(vec, row["vec"])) if not printed_match(v, g, TM["printed_digits"])]
