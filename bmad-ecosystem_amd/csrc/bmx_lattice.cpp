@@ -563,8 +563,6 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
     if (e.p0c != p0c || e.p0c_start != p0c || e.e_tot != e_tot)
       return fail("reference energy changes along the lattice (only rfcavity-style lattices are supported)");
     if (e.orientation != 1 && e.orientation != -1) return fail("orientation must be +1 or -1");
-    if (e.orientation == -1 && e.key == BMX_KEY_SBEND)
-      return fail("sbend with orientation -1 is not supported on the device path");
     // charge_dir of the magnetic fields: orientation x direction x
     // rel_tracking_charge_to_mass; with orientation x direction = -1 the
     // particle meets the element's exit end at its first track edge

@@ -183,17 +183,13 @@ def test_element_parity(ctx, name, e_tot):
 @pytest.mark.parametrize("name", sorted(ELEMENT_CASES))
 @pytest.mark.parametrize("variant", ["reversed", "reversed_anti", "anti"])
 def test_element_parity_reversed_and_antiparticle(ctx, name, variant):
-    """Every element case with orientation -1 (sbends keep +1: the device
-    path refuses reversed bends), tracked as the reference species or as its
+    """Every element case with orientation -1 (sector bends included), tracked as the reference species or as its
     antiparticle (a positron through an electron lattice), against the
     oracle's offset_particle / fringe_here / charge_dir logic for those."""
     eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=1e9)
     if variant.startswith("reversed"):
-        if all(e.key == abi.KEY_SBEND for e in eles):
-            pytest.skip("sbend orientation -1 is refused by the device path")
         for e in eles:
-            if e.key != abi.KEY_SBEND:
-                e.orientation = -1
+            e.orientation = -1
     tracked = abi.POSITRON if variant.endswith("anti") else None
     coords = _coords(4096, 13, SCALE)
     g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=tracked)
@@ -212,11 +208,8 @@ def test_element_parity_backward(ctx, name, variant):
     eles = lattice.set_reference([lattice.EleRec_copy(e) for e in ELEMENT_CASES[name]], abi.ELECTRON, e_tot=1e9)
     tracked = None
     if variant == "backward_reversed_anti":
-        if all(e.key == abi.KEY_SBEND for e in eles):
-            pytest.skip("sbend orientation -1 is refused by the device path")
         for e in eles:
-            if e.key != abi.KEY_SBEND:
-                e.orientation = -1
+            e.orientation = -1
         tracked = abi.POSITRON
     coords = _coords(4096, 17, SCALE)
     g = _gpu_track(ctx, eles, abi.ELECTRON, coords, 1, tracked_species=tracked, direction=-1)

@@ -69,7 +69,6 @@ def test_misaligned_bend_adds_frame_changes(kw, extra):
 @pytest.mark.parametrize("e,needle", [
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, b_elec={0: 1.0, 6: 1.0}), "6 in a bend"),
     (lattice.ele("quadrupole", l=0.5, k1=0.8, num_steps=1, b_elec={0: 1.0, 7: 1.0}), "more than 7 orders"),
-    (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, orientation=-1), "sbend with orientation -1"),
     (lattice.ele("drift", l=1.0, orientation=2), "orientation must be +1 or -1"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_FULL), "fringe_type full"),
     (lattice.ele("sbend", l=1.0, angle=0.02, num_steps=1, fringe_type=abi.FRINGE_SAD_FULL), "fringe_type full"),

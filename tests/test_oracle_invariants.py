@@ -183,3 +183,41 @@ def test_oracle_refuses_unmodelled_bend_edges(ft):
     e[0].fringe_at = abi.NO_END
     r = oracle.track_bunch(e, abi.ELECTRON, 1, np.zeros((6, 4)))
     assert np.all(r["state"] == abi.ALIVE)
+
+
+@pytest.mark.parametrize("kind", ["hwang", "hard_edge", "linear_edge", "dg", "k2", "k1"])
+def test_reversed_sbend_is_the_mirrored_bend_for_the_antiparticle(kind):
+    """A sector bend turned around (orientation -1) and crossed by the
+    antiparticle meets the physical exit face first and the same field
+    relative to its charge: it equals the bend with its two faces swapped
+    (e1 <-> e2, fint <-> fintx, hgap <-> hgapx) crossed by the particle.
+    Exact for the edges (Hwang, hard edge, linear), dg and multipoles; for the
+    k1 body every coordinate but z is exact, and z differs by the constant
+    path-length term the reference signs with orientation x direction
+    (track_a_bend.f90:655), a function of pz alone."""
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal((6, 2000)) * np.array([2e-3, 2e-4, 2e-3, 2e-4, 2e-3, 2e-3])[:, None]
+    base = dict(l=1.2, angle=0.05, e1=0.03, e2=-0.02, fint=0.6, hgap=0.02, fintx=0.4, hgapx=0.03, num_steps=1)
+    extra = {"hwang": {}, "hard_edge": dict(fringe_type=abi.FRINGE_HARD_EDGE_ONLY),
+             "linear_edge": dict(fringe_type=abi.FRINGE_LINEAR_EDGE), "dg": dict(dg=0.003),
+             "k2": dict(k2=2.0, num_steps=4), "k1": dict(k1=0.3, num_steps=4)}[kind]
+    kw = dict(base, **extra)
+    mirrored = dict(kw, e1=kw["e2"], e2=kw["e1"], fint=kw["fintx"], fintx=kw["fint"], hgap=kw["hgapx"],
+                    hgapx=kw["hgap"])
+    rev = lattice.set_reference([lattice.ele("sbend", orientation=-1, **kw)], abi.ELECTRON, e_tot=1e9)
+    mir = lattice.set_reference([lattice.ele("sbend", **mirrored)], abi.ELECTRON, e_tot=1e9)
+    a = oracle.track_bunch(rev, abi.ELECTRON, 1, c, tracked_species=abi.POSITRON)
+    b = oracle.track_bunch(mir, abi.ELECTRON, 1, c)
+    assert np.array_equal(a["state"], b["state"])
+    for k in ("x", "px", "y", "py", "pz") + (("z",) if kind != "k1" else ()):
+        assert np.max(np.abs(a[k] - b[k])) <= 1e-15 * np.max(np.abs(b[k])), k
+    if kind == "k1":
+        dz = a["z"] - b["z"]
+        assert np.max(np.abs(dz)) > 1e-8  # the orientation-signed term is there
+        # the same pz with other transverse coordinates: the same z difference
+        c2 = c.copy()
+        c2[:5] = rng.standard_normal((5, c.shape[1])) * np.array([2e-3, 2e-4, 2e-3, 2e-4, 2e-3])[:, None]
+        a2 = oracle.track_bunch(rev, abi.ELECTRON, 1, c2, tracked_species=abi.POSITRON)
+        b2 = oracle.track_bunch(mir, abi.ELECTRON, 1, c2)
+        dz2 = a2["z"] - b2["z"]
+        assert np.max(np.abs(dz2 - dz)) <= 1e-12 * np.max(np.abs(dz))
