@@ -10,12 +10,15 @@
  * statement by statement, in the reference's operation order, written anew in
  * C.  Paths are relative to the bmad-ecosystem root.  Supported: forward and
  * backward direction (time_dir = +1), both element orientations, the
- * elements enumerated in include/bmx.h.  Time t is carried in double (the
- * reference uses quad precision, bmad/modules/bmad_struct.f90:549).
+ * elements enumerated in include/bmx.h.  Time t is carried in long double
+ * (the reference uses quad precision, bmad/modules/bmad_struct.f90:549).
  *
- * Pinning (see tests/test_oracle_golden.py): per-element maps by the
- * tracking_method_test Bmad_Standard rows, the Hwang edge by fringe_test,
- * apertures by aperture_test; the rest by closed forms and invariants.
+ * Pinning (tests/test_oracle_golden.py, tests/test_oracle_invariants.py):
+ * per-element maps by the tracking_method_test Bmad_Standard rows (absolute
+ * time and backward rf included), multipole_ele_to_ab by multipole_test, the
+ * Hwang edge by fringe_test, apertures by aperture_test; the linear edge,
+ * the coupler angle, reversed bends and every branch switch by closed forms,
+ * symmetries and high-precision evaluation of the other branch.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -2308,6 +2311,37 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
     state[i] = a.state;
   }
   return 0;
+}
+
+/* The special functions alone, for the branch-switch checks of the tests:
+ * which = 0 sinc, 1 cosc, 2 sincc, 3 sqrt_one. */
+ORACLE_API double oracle_special(int which, double x) {
+  switch (which) {
+    case 0: return RV(sinc((real)(x)));
+    case 1: return RV(cosc((real)(x)));
+    case 2: return RV(sincc((real)(x)));
+    case 3: return RV(sqrt_one((real)(x)));
+  }
+  return 0.0;
+}
+
+/* low_energy_z_correction alone (low_energy_z_correction.f90:22-58) for a
+ * particle of `species` at pz (beta from convert_pc_to) in an element of
+ * reference momentum p0c and energy e_tot, over the length ds. */
+ORACLE_API double oracle_low_energy_z(int species, double p0c, double e_tot, double pz, double ds) {
+  o_coord a;
+  bmx_ele_rec e;
+  real pc, mc2 = mass_of(species);
+  memset(&e, 0, sizeof(e));
+  e.p0c = p0c;
+  e.e_tot = e_tot;
+  memset(&a, 0, sizeof(a));
+  a.vec[5] = pz;
+  a.species = species;
+  a.p0c = p0c;
+  pc = (real)(p0c) * (1 + a.vec[5]);
+  a.beta = pc / sqrt(pc * pc + mc2 * mc2);
+  return RV(low_energy_z_correction(&a, &e, (real)(ds)));
 }
 
 #ifdef ORACLE_DUAL

@@ -158,9 +158,41 @@ def aperture():
                 elements={"EC": dict(key="ecollimator", **coll), "RC": dict(key="rcollimator", **coll)}, rows=out)
 
 
+def multipole():
+    """MULT rows of regression_tests/multipole_test for the lord elements
+    QQ, BB, MM: a1, a3, a5, b2, b4, b6 of multipole_ele_to_ab (magnetic), then
+    the electric call's ix_pole_max, for use_ele_tilt (first digit 2) and
+    include_kicks (second digit 2) (multipole_test.f90:34-47)."""
+    path = os.path.join(REF, "regression_tests/multipole_test/output.correct")
+    out = {}
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = re.match(r'^"MULT-([12])([12])-(QQ|BB|MM)"\s+ABS\s+(\S+)\s+(.*)$', line.strip())
+            if m:
+                vals = m.group(5).split()
+                out[f"{m.group(3)}-{m.group(1)}{m.group(2)}"] = dict(
+                    line=lineno, use_ele_tilt=m.group(1) == "2", include_kicks=m.group(2) == "2",
+                    vals=[float(v) for v in vals[:6]], ix_elec=int(vals[6]))
+    # multipole.bmad:9-26 (e_tot 1e9, electron); bb's "a5 = 5, a5 = 6" leaves a4 = 0 and a5 = 6
+    elements = {
+        "QQ": dict(key="quadrupole", l=1e-3, k1=0.1, tilt=PI / 4, scale_multipoles=0,
+                   a={0: 1, 1: 2, 2: 3, 3: 4, 4: 5, 5: 6}, b={0: 0.1, 1: 0.2, 2: 0.3, 3: 0.4, 4: 0.5, 5: 0.6},
+                   a_elec={0: 1e-2, 1: 2e-2, 2: 3e-2, 3: 4e-2, 4: 5e-2, 5: 6e-2},
+                   b_elec={0: 1e-3, 1: 2e-3, 2: 3e-3, 3: 4e-3, 4: 5e-3, 5: 6e-3}, num_steps=1),
+        "BB": dict(key="sbend", l=1e-3, g=1e-2, k1=0.1, k2=0.2, ref_tilt=PI / 2,
+                   a={0: 1, 1: 2, 2: 3, 3: 4, 5: 6}, b={0: 0.1, 1: 0.2, 2: 0.3, 3: 0.4, 4: 0.5, 5: 0.6},
+                   a_elec={0: 1e-2, 1: 2e-2, 2: 3e-2, 3: 4e-2, 4: 5e-2, 5: 6e-2},
+                   b_elec={0: 1e-3, 1: 2e-3, 2: 3e-3, 3: 4e-3, 4: 5e-3, 5: 6e-3}, num_steps=1),
+        "MM": dict(key="multipole", knl={6: 1, 1: 2, 2: 3, 3: 4, 4: 5, 5: 6},
+                   tn={0: 0.1, 1: 0.2, 2: 0.3, 3: 0.4, 4: 0.5, 5: 0.6}),
+    }
+    return dict(source="regression_tests/multipole_test/output.correct", particle="electron", e_tot=1e9,
+                printed_digits=5, elements=elements, rows=out)
+
+
 if __name__ == "__main__":
     for name, fn in (("tracking_method_bmad_standard.json", tracking_method), ("fringe_test.json", fringe),
-                     ("aperture_test.json", aperture)):
+                     ("aperture_test.json", aperture), ("multipole_test.json", multipole)):
         d = fn()
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(d, f, indent=1, sort_keys=True)

@@ -237,13 +237,13 @@ def test_rf_parity_absolute_time(ctx, name, direction):
     _compare_times(g, r)
 
 
-def _spread_tol(eles, n_turns, coords, **kw):
+def _spread_tol(eles, n_turns, coords, species=abi.ELECTRON, **kw):
     """Parity bar over several turns: 4 x the oracle's own spread between two
     FP-legal evaluation orders (plain and FMA-contracted builds) on the same
     particles, never below the one-turn bar."""
     from helpers import coord_errors
-    a = oracle.track_bunch(eles, abi.ELECTRON, n_turns, coords, **kw)
-    b = oracle.track_bunch(eles, abi.ELECTRON, n_turns, coords, fma=True, **kw)
+    a = oracle.track_bunch(eles, species, n_turns, coords, **kw)
+    b = oracle.track_bunch(eles, species, n_turns, coords, fma=True, **kw)
     m = (a["state"] == abi.ALIVE) & (b["state"] == abi.ALIVE)
     spread = max(coord_errors(a, b, m, _path(eles, n_turns)).values())
     print(f"parity: oracle FP-order spread over {n_turns} turns {spread:.2e}", flush=True)
@@ -565,3 +565,43 @@ def test_cell_kernels_match_interpreter(ctx, ring, monkeypatch):
     print("cell vs interpreter, alive:", {k: f"{v:.1e}" for k, v in errs.items()},
           "lost:", {k: f"{v:.1e}" for k, v in errs_lost.items()}, flush=True)
     assert max(errs.values()) <= TOL and max(errs_lost.values()) <= TOL
+
+
+@pytest.mark.parametrize("case", ["near_zero_orbit", "k1_small_argument", "low_energy_z", "sincc_cutoff"])
+def test_branch_switches_on_the_device(ctx, case):
+    """The device on both sides of the reference's branch switches, against
+    the oracle (whose two sides tests/test_oracle_invariants.py pins against
+    the mathematics): the sbend's near-zero-orbit map (track_a_bend.f90:
+    136-137), sbend_body_with_k1_map's small-argument branch (:546-559, with
+    its z2 sign), low_energy_z_correction's series (low_energy_z_correction.f90:
+    44, protons at 1 GeV) and the sincc cut-off (sincc.f90, bend angle per
+    step across 0.1)."""
+    species, e_tot = abi.ELECTRON, 1e9
+    if case == "near_zero_orbit":
+        variants = [lattice.ele("sbend", l=1.0, g=0.01, fringe_type=abi.FRINGE_NONE, num_steps=1)]
+        rng = np.random.default_rng(31)
+        coords = rng.uniform(-1.0, 1.0, (6, 4096)) * np.array([1e-9, 2e-9, 1e-9, 2e-9, 1e-9, 2e-9])[:, None]
+    elif case == "k1_small_argument":
+        variants = [lattice.ele("sbend", l=1.0, g=0.01, k1=-1e-4 + 1e-12 * f, fringe_type=abi.FRINGE_NONE,
+                                num_steps=1) for f in (0.999, 1.001)]
+        coords = _coords(4096, 37, SCALE)
+        coords[5] = 0.0
+    elif case == "low_energy_z":
+        species = abi.PROTON
+        variants = [lattice.ele("quadrupole", l=0.5, k1=0.3, num_steps=1)]
+        thr = 1.6347376651828182e-3
+        rng = np.random.default_rng(41)
+        coords = _coords(4096, 41, [1e-3, 1e-4, 1e-3, 1e-4, 1e-3, 0.0])
+        coords[5] = thr * rng.uniform(0.98, 1.02, 4096)
+    else:
+        variants = [lattice.ele("sbend", l=1.0, g=a, num_steps=1, fringe_type=abi.FRINGE_NONE) for a in (0.0999, 0.1001)]
+        coords = _coords(4096, 43, SCALE)
+    for e in variants:
+        eles = lattice.set_reference([e], species, e_tot=e_tot)
+        g = _gpu_track(ctx, eles, species, coords, 1)
+        r = oracle.track_bunch(eles, species, 1, coords)
+        # 1e-9 orbits: the exact body (taken by the larger of them) works in
+        # units of 1/g, so its rounding is absolute, not relative to the orbit;
+        # the bar there is the oracle's own FP-order spread
+        tol = _spread_tol(eles, 1, coords) if case == "near_zero_orbit" else TOL
+        _compare(g, r, _path(eles, 1), tol=tol)

@@ -174,3 +174,30 @@ def test_backward_bunch_tracking_matches_the_golden_rows(row):
     vec = [out[k][0] for k in ("x", "px", "y", "py", "z", "pz")]
     bad = [(i, v, g) for i, (v, g) in enumerate(zip(vec, row["vec"])) if not printed_match(v, g, TM["printed_digits"])]
     assert not bad, f"coordinates off the printed golden value: {bad}"
+
+
+MP = _load("multipole_test.json")
+
+
+@pytest.mark.parametrize("key", sorted(MP["rows"]))
+def test_multipole_ele_to_ab_rows(key):
+    """multipole_ele_to_ab against regression_tests/multipole_test (rows
+    MULT-<use_ele_tilt><include_kicks>-QQ/BB/MM, printed to 5 digits, ABS 0
+    there): a1, a3, a5, b2, b4, b6 of the magnetic poles (tilt rotation by
+    (n+1) tilt, ref_tilt for the bend; k1 L, k2 L / 2 pulled in with the
+    kicks; the bend's l (g + dg) scaling; the kt -> ab form of a multipole
+    element) and the order of the electric poles."""
+    row = MP["rows"][key]
+    name = key.split("-")[0]
+    sp = SPECIES[MP["particle"]]
+    e = ele_from_spec(MP["elements"][name], sp, e_tot=MP["e_tot"])
+    import ctypes as C
+    a, b = (C.c_double * abi.N_POLE)(), (C.c_double * abi.N_POLE)()
+    L = oracle.lib()
+    L.oracle_multipole_ele_to_ab(C.byref(e), int(row["use_ele_tilt"]), 1, int(row["include_kicks"]), a, b, None)
+    got = [a[1], a[3], a[5], b[2], b[4], b[6]]
+    for v, g in zip(got, row["vals"]):
+        assert printed_match(v, g, MP["printed_digits"]), (got, row["vals"])
+    ix = L.oracle_multipole_ele_to_ab(C.byref(e), int(row["use_ele_tilt"]), 2, int(row["include_kicks"]), a, b,
+                                      None)
+    assert ix == row["ix_elec"]
