@@ -58,39 +58,73 @@ def cpu_baseline(eles, tw, budget_s=15.0):
             "pet_per_s": n * turns * len(eles) / dt}
 
 
-DA_SPEC = dict(n_turns=512, n_angle=64, n_amp=128, n_pz=5, amp_max_x=0.01, amp_ratio_y=0.5, pz_min=-0.002,
-               pz_max=0.002, n_cells=50)
+# The DA workload: a C2-style error-seeded ring (125 cells = 1,000 elements of
+# the C1 cell pattern, 8 rf cavities, quad k1 x (1 + 1e-3 g), sextupole
+# k2 x (1 + 5e-3 g), 50 um offsets) under 32 sextupole-family knobs
+# U(+-20 %), polar grid 64 angles x 128 amplitudes (0..30 sigma_x, y at the
+# same amplitude) x 5 pz in [-2 %, +2 %], 1,024 turns, FP64.
+DA_CELLS = 125
+DA_TURNS = 1024
 
 
-def da_scan(ctx, n_train=2048, n_test=512, epochs=8):
+def da_spec():
+    from bmx import beam, lattice, surrogate
+    tw = beam.periodic_twiss(lattice.ring_c1(n_cells=DA_CELLS, n_rf=0)[:8])
+    amp = 30.0 * (10e-9 * tw[0]) ** 0.5
+    return surrogate.ScanSpec(n_turns=DA_TURNS, n_angle=64, n_amp=128, n_pz=5, amp_max_x=amp, amp_ratio_y=1.0,
+                              pz_min=-0.02, pz_max=0.02, n_cells=DA_CELLS, n_rf=8, error_seed=20261204)
+
+
+def da_scan(ctx, n_train=4096, n_test=1024, train_ics=8192, epochs=6):
     """DA lattices/hour, surrogate-screened vs plain FP64 grid tracking, on
-    unseen sextupole-knob settings of a 50-cell (400-element) ring: polar grid
-    64 angles x 128 amplitudes (0..10 mm, y at half the x amplitude) x 5 pz in
-    [-0.2 %, +0.2 %], 512 turns.  The network (4 x 256 SiLU, BF16 MFMA) is
-    trained first on a plain scan of n_train settings; that one-time cost and
-    the number of scanned lattices that repays it are reported beside the
-    rates.  The screened scan runs in bracket mode: FP64 tracking of the
-    predicted boundary and the amplitude below it on every ray, one more
-    round of probes (4 up / 3 down) or bracket fill where that does not pin
-    the first loss, then the rays still open take the boundary of their
-    resolved neighbours in angle.  It resolves the lattices in two chunks on
-    two streams, so that the rounds of one overlap the other's work."""
+    n_test unseen knob settings of the DA workload above.  The network (4 x
+    256 SiLU, BF16 MFMA, FP32 accumulation) is trained first on FP64 tracking
+    of train_ics random grid ICs of each of n_train other settings; that
+    one-time cost and the number of scanned lattices that repays it are
+    reported beside the rates.  Headline (band mode): every IC within +-10 %
+    of the predicted boundary amplitude is re-tracked in FP64, in one round,
+    and decides the ray, the network standing for the untracked amplitudes.
+    Second figure (bracket mode, labelled): two FP64 rounds around the
+    predicted boundary, rays still open from resolved neighbours."""
     from bmx import surrogate
-    spec = surrogate.ScanSpec(**DA_SPEC)
-    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs, workers=2, window=(1, 0),
-                                probe_up=4, probe_down=3, max_rounds=2)
-    keep = ("plain_lattices_per_hour", "surrogate_lattices_per_hour", "speedup", "da_area_err_mean",
-            "da_area_err_max", "frac_within_2pct", "rays_identical", "rays_open",
-            "da_area_err_all_slices_frac_within_2pct", "da_area_err_all_slices_abs_max_rel_mean_area",
-            "ics_tracked_plain", "ics_tracked_surrogate",
-            "pet_plain", "pet_surrogate", "plain_s", "surrogate_s", "train_scan_s", "train_s", "break_even_lattices",
-            "pred_boundary_within_band", "mlp_forward_tflops", "mlp_train_step_tflops", "n_train", "n_test")
+    spec = da_spec()
+    r = surrogate.compare_scans(ctx, spec, n_train=n_train, n_test=n_test, epochs=epochs, workers=2,
+                                train_ics=train_ics, modes=[surrogate.BAND_MODE, surrogate.BRACKET_MODE],
+                                scan_chunk=256)
+    band, bracket = r["modes"]
+    keep = ("plain_lattices_per_hour", "plain_s", "ics_tracked_plain", "pet_plain", "n_train", "n_test",
+            "train_ics_per_lattice", "train_samples", "train_scan_s", "train_data_s", "train_s", "loss_first",
+            "loss_last", "mlp_forward_tflops", "mlp_train_step_tflops")
     out = {k: r[k] for k in keep}
-    out["rounds"] = r["rounds"]
-    out["workload"] = ("50-cell ring (400 elements), 32 sextupole-family knobs U(+-20%), grid 64 x 128 x 5 "
-                       "(40,960 ICs/lattice), 512 turns, FP64; DA area = on-momentum slice; bracket mode: "
-                       "2 FP64 rounds around the predicted boundary, open rays from resolved neighbours")
+    out["band"] = band
+    out["bracket"] = {k: bracket[k] for k in bracket if k != "rounds"}
+    out["surrogate_lattices_per_hour"] = band["surrogate_lattices_per_hour"]
+    out["speedup"] = band["speedup"]
+    out["da_area_err_mean"] = band["da_area_err_mean"]
+    out["da_area_err_max"] = band["da_area_err_max"]
+    out["method"] = ("band: every IC within +-10 % of the predicted boundary amplitude re-tracked in FP64 in one "
+                     "round, the network standing for the untracked amplitudes; no neighbour estimate")
+    out["workload"] = (f"error-seeded {8 * DA_CELLS}-element ring with 8 rf cavities, 32 sextupole-family knobs "
+                       f"U(+-20%), grid 64 x 128 x 5 (0..{spec.amp_max_x * 1e3:.2f} mm = 30 sigma_x, pz -2..2 %), "
+                       f"{DA_TURNS} turns, FP64; training set {n_train} lattices x {train_ics} ICs "
+                       f"(C3 asks 20,000 x 8,192)")
     return out
+
+
+def opt_search(ctx, n_gen, n_init=2048):
+    """The surrogate-guided sextupole optimisation (bmx.optimize) on the DA
+    workload tracked for 2,048 turns: 4,096 candidates per generation
+    screened by the network, the top 5 % tracked in FP64 on the full grid,
+    the network retrained every 5 generations; then the plain FP64-only
+    search with the same wall time.  n_gen = 20 is the full loop."""
+    from bmx import optimize
+    spec = da_spec()
+    spec.n_turns = 2 * DA_TURNS
+    r = optimize.compare_search(ctx, spec, n_gen=n_gen, n_init=n_init)
+    r["workload"] = (f"error-seeded {8 * DA_CELLS}-element ring with 8 rf cavities, 32 sextupole-family knobs, "
+                     f"{n_gen} generations x 4096 candidates, top 5 % tracked in FP64 on the 64 x 128 x 5 grid for "
+                     f"{spec.n_turns} turns, retraining every 5 generations; plain search: FP64 only, same wall time")
+    return r
 
 
 def error_study_grid(ctx, n_var):
@@ -153,6 +187,15 @@ def main():
                     help="only run a plain 512 MiB device copy (reference for HBM byte counters) and exit")
     ap.add_argument("--da-only", action="store_true",
                     help="only run the surrogate-vs-plain DA scan comparison and print its record")
+    ap.add_argument("--da-train", type=int, default=4096, help="training lattices of the DA comparison")
+    ap.add_argument("--da-test", type=int, default=1024, help="test lattices of the DA comparison")
+    ap.add_argument("--da-epochs", type=int, default=6, help="training epochs of the DA comparison")
+    ap.add_argument("--opt-init", type=int, default=2048,
+                    help="training lattices (x 8,192 ICs) of the optimisation's initial network")
+    ap.add_argument("--opt-gens", type=int, default=3,
+                    help="generations of the surrogate-guided optimisation (20: the full loop; 0: skip it)")
+    ap.add_argument("--opt-only", action="store_true",
+                    help="only run the surrogate-guided optimisation and print its record")
     ap.add_argument("--error-study-sample", type=int, default=0, metavar="N_VAR",
                     help="only run the batched-DA error study on its first N_VAR variants and print its record")
     args = ap.parse_args()
@@ -175,7 +218,11 @@ def main():
         return
 
     if args.da_only:
-        print(json.dumps({"da_scan": da_scan(tracker.Context(0))}), flush=True)
+        print(json.dumps({"da_scan": da_scan(tracker.Context(0), n_train=args.da_train, n_test=args.da_test,
+                                             epochs=args.da_epochs)}), flush=True)
+        return
+    if args.opt_only:
+        print(json.dumps({"opt_search": opt_search(tracker.Context(0), args.opt_gens, args.opt_init)}), flush=True)
         return
     if args.error_study_sample > 0:
         print(json.dumps({"error_study": error_study_grid(tracker.Context(0), args.error_study_sample)}), flush=True)
@@ -239,7 +286,9 @@ def main():
     }
     lat.close()
     if not args.no_da:
-        result["da_scan"] = da_scan(ctx)
+        result["da_scan"] = da_scan(ctx, n_train=args.da_train, n_test=args.da_test, epochs=args.da_epochs)
+        if args.opt_gens > 0:
+            result["opt_search"] = opt_search(ctx, args.opt_gens, 512)
     ctx.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(eles, tw)

@@ -32,6 +32,7 @@ FRINGE_NONE, FRINGE_SOFT_EDGE_ONLY, FRINGE_HARD_EDGE_ONLY, FRINGE_FULL, FRINGE_S
 ENTRANCE_END, EXIT_END, BOTH_ENDS, NO_END = 1, 2, 3, 4
 APERTURE_RECTANGULAR, APERTURE_ELLIPTICAL = 2, 3
 ELECTRON, POSITRON, PROTON, ANTIPROTON, MUON, ANTIMUON = 1, 2, 3, 4, 5, 6
+ATTR_K1, ATTR_K2, ATTR_K3 = 1, 2, 3
 
 # physical constants (sim_utils/interfaces/physical_constants.f90:18-34)
 C_LIGHT = 2.99792458e8
@@ -96,7 +97,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 # every entry point declared in include/bmx.h
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
-    "bmx_lattice_upload", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch",
+    "bmx_lattice_upload", "bmx_lattice_upload_scaled", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch",
     "bmx_track_bunch_noalloc", "bmx_track_multi", "bmx_track_grid",
     "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_one_turn_matrix", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
@@ -127,6 +128,10 @@ def load_library(path=None):
     lib.bmx_lattice_upload.restype = C.c_int32
     lib.bmx_lattice_upload.argtypes = [C.c_void_p, C.POINTER(EleRec), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
                                        C.POINTER(Ref), C.POINTER(Config), C.POINTER(C.c_void_p)]
+    lib.bmx_lattice_upload_scaled.restype = C.c_int32
+    lib.bmx_lattice_upload_scaled.argtypes = [C.c_void_p, C.POINTER(EleRec), C.c_int32, C.c_int32, C.c_int32,
+                                              C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double),
+                                              C.POINTER(Ref), C.POINTER(Config), C.POINTER(C.c_void_p)]
     lib.bmx_lattice_destroy.restype = None
     lib.bmx_lattice_destroy.argtypes = [C.c_void_p]
     lib.bmx_lattice_check.restype = C.c_int32

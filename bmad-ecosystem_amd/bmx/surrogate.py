@@ -230,13 +230,16 @@ KNOB_RANGE = 0.2
 class ScanSpec:
     """Polar grid of a DA scan (the bmx_track_grid layout): n_angle angles in
     [0, pi], n_amp amplitudes amp_max_x (k + 1) / n_amp, n_pz momentum offsets
-    in [pz_min, pz_max], n_turns turns."""
+    in [pz_min, pz_max], n_turns turns; the ring: n_cells C1 cells with n_rf
+    rf cavities, and with error_seed the seeded errors of the batched-DA
+    variants (quad k1 x (1 + 1e-3 g), sextupole k2 x (1 + 5e-3 g), 50 um
+    quad and sextupole offsets) under the knobs."""
 
     def __init__(self, n_turns=256, n_angle=16, n_amp=32, n_pz=3, amp_max_x=0.02, amp_ratio_y=0.5, pz_min=-0.01,
-                 pz_max=0.01, n_cells=50):
+                 pz_max=0.01, n_cells=50, n_rf=0, error_seed=None):
         self.n_turns, self.n_angle, self.n_amp, self.n_pz = n_turns, n_angle, n_amp, n_pz
         self.amp_max_x, self.amp_ratio_y, self.pz_min, self.pz_max = amp_max_x, amp_ratio_y, pz_min, pz_max
-        self.n_cells = n_cells
+        self.n_cells, self.n_rf, self.error_seed = n_cells, n_rf, error_seed
 
     @property
     def n_ic(self):
@@ -260,36 +263,65 @@ class ScanSpec:
         return out
 
 
-def knob_lattice(knobs, spec, n_rf=0):
-    """C1-cell ring of spec.n_cells cells whose sextupole i has k2 x (1 + knobs[i % 32])."""
+def knob_lattice(knobs, spec, n_rf=None):
+    """C1-cell ring of spec.n_cells cells (spec.n_rf cavities, the seeded
+    errors of spec.error_seed) whose sextupole i has its k2 multiplied by
+    (1 + knobs[i % 32])."""
+    from . import abi
     from . import lattice as latmod
-    ns = 2 * spec.n_cells
-    k2e = np.array([knobs[i % N_KNOBS] for i in range(ns)])
-    return latmod.ring_c1(n_cells=spec.n_cells, n_rf=n_rf, k2_err=k2e)
+    ns = nq = 2 * spec.n_cells
+    n_rf = spec.n_rf if n_rf is None else n_rf
+    if spec.error_seed is None:
+        eles = latmod.ring_c1(n_cells=spec.n_cells, n_rf=n_rf)
+    else:
+        rng = np.random.default_rng(spec.error_seed)
+        k1e = 1e-3 * rng.standard_normal(nq)
+        k2e = 5e-3 * rng.standard_normal(ns)
+        offs = {}
+        for i in range(nq):
+            offs[("q", i)] = tuple(50e-6 * rng.standard_normal(2))
+        for i in range(ns):
+            offs[("s", i)] = tuple(50e-6 * rng.standard_normal(2))
+        eles = latmod.ring_c1(n_cells=spec.n_cells, n_rf=n_rf, k1_err=k1e, k2_err=k2e, offsets=offs,
+                              seed=spec.error_seed + 1)
+    i = 0
+    for e in eles:
+        if e.key == abi.KEY_SEXTUPOLE:
+            e.k2 = float(np.float64(e.k2) * (1.0 + np.float64(knobs[i % N_KNOBS])))
+            i += 1
+    return eles
 
 
 def upload_variants(ctx, knob_sets, spec):
-    """One lattice upload holding a knob_lattice per knob set.  The element
-    records of the nominal ring are replicated with numpy and only the
-    sextupole k2 fields are rewritten (k2 = k2_nominal x (1 + knob), the same
-    product knob_lattice forms)."""
+    """One lattice upload holding a knob_lattice per knob set: the nominal
+    ring once, its sextupole k2 scaled per set by (1 + knob) in the library
+    (bmx_lattice_upload_scaled: the same product knob_lattice forms)."""
+    from . import abi, tracker
+    knob_sets = np.asarray(knob_sets, dtype=np.float64)
+    base = knob_lattice(np.zeros(N_KNOBS), spec)
+    sext = np.array([i for i, e in enumerate(base) if e.key == abi.KEY_SEXTUPOLE], dtype=np.int32)
+    fam = np.arange(len(sext)) % N_KNOBS
+    factors = 1.0 + knob_sets[:, fam]
+    if ctx is None or not hasattr(ctx, "h"):  # a stand-in context (tests): the records themselves
+        return _upload_records(ctx, base, sext, factors)
+    return tracker.Lattice.scaled(ctx, base, abi.ELECTRON, abi.ATTR_K2, sext, factors)
+
+
+def _upload_records(ctx, base, sext, factors):
+    """The scaled variants as one list of element records, uploaded through
+    ctx.lattice (the records bmx_lattice_upload_scaled forms)."""
     import ctypes as Cc
     from . import abi
     from . import lattice as latmod
-    knob_sets = np.asarray(knob_sets, dtype=np.float64)
-    base = knob_lattice(np.zeros(N_KNOBS), spec)
     arr0 = latmod.to_array(base)
     size = Cc.sizeof(abi.EleRec)
     n_ele = len(base)
     rec = np.frombuffer(arr0, dtype=np.uint8).reshape(n_ele, size)
-    sext = np.array([i for i, e in enumerate(base) if e.key == abi.KEY_SEXTUPOLE])
-    k2_nom = np.array([base[i].k2 for i in sext])
-    fam = np.arange(len(sext)) % N_KNOBS
-    n_var = len(knob_sets)
+    n_var = len(factors)
     big = np.ascontiguousarray(np.tile(rec, (n_var, 1)))
     k2_off = abi.EleRec.k2.offset
     k2 = big[:, k2_off:k2_off + 8].copy().view(np.float64).reshape(n_var, n_ele)
-    k2[:, sext] = k2_nom[None, :] * (1.0 + knob_sets[:, fam])
+    k2[:, sext] = k2[:, sext] * factors
     big[:, k2_off:k2_off + 8] = k2.reshape(-1, 1).view(np.uint8)
     arr = (abi.EleRec * (n_var * n_ele)).from_buffer(big)
     offs = [v * n_ele for v in range(n_var + 1)]
@@ -344,6 +376,47 @@ def plain_scan(ctx, knob_sets, spec, turns_per_block=0):
     lat.close()
     area = np.array([slice_area(s[spec.n_pz // 2], spec) for s in surv])
     return surv, area, dict(total=t2 - t0, upload=t1 - t0, track=t2 - t1, pet=st["pet"])
+
+
+def training_data(ctx, knob_sets, spec, n_ic, seed=0, chunk=512, gamma=TARGET_GAMMA):
+    """Training set of the surrogate from FP64 tracking: for every knob set,
+    n_ic grid initial conditions drawn at random (without replacement) from
+    the spec's grid, tracked for spec.n_turns turns (bmx_track_ics, chunk
+    lattices per launch).  Returns (X [n_lat * n_ic, 35] float32 features,
+    Y [n_lat * n_ic] float32 targets, timing dict total / track seconds and
+    the live particle-element-turns)."""
+    import time
+    from . import tracker
+    knob_sets = np.asarray(knob_sets, dtype=np.float64)
+    n_lat = len(knob_sets)
+    rng = np.random.default_rng(seed)
+    off = spec.offsets().reshape(-1, 3)
+    geo = geometry(spec)
+    X = np.empty((n_lat, n_ic, N_KNOBS + 3), dtype=np.float32)
+    Y = np.empty((n_lat, n_ic), dtype=np.float32)
+    t = dict(total=0.0, track=0.0, pet=0.0)
+    for a in range(0, n_lat, chunk):
+        b = min(n_lat, a + chunk)
+        t0 = time.perf_counter()
+        lat = upload_variants(ctx, knob_sets[a:b], spec)
+        picks = [rng.choice(spec.n_ic, size=n_ic, replace=False) for _ in range(b - a)]
+        per = []
+        for idx in picks:
+            c = np.zeros((6, n_ic))
+            c[0], c[2], c[5] = off[idx, 0], off[idx, 1], off[idx, 2]
+            per.append(c)
+        t1 = time.perf_counter()
+        surv, st = tracker.track_ics(lat, per, spec.n_turns, return_stats=True)
+        t2 = time.perf_counter()
+        lat.close()
+        for j, (idx, sv) in enumerate(zip(picks, surv)):
+            X[a + j, :, :N_KNOBS] = knob_sets[a + j] / KNOB_RANGE
+            X[a + j, :, N_KNOBS:] = geo[idx]
+            Y[a + j] = target(sv, spec.n_turns, gamma)
+        t["total"] += t2 - t0
+        t["track"] += t2 - t1
+        t["pet"] += st["pet"]
+    return X.reshape(-1, N_KNOBS + 3), Y.reshape(-1), t
 
 
 def slice_area(surv_slice, spec):
@@ -641,6 +714,13 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
     bounds the scan time; the rays left open are a few per cent, scattered,
     and their neighbours in angle carry the boundary.
 
+    Band mode (window None, max_rounds = 1): the one FP64 round of the band
+    decides every ray, with the network standing for the untracked
+    amplitudes: below the band they survive, so the surviving prefix ends at
+    the first tracked loss; a band without a loss puts the boundary just
+    above it (the lowest amplitude the FP64 round leaves open), capped at the
+    grid's edge.
+
     workers > 1 splits the lattices into that many chunks resolved
     concurrently, each on its own context (HIP stream), so the
     latency-bound late rounds of one chunk overlap the others' work; their
@@ -718,8 +798,12 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
     a_top = np.concatenate([r[1] for r in results], 0)
     n_tracked = sum(r[2] for r in results)
     rounds = [x for r in results for x in r[3]]
-    runs = neighbour_estimate(bnd, a_top, kb)
     n_open = int(np.sum(a_top != bnd - 1))
+    if window is None and max_rounds == 1:  # band mode
+        hi = np.concatenate([boxes[w]["hi"].reshape(bnd[bounds[w]:bounds[w + 1]].shape) for w in range(workers)], 0)
+        runs = np.where(bnd < spec.n_amp, bnd, np.minimum(hi + 1, spec.n_amp))
+    else:
+        runs = neighbour_estimate(bnd, a_top, kb)
     area = np.array([area_from_runs(runs[v, spec.n_pz // 2], spec) for v in range(n_lat)])
     t1 = time.perf_counter()
     timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb, n_open=n_open)
@@ -757,72 +841,126 @@ def inference_throughput(net, n=1 << 18, reps=10):
             "forward_ms": fwd * 1e3, "train_step_ms": trn * 1e3, "batch": n}
 
 
+def area_errors(runs, plain_runs, spec):
+    """DA-area errors of screened runs [n_lat, n_pz, n_angle] against plain
+    ones: of the on-momentum slice (relative), of every pz slice (relative
+    where the plain slice has area; absolute in units of the mean
+    on-momentum area everywhere) and of the areas summed over the slices."""
+    n_lat = len(runs)
+    a_s = np.array([[area_from_runs(runs[v, p], spec) for p in range(spec.n_pz)] for v in range(n_lat)])
+    a_p = np.array([[area_from_runs(plain_runs[v, p], spec) for p in range(spec.n_pz)] for v in range(n_lat)])
+    mid = spec.n_pz // 2
+    # relative errors where the plain area is not zero; lattices with no
+    # on-momentum aperture at all are counted apart
+    nz = a_p[:, mid] > 0
+    err = np.abs(a_s[nz, mid] - a_p[nz, mid]) / a_p[nz, mid]
+    if err.size == 0:
+        err = np.zeros(1)
+    has = a_p > 0
+    err_all = np.abs(a_s - a_p)[has] / a_p[has]
+    tot_s, tot_p = a_s.sum(1), a_p.sum(1)
+    err_tot = np.abs(tot_s - tot_p) / np.maximum(tot_p, 1e-300)
+    return {
+        "da_area_err_mean": float(err.mean()), "da_area_err_max": float(err.max()),
+        "frac_within_2pct": float(np.mean(err <= 0.02)),
+        "da_area_err_all_slices_mean": float(err_all.mean()) if err_all.size else 0.0,
+        "da_area_err_all_slices_frac_within_2pct": float(np.mean(err_all <= 0.02)) if err_all.size else 1.0,
+        "da_area_err_all_slices_abs_max_rel_mean_area": float(np.abs(a_s - a_p).max() / np.mean(a_p[:, mid])),
+        "da_area_err_summed_slices_mean": float(err_tot.mean()), "da_area_err_summed_slices_max": float(err_tot.max()),
+        "rays_identical": float(np.mean(runs == plain_runs)),
+        "area_plain_mean_mm2": float(a_p[:, mid].mean() * 1e6),
+        "n_zero_area_plain": int(np.sum(~nz)), "zero_area_found": int(np.sum(~nz & (a_s[:, mid] == 0))),
+    }
+
+
+BAND_MODE = dict(window=None, max_rounds=1)
+BRACKET_MODE = dict(window=(1, 0), probe_up=4, probe_down=3, max_rounds=2)
+
+
 def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed=20261205, band=0.10,
                   gamma=TARGET_GAMMA, workers=1, refine=6, window=None, probe_up=4, probe_down=3, max_rounds=64,
-                  scan_chunk=512):
-    """Surrogate-screened vs plain DA scans of n_test unseen knob settings:
-    trains a fresh 4 x 256 network on a plain scan of n_train settings (in
-    chunks of scan_chunk lattices), then scans the test settings both ways
-    (surrogate_scan's exact mode, or its bracket mode with window, probe_up,
-    probe_down and max_rounds).  Returns a dict of timings, lattices/hour
-    (lattice upload + tracking for plain; upload + inference + FP64
-    re-tracking for the surrogate), FP64-tracked IC counts, the rays left to
-    the neighbour estimate and the DA-area error statistics: of the on-momentum
-    slice (the DA area) and of every pz slice."""
+                  scan_chunk=512, train_ics=None, modes=None, lr=2e-3):
+    """Surrogate-screened vs plain DA scans of n_test unseen knob settings.
+    Trains a fresh 4 x 256 network on FP64 tracking of n_train settings:
+    train_ics random grid ICs per setting (training_data; None: the whole
+    grid, as plain scans in chunks of scan_chunk lattices).  Then scans the
+    test settings plainly and with the network, once per entry of `modes`
+    (dicts of surrogate_scan arguments; None: one mode from the window /
+    max_rounds / probe arguments), on the same network and lattices.
+    Returns a dict of timings, lattices/hour (lattice upload + tracking for
+    plain; upload + inference + FP64 re-tracking for the surrogate),
+    FP64-tracked IC counts and the DA-area errors (area_errors); with several
+    modes their records are under "modes" and the first one's are also at
+    the top level."""
+    import sys
     import time
     rng = np.random.default_rng(seed)
     k_train = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_train, N_KNOBS))
     k_test = rng.uniform(-KNOB_RANGE, KNOB_RANGE, (n_test, N_KNOBS))
-    Y, t_gen = [], dict(total=0.0, track=0.0)
-    for a in range(0, n_train, scan_chunk):
-        s_tr, _, t = plain_scan(ctx, k_train[a:a + scan_chunk], spec)
-        Y.append(target(s_tr.reshape(-1), spec.n_turns, gamma))
-        t_gen["total"] += t["total"]
-        t_gen["track"] += t["track"]
-        del s_tr
-    Y = np.concatenate(Y)
+
+    def say(msg):
+        print(f"da comparison: {msg}", file=sys.stderr, flush=True)
+
+    say(f"training data: {n_train} lattices x {train_ics or spec.n_ic} ICs, {spec.n_turns} turns")
+    if train_ics is None:
+        Y, t_gen = [], dict(total=0.0, track=0.0)
+        for a in range(0, n_train, scan_chunk):
+            s_tr, _, t = plain_scan(ctx, k_train[a:a + scan_chunk], spec)
+            Y.append(target(s_tr.reshape(-1), spec.n_turns, gamma))
+            t_gen["total"] += t["total"]
+            t_gen["track"] += t["track"]
+            del s_tr
+        Y = np.concatenate(Y)
+        X = features(k_train, spec)
+        n_ic_train = spec.n_ic
+    else:
+        X, Y, t_gen = training_data(ctx, k_train, spec, train_ics, seed=seed + 1, chunk=scan_chunk, gamma=gamma)
+        n_ic_train = train_ics
+    say(f"training data done in {t_gen['total']:.1f} s; training {epochs} epochs on {len(Y)} samples")
     t0 = time.perf_counter()
     net = MLP(ctx, [N_KNOBS + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
-    X = features(k_train, spec)
-    losses = train(net, X, Y, epochs=epochs, batch=batch)
+    losses = train(net, X, Y, epochs=epochs, batch=batch, lr=lr)
     del X, Y
     t_train = time.perf_counter() - t0
+    say(f"trained in {t_train:.1f} s (loss {losses[0]:.4g} -> {losses[-1]:.4g}); plain scan of {n_test} lattices")
     s_te, a_plain, t_plain = plain_scan(ctx, k_test, spec)
     plain_runs = np.cumprod(s_te >= spec.n_turns, axis=-1).sum(-1)
     del s_te
-    runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band, workers=workers, refine=refine,
-                                                   window=window, probe_up=probe_up, probe_down=probe_down,
-                                                   max_rounds=max_rounds)
-    err = np.abs(a_sur - a_plain) / np.maximum(a_plain, 1e-300)
-    # every pz slice: relative area error where the slice has area, absolute (in units of the mean
-    # on-momentum area) everywhere
-    a_all_p = np.array([[area_from_runs(plain_runs[v, p], spec) for p in range(spec.n_pz)] for v in range(n_test)])
-    a_all_s = np.array([[area_from_runs(runs[v, p], spec) for p in range(spec.n_pz)] for v in range(n_test)])
-    has = a_all_p > 0
-    err_all = np.abs(a_all_s - a_all_p)[has] / a_all_p[has]
+    if modes is None:
+        modes = [dict(window=window, probe_up=probe_up, probe_down=probe_down, max_rounds=max_rounds)]
     plain_s = t_plain["total"]
-    sur_s = t_sur["total"]
-    per_lat_gain = plain_s / n_test - sur_s / n_test
+    out_modes = []
+    say(f"plain scan done in {t_plain['total']:.1f} s")
+    for m in modes:
+        runs, a_sur, t_sur, n_tracked = surrogate_scan(ctx, net, k_test, spec, band=band, workers=workers,
+                                                       refine=refine, **m)
+        say(f"surrogate scan {m} done in {t_sur['total']:.2f} s")
+        sur_s = t_sur["total"]
+        per_lat_gain = plain_s / n_test - sur_s / n_test
+        rec = {
+            "surrogate_s": sur_s, "surrogate_lattices_per_hour": n_test / sur_s * 3600, "speedup": plain_s / sur_s,
+            "ics_tracked_surrogate": int(n_tracked), "rays_open": t_sur["n_open"] / runs.size,
+            "pred_boundary_err_mean": float(np.mean(np.abs(t_sur["kb"] - plain_runs))),
+            "pred_boundary_exact": float(np.mean(t_sur["kb"] == plain_runs)),
+            "pred_boundary_err_q90": float(np.quantile(np.abs(t_sur["kb"] - plain_runs), 0.9)),
+            "pred_boundary_within_band": float(np.mean(np.abs(t_sur["kb"] - plain_runs)
+                                                       <= np.maximum(1, band * plain_runs))),
+            "rounds": [[int(n), round(t, 4), float(p)] for n, t, p in t_sur["rounds"]],
+            "predict_s": t_sur["predict"], "pet_surrogate": float(sum(p for _, _, p in t_sur["rounds"])),
+            "break_even_lattices": (t_gen["total"] + t_train) / per_lat_gain if per_lat_gain > 0 else None,
+        }
+        rec.update(area_errors(runs, plain_runs, spec))
+        out_modes.append(rec)
     inf = inference_throughput(net)
-    return {
-        "n_train": n_train, "n_test": n_test, "ics_per_lattice": spec.n_ic, "train_samples": n_train * spec.n_ic,
-        "train_scan_s": t_gen["track"], "train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
-        "plain_s": plain_s, "surrogate_s": sur_s,
-        "plain_lattices_per_hour": n_test / plain_s * 3600, "surrogate_lattices_per_hour": n_test / sur_s * 3600,
-        "speedup": plain_s / sur_s,
-        "ics_tracked_plain": n_test * spec.n_ic, "ics_tracked_surrogate": int(n_tracked),
-        "rays_identical": float(np.mean(runs == plain_runs)),
-        "rays_open": t_sur["n_open"] / runs.size,
-        "da_area_err_all_slices_frac_within_2pct": float(np.mean(err_all <= 0.02)),
-        "da_area_err_all_slices_abs_max_rel_mean_area": float(np.abs(a_all_s - a_all_p).max() / np.mean(a_plain)),
-        "pred_boundary_err_mean": float(np.mean(np.abs(t_sur["kb"] - plain_runs))),
-        "pred_boundary_exact": float(np.mean(t_sur["kb"] == plain_runs)),
-        "pred_boundary_err_q90": float(np.quantile(np.abs(t_sur["kb"] - plain_runs), 0.9)),
-        "pred_boundary_within_band": float(np.mean(np.abs(t_sur["kb"] - plain_runs) <= np.maximum(1, band * plain_runs))),
-        "da_area_err_mean": float(err.mean()), "da_area_err_max": float(err.max()),
-        "frac_within_2pct": float(np.mean(err <= 0.02)),
-        "rounds": [[int(n), round(t, 4), float(p)] for n, t, p in t_sur["rounds"]], "predict_s": t_sur["predict"],
-        "pet_plain": t_plain["pet"], "pet_surrogate": float(sum(p for _, _, p in t_sur["rounds"])),
-        "break_even_lattices": (t_gen["total"] + t_train) / per_lat_gain if per_lat_gain > 0 else None,
+    res = {
+        "n_train": n_train, "n_test": n_test, "ics_per_lattice": spec.n_ic, "train_ics_per_lattice": n_ic_train,
+        "train_samples": n_train * n_ic_train, "train_scan_s": t_gen["track"], "train_data_s": t_gen["total"],
+        "train_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
+        "plain_s": plain_s, "plain_lattices_per_hour": n_test / plain_s * 3600,
+        "ics_tracked_plain": n_test * spec.n_ic, "pet_plain": t_plain["pet"],
         "mlp_forward_tflops": inf["forward_tflops"], "mlp_train_step_tflops": inf["train_tflops"],
     }
+    res.update(out_modes[0])
+    if len(out_modes) > 1:
+        res["modes"] = out_modes
+    return res

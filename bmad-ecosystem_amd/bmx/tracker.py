@@ -117,6 +117,37 @@ class Lattice:
         _check(ctx.h, rc, "bmx_lattice_upload")
         self.h = h
 
+    @classmethod
+    def scaled(cls, ctx, eles, species, attr, sel, factors, aperture_limit_on=True, max_aperture_limit=1e3):
+        """Variants of one element list with attribute attr (abi.ATTR_K1/K2/K3)
+        of elements sel scaled by factors [n_variants, len(sel)]
+        (bmx_lattice_upload_scaled)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.species = self.tracked_species = species
+        arr = eles if isinstance(eles, C.Array) else latmod.to_array(eles)
+        f = np.ascontiguousarray(np.asarray(factors, dtype=np.float64))
+        s = np.ascontiguousarray(np.asarray(sel, dtype=np.int32))
+        n_var = f.shape[0]
+        if f.shape[1] != len(s):
+            raise ValueError("factors must be [n_variants, len(sel)]")
+        self.n_ele_total = len(arr) * n_var
+        self.p0c = arr[0].p0c
+        self.e_tot = arr[0].e_tot
+        self.n_ele = [len(arr)] * n_var
+        self.n_variants = n_var
+        self.absolute_time_tracking = False
+        ref = abi.Ref(species, 0)
+        cfg = abi.Config(1 if aperture_limit_on else 0, 0, max_aperture_limit, 0, 0)
+        h = C.c_void_p()
+        rc = ctx.lib.bmx_lattice_upload_scaled(ctx.h, arr, len(arr), n_var, int(attr),
+                                               s.ctypes.data_as(C.POINTER(C.c_int32)), len(s),
+                                               f.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ref), C.byref(cfg),
+                                               C.byref(h))
+        _check(ctx.h, rc, "bmx_lattice_upload_scaled")
+        self.h = h
+        return self
+
     def cells(self):
         """(ops per repeating cell, runs of it) tracked by the lattice's
         runtime-compiled cell kernels; (0, 0) when it runs on the op

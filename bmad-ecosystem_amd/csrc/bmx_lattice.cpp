@@ -80,7 +80,7 @@ bool magnetic_ab(const bmx_ele_rec& e, bool include_kicks, AB* out, double* b1, 
   r.ix_max = -1;
   if (b1) *b1 = 0;
   if (!e.is_on || e.key == BMX_KEY_PIPE) return true;
-  static double nfact[BMX_N_POLE];
+  double nfact[BMX_N_POLE];  // n! (a local table: variants compile on several threads)
   nfact[0] = 1;
   for (int n = 1; n < BMX_N_POLE; n++) nfact[n] = n * nfact[n - 1];
 

@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -208,32 +210,68 @@ void bmx_ctx_destroy(bmx_ctx* ctx) {
 
 const char* bmx_last_error(const bmx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_total, const int32_t* variant_offsets,
-                           int32_t n_variants, const bmx_ref* ref, const bmx_config* cfg, bmx_lattice** out) {
-  if (!ctx || !eles || !ref || !out || n_ele_total <= 0 || n_variants <= 0) return set_err(ctx, "invalid arguments");
-  *out = nullptr;
-  bmx_config c = cfg ? *cfg : bmx_config{1, 0, 1e3};
+}  // extern "C"
+
+// The element list of variant v: *p, *n, built in buf when it has to be
+// (a worker thread's own buffer).
+using VariantSource = std::function<void(int v, std::vector<bmx_ele_rec>& buf, const bmx_ele_rec** p, int* n)>;
+
+// Compile, upload and set up every variant of a lattice (bmx_lattice_upload,
+// bmx_lattice_upload_scaled).
+static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource& src, const bmx_ref* ref,
+                           const bmx_config* cfg, bmx_lattice** out) {
+  bmx_config c = cfg ? *cfg : bmx_config{1, 0, 1e3, 0, 0};
   c.direction = 1;  // the forward program; the backward one of variant 0 is compiled below
-  std::vector<int32_t> offs;
-  if (variant_offsets) offs.assign(variant_offsets, variant_offsets + n_variants + 1);
-  else if (n_variants == 1) offs = {0, n_ele_total};
-  else return set_err(ctx, "variant_offsets is required for n_variants > 1");
-  for (int v = 0; v < n_variants; v++)
-    if (offs[v] < 0 || offs[v + 1] <= offs[v] || offs[v + 1] > n_ele_total) return set_err(ctx, "bad variant_offsets");
   bmx_lattice* lat = new bmx_lattice();
   lat->device = ctx->device;
   lat->n_variants = n_variants;
-  std::vector<DevOp> prog;
-  for (int v = 0; v < n_variants; v++) {
-    lat->var_op_begin.push_back((int32_t)prog.size());
+  // the variants compile independently, on up to 16 host threads, then
+  // their programs are concatenated in variant order
+  struct VarOut {
+    std::vector<DevOp> prog;
     std::vector<int32_t> eb;
     LatticeConsts lc;
-    std::memset(&lc, 0, sizeof(lc));
     std::string err;
-    if (!compile_variant(eles + offs[v], offs[v + 1] - offs[v], *ref, c, prog, eb, &lc, &err)) {
+    int n_ele = 0;
+    bool ok = false;
+  };
+  std::vector<VarOut> outs(n_variants);
+  {
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+      std::vector<bmx_ele_rec> buf;
+      for (int v; (v = next.fetch_add(1)) < n_variants;) {
+        VarOut& o = outs[v];
+        std::memset(&o.lc, 0, sizeof(o.lc));
+        const bmx_ele_rec* e;
+        int n;
+        src(v, buf, &e, &n);
+        o.n_ele = n;
+        o.ok = compile_variant(e, n, *ref, c, o.prog, o.eb, &o.lc, &o.err);
+      }
+    };
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    int nt = (int)std::min<unsigned>(std::min<unsigned>(hw, 16u), (unsigned)n_variants);
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+  }
+  std::vector<DevOp> prog;
+  size_t total_ops = 0;
+  for (const VarOut& o : outs) total_ops += o.prog.size();
+  prog.reserve(total_ops);
+  for (int v = 0; v < n_variants; v++) {
+    VarOut& o = outs[v];
+    if (!o.ok) {
       delete lat;
-      return set_err(ctx, "variant " + std::to_string(v) + ": " + err);
+      return set_err(ctx, "variant " + std::to_string(v) + ": " + o.err);
     }
+    int32_t base = (int32_t)prog.size();
+    lat->var_op_begin.push_back(base);
+    prog.insert(prog.end(), o.prog.begin(), o.prog.end());
+    for (int32_t& x : o.eb) x += base;
+    LatticeConsts& lc = o.lc;
     lc.max_aperture_limit = c.max_aperture_limit;
     lc.two_max_ap = 2.0 * c.max_aperture_limit;
     lc.aperture_limit_on = c.aperture_limit_on;
@@ -242,8 +280,9 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
       delete lat;
       return set_err(ctx, "all variants must share the reference energy and species");
     }
-    lat->ele_op_begin.push_back(eb);
-    lat->var_n_ele.push_back(offs[v + 1] - offs[v]);
+    lat->ele_op_begin.push_back(std::move(o.eb));
+    lat->var_n_ele.push_back(o.n_ele);
+    std::vector<DevOp>().swap(o.prog);
   }
   lat->var_op_begin.push_back((int32_t)prog.size());
   lat->n_ops = (int64_t)prog.size();
@@ -276,9 +315,11 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
       std::call_once(once, [&] { std::fprintf(stderr, "bmx: cell kernels unavailable, using the interpreter: %s\n", jerr.c_str()); });
     }
   }
+  std::vector<bmx_ele_rec> buf0;
+  const bmx_ele_rec* e0;
+  int n0;
+  src(0, buf0, &e0, &n0);
   {  // reference times of variant 0's element boundaries (particle time t, loss edges)
-    const bmx_ele_rec* e0 = eles + offs[0];
-    int n0 = offs[1] - offs[0];
     lat->rt.resize(n0 + 1);
     lat->rt[0] = e0[0].ref_time - e0[0].delta_ref_time;
     for (int j = 1; j <= n0; j++) lat->rt[j] = e0[j - 1].ref_time;
@@ -296,7 +337,7 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
     LatticeConsts lb;
     std::memset(&lb, 0, sizeof(lb));
     std::string berr;
-    if (compile_variant(eles + offs[0], offs[1] - offs[0], *ref, cb, bprog, lat->eb_bwd, &lb, &berr)) {
+    if (compile_variant(e0, n0, *ref, cb, bprog, lat->eb_bwd, &lb, &berr)) {
       lb.max_aperture_limit = cb.max_aperture_limit;
       lb.two_max_ap = 2.0 * cb.max_aperture_limit;
       lb.aperture_limit_on = cb.aperture_limit_on;
@@ -319,6 +360,46 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
   *out = lat;
   return 0;
 }
+
+extern "C" {
+
+int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_total, const int32_t* variant_offsets,
+                           int32_t n_variants, const bmx_ref* ref, const bmx_config* cfg, bmx_lattice** out) {
+  if (!ctx || !eles || !ref || !out || n_ele_total <= 0 || n_variants <= 0) return set_err(ctx, "invalid arguments");
+  *out = nullptr;
+  std::vector<int32_t> offs;
+  if (variant_offsets) offs.assign(variant_offsets, variant_offsets + n_variants + 1);
+  else if (n_variants == 1) offs = {0, n_ele_total};
+  else return set_err(ctx, "variant_offsets is required for n_variants > 1");
+  for (int v = 0; v < n_variants; v++)
+    if (offs[v] < 0 || offs[v + 1] <= offs[v] || offs[v + 1] > n_ele_total) return set_err(ctx, "bad variant_offsets");
+  return upload_impl(ctx, n_variants, [&](int v, std::vector<bmx_ele_rec>&, const bmx_ele_rec** p, int* n) {
+    *p = eles + offs[v];
+    *n = offs[v + 1] - offs[v];
+  }, ref, cfg, out);
+}
+
+int32_t bmx_lattice_upload_scaled(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele, int32_t n_variants,
+                                  int32_t attr, const int32_t* sel, int32_t n_sel, const double* factors,
+                                  const bmx_ref* ref, const bmx_config* cfg, bmx_lattice** out) {
+  if (!ctx || !eles || !ref || !out || n_ele <= 0 || n_variants <= 0 || n_sel < 0 || (n_sel > 0 && (!sel || !factors)))
+    return set_err(ctx, "invalid arguments");
+  *out = nullptr;
+  double bmx_ele_rec::*field = attr == BMX_ATTR_K1 ? &bmx_ele_rec::k1
+                             : attr == BMX_ATTR_K2 ? &bmx_ele_rec::k2
+                             : attr == BMX_ATTR_K3 ? &bmx_ele_rec::k3 : nullptr;
+  if (!field) return set_err(ctx, "attr must be BMX_ATTR_K1, BMX_ATTR_K2 or BMX_ATTR_K3");
+  for (int j = 0; j < n_sel; j++)
+    if (sel[j] < 0 || sel[j] >= n_ele) return set_err(ctx, "selected element index out of range");
+  return upload_impl(ctx, n_variants, [&](int v, std::vector<bmx_ele_rec>& buf, const bmx_ele_rec** p, int* n) {
+    buf.assign(eles, eles + n_ele);
+    const double* f = factors + (size_t)v * n_sel;
+    for (int j = 0; j < n_sel; j++) buf[sel[j]].*field = buf[sel[j]].*field * f[j];
+    *p = buf.data();
+    *n = n_ele;
+  }, ref, cfg, out);
+}
+
 
 int32_t bmx_lattice_check(const bmx_ele_rec* eles, int32_t n_ele, const bmx_ref* ref, const bmx_config* cfg,
                           int64_t* n_ops, char* msg, int32_t msg_len) {

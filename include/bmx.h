@@ -243,6 +243,21 @@ int32_t bmx_lattice_upload(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele_
                            const bmx_ref *ref, const bmx_config *cfg, bmx_lattice **out);
 void bmx_lattice_destroy(bmx_lattice *lat);
 
+/* Strength attributes bmx_lattice_upload_scaled can scale. */
+enum { BMX_ATTR_K1 = 1, BMX_ATTR_K2 = 2, BMX_ATTR_K3 = 3 };
+
+/* n_variants variants of one element list without building them all on the
+ * caller's side: variant v is eles[0 .. n_ele) with the attribute `attr`
+ * (BMX_ATTR_K1, K2 or K3) of element sel[j] multiplied by
+ * factors[v * n_sel + j] -- knob or error studies of magnet strengths, the
+ * same lattice as bmx_lattice_upload of the scaled copies, record for record.
+ * The variants compile on several host threads.  No reference counterpart
+ * (the reference sets each value with set_flags_for_changed_attribute and
+ * lattice_bookkeeper, lattice by lattice). */
+int32_t bmx_lattice_upload_scaled(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele, int32_t n_variants,
+                                  int32_t attr, const int32_t *sel, int32_t n_sel, const double *factors,
+                                  const bmx_ref *ref, const bmx_config *cfg, bmx_lattice **out);
+
 /* Host-only check of one element list against the device path, with no
  * device, context or allocation on it: 0 when bmx_lattice_upload would
  * accept it (*n_ops: the ops it compiles to), else -1 with the reason (the

@@ -210,3 +210,28 @@ def test_tune_grid_picks_and_stores_a_schedule(ctx):
     assert np.array_equal(tuned, ref)
     ctx.set_schedule(saved["turns_per_block"], saved["graph_depth"])
     lat.close()
+
+
+def test_scaled_upload_equals_built_variants(ctx):
+    """bmx_lattice_upload_scaled (knob variants scaled in the library) tracks
+    exactly as bmx_lattice_upload of the variants built one by one
+    (surrogate.knob_lattice on the error-seeded rf ring), over 20 turns."""
+    from bmx import surrogate
+    spec = surrogate.ScanSpec(n_cells=16, n_rf=8, error_seed=77)
+    knobs = np.random.default_rng(12).uniform(-0.2, 0.2, (5, surrogate.N_KNOBS))
+    lat_a = surrogate.upload_variants(ctx, knobs, spec)
+    built = [surrogate.knob_lattice(k, spec) for k in knobs]
+    offs = [0]
+    for b in built:
+        offs.append(offs[-1] + len(b))
+    lat_b = ctx.lattice([e for b in built for e in b], abi.ELECTRON, variant_offsets=offs)
+    rng = np.random.default_rng(13)
+    coords = [rng.standard_normal((6, 512)) * np.array([2e-3, 2e-4, 2e-3, 2e-4, 2e-3, 5e-3])[:, None]
+              for _ in knobs]
+    a = tracker.track_multi(lat_a, coords, n_turns=20)
+    b = tracker.track_multi(lat_b, coords, n_turns=20)
+    lat_a.close()
+    lat_b.close()
+    for va, vb in zip(a, b):
+        for k in ("x", "px", "y", "py", "z", "pz", "state", "ix_lost", "turn_lost"):
+            assert np.array_equal(va[k], vb[k]), k

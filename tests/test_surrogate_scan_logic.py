@@ -191,3 +191,49 @@ def test_bracket_mode_mixed_misses_across_lattices(monkeypatch):
                                          max_rounds=2)
     assert timing["n_open"] == 0
     assert np.array_equal(runs, truth)
+
+
+@pytest.mark.parametrize("shift", [0, 1, -1, 2, 8, -8])
+def test_band_mode_single_round(monkeypatch, shift):
+    """Band mode (one FP64 round of the +-10 % band, max_rounds = 1): where the
+    true boundary lies in the band the ray is exact; a boundary above the
+    band reads as the band's top + 1, one below it as the band's bottom (the
+    network stands for the untracked amplitudes), never a neighbour
+    estimate."""
+    rng = np.random.default_rng(abs(shift) + 21)
+    truth = rng.integers(12, 35, size=(4, 3, 6))
+    runs, n_tracked, spec, timing = _run(monkeypatch, truth, shift, window=None, max_rounds=1)
+    assert len(timing["rounds"]) == 1
+    kb = np.clip(truth + shift, 0, spec.n_amp)
+    amps = spec.amp_max_x * (np.arange(spec.n_amp) + 1) / spec.n_amp
+    a_b = spec.amp_max_x * np.maximum(kb, 1) / spec.n_amp
+    lo = np.clip(np.minimum(np.searchsorted(amps, a_b * 0.9, side="left"), kb - 1), 0, spec.n_amp - 1)
+    hi = np.clip(np.maximum(np.searchsorted(amps, a_b * 1.1, side="right") - 1, kb), 0, spec.n_amp - 1)
+    want = np.where(truth > hi, np.minimum(hi + 1, spec.n_amp), np.where(truth < lo, lo, truth))
+    assert np.array_equal(runs, want)
+    assert n_tracked == int(np.sum(hi - lo + 1))
+    if abs(shift) <= 1:
+        assert np.array_equal(runs, truth)
+
+
+def test_replicated_error_seeded_knob_lattices_equal_built_ones():
+    """The same byte-for-byte replication on the error-seeded rf ring."""
+    import ctypes as C
+    from bmx import abi, lattice
+
+    captured = {}
+
+    class _Ctx:
+        def lattice(self, arr, species, variant_offsets=None):
+            captured["arr"] = bytes(memoryview(arr).cast("B"))
+            captured["offs"] = variant_offsets
+            return _FakeLat()
+
+    spec = surrogate.ScanSpec(n_cells=16, n_rf=8, error_seed=77)
+    knobs = np.random.default_rng(6).uniform(-0.2, 0.2, (3, surrogate.N_KNOBS))
+    surrogate.upload_variants(_Ctx(), knobs, spec)
+    size = C.sizeof(abi.EleRec)
+    for v in range(3):
+        want = bytes(memoryview(lattice.to_array(surrogate.knob_lattice(knobs[v], spec))).cast("B"))
+        a, b = captured["offs"][v] * size, captured["offs"][v + 1] * size
+        assert captured["arr"][a:b] == want
