@@ -17,6 +17,8 @@ use, intrinsic :: iso_c_binding
 implicit none
 
 integer(c_int32_t), parameter :: bmx_n_pole = 22
+integer(c_int32_t), parameter :: bmx_noalloc_max_particles = 65536
+integer(c_int32_t), parameter :: bmx_attr_k1 = 1, bmx_attr_k2 = 2, bmx_attr_k3 = 3
 
 ! particle states (bmad/modules/bmad_struct.f90:520-527)
 integer(c_int32_t), parameter :: bmx_alive = 1, bmx_lost = 2, bmx_lost_neg_x = 3, bmx_lost_pos_x = 4, &
@@ -130,6 +132,21 @@ interface
     integer(c_int32_t) :: rc
   end function
 
+  function bmx_lattice_upload_scaled(ctx, eles, n_ele, n_variants, attr, sel, n_sel, factors, ref, cfg, lat) &
+                                     bind(C, name = 'bmx_lattice_upload_scaled') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    type(bmx_ele_rec) :: eles(*)
+    integer(c_int32_t), value :: n_ele, n_variants, attr
+    integer(c_int32_t) :: sel(*)
+    integer(c_int32_t), value :: n_sel
+    real(c_double) :: factors(*)
+    type(bmx_ref) :: ref
+    type(bmx_config) :: cfg
+    type(c_ptr) :: lat
+    integer(c_int32_t) :: rc
+  end function
+
   subroutine bmx_lattice_destroy(lat) bind(C, name = 'bmx_lattice_destroy')
     import
     type(c_ptr), value :: lat
@@ -158,6 +175,16 @@ interface
 
   function bmx_track_bunch(ctx, lat, ix_start, ix_end, n_turns, direction, p, stats) &
                            bind(C, name = 'bmx_track_bunch') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    integer(c_int32_t), value :: ix_start, ix_end, n_turns, direction
+    type(bmx_particles) :: p
+    type(bmx_stats) :: stats
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_track_bunch_noalloc(ctx, lat, ix_start, ix_end, n_turns, direction, p, stats) &
+                                   bind(C, name = 'bmx_track_bunch_noalloc') result(rc)
     import
     type(c_ptr), value :: ctx, lat
     integer(c_int32_t), value :: ix_start, ix_end, n_turns, direction
