@@ -7,11 +7,11 @@
 // track_a_rfcavity (bmad/low_level/track_a_rfcavity.f90:17-80).
 //
 // Scope of the device path: forward and backward tracking (direction = +1 or
-// -1, time_dir = +1; backward without rf cavities or misaligned bends),
-// element orientation +1 and -1 (sbend +1), the reference species or its
-// antiparticle, relative-time RF.  Anything
-// else is rejected with an error message naming the element; the CPU side
-// never silently substitutes another path.
+// -1, time_dir = +1), element orientation +1 and -1 for every key, the
+// reference species or its antiparticle, relative-time and absolute-time RF
+// (bmad_com%absolute_time_tracking), a constant reference energy apart from
+// rf cavities.  Anything else is rejected with an error message naming the
+// element; the CPU side never silently substitutes another path.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
