@@ -12,6 +12,21 @@
 #pragma once
 #include <stdint.h>
 
+// Particles per lane of the tracking kernels (bmx_kernels.h).
+#ifndef BMX_NP
+#define BMX_NP 1
+#endif
+
+// Lanes per workgroup of the survival-scan kernel (bmx_grid_block_kernel),
+// also its LDS chunk of ops.  The host launches it with this many lanes and
+// passes the value (and BMX_NP) to the runtime-compiled kernels.  On scans
+// whose variants keep a few hundred live particles each, 64- and 128-lane
+// workgroups ran 2-7 % slower than 256 (more op staging per particle; the
+// partly filled last workgroup of a variant was not what limited them).
+#ifndef BMX_GRID_BLOCK
+#define BMX_GRID_BLOCK 256
+#endif
+
 namespace bmx {
 
 enum OpType : int32_t {

@@ -212,8 +212,11 @@ const JitKernels* jit_kernels(int device, const std::vector<DevOp>& prog, const 
       if (ch == ' ') { if (!w.empty()) extra.push_back(w); w.clear(); } else w += ch;
     }
   }
+  // the launch geometry the host uses for these kernels
+  std::string np_opt = "-DBMX_NP=" + std::to_string(BMX_NP);
+  std::string gb_opt = "-DBMX_GRID_BLOCK=" + std::to_string(BMX_GRID_BLOCK);
   std::vector<const char*> opts = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                                   "-packed-fp32-ops"};
+                                   "-packed-fp32-ops", np_opt.c_str(), gb_opt.c_str()};
   for (auto& x : extra) opts.push_back(x.c_str());
   hiprtcResult r = hiprtcCompileProgram(p, (int)opts.size(), opts.data());
   if (r != HIPRTC_SUCCESS) {

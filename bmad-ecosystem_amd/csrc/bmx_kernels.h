@@ -14,13 +14,12 @@
 #endif
 constexpr int OPS_CHUNK = BMX_OPS_CHUNK;
 
-// Particles per lane.  Each lane carries NP particles through the element
-// program together, so the op dispatch, the LDS reads of the op constants
-// and the loop control are paid once for NP particles, and the NP particle
-// updates of an op sit in one basic block where they can be interleaved.
-#ifndef BMX_NP
-#define BMX_NP 1
-#endif
+// Particles per lane (BMX_NP, bmx_internal.h).  Each lane carries NP
+// particles through the element program together, so the op dispatch, the
+// LDS reads of the op constants and the loop control are paid once for NP
+// particles, and the NP particle updates of an op sit in one basic block
+// where they can be interleaved.
+constexpr int GRID_BLOCK = BMX_GRID_BLOCK;
 
 // The particles of one lane.  live[j]: still tracked.  A particle is lost at
 // the op whose step set its state: on_loss() gets its coordinates at that
@@ -78,7 +77,7 @@ __device__ __forceinline__ double edge_time(const LatticeConsts& lc, int ie, int
 // turn0: the pass number of the first turn (survival scans in turn blocks);
 // the reference time of each pass (TurnTime, absolute-time rf) follows from
 // it, the turn and whether the segment lies past the lattice end.
-template <int NP, class OnLoss>
+template <int NP, int CHUNK, class OnLoss>
 __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops, int lo1, int hi1, int lo2, int hi2,
                                      int turns, int turn0, const LatticeConsts& lc, DevOp* sops, OnLoss on_loss) {
   for (int t = 0; t < turns; t++) {
@@ -88,7 +87,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
       tt.k = lc.tt_sign * (turn0 + t + seg);
       tt.r = (double)(lc.tt_sign * (turn0 + t)) * lc.tt_d;
       for (int c = lo; c < hi;) {
-        int m = min(OPS_CHUNK, hi - c);
+        int m = min(CHUNK, hi - c);
 #ifdef BMX_CELL_LEN
         {
           int s = lc.cell_prv[c + m - 1];
@@ -203,7 +202,7 @@ __device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const Dev
       }
     }
   }
-  walk<NP>(P, ops, lo1, hi1, lo2, hi2, n_turns, 0, lc, sops, [&](int j, const Lane& L, int t, int ie) {
+  walk<NP, OPS_CHUNK>(P, ops, lo1, hi1, lo2, hi2, n_turns, 0, lc, sops, [&](int j, const Lane& L, int t, int ie) {
     int64_t i = idx[j];
     p.x[i] = L.x; p.px[i] = L.px; p.y[i] = L.y; p.py[i] = L.py; p.z[i] = L.z; p.pz[i] = L.pz; p.beta[i] = L.beta;
     if (p.t) p.t[i] = edge_time(lc, ie, t) + p.t[i] - L.z / (L.beta * D_C_LIGHT);
@@ -303,12 +302,12 @@ __global__ __launch_bounds__(256) void bmx_ics_init_kernel(LatticeConsts lc, Gri
   survival[k] = n_turns;
 }
 
-extern "C" __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
+extern "C" __global__ __launch_bounds__(GRID_BLOCK, BMX_WAVES_PER_EU) void bmx_grid_block_kernel(
     const DevOp* __restrict__ ops, const int32_t* __restrict__ var_op_begin, const int32_t* __restrict__ var_n_ele,
     LatticeConsts lc, int n_ic, GridSoA src, GridSoA dst, const int32_t* __restrict__ count_in,
     int32_t* __restrict__ count_out, const int32_t* __restrict__ d_turn0, int turns_per_block, int n_turns,
     int32_t* __restrict__ survival, unsigned long long* __restrict__ pet) {
-  __shared__ DevOp sops[OPS_CHUNK];
+  __shared__ DevOp sops[GRID_BLOCK];
   // the block's first turn lives in device memory so that captured graphs of
   // several blocks replay unchanged; every thread reads the same value, so a
   // finished scan leaves the whole grid together
@@ -346,7 +345,7 @@ extern "C" __global__ __launch_bounds__(256, BMX_WAVES_PER_EU) void bmx_grid_blo
     }
   }
   int lo = var_op_begin[v], hi = var_op_begin[v + 1];
-  walk<NP>(P, ops, lo, hi, 0, 0, turns, turn0, lc, sops,
+  walk<NP, GRID_BLOCK>(P, ops, lo, hi, 0, 0, turns, turn0, lc, sops,
            [&](int j, const Lane&, int t, int) { survival[base + ic[j]] = turn0 + t; });
   // live particle-element-turns of this lane
   unsigned long long my_pet = 0;

@@ -708,7 +708,7 @@ struct SurvivalRun {
 static int run_survival(bmx_ctx* ctx, const bmx_lattice* lat, SurvivalRun& R, std::vector<int32_t> h_count,
                         int n_turns, int tpb, int graph_depth, int32_t* h_surv, bmx_stats* out,
                         const std::function<void()>& init) {
-  const int block = 256;
+  const int block = GRID_BLOCK;
   int nv = R.nv;
   if (graph_depth > 1 && (graph_depth & 1)) graph_depth++;
   (void)hipMemcpyAsync(R.d_count, h_count.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);

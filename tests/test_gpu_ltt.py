@@ -31,6 +31,14 @@ def test_bunch_moments_match_numpy(ctx):
 
 
 def test_run_beam_mode_matches_oracle_turn_loop(ctx):
+    """12 turns in launches of 3 with a record every 4: the live count of
+    every record equals the oracle's (particles alive after that many turns)
+    up to the chaotic band -- particles whose survival or loss element
+    differs between the plain and the FMA-contracted oracle builds, which
+    the count may or may not include; final states and loss elements
+    identical outside the band; the final centroid within 4x the oracle's
+    own plain-vs-FMA spread of the survivors' coordinates."""
+    from helpers import coord_errors
     eles = lattice.fodo_ring_c0(n_cells=20)
     coords, _ = beam.c0_bunch(eles[:7], n=3000, seed=9)
     coords *= 12.0
@@ -39,18 +47,30 @@ def test_run_beam_mode_matches_oracle_turn_loop(ctx):
     recs, reason = ltt.run_beam_mode(lat, b, n_turns=12, output_every=4, dead_cutoff=1.0, turns_per_launch=3)
     print("records:", [(r["turn"], r["n_live"]) for r in recs], reason, flush=True)
     assert [r["turn"] for r in recs] == [0, 4, 8, 12]
-    c = coords.copy()
-    state = np.ones(c.shape[1], dtype=np.int32)
-    for r in recs[1:]:
-        o = oracle.track_bunch(eles, abi.ELECTRON, 4, c, n_threads=8)
-        c = np.stack([o[k] for k in ("x", "px", "y", "py", "z", "pz")])
-        state = np.where(state == abi.ALIVE, o["state"], state)
-        # particles lost earlier stay lost: the oracle tracked them again from
-        # their loss coordinates, which the survivors' count ignores
-        n_ref = int((state == abi.ALIVE).sum())
-        assert abs(r["n_live"] - n_ref) <= 2, (r["turn"], r["n_live"], n_ref)
-    mean_ref = c[:, state == abi.ALIVE].mean(1)
-    assert np.allclose(recs[-1]["centroid"], mean_ref, rtol=1e-6, atol=1e-9)
+    r = oracle.track_bunch(eles, abi.ELECTRON, 12, coords, n_threads=8)
+    f = oracle.track_bunch(eles, abi.ELECTRON, 12, coords, n_threads=8, fma=True)
+    surv_r = np.where(r["state"] == abi.ALIVE, 12, r["turn_lost"])
+    surv_f = np.where(f["state"] == abi.ALIVE, 12, f["turn_lost"])
+    band = (surv_r != surv_f) | (r["ix_lost"] != f["ix_lost"])
+    nb = int(band.sum())
+    print(f"chaotic band {nb}", flush=True)
+    for rec in recs:
+        n_ref = int(np.sum(surv_r >= rec["turn"]))
+        assert abs(rec["n_live"] - n_ref) <= nb, (rec["turn"], rec["n_live"], n_ref)
+    g = b.numpy()
+    out = ~band
+    assert np.array_equal(g["state"][out], r["state"][out])
+    lost = out & (r["state"] != abi.ALIVE)
+    assert lost.sum() > 100 and np.array_equal(g["ix_lost"][lost], r["ix_lost"][lost])
+    alive = out & (r["state"] == abi.ALIVE)
+    path = 12 * sum(e.l for e in eles)
+    tol = max(1e-12, 4 * max(coord_errors(f, r, alive, path).values()))
+    assert max(coord_errors(g, r, alive, path).values()) <= tol
+    if nb == 0:  # the mean of the survivors' coordinates moves by at most their largest error
+        v = np.stack([r[k][alive] for k in ("x", "px", "y", "py", "z", "pz")])
+        scale = np.max(np.abs(v), axis=1)
+        scale[4] = max(scale[4], 1e-3 * path)
+        assert np.all(np.abs(recs[-1]["centroid"] - v.mean(1)) <= tol * scale * (1 + 1e-9))
     lat.close()
 
 
