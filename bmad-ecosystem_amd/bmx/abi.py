@@ -99,7 +99,7 @@ EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
     "bmx_lattice_upload", "bmx_lattice_upload_scaled", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch",
     "bmx_track_bunch_noalloc", "bmx_track_multi", "bmx_track_grid",
-    "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_one_turn_matrix", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
+    "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_one_turn_matrix", "bmx_transfer_matrix", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
     "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_step_fit", "bmx_ctx_sync",
 ]
@@ -168,6 +168,10 @@ def load_library(path=None):
     lib.bmx_one_turn_matrix.restype = C.c_int32
     lib.bmx_one_turn_matrix.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(Stats)]
+    lib.bmx_transfer_matrix.restype = C.c_int32
+    lib.bmx_transfer_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int32)]
     lib.bmx_gemm_bf16.restype = C.c_int32
     lib.bmx_gemm_bf16.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                   C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,

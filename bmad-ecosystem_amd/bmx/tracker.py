@@ -341,6 +341,24 @@ def one_turn_matrix(lat, orbit=None, d_orb=None):
     return m.reshape(nv, 6, 6), status
 
 
+def transfer_matrix(lat, orbits, ix_start=0, ix_end=0, n_turns=1):
+    """Exact transfer matrices (bmx_transfer_matrix): orbits [n, 6] (or one
+    orbit [6]) tracked forward through elements ix_start+1 .. ix_end (whole
+    turns when equal) for n_turns passes.  Returns (out [n, 6], mat [n, 6, 6]
+    with mat[i, r, q] = d out_r / d in_q, state [n])."""
+    v = np.ascontiguousarray(np.asarray(orbits, dtype=np.float64).reshape(-1, 6))
+    n = v.shape[0]
+    out = np.zeros_like(v)
+    mat = np.zeros((n, 36))
+    state = np.zeros(n, dtype=np.int32)
+    dp = C.POINTER(C.c_double)
+    rc = lat.ctx.lib.bmx_transfer_matrix(lat.ctx.h, lat.h, int(ix_start), int(ix_end), int(n_turns), n,
+                                         v.ctypes.data_as(dp), out.ctypes.data_as(dp), mat.ctypes.data_as(dp),
+                                         state.ctypes.data_as(C.POINTER(C.c_int32)))
+    _check(lat.ctx.h, rc, "bmx_transfer_matrix")
+    return out, mat.reshape(n, 6, 6), state
+
+
 def track_ics(lat, ics_per_variant, n_turns, turns_per_block=0, return_stats=False):
     """Survival turns of explicit initial conditions, one [6, n_v] host array
     per variant (bmx_track_ics).  Returns a list of int32 arrays."""

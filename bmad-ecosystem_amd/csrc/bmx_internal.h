@@ -29,6 +29,12 @@
 
 namespace bmx {
 
+// FP64 for values that are never a particle coordinate (op constants, series
+// coefficients).  The particle arithmetic of bmx_ops.h is written with
+// `double`, which the transfer-matrix build redefines as a dual number; this
+// name stays plain FP64 there.
+typedef double bmx_f64;
+
 enum OpType : int32_t {
   OP_DRIFT = 1,          // exact drift, c[0] = L
   OP_KICK = 2,           // thin ab multipole kick, complex Horner over n = 0..nmax

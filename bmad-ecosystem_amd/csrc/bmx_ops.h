@@ -40,7 +40,7 @@ struct Lane {
 // so the loads cannot be hoisted and become a few s_load_dwordx8 into
 // SGPRs, and sg() keeps each loaded value in its SGPR pair, where the FMA
 // takes it as an operand.  Per block: the highest-order coefficient last.
-typedef __attribute__((address_space(4))) const double cdouble;
+typedef __attribute__((address_space(4))) const bmx_f64 cdouble;
 enum : int {
   CS_DRIFT = 0,   // 5/16, 35/128, 63/256: binomial series of 1/sqrt(1-u) - 1 from u^3
   CS_QUAD = 3,    // (4m)!, (4m+2)!, (4m+1)!, (4m+3)! reciprocals, m = 0..5
@@ -51,7 +51,7 @@ enum : int {
   CS_ISINC = 49,  // x / sin(x) = 1 + x^2 P(x^2)
   CS_END = 56
 };
-__constant__ double kSeries[CS_END] = {
+__constant__ bmx_f64 kSeries[CS_END] = {
     // CS_DRIFT
     0.3125, 0.2734375, 0.24609375,
     // CS_QUAD
@@ -81,7 +81,7 @@ __device__ __forceinline__ const cdouble* series(int block) {
   return p;
 }
 
-__device__ __forceinline__ double sg(double c) {
+__device__ __forceinline__ bmx_f64 sg(bmx_f64 c) {
   asm("" : "+s"(c));
   return c;
 }
@@ -215,7 +215,7 @@ __device__ __forceinline__ void cmul(double& kr, double& ki, double ar, double a
 // c holds nh + 1 complex coefficients for orders lo .. lo + nh; the common
 // factor (x + i y)^lo is applied as powers of (x + i y)^2, without a loop for
 // lo <= 3 (nh and lo are wave-uniform).
-__device__ __forceinline__ void kick_horner(const double* c, int nh, int lo, double x, double y, double& kr,
+__device__ __forceinline__ void kick_horner(const bmx_f64* c, int nh, int lo, double x, double y, double& kr,
                                             double& ki) {
   kr = c[2 * nh];
   ki = c[2 * nh + 1];
@@ -456,7 +456,13 @@ __device__ __noinline__ BendExit bend_exit_exact(double phi_1, double L_u, doubl
 // sequences the compiler emits for sqrt() and '/', without their range
 // scaling and special-value fix-ups.  From v_rsq / v_rcp (relative error
 // below 2^-22) two Newton steps and an FMA residual correction give the
-// correctly rounded result.
+// correctly rounded result.  The transfer-matrix build (BMX_TANGENT) takes
+// the plain operations, whose derivatives its number type carries.
+#if BMX_TANGENT
+__device__ __forceinline__ double sqrt_mid(double x) { return sqrt(x); }
+__device__ __forceinline__ double rcp_mid(double b) { return 1.0 / b; }
+__device__ __forceinline__ double div_mid(double a, double b) { return a / b; }
+#else
 __device__ __forceinline__ double sqrt_mid(double x) {
   double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = 0.5 * y;
@@ -487,6 +493,7 @@ __device__ __forceinline__ double div_mid(double a, double b) {
   double q = a * r;
   return fma(fma(-b, q, a), r, q);
 }
+#endif
 
 // Exact sector-bend body step (track_a_bend.f90:120-209), in the reference's
 // order of operations: phi_1 = asin(px/pt), sin/cos(angle + phi_1), the exit
@@ -498,7 +505,7 @@ __device__ __forceinline__ double div_mid(double a, double b) {
 // beta_ref as beta (sl / beta_ref), each within an ulp of the reference's
 // quotient, so that on the common path both forms give the same bits.
 // c: 0 step_len, 1 g, 2 g_tot, 3 angle, 4 cos a, 5 sin a, 6 sinc a, 7 cosc a, 8 sincc a, 9 beta_ref, 10 dg
-__device__ __forceinline__ Lane bend_general(Lane L, const double* c, double p0c, double mc2sq, double mal,
+__device__ __forceinline__ Lane bend_general(Lane L, const bmx_f64* c, double p0c, double mc2sq, double mal,
                                           double two_mal) {
   double sl = c[0], g = c[1], g_tot = c[2], angle = c[3];
   double cos_a = c[4], sin_a = c[5], sinc_a = c[6], cosc_a = c[7];
@@ -755,7 +762,7 @@ __device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set, in
     // a misaligned bend (offset_particle.f90:208-235, :409-440): its chain of
     // arc shifts, offsets and rotations composed on the host, acting on
     // (x, y, 0) and the momentum (px, py, pz_s) with pz_s = sqrt((1+pz)^2 - px^2 - py^2)
-    const double* w = &op.c[4];
+    const bmx_f64* w = &op.c[4];
     double pzz = rel_p * rel_p - L.px * L.px - L.py * L.py;
     if (pzz <= 0.0) { L.state = BMX_LOST_PZ; return; }
     double x = L.x, y = L.y;
@@ -785,7 +792,7 @@ __device__ __forceinline__ void op_offset(Lane& L, const DevOp& op, bool set, in
   }
   double s_pos;  // s_body (set) or s_lab (unset)
   if (flags & F_OFFSET_PITCH) {
-    const double* w = &op.c[4];
+    const bmx_f64* w = &op.c[4];
     double n0 = w[0] * r0 + w[1] * r1 + w[2] * r2;
     double n1 = w[3] * r0 + w[4] * r1 + w[5] * r2;
     double n2 = w[6] * r0 + w[7] * r1 + w[8] * r2;
@@ -980,7 +987,7 @@ __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
 // ps = px, py, z, pz (width2, center) pairs; an ecollimator takes each of
 // (px, py) and (z, pz) as an ellipse when both widths are set.  uf: the
 // unstable factor the x, y limits are compared with.
-__device__ void ps_limits(Lane& L, const double* ps, bool ell, double& uf) {
+__device__ void ps_limits(Lane& L, const bmx_f64* ps, bool ell, double& uf) {
   auto one = [&](double v, double w2, double c0) {
     if (w2 == 0.0) return;
     if (v > c0 - w2 && v < c0 + w2) return;
@@ -988,7 +995,7 @@ __device__ void ps_limits(Lane& L, const double* ps, bool ell, double& uf) {
     double d1 = (c0 - w2) - v, d2 = v - (c0 + w2);
     uf = d1 > d2 ? d1 : d2;
   };
-  auto pair = [&](double u, double v, const double* q) {
+  auto pair = [&](double u, double v, const bmx_f64* q) {
     if (!ell || q[0] == 0.0 || q[2] == 0.0) {
       one(u, q[0], q[1]);
       one(v, q[2], q[3]);
@@ -1006,9 +1013,9 @@ __device__ void ps_limits(Lane& L, const double* ps, bool ell, double& uf) {
 }
 
 // true when ps_limits would leave the particle alone (the same tests)
-__device__ __forceinline__ bool ps_inside(const Lane& L, const double* ps, bool ell) {
+__device__ __forceinline__ bool ps_inside(const Lane& L, const bmx_f64* ps, bool ell) {
   auto one = [&](double v, double w2, double c0) { return w2 == 0.0 || (v > c0 - w2 && v < c0 + w2); };
-  auto pair = [&](double u, double v, const double* q) {
+  auto pair = [&](double u, double v, const bmx_f64* q) {
     if (!ell || q[0] == 0.0 || q[2] == 0.0) return one(u, q[0], q[1]) && one(v, q[2], q[3]);
     double a = (u - q[1]) / q[0], b = (v - q[3]) / q[2];
     return !(a * a + b * b > 1.0);
@@ -1016,7 +1023,7 @@ __device__ __forceinline__ bool ps_inside(const Lane& L, const double* ps, bool 
   return pair(L.px, L.py, ps) && pair(L.z, L.pz, ps + 4);
 }
 
-__device__ __forceinline__ void aperture(Lane& L, const double* c, bool elliptical, double uf = 0.0) {
+__device__ __forceinline__ void aperture(Lane& L, const bmx_f64* c, bool elliptical, double uf = 0.0) {
   if (elliptical) {
     double xp = L.x - c[0], yp = L.y - c[1];
     // screen with the reciprocal widths: it is within a few ulp of the
@@ -1074,8 +1081,8 @@ __device__ __forceinline__ void op_sbend(Lane& L, const DevOp& op, int flags, bo
 // every screen and takes the ordered tests, as in the reference.
 // c: the aperture constants (as aperture()); ps: collimator phase-space
 // limits (F_END_PS).
-__device__ __forceinline__ void ele_end(Lane& L, const double* c, int flags, const LatticeConsts& lc,
-                                        const double* ps) {
+__device__ __forceinline__ void ele_end(Lane& L, const bmx_f64* c, int flags, const LatticeConsts& lc,
+                                        const bmx_f64* ps) {
   double mal = lc.max_aperture_limit;
   double rel_p = 1.0 + L.pz;
   double f = L.px * L.px + L.py * L.py - rel_p * rel_p;

@@ -30,6 +30,7 @@
 #include "../../include/bmx.h"
 #include "bmx_internal.h"
 #include "bmx_jit.h"
+#include "bmx_tangent.h"
 #include <tuple>
 
 namespace bmx {
@@ -119,6 +120,7 @@ struct bmx_lattice {
   // reference times of variant 0's element boundaries (LatticeConsts::rt), host and device
   std::vector<double> rt;
   double* d_rt = nullptr;
+  bool abs_time = false;  // uploaded with absolute-time rf
 };
 
 // Launch a tracking kernel: the lattice's runtime-compiled cell kernel when
@@ -225,6 +227,7 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
   bmx_lattice* lat = new bmx_lattice();
   lat->device = ctx->device;
   lat->n_variants = n_variants;
+  lat->abs_time = c.absolute_time_tracking != 0;
   // the variants compile independently, on up to 16 host threads, then
   // their programs are concatenated in variant order
   struct VarOut {
@@ -1153,6 +1156,48 @@ int32_t bmx_one_turn_matrix(bmx_ctx* ctx, const bmx_lattice* lat, const double* 
             lost ? 0.0 : (c[(size_t)i * n + kp] - c[(size_t)i * n + km]) / (2.0 * d[q]);
       }
   }
+  return 0;
+}
+
+int32_t bmx_transfer_matrix(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
+                            int64_t n, const double* vec_in, double* vec_out, double* mat6, int32_t* state) {
+  if (!ctx || !lat || !vec_in || !vec_out || !mat6 || !state || n < 0 || n_turns < 0)
+    return set_err(ctx, "invalid arguments");
+  if (lat->abs_time) return set_err(ctx, "bmx_transfer_matrix: lattices with absolute-time rf are not supported");
+  int n_ele = lat->var_n_ele[0];
+  if (ix_start < 0 || ix_start > n_ele || ix_end < 0 || ix_end > n_ele) return set_err(ctx, "ix_start/ix_end out of range");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const std::vector<int32_t>& eb = lat->ele_op_begin[0];
+  int lo1, hi1, lo2 = 0, hi2 = 0;
+  if (ix_end > ix_start) { lo1 = eb[ix_start]; hi1 = eb[ix_end]; }
+  else { lo1 = eb[ix_start]; hi1 = eb[n_ele]; lo2 = eb[0]; hi2 = eb[ix_end]; }
+  if (n == 0 || n_turns == 0) {
+    for (int64_t i = 0; i < n; i++) {
+      state[i] = BMX_ALIVE;
+      for (int r = 0; r < 6; r++) {
+        vec_out[i * 6 + r] = vec_in[i * 6 + r];
+        for (int q = 0; q < 6; q++) mat6[i * 36 + r * 6 + q] = r == q ? 1.0 : 0.0;
+      }
+    }
+    return 0;
+  }
+  double* d = nullptr;
+  size_t nb = (size_t)n * (6 + 6 + 36) * sizeof(double) + (size_t)n * sizeof(int32_t);
+  HIP_TRY(ctx, hipMalloc(&d, nb));
+  double *d_in = d, *d_out = d + 6 * n, *d_mat = d + 12 * n;
+  int32_t* d_state = reinterpret_cast<int32_t*>(d + 48 * n);
+  hipError_t e = hipMemcpyAsync(d_in, vec_in, (size_t)n * 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_transfer(ctx->stream, lat->d_ops, lo1, hi1, lo2, hi2, lat->lc, n_turns, d_in, d_out, d_mat, d_state, n);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(vec_out, d_out, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(mat6, d_mat, (size_t)n * 36 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(state, d_state, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_err(ctx, std::string("bmx_transfer_matrix: ") + hipGetErrorString(e));
   return 0;
 }
 

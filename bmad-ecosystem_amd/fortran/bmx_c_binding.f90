@@ -281,6 +281,22 @@ interface
     type(bmx_stats) :: stats
     integer(c_int32_t) :: rc
   end function
+
+  ! exact transfer matrices (make_mat6 / transfer_matrix_calc over a range):
+  ! vec_in, vec_out (6, n) as coord_struct%vec; mat6 (6, 6, n) holds in
+  ! Fortran order the transpose of each C row-major matrix, so ele%mat6 =
+  ! transpose(mat6(:, :, i))
+  function bmx_transfer_matrix(ctx, lat, ix_start, ix_end, n_turns, n, vec_in, vec_out, mat6, state) &
+                               bind(C, name = 'bmx_transfer_matrix') result(rc)
+    import
+    type(c_ptr), value :: ctx, lat
+    integer(c_int32_t), value :: ix_start, ix_end, n_turns
+    integer(c_int64_t), value :: n
+    real(c_double), intent(in) :: vec_in(*)
+    real(c_double), intent(out) :: vec_out(*), mat6(*)
+    integer(c_int32_t), intent(out) :: state(*)
+    integer(c_int32_t) :: rc
+  end function
 end interface
 
 contains

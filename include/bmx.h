@@ -365,6 +365,23 @@ int32_t bmx_closed_orbit(bmx_ctx *ctx, const bmx_lattice *lat, int32_t n_dim, do
 int32_t bmx_one_turn_matrix(bmx_ctx *ctx, const bmx_lattice *lat, const double *orbit, const double *d_orb,
                             double *mat6, int32_t *status, bmx_stats *out);
 
+/* Exact transfer matrices: for each of n orbits (vec_in, n x 6, host), the
+ * 6x6 Jacobian mat6[i][r][q] = d out_r / d in_q (row-major, n x 36) of
+ * tracking it forward through elements ix_start+1 .. ix_end of variant 0 for
+ * n_turns passes (the element range of bmx_track_bunch; ix_start == ix_end:
+ * whole turns), and the tracked orbit (vec_out, n x 6).  Replaces make_mat6
+ * (bmad/code/make_mat6.f90:23) with the make_matrix branches of the
+ * track_a_* routines (e.g. bmad/low_level/track_a_drift.f90:94-114), chained
+ * over the range as transfer_matrix_calc does; the derivatives are those of
+ * libbmx's own tracking map, carried by forward-mode differentiation, so one
+ * element or a thousand turns are exact to rounding.  beta at entry follows
+ * init_coord (the reference beta at pz = 0).  state[i]: BMX_ALIVE, or the
+ * loss state (its matrix is zeros, vec_out the coordinates at the loss).
+ * Relative-time rf only. */
+int32_t bmx_transfer_matrix(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
+                            int32_t n_turns, int64_t n, const double *vec_in, double *vec_out, double *mat6,
+                            int32_t *state);
+
 /* ------------------------------------------------------------------------
  * Dynamic-aperture surrogate kernels (no counterpart in the reference; they
  * serve the learned screening of lattice variants).  Device pointers; work
