@@ -94,7 +94,7 @@ def da_scan(ctx, n_train=4096, n_test=1024, train_ics=8192, epochs=6):
     band, bracket = r["modes"]
     keep = ("plain_lattices_per_hour", "plain_s", "ics_tracked_plain", "pet_plain", "n_train", "n_test",
             "train_ics_per_lattice", "train_samples", "train_scan_s", "train_data_s", "train_s", "loss_first",
-            "loss_last", "mlp_forward_tflops", "mlp_train_step_tflops")
+            "loss_last", "mlp_forward_tflops", "mlp_forward_fused_tflops", "mlp_train_step_tflops")
     out = {k: r[k] for k in keep}
     out["band"] = band
     out["bracket"] = {k: bracket[k] for k in bracket if k != "rounds"}

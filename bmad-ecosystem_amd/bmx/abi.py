@@ -101,7 +101,9 @@ EXPORTS = [
     "bmx_track_bunch_noalloc", "bmx_track_multi", "bmx_track_grid",
     "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_one_turn_matrix", "bmx_transfer_matrix", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
-    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_step_fit", "bmx_ctx_sync",
+    "bmx_gemm_bf16", "bmx_colsum_f32", "bmx_adam_step", "bmx_f32_to_bf16", "bmx_mse_grad", "bmx_da_features", "bmx_step_fit", "bmx_mlp_forward", "bmx_da_screen", "bmx_ctx_sync",
+    "bmx_surrogate_create", "bmx_surrogate_destroy", "bmx_surrogate_set_params", "bmx_surrogate_get_params",
+    "bmx_surrogate_train", "bmx_surrogate_infer", "bmx_surrogate_screen",
 ]
 
 _lib = None
@@ -168,6 +170,30 @@ def load_library(path=None):
     lib.bmx_one_turn_matrix.restype = C.c_int32
     lib.bmx_one_turn_matrix.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(Stats)]
+    lib.bmx_mlp_forward.restype = C.c_int32
+    lib.bmx_mlp_forward.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int64, C.c_void_p]
+    lib.bmx_da_screen.restype = C.c_int32
+    lib.bmx_da_screen.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_int32,
+                                  C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                                  C.c_float, C.c_void_p]
+    lib.bmx_surrogate_create.restype = C.c_int32
+    lib.bmx_surrogate_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_void_p)]
+    lib.bmx_surrogate_destroy.restype = None
+    lib.bmx_surrogate_destroy.argtypes = [C.c_void_p]
+    lib.bmx_surrogate_set_params.restype = C.c_int32
+    lib.bmx_surrogate_set_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    lib.bmx_surrogate_get_params.restype = C.c_int32
+    lib.bmx_surrogate_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    lib.bmx_surrogate_train.restype = C.c_int32
+    lib.bmx_surrogate_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                        C.c_float, C.c_uint64, C.c_int32, C.POINTER(C.c_float)]
+    lib.bmx_surrogate_infer.restype = C.c_int32
+    lib.bmx_surrogate_infer.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    lib.bmx_surrogate_screen.restype = C.c_int32
+    lib.bmx_surrogate_screen.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32,
+                                         C.c_void_p, C.c_int32, C.c_float, C.c_void_p]
     lib.bmx_transfer_matrix.restype = C.c_int32
     lib.bmx_transfer_matrix.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
                                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),

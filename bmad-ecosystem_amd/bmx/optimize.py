@@ -123,7 +123,7 @@ def surrogate_search(ctx, spec, n_gen=20, n_cand=4096, top_frac=0.05, retrain_ev
     _say(f"initial training data: {n_init} lattices x {init_ics} ICs, {spec.n_turns} turns")
     k0 = rng.uniform(-R, R, (n_init, NK))
     X0, Y0, t_gen = surrogate.training_data(ctx, k0, spec, init_ics, seed=seed + 1, chunk=256)
-    net = surrogate.MLP(ctx, [NK + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
+    net = surrogate.Net(ctx, [NK + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
     t0 = time.perf_counter()
     surrogate.train(net, X0, Y0, epochs=init_epochs, batch=batch)
     t_init_train = time.perf_counter() - t0

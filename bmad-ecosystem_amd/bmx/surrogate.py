@@ -193,6 +193,29 @@ class MLP:
                 cur = nxt
         return self.loss
 
+    def fused_ok(self):
+        """True when the one-kernel forward pass takes this network
+        (bmx_mlp_forward: dims[0] <= 64, hidden layers 256 wide)."""
+        d = self.dims
+        return len(d) >= 3 and 1 <= d[0] <= 64 and all(w == 256 for w in d[1:-1])
+
+    def _dims_c(self):
+        return (C.c_int32 * len(self.dims))(*self.dims)
+
+    def forward_fused(self, x, y=None):
+        """Inference-only forward pass in one kernel (bmx_mlp_forward), equal
+        bit for bit to forward(): x BF16 device tensor [n, dims[0]]; returns
+        the FP32 device tensor y [n] (valid after sync())."""
+        n = x.shape[0]
+        if x.dtype != torch.bfloat16 or not x.is_contiguous() or x.shape[1] != self.dims[0]:
+            raise ValueError("x must be a contiguous bfloat16 [n, n_in] tensor")
+        if y is None:
+            y = torch.empty(n, dtype=torch.float32, device=self.dev)
+        rc = self.lib.bmx_mlp_forward(self.ctx.h, len(self.dims), self._dims_c(), _p(self.w), _p(self.wb), _p(x), n,
+                                      _p(y))
+        self._check(rc, "bmx_mlp_forward")
+        return y
+
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         self.step_count += 1
         rc = self.lib.bmx_adam_step(self.ctx.h, self.n_params, _p(self.w), _p(self.wb), _p(self.g), _p(self.m),
@@ -204,6 +227,102 @@ class MLP:
         loss = self.backward(t)
         self.adam(lr)
         return loss
+
+
+class Net:
+    """A surrogate network held by the library (bmx_surrogate_*): the same
+    network and kernels as MLP, with its training loop, inference and DA
+    screening in C++ (each training step replayed as a hipGraph), so that a
+    caller needs no Python.  Interface as far as the DA pipeline uses MLP:
+    dims, fused_ok(), params(), and train() / predict_boundaries() take it."""
+
+    def __init__(self, ctx, dims, params=None, seed=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.dims = list(dims)
+        self.dev = torch.device(f"cuda:{ctx.device}")
+        flat = _flatten(dims, _init_params(dims, seed) if params is None else params)
+        n = C.c_int64(0)
+        h = C.c_void_p()
+        self.h = None
+        rc = self.lib.bmx_surrogate_create(ctx.h, len(self.dims), self._dims_c(),
+                                           flat.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n), C.byref(h))
+        self._check(rc, "bmx_surrogate_create")
+        self.h = h
+        self.n_params = n.value
+        ctx._nets.add(self)
+
+    def _dims_c(self):
+        return (C.c_int32 * len(self.dims))(*self.dims)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: {self.lib.bmx_last_error(self.ctx.h).decode()}")
+
+    def fused_ok(self):
+        d = self.dims
+        return len(d) >= 3 and 1 <= d[0] <= 64 and all(w == 256 for w in d[1:-1])
+
+    def params(self):
+        a = np.zeros(self.n_params, dtype=np.float32)
+        self._check(self.lib.bmx_surrogate_get_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float))),
+                    "bmx_surrogate_get_params")
+        out, off = [], 0
+        for i in range(len(self.dims) - 1):
+            n_in, n_out = self.dims[i], self.dims[i + 1]
+            W = a[off:off + n_in * n_out].astype(np.float64).reshape(n_out, n_in)
+            off += n_in * n_out
+            out.append((W, a[off:off + n_out].astype(np.float64)))
+            off += n_out
+        return out
+
+    def set_params(self, params):
+        flat = _flatten(self.dims, params)
+        self._check(self.lib.bmx_surrogate_set_params(self.h, flat.ctypes.data_as(C.POINTER(C.c_float))),
+                    "bmx_surrogate_set_params")
+
+    def train(self, X, Y, epochs=20, batch=4096, lr=2e-3, lr_final=2e-4, seed=0, eager=False):
+        """bmx_surrogate_train on (X, Y) (numpy float32 or device tensors);
+        returns the per-epoch mean sampled losses."""
+        Xd = X if torch.is_tensor(X) else torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32))
+        Yd = Y if torch.is_tensor(Y) else torch.from_numpy(np.ascontiguousarray(Y, dtype=np.float32))
+        Xd = Xd.to(self.dev, torch.float32).contiguous()
+        Yd = Yd.to(self.dev, torch.float32).contiguous()
+        torch.cuda.synchronize(self.dev)
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        rc = self.lib.bmx_surrogate_train(self.h, _p(Xd), _p(Yd), Xd.shape[0], int(epochs), int(batch), float(lr),
+                                          float(lr_final), int(seed) & ((1 << 64) - 1), 1 if eager else 0,
+                                          losses.ctypes.data_as(C.POINTER(C.c_float)))
+        self._check(rc, "bmx_surrogate_train")
+        return [float(v) for v in losses[:epochs]]
+
+    def infer(self, xb):
+        """Network output for BF16 device rows xb [n, dims[0]] -> FP32 device tensor [n]."""
+        y = torch.empty(xb.shape[0], dtype=torch.float32, device=self.dev)
+        torch.cuda.synchronize(self.dev)
+        self._check(self.lib.bmx_surrogate_infer(self.h, _p(xb), xb.shape[0], _p(y)), "bmx_surrogate_infer")
+        self._check(self.lib.bmx_ctx_sync(self.ctx.h), "bmx_ctx_sync")
+        return y
+
+    def close(self):
+        if self.h:
+            self.lib.bmx_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _flatten(dims, params):
+    """Per layer W [out, in] then b [out], FP32."""
+    parts = []
+    for (W, b), n_in, n_out in zip(params, dims[:-1], dims[1:]):
+        parts.append(np.asarray(W, dtype=np.float32).reshape(n_out, n_in).reshape(-1))
+        parts.append(np.asarray(b, dtype=np.float32).reshape(n_out))
+    return np.ascontiguousarray(np.concatenate(parts))
 
 
 def _init_params(dims, seed):
@@ -441,6 +560,8 @@ def train(net, X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, lr_final=2e-4, loss
     the network's stream, which the host joins every `loss_every` steps (and
     at the end of the epoch) to read the loss.  Returns the per-epoch means
     of those sampled losses."""
+    if isinstance(net, Net):
+        return net.train(X, Y, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final, seed=seed)
     dev = net.dev
     Xd = torch.from_numpy(np.ascontiguousarray(X)).to(dev)
     Yd = torch.from_numpy(np.ascontiguousarray(Y)).to(dev)
@@ -540,12 +661,13 @@ def predict_knobs(net, knob_sets, spec, lat_chunk=None):
     return out.cpu().numpy().reshape(n_lat, spec.n_pz, spec.n_angle, spec.n_amp)
 
 
-def predict_boundaries(net, knob_sets, spec, lat_chunk=None):
+def predict_boundaries(net, knob_sets, spec, lat_chunk=None, layered=False):
     """Predicted boundary index of every ray [n_lat, n_pz, n_angle] (int64,
     0..n_amp): the network output for every grid IC and its step fit
-    (step_fit_boundary at the surviving level 1), all on the device
-    (bmx_da_features, the MLP forward, bmx_step_fit); only the indices come
-    back to the host."""
+    (step_fit_boundary at the surviving level 1), all on the device; only the
+    indices come back to the host.  A network the one-kernel forward pass
+    takes runs through bmx_da_screen; otherwise (or with layered=True) the
+    calls are bmx_da_features, the layer-by-layer forward and bmx_step_fit."""
     knob_sets = np.asarray(knob_sets)
     n_lat = len(knob_sets)
     if lat_chunk is None:
@@ -555,6 +677,20 @@ def predict_boundaries(net, knob_sets, spec, lat_chunk=None):
     kd = torch.from_numpy(knob_sets.astype(np.float32)).to(dev)
     n_ray_lat = spec.n_pz * spec.n_angle
     kb = torch.empty(n_lat * n_ray_lat, dtype=torch.int32, device=dev)
+    if isinstance(net, Net):
+        torch.cuda.synchronize(dev)
+        rc = net.lib.bmx_surrogate_screen(net.h, n_lat, N_KNOBS, _p(kd), KNOB_RANGE, spec.n_ic, 3, _p(geo),
+                                          spec.n_amp, 1.0, _p(kb))
+        net._check(rc, "bmx_surrogate_screen")
+        net._check(net.lib.bmx_ctx_sync(net.ctx.h), "bmx_ctx_sync")
+        return kb.cpu().numpy().astype(np.int64).reshape(n_lat, spec.n_pz, spec.n_angle)
+    if net.fused_ok() and not layered:  # features, forward pass and step fit in the library
+        torch.cuda.synchronize(dev)
+        rc = net.lib.bmx_da_screen(net.ctx.h, len(net.dims), net._dims_c(), _p(net.w), _p(net.wb), n_lat, N_KNOBS,
+                                   _p(kd), KNOB_RANGE, spec.n_ic, 3, _p(geo), spec.n_amp, 1.0, _p(kb))
+        net._check(rc, "bmx_da_screen")
+        net.sync()
+        return kb.cpu().numpy().astype(np.int64).reshape(n_lat, spec.n_pz, spec.n_angle)
     X = torch.empty(min(lat_chunk, n_lat) * spec.n_ic, N_KNOBS + 3, dtype=torch.bfloat16, device=dev)
     torch.cuda.synchronize(dev)
     for a in range(0, n_lat, lat_chunk):
@@ -811,34 +947,38 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
 
 
 def inference_throughput(net, n=1 << 18, reps=10):
-    """Dense MFMA throughput of the network (algorithmic BF16 flops of the
-    GEMMs / wall time between stream syncs) for forward passes and training
-    steps at batch n."""
+    """Dense MFMA throughput of the network (algorithmic BF16 flops / wall
+    time between stream syncs) at batch n: the forward pass layer by layer
+    (bmx_gemm_bf16 chain) and in one kernel (bmx_mlp_forward, when the shape
+    allows), and training steps (forward, backward, Adam).  Measured on a
+    copy of the network's parameters, which it leaves unchanged."""
     import time
     dims = net.dims
+    m = MLP(net.ctx, dims, params=net.params())
     flop = 2 * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
-    x = torch.rand(n, dims[0], device=net.dev, dtype=torch.float32)
-    t = torch.rand(n, device=net.dev, dtype=torch.float32)
-    torch.cuda.synchronize(net.dev)
-    net.forward(x)
-    net.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        net.forward(x)
-    net.sync()
-    fwd = (time.perf_counter() - t0) / reps
-    saved = (net.w.clone(), net.wb.clone(), net.m.clone(), net.v.clone(), net.step_count)
-    net.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        net.train_step(x, t, 0.0)
-    net.sync()
-    trn = (time.perf_counter() - t0) / reps
-    net.w.copy_(saved[0]); net.wb.copy_(saved[1]); net.m.copy_(saved[2]); net.v.copy_(saved[3])
-    net.step_count = saved[4]
-    torch.cuda.synchronize(net.dev)
-    return {"forward_tflops": n * flop / fwd / 1e12, "train_tflops": 3 * n * flop / trn / 1e12,
-            "forward_ms": fwd * 1e3, "train_step_ms": trn * 1e3, "batch": n}
+    x = torch.rand(n, dims[0], device=m.dev, dtype=torch.float32)
+    xb = x.to(torch.bfloat16)
+    t = torch.rand(n, device=m.dev, dtype=torch.float32)
+    y = torch.empty(n, device=m.dev, dtype=torch.float32)
+    torch.cuda.synchronize(m.dev)
+
+    def timed(f):
+        f()
+        m.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        m.sync()
+        return (time.perf_counter() - t0) / reps
+
+    fwd = timed(lambda: m.forward(xb))
+    fused = timed(lambda: m.forward_fused(xb, y)) if m.fused_ok() else None
+    trn = timed(lambda: m.train_step(x, t, 0.0))
+    out = {"forward_tflops": n * flop / fwd / 1e12, "train_tflops": 3 * n * flop / trn / 1e12,
+           "forward_ms": fwd * 1e3, "train_step_ms": trn * 1e3, "batch": n}
+    if fused is not None:
+        out.update(forward_fused_tflops=n * flop / fused / 1e12, forward_fused_ms=fused * 1e3)
+    return out
 
 
 def area_errors(runs, plain_runs, spec):
@@ -918,7 +1058,7 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
         n_ic_train = train_ics
     say(f"training data done in {t_gen['total']:.1f} s; training {epochs} epochs on {len(Y)} samples")
     t0 = time.perf_counter()
-    net = MLP(ctx, [N_KNOBS + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
+    net = Net(ctx, [N_KNOBS + 3, 256, 256, 256, 256, 1], seed=seed % 1000)
     losses = train(net, X, Y, epochs=epochs, batch=batch, lr=lr)
     del X, Y
     t_train = time.perf_counter() - t0
@@ -959,6 +1099,7 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
         "plain_s": plain_s, "plain_lattices_per_hour": n_test / plain_s * 3600,
         "ics_tracked_plain": n_test * spec.n_ic, "pet_plain": t_plain["pet"],
         "mlp_forward_tflops": inf["forward_tflops"], "mlp_train_step_tflops": inf["train_tflops"],
+        "mlp_forward_fused_tflops": inf.get("forward_fused_tflops"),
     }
     res.update(out_modes[0])
     if len(out_modes) > 1:

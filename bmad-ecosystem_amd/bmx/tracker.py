@@ -7,6 +7,7 @@ px, y, py, z, pz, beta; int32 state, ix_lost, turn_lost).  PyTorch is used
 only for device memory; every computation runs in libbmx's HIP kernels.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -41,9 +42,12 @@ class Context:
             raise BmxError(f"bmx_ctx_create(device={device}) failed with rc={rc} (no HIP device?)")
         self.h = h
         self.device = device
+        self._nets = weakref.WeakSet()  # library-held networks (surrogate.Net), closed before the context
 
     def close(self):
         if self.h:
+            for net in list(self._nets):
+                net.close()
             self.lib.bmx_ctx_destroy(self.h)
             self.h = None
 

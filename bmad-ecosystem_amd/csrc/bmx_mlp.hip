@@ -9,7 +9,11 @@
 // v_mfma_f32_16x16x32_bf16 (A: 16 x 32, B: 32 x 16, C: 16 x 16 FP32).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
+#include <vector>
+#include <random>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -305,16 +309,187 @@ __global__ __launch_bounds__(RAYS) void step_fit_kernel(int n_rays, int K, const
   out[r0 + threadIdx.x] = kbest;
 }
 
+// ---------------------------------------------------------------------------
+// The whole forward pass in one kernel, for inference: a workgroup takes 64
+// rows and keeps their activations in LDS through every layer, so that only
+// the inputs are read from and one output per row written to HBM.  Hidden
+// layers are FH = 256 wide; the input is at most FK0 = 64 wide (zero-padded),
+// given as BF16 rows or formed in place from a DA scan's knob vectors and
+// grid coordinates (as da_features_kernel forms them).  Wave w computes output
+// columns 64 w .. 64 w + 63 of every layer for all 64 rows as 4 x 4 blocks of
+// v_mfma_f32_16x16x32_bf16, A from LDS, B (the weights, W[n][k] row-major)
+// straight from L2; the next k-step's weights are loaded before this one's
+// MFMAs.  The MFMA operands, their k order and the epilogues (bias, SiLU,
+// BF16 rounding) are those of gemm_bf16_kernel, so the outputs equal the
+// layer-by-layer pass bit for bit.  The linear head (one output) is a 16 x 16
+// block per wave whose B holds the head weights in column 0.
+constexpr int FH = 256, FK0 = 64, FP = FH + 8;
+
+struct FusedNet {
+  int d0;            // input width, <= FK0
+  int n_hidden;      // hidden layers, each FH wide
+  const __bf16* wb;  // BF16 parameters, per layer W [out][in] then b [out]
+  const float* w;    // FP32 master parameters (the biases are read here)
+};
+
+struct FusedIn {
+  const __bf16* X;  // rows of d0 BF16 values; null: DA features below
+  const float* knobs;
+  float inv_range;
+  int n_knob;
+  const float* geo;
+  int n_geo;
+  int n_ic;
+};
+
+__device__ __forceinline__ bf16x8_t load_w8(const __bf16* __restrict__ W, int64_t base, int k, int n_in, bool vec) {
+  if (vec) return *reinterpret_cast<const bf16x8_t*>(W + base + k);
+  bf16x8_t v;
+  for (int e = 0; e < 8; e++) v[e] = (k + e < n_in) ? W[base + k + e] : (__bf16)0.0f;
+  return v;
+}
+
+template <int RB>  // row blocks of 16 per workgroup: FR = 16 RB rows
+__global__ __launch_bounds__(256) void mlp_fused_forward_kernel(int64_t n, FusedNet net, FusedIn in,
+                                                               float* __restrict__ y) {
+  constexpr int FR = 16 * RB;
+  __shared__ __bf16 act[FR * FP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * FR;
+  for (int e = tid; e < FR * FK0; e += 256) {
+    int m = e / FK0, k = e % FK0;
+    int64_t row = row0 + m;
+    __bf16 v = (__bf16)0.0f;
+    if (row < n && k < net.d0) {
+      if (in.X) {
+        v = in.X[row * net.d0 + k];
+      } else {
+        int64_t l = row / in.n_ic;
+        int i = (int)(row % in.n_ic);
+        v = (k < in.n_knob) ? (__bf16)(in.knobs[l * in.n_knob + k] * in.inv_range)
+                            : (__bf16)in.geo[(int64_t)i * in.n_geo + (k - in.n_knob)];
+      }
+    }
+    act[m * FP + k] = v;
+  }
+  __syncthreads();
+  const int r = lane & 15, q = lane >> 4;
+  int64_t off = 0;
+  int n_in = net.d0;
+  for (int l = 0; l < net.n_hidden; l++) {
+    const __bf16* W = net.wb + off;
+    const float* bias = net.w + off + (int64_t)FH * n_in;
+    const int K = (l == 0) ? FK0 : FH;
+    const bool vec = l > 0;
+    f32x4_t acc[RB][4];
+    for (int i = 0; i < RB; i++)
+      for (int j = 0; j < 4; j++) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    bf16x8_t bfr[4], nxt[4];
+    for (int j = 0; j < 4; j++) bfr[j] = load_w8(W, (int64_t)(64 * wave + 16 * j + r) * n_in, 8 * q, n_in, vec);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      if (k0 + 32 < K)
+        for (int j = 0; j < 4; j++)
+          nxt[j] = load_w8(W, (int64_t)(64 * wave + 16 * j + r) * n_in, k0 + 32 + 8 * q, n_in, vec);
+      for (int i = 0; i < RB; i++) {
+        bf16x8_t af = *reinterpret_cast<const bf16x8_t*>(&act[(16 * i + r) * FP + k0 + 8 * q]);
+        for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[j], acc[i][j], 0, 0, 0);
+      }
+      for (int j = 0; j < 4; j++) bfr[j] = nxt[j];
+    }
+    __syncthreads();  // every wave has read this layer's input
+    for (int i = 0; i < RB; i++)
+      for (int j = 0; j < 4; j++) {
+        int nn = 64 * wave + 16 * j + r;
+        float b = bias[nn];
+        for (int t = 0; t < 4; t++) act[(16 * i + 4 * q + t) * FP + nn] = (__bf16)silu(acc[i][j][t] + b);
+      }
+    __syncthreads();
+    off += (int64_t)FH * n_in + FH;
+    n_in = FH;
+  }
+  const __bf16* Wh = net.wb + off;
+  const float bh = net.w[off + FH];
+  for (int ib = wave; ib < RB; ib += 4) {
+    f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < FH; k0 += 32) {
+      bf16x8_t af = *reinterpret_cast<const bf16x8_t*>(&act[(16 * ib + r) * FP + k0 + 8 * q]);
+      bf16x8_t b = {};
+      if (r == 0) b = *reinterpret_cast<const bf16x8_t*>(Wh + k0 + 8 * q);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b, acc, 0, 0, 0);
+    }
+    if (r == 0)
+      for (int t = 0; t < 4; t++) {
+        int64_t row = row0 + 16 * ib + 4 * q + t;
+        if (row < n) y[row] = acc[t] + bh;
+      }
+  }
+}
+
+// Training steps of a bmx_net as a replayable graph: the step's indices live
+// in device memory (TrainState), so that the captured kernels read the batch,
+// the learning rate and Adam's bias corrections for the step they run as.
+struct TrainState {
+  int64_t t;  // global step count before this step (Adam's step is t + 1)
+  int32_t k;  // step within the epoch
+};
+
+// xb[i] = bf16(X[perm[k batch + i]]), yb[i] = Y[perm[k batch + i]]
+__global__ __launch_bounds__(256) void gather_batch_kernel(const float* __restrict__ X, const float* __restrict__ Y,
+                                                          const int64_t* __restrict__ perm,
+                                                          const TrainState* __restrict__ st, int batch, int d0,
+                                                          __bf16* __restrict__ xb, float* __restrict__ yb) {
+  int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)batch * d0) return;
+  int i = (int)(e / d0), c = (int)(e % d0);
+  int64_t src = perm[(int64_t)st->k * batch + i];
+  xb[e] = (__bf16)X[src * d0 + c];
+  if (c == 0) yb[i] = Y[src];
+}
+
+// adam_kernel with the step's learning rate (lr_tab[t - t0]) and bias
+// corrections taken from the step count in device memory
+__global__ __launch_bounds__(256) void adam_graph_kernel(int64_t n, float* __restrict__ w, __bf16* __restrict__ wb,
+                                                        const float* __restrict__ g, float* __restrict__ m,
+                                                        float* __restrict__ v, const float* __restrict__ lr_tab,
+                                                        int64_t t0, const TrainState* __restrict__ st, float b1,
+                                                        float b2, float eps) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int64_t t = st->t;
+  float lr = lr_tab[t - t0];
+  float c1 = 1.f - powf(b1, (float)(t + 1)), c2 = 1.f - powf(b2, (float)(t + 1));
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  wb[i] = (__bf16)wi;
+}
+
+__global__ void train_advance_kernel(TrainState* st) {
+  st->t += 1;
+  st->k += 1;
+}
+
+__global__ void train_reset_kernel(TrainState* st, int64_t t) {
+  st->t = t;
+  st->k = 0;
+}
+
 }  // namespace
 
 // defined in bmx_track.hip (hidden visibility)
 extern "C" hipStream_t bmx_ctx_stream(const bmx_ctx* ctx);
+extern "C" int bmx_ctx_device(const bmx_ctx* ctx);
 extern "C" int32_t bmx_ctx_set_error(bmx_ctx* ctx, const char* msg);
 
 // per-context workspaces (slot 0: split-K partial products, slot 1:
-// column-sum partials), grown on demand, released by bmx_ctx_destroy
+// column-sum partials, slot 2: network outputs of a screening chunk), grown
+// on demand, released by bmx_ctx_destroy
 static std::mutex g_ws_mu;
-static std::unordered_map<const bmx_ctx*, std::array<std::pair<float*, size_t>, 2>> g_ws;
+static std::unordered_map<const bmx_ctx*, std::array<std::pair<float*, size_t>, 3>> g_ws;
 
 static float* splitk_workspace(const bmx_ctx* ctx, size_t floats, int slot = 0) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
@@ -351,6 +526,40 @@ extern "C" __attribute__((visibility("hidden"))) void bmx_mlp_release(const bmx_
   for (auto& e : it->second)
     if (e.first) (void)hipFree(e.first);
   g_ws.erase(it);
+}
+
+// dims[0] <= FK0 -> FH -> ... -> FH -> 1 (at least one hidden layer): the shapes the fused pass takes
+static bool fused_shape(int32_t n_dims, const int32_t* dims, FusedNet* net) {
+  if (!dims || n_dims < 3 || dims[0] < 1 || dims[0] > FK0 || dims[n_dims - 1] != 1) return false;
+  for (int i = 1; i < n_dims - 1; i++)
+    if (dims[i] != FH) return false;
+  net->d0 = dims[0];
+  net->n_hidden = n_dims - 2;
+  return true;
+}
+
+#ifndef BMX_FUSED_RB
+#define BMX_FUSED_RB 4
+#endif
+static void launch_fused(hipStream_t s, int64_t n, const FusedNet& net, const FusedIn& in, float* y) {
+  constexpr int RB = BMX_FUSED_RB, FR = 16 * RB;
+  hipLaunchKernelGGL(mlp_fused_forward_kernel<RB>, dim3((unsigned)((n + FR - 1) / FR)), dim3(256), 0, s, n, net, in,
+                     y);
+}
+
+static void launch_step_fit(hipStream_t st, int32_t n_rays, int32_t n_amp, const float* pred, float alive_level,
+                            int32_t* k_out) {
+  // rays per workgroup so that the staged profiles stay within 33 KiB of LDS
+  size_t row = (size_t)(n_amp + 1) * sizeof(float);
+  if (n_amp <= 128)
+    hipLaunchKernelGGL(step_fit_kernel<64>, dim3((unsigned)((n_rays + 63) / 64)), dim3(64), 64 * row, st, n_rays,
+                       n_amp, pred, alive_level, k_out);
+  else if (n_amp <= 256)
+    hipLaunchKernelGGL(step_fit_kernel<32>, dim3((unsigned)((n_rays + 31) / 32)), dim3(32), 32 * row, st, n_rays,
+                       n_amp, pred, alive_level, k_out);
+  else
+    hipLaunchKernelGGL(step_fit_kernel<4>, dim3((unsigned)((n_rays + 3) / 4)), dim3(4), 4 * row, st, n_rays, n_amp,
+                       pred, alive_level, k_out);
 }
 
 extern "C" {
@@ -464,20 +673,370 @@ int32_t bmx_step_fit(bmx_ctx* ctx, int32_t n_rays, int32_t n_amp, const float* p
                      int32_t* k_out) {
   if (!ctx || !pred || !k_out || n_rays <= 0 || n_amp <= 0 || n_amp > 2048)
     return bmx_ctx_set_error(ctx, "bmx_step_fit: invalid arguments");
-  // rays per workgroup so that the staged profiles stay within 33 KiB of LDS
-  hipStream_t st = bmx_ctx_stream(ctx);
-  size_t row = (size_t)(n_amp + 1) * sizeof(float);
-  if (n_amp <= 128)
-    hipLaunchKernelGGL(step_fit_kernel<64>, dim3((unsigned)((n_rays + 63) / 64)), dim3(64), 64 * row, st, n_rays,
-                       n_amp, pred, alive_level, k_out);
-  else if (n_amp <= 256)
-    hipLaunchKernelGGL(step_fit_kernel<32>, dim3((unsigned)((n_rays + 31) / 32)), dim3(32), 32 * row, st, n_rays,
-                       n_amp, pred, alive_level, k_out);
-  else
-    hipLaunchKernelGGL(step_fit_kernel<4>, dim3((unsigned)((n_rays + 3) / 4)), dim3(4), 4 * row, st, n_rays, n_amp,
-                       pred, alive_level, k_out);
+  launch_step_fit(bmx_ctx_stream(ctx), n_rays, n_amp, pred, alive_level, k_out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+}
+
+int32_t bmx_mlp_forward(bmx_ctx* ctx, int32_t n_dims, const int32_t* dims, const float* w, const void* w_bf16,
+                        const void* X_bf16, int64_t n, float* y) {
+  FusedNet net;
+  if (!ctx || !w || !w_bf16 || !X_bf16 || !y || n <= 0) return bmx_ctx_set_error(ctx, "bmx_mlp_forward: invalid arguments");
+  if (!fused_shape(n_dims, dims, &net))
+    return bmx_ctx_set_error(ctx, "bmx_mlp_forward: the network must be dims[0] <= 64 -> 256 -> ... -> 256 -> 1");
+  if (((uintptr_t)w_bf16 & 15) != 0) return bmx_ctx_set_error(ctx, "bmx_mlp_forward: w_bf16 must be 16-byte aligned");
+  net.wb = (const __bf16*)w_bf16;
+  net.w = w;
+  FusedIn in{};
+  in.X = (const __bf16*)X_bf16;
+  launch_fused(bmx_ctx_stream(ctx), n, net, in, y);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+}
+
+int32_t bmx_da_screen(bmx_ctx* ctx, int32_t n_dims, const int32_t* dims, const float* w, const void* w_bf16,
+                      int32_t n_lat, int32_t n_knob, const float* knobs, float knob_range, int32_t n_ic, int32_t n_geo,
+                      const float* geo, int32_t n_amp, float alive_level, int32_t* kb) {
+  FusedNet net;
+  if (!ctx || !w || !w_bf16 || !knobs || !geo || !kb || n_lat <= 0 || n_knob < 0 || n_geo < 0 || n_ic <= 0 ||
+      n_amp <= 0 || n_amp > 2048 || n_ic % n_amp != 0 || knob_range <= 0)
+    return bmx_ctx_set_error(ctx, "bmx_da_screen: invalid arguments");
+  if (!fused_shape(n_dims, dims, &net) || n_knob + n_geo != dims[0])
+    return bmx_ctx_set_error(ctx, "bmx_da_screen: the network must be (n_knob + n_geo <= 64) -> 256 -> ... -> 256 -> 1");
+  if (((uintptr_t)w_bf16 & 15) != 0) return bmx_ctx_set_error(ctx, "bmx_da_screen: w_bf16 must be 16-byte aligned");
+  net.wb = (const __bf16*)w_bf16;
+  net.w = w;
+  // lattices per chunk: at most 2^22 network outputs in the workspace
+  int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 22) / n_ic);
+  chunk = std::min<int64_t>(chunk, n_lat);
+  float* yw = splitk_workspace(ctx, (size_t)(chunk * n_ic), 2);
+  if (!yw) return bmx_ctx_set_error(ctx, "bmx_da_screen: workspace allocation failed");
+  hipStream_t s = bmx_ctx_stream(ctx);
+  int rays_lat = n_ic / n_amp;
+  for (int64_t a = 0; a < n_lat; a += chunk) {
+    int64_t nl = std::min<int64_t>(chunk, n_lat - a);
+    FusedIn in{};
+    in.knobs = knobs + a * n_knob;
+    in.inv_range = 1.0f / knob_range;
+    in.n_knob = n_knob;
+    in.geo = geo;
+    in.n_geo = n_geo;
+    in.n_ic = n_ic;
+    launch_fused(s, nl * n_ic, net, in, yw);
+    launch_step_fit(s, (int32_t)(nl * rays_lat), n_amp, yw, alive_level, kb + a * rays_lat);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// bmx_net: a surrogate network owned by the library
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct bmx_net {
+  bmx_ctx* ctx = nullptr;
+  int device = 0;
+  std::vector<int32_t> dims;
+  struct Layer {
+    int64_t wo, bo;
+    int n_in, n_out;
+  };
+  std::vector<Layer> layout;
+  int64_t n_params = 0;
+  float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
+  __bf16* wb = nullptr;
+  int64_t step = 0;  // Adam steps taken
+  // one resident batch size
+  int nb = 0;
+  __bf16 *xb = nullptr, *db = nullptr, *dpre_b[2] = {nullptr, nullptr};
+  std::vector<float*> pre;
+  std::vector<__bf16*> h;
+  float *y = nullptr, *d = nullptr, *yb = nullptr, *dpre[2] = {nullptr, nullptr};
+  std::vector<void*> batch_bufs;
+  ~bmx_net() {
+    for (void* p : batch_bufs) (void)hipFree(p);
+    for (void* p : {(void*)w, (void*)g, (void*)m, (void*)v, (void*)loss, (void*)wb})
+      if (p) (void)hipFree(p);
+  }
+  template <class T>
+  bool alloc(T** p, size_t n) {
+    if (hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return false;
+    batch_bufs.push_back(*p);
+    return true;
+  }
+  bool buffers(int n) {
+    if (n == nb) return true;
+    for (void* p : batch_bufs) (void)hipFree(p);
+    batch_bufs.clear();
+    pre.clear();
+    h.clear();
+    nb = 0;
+    int wmax = 1;
+    for (size_t l = 1; l + 1 < dims.size(); l++) wmax = std::max(wmax, dims[l]);
+    bool ok = alloc(&xb, (size_t)n * dims[0]) && alloc(&y, n) && alloc(&yb, n) && alloc(&d, n) && alloc(&db, n);
+    for (int k = 0; k < 2 && ok; k++) ok = alloc(&dpre[k], (size_t)n * wmax) && alloc(&dpre_b[k], (size_t)n * wmax);
+    for (size_t l = 1; l + 1 < dims.size() && ok; l++) {
+      float* p;
+      __bf16* q;
+      ok = alloc(&p, (size_t)n * dims[l]) && alloc(&q, (size_t)n * dims[l]);
+      if (ok) {
+        pre.push_back(p);
+        h.push_back(q);
+      }
+    }
+    if (ok) nb = n;
+    return ok;
+  }
+};
+
+namespace {
+
+// The layer-by-layer forward pass of the training step (as the Python MLP
+// forms it) on the batch in xb: hidden layers pre = h W^T + b, h = bf16(SiLU),
+// then the linear head into y.
+int32_t net_forward(bmx_net* net, int n) {
+  int L = (int)net->layout.size() - 1;
+  const __bf16* hin = net->xb;
+  for (int l = 0; l < L; l++) {
+    const auto& ly = net->layout[l];
+    if (bmx_gemm_bf16(net->ctx, 0, 1, n, ly.n_out, ly.n_in, hin, ly.n_in, net->wb + ly.wo, ly.n_in, net->pre[l],
+                      net->h[l], ly.n_out, net->w + ly.bo, 2, nullptr) != 0)
+      return -1;
+    hin = net->h[l];
+  }
+  const auto& hd = net->layout[L];
+  return bmx_gemm_bf16(net->ctx, 0, 1, n, 1, hd.n_in, hin, hd.n_in, net->wb + hd.wo, hd.n_in, net->y, nullptr, 1,
+                       net->w + hd.bo, 1, nullptr);
+}
+
+// MSE loss against yb and the gradients of every parameter into g.
+int32_t net_backward(bmx_net* net, int n) {
+  bmx_ctx* ctx = net->ctx;
+  int L = (int)net->layout.size() - 1;
+  if (bmx_mse_grad(ctx, n, net->y, net->yb, net->d, net->db, net->loss) != 0) return -1;
+  const auto& hd = net->layout[L];
+  const __bf16* h_in = L > 0 ? net->h[L - 1] : net->xb;
+  if (bmx_gemm_bf16(ctx, 1, 0, 1, hd.n_in, n, net->db, 1, h_in, hd.n_in, net->g + hd.wo, nullptr, hd.n_in, nullptr, 0,
+                    nullptr) != 0 ||
+      bmx_colsum_f32(ctx, n, 1, net->d, 1, net->g + hd.bo) != 0)
+    return -1;
+  if (L == 0) return 0;
+  int cur = 0;
+  if (bmx_gemm_bf16(ctx, 0, 0, n, hd.n_in, 1, net->db, 1, net->wb + hd.wo, hd.n_in, net->dpre[cur], net->dpre_b[cur],
+                    hd.n_in, nullptr, 3, net->pre[L - 1]) != 0)
+    return -1;
+  for (int l = L - 1; l >= 0; l--) {
+    const auto& ly = net->layout[l];
+    const __bf16* hi = l > 0 ? net->h[l - 1] : net->xb;
+    if (bmx_gemm_bf16(ctx, 1, 0, ly.n_out, ly.n_in, n, net->dpre_b[cur], ly.n_out, hi, ly.n_in, net->g + ly.wo,
+                      nullptr, ly.n_in, nullptr, 0, nullptr) != 0 ||
+        bmx_colsum_f32(ctx, n, ly.n_out, net->dpre[cur], ly.n_out, net->g + ly.bo) != 0)
+      return -1;
+    if (l > 0) {
+      int nxt = 1 - cur;
+      if (bmx_gemm_bf16(ctx, 0, 0, n, ly.n_in, ly.n_out, net->dpre_b[cur], ly.n_out, net->wb + ly.wo, ly.n_in,
+                        net->dpre[nxt], net->dpre_b[nxt], ly.n_in, nullptr, 3, net->pre[l - 1]) != 0)
+        return -1;
+      cur = nxt;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bmx_surrogate_create(bmx_ctx* ctx, int32_t n_dims, const int32_t* dims, const float* params,
+                             int64_t* n_params, bmx_net** out) {
+  if (!ctx || !dims || !params || !out || n_dims < 2 || dims[n_dims - 1] != 1)
+    return bmx_ctx_set_error(ctx, "bmx_surrogate_create: invalid arguments (the head must have one output)");
+  for (int i = 0; i < n_dims; i++)
+    if (dims[i] < 1 || dims[i] > 65536) return bmx_ctx_set_error(ctx, "bmx_surrogate_create: bad layer width");
+  if (hipSetDevice(bmx_ctx_device(ctx)) != hipSuccess) return bmx_ctx_set_error(ctx, "bmx_surrogate_create: device");
+  bmx_net* net = new bmx_net();
+  net->ctx = ctx;
+  net->device = bmx_ctx_device(ctx);
+  net->dims.assign(dims, dims + n_dims);
+  int64_t off = 0;
+  for (int i = 0; i + 1 < n_dims; i++) {
+    bmx_net::Layer ly{off, off + (int64_t)dims[i + 1] * dims[i], dims[i], dims[i + 1]};
+    net->layout.push_back(ly);
+    off = ly.bo + dims[i + 1];
+  }
+  net->n_params = off;
+  size_t nf = (size_t)off * sizeof(float);
+  bool ok = hipMalloc(&net->w, nf) == hipSuccess && hipMalloc(&net->g, nf) == hipSuccess &&
+            hipMalloc(&net->m, nf) == hipSuccess && hipMalloc(&net->v, nf) == hipSuccess &&
+            hipMalloc(&net->loss, sizeof(float)) == hipSuccess &&
+            hipMalloc(&net->wb, (size_t)off * sizeof(__bf16)) == hipSuccess;
+  if (!ok) {
+    delete net;
+    return bmx_ctx_set_error(ctx, "bmx_surrogate_create: device allocation failed");
+  }
+  *out = net;
+  if (n_params) *n_params = off;
+  return bmx_surrogate_set_params(net, params);
+}
+
+void bmx_surrogate_destroy(bmx_net* net) {  // hipFree waits for the device; the context is not needed
+  if (!net) return;
+  (void)hipSetDevice(net->device);
+  delete net;
+}
+
+int32_t bmx_surrogate_set_params(bmx_net* net, const float* params) {
+  if (!net || !params) return -1;
+  bmx_ctx* ctx = net->ctx;
+  hipStream_t s = bmx_ctx_stream(ctx);
+  size_t nf = (size_t)net->n_params * sizeof(float);
+  hipError_t e = hipMemcpyAsync(net->w, params, nf, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(net->m, 0, nf, s);
+  if (e == hipSuccess) e = hipMemsetAsync(net->v, 0, nf, s);
+  if (e == hipSuccess) e = hipMemsetAsync(net->g, 0, nf, s);
+  if (e == hipSuccess)
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)((net->n_params + 255) / 256)), dim3(256), 0, s,
+                       net->n_params, net->w, net->wb);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  net->step = 0;
+  return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+}
+
+int32_t bmx_surrogate_get_params(const bmx_net* net, float* params) {
+  if (!net || !params) return -1;
+  hipStream_t s = bmx_ctx_stream(net->ctx);
+  hipError_t e = hipMemcpyAsync(params, net->w, (size_t)net->n_params * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : bmx_ctx_set_error(net->ctx, hipGetErrorString(e));
+}
+
+int32_t bmx_surrogate_train(bmx_net* net, const float* X, const float* Y, int64_t n, int32_t epochs, int32_t batch,
+                            float lr, float lr_final, uint64_t seed, int32_t flags, float* epoch_loss) {
+  if (!net) return -1;
+  bmx_ctx* ctx = net->ctx;
+  if (!X || !Y || n <= 0 || epochs < 0 || batch <= 0 || batch > n || !(lr > 0) || !(lr_final > 0))
+    return bmx_ctx_set_error(ctx, "bmx_surrogate_train: invalid arguments");
+  if (hipSetDevice(bmx_ctx_device(ctx)) != hipSuccess) return bmx_ctx_set_error(ctx, "bmx_surrogate_train: device");
+  if (!net->buffers(batch)) return bmx_ctx_set_error(ctx, "bmx_surrogate_train: batch buffer allocation failed");
+  hipStream_t s = bmx_ctx_stream(ctx);
+  const int d0 = net->dims[0];
+  const int64_t spe = n / batch;  // steps per epoch (the remainder of a shuffle is left out)
+  const int64_t total = spe * epochs;
+  if (total == 0) return 0;
+  // cosine learning-rate decay from lr to lr_final over all steps
+  std::vector<float> lr_h(total);
+  for (int64_t t = 0; t < total; t++) {
+    double frac = total > 1 ? (double)t / (double)(total - 1) : 0.0;
+    lr_h[t] = (float)(lr_final + 0.5 * ((double)lr - lr_final) * (1.0 + std::cos(3.14159265358979323846 * frac)));
+  }
+  float* lr_d = nullptr;
+  int64_t* perm_d = nullptr;
+  TrainState* st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+  auto hip_ok = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && err.empty()) err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipSuccess;
+  };
+  std::vector<int64_t> perm(n);
+  std::mt19937_64 rng(seed);
+  const int64_t t0 = net->step;
+  const int64_t n_params = net->n_params;
+  auto step_kernels = [&]() -> bool {  // one training step, all on stream s
+    hipLaunchKernelGGL(gather_batch_kernel, dim3((unsigned)(((int64_t)batch * d0 + 255) / 256)), dim3(256), 0, s, X,
+                       Y, perm_d, st, batch, d0, net->xb, net->yb);
+    if (net_forward(net, batch) != 0 || net_backward(net, batch) != 0) return false;
+    hipLaunchKernelGGL(adam_graph_kernel, dim3((unsigned)((n_params + 255) / 256)), dim3(256), 0, s, n_params, net->w,
+                       net->wb, net->g, net->m, net->v, lr_d, t0, st, 0.9f, 0.999f, 1e-8f);
+    hipLaunchKernelGGL(train_advance_kernel, dim3(1), dim3(1), 0, s, st);
+    return hip_ok(hipGetLastError(), "training step");
+  };
+  bool ok = hip_ok(hipMalloc(&lr_d, total * sizeof(float)), "allocation") &&
+            hip_ok(hipMalloc(&perm_d, n * sizeof(int64_t)), "allocation") &&
+            hip_ok(hipMalloc(&st, sizeof(TrainState)), "allocation") &&
+            hip_ok(hipMemcpyAsync(lr_d, lr_h.data(), total * sizeof(float), hipMemcpyHostToDevice, s), "copy");
+  bool use_graph = !(flags & 1);
+  for (int ep = 0; ep < epochs && ok; ep++) {
+    for (int64_t i = 0; i < n; i++) perm[i] = i;
+    for (int64_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[(int64_t)(rng() % (uint64_t)(i + 1))]);
+    ok = hip_ok(hipMemcpyAsync(perm_d, perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s), "copy");
+    hipLaunchKernelGGL(train_reset_kernel, dim3(1), dim3(1), 0, s, st, net->step);
+    ok = ok && hip_ok(hipStreamSynchronize(s), "synchronize");
+    double acc = 0.0;
+    int cnt = 0;
+    for (int64_t k = 0; k < spe && ok; k++) {
+      bool eager = !use_graph || (ep == 0 && k == 0);  // the first step runs eagerly: it sizes the workspaces
+      if (eager) {
+        ok = step_kernels();
+      } else {
+        if (!exec) {
+          ok = hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+          bool built = ok && step_kernels();
+          hipError_t ce = hipStreamEndCapture(s, &graph);
+          ok = built && hip_ok(ce, "capture") && hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph");
+          if (!ok) break;
+        }
+        ok = hip_ok(hipGraphLaunch(exec, s), "graph launch");
+      }
+      net->step++;
+      if (ok && (k % 16 == 15 || k == spe - 1)) {
+        float l = 0.f;
+        ok = hip_ok(hipMemcpyAsync(&l, net->loss, sizeof(float), hipMemcpyDeviceToHost, s), "copy") &&
+             hip_ok(hipStreamSynchronize(s), "synchronize");
+        acc += l;
+        cnt++;
+      }
+    }
+    if (epoch_loss) epoch_loss[ep] = cnt ? (float)(acc / cnt) : 0.f;
+  }
+  (void)hipStreamSynchronize(s);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (lr_d) (void)hipFree(lr_d);
+  if (perm_d) (void)hipFree(perm_d);
+  if (st) (void)hipFree(st);
+  if (!ok) return bmx_ctx_set_error(ctx, ("bmx_surrogate_train: " + (err.empty() ? std::string(bmx_last_error(ctx)) : err)).c_str());
+  return 0;
+}
+
+int32_t bmx_surrogate_infer(bmx_net* net, const void* X_bf16, int64_t n, float* y) {
+  if (!net) return -1;
+  bmx_ctx* ctx = net->ctx;
+  if (!X_bf16 || !y || n <= 0) return bmx_ctx_set_error(ctx, "bmx_surrogate_infer: invalid arguments");
+  FusedNet fn;
+  if (fused_shape((int32_t)net->dims.size(), net->dims.data(), &fn)) {
+    fn.wb = net->wb;
+    fn.w = net->w;
+    FusedIn in{};
+    in.X = (const __bf16*)X_bf16;
+    launch_fused(bmx_ctx_stream(ctx), n, fn, in, y);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : bmx_ctx_set_error(ctx, hipGetErrorString(e));
+  }
+  // other shapes: layer by layer through the batch buffers, in batches
+  const int64_t B = 1 << 16;
+  if (!net->buffers((int)std::min<int64_t>(B, n))) return bmx_ctx_set_error(ctx, "bmx_surrogate_infer: allocation failed");
+  hipStream_t s = bmx_ctx_stream(ctx);
+  for (int64_t a = 0; a < n; a += net->nb) {
+    int m = (int)std::min<int64_t>(net->nb, n - a);
+    if (hipMemcpyAsync(net->xb, (const __bf16*)X_bf16 + a * net->dims[0], (size_t)m * net->dims[0] * sizeof(__bf16),
+                       hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        net_forward(net, m) != 0 ||
+        hipMemcpyAsync(y + a, net->y, (size_t)m * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return bmx_ctx_set_error(ctx, "bmx_surrogate_infer: forward pass failed");
+  }
+  return 0;
+}
+
+int32_t bmx_surrogate_screen(bmx_net* net, int32_t n_lat, int32_t n_knob, const float* knobs, float knob_range,
+                             int32_t n_ic, int32_t n_geo, const float* geo, int32_t n_amp, float alive_level,
+                             int32_t* kb) {
+  if (!net) return -1;
+  return bmx_da_screen(net->ctx, (int32_t)net->dims.size(), net->dims.data(), net->w, net->wb, n_lat, n_knob, knobs,
+                       knob_range, n_ic, n_geo, geo, n_amp, alive_level, kb);
 }
 
 int32_t bmx_ctx_sync(bmx_ctx* ctx) {

@@ -154,6 +154,7 @@ static int set_err(bmx_ctx* ctx, const std::string& s) {
 
 // Shared with bmx_mlp.hip (hidden: not part of the exported ABI).
 extern "C" __attribute__((visibility("hidden"))) hipStream_t bmx_ctx_stream(const bmx_ctx* ctx) { return ctx->stream; }
+extern "C" __attribute__((visibility("hidden"))) int bmx_ctx_device(const bmx_ctx* ctx) { return ctx->device; }
 // defined in bmx_mlp.hip: frees the context's GEMM workspace
 extern "C" void bmx_mlp_release(const bmx_ctx* ctx);
 extern "C" __attribute__((visibility("hidden"))) int32_t bmx_ctx_set_error(bmx_ctx* ctx, const char* msg) {

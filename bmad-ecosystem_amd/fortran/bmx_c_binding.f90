@@ -282,6 +282,75 @@ interface
     integer(c_int32_t) :: rc
   end function
 
+  ! DA surrogate held by the library: parameters per layer W(out, in) stored
+  ! row-major (in Fortran: W(in, out) column-major) then b(out), FP32; X, Y,
+  ! knobs, geo and kb are device pointers
+  function bmx_surrogate_create(ctx, n_dims, dims, params, n_params, net) &
+                                bind(C, name = 'bmx_surrogate_create') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: n_dims
+    integer(c_int32_t), intent(in) :: dims(*)
+    real(c_float), intent(in) :: params(*)
+    integer(c_int64_t), intent(out) :: n_params
+    type(c_ptr) :: net
+    integer(c_int32_t) :: rc
+  end function
+
+  subroutine bmx_surrogate_destroy(net) bind(C, name = 'bmx_surrogate_destroy')
+    import
+    type(c_ptr), value :: net
+  end subroutine
+
+  function bmx_surrogate_set_params(net, params) bind(C, name = 'bmx_surrogate_set_params') result(rc)
+    import
+    type(c_ptr), value :: net
+    real(c_float), intent(in) :: params(*)
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_surrogate_get_params(net, params) bind(C, name = 'bmx_surrogate_get_params') result(rc)
+    import
+    type(c_ptr), value :: net
+    real(c_float), intent(out) :: params(*)
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_surrogate_train(net, x, y, n, epochs, batch, lr, lr_final, seed, flags, epoch_loss) &
+                               bind(C, name = 'bmx_surrogate_train') result(rc)
+    import
+    type(c_ptr), value :: net, x, y
+    integer(c_int64_t), value :: n
+    integer(c_int32_t), value :: epochs, batch
+    real(c_float), value :: lr, lr_final
+    integer(c_int64_t), value :: seed
+    integer(c_int32_t), value :: flags
+    real(c_float), intent(out) :: epoch_loss(*)
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_surrogate_infer(net, x_bf16, n, y) bind(C, name = 'bmx_surrogate_infer') result(rc)
+    import
+    type(c_ptr), value :: net, x_bf16, y
+    integer(c_int64_t), value :: n
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_surrogate_screen(net, n_lat, n_knob, knobs, knob_range, n_ic, n_geo, geo, n_amp, alive_level, kb) &
+                                bind(C, name = 'bmx_surrogate_screen') result(rc)
+    import
+    type(c_ptr), value :: net
+    integer(c_int32_t), value :: n_lat, n_knob
+    type(c_ptr), value :: knobs
+    real(c_float), value :: knob_range
+    integer(c_int32_t), value :: n_ic, n_geo
+    type(c_ptr), value :: geo
+    integer(c_int32_t), value :: n_amp
+    real(c_float), value :: alive_level
+    type(c_ptr), value :: kb
+    integer(c_int32_t) :: rc
+  end function
+
   ! exact transfer matrices (make_mat6 / transfer_matrix_calc over a range):
   ! vec_in, vec_out (6, n) as coord_struct%vec; mat6 (6, 6, n) holds in
   ! Fortran order the transpose of each C row-major matrix, so ele%mat6 =

@@ -426,6 +426,54 @@ int32_t bmx_da_features(bmx_ctx *ctx, int32_t n_lat, int32_t n_ic, int32_t n_kno
 int32_t bmx_step_fit(bmx_ctx *ctx, int32_t n_rays, int32_t n_amp, const float *pred, float alive_level,
                      int32_t *k_out);
 
+/* The whole forward pass of a surrogate network dims[0] -> 256 -> ... -> 256
+ * -> 1 (dims[0] <= 64, SiLU after each hidden layer; n_dims entries, at
+ * least one hidden layer) in one kernel, activations kept on chip: y[i] for
+ * the n BF16 input rows X (n x dims[0]).  w: FP32 parameters, w_bf16 their
+ * BF16 copy (16-byte aligned), both laid out per layer as W [out][in] then
+ * b [out].  Equal bit for bit to the layer-by-layer bmx_gemm_bf16 pass. */
+int32_t bmx_mlp_forward(bmx_ctx *ctx, int32_t n_dims, const int32_t *dims, const float *w, const void *w_bf16,
+                        const void *X_bf16, int64_t n, float *y);
+
+/* Surrogate screening of a DA scan: for each of n_lat knob vectors (knobs,
+ * n_lat x n_knob) and each of the n_ic grid ICs (geo, n_ic x n_geo, ordered
+ * ray by ray with n_amp amplitudes per ray), the network's prediction
+ * (inputs as bmx_da_features forms them, forward pass as bmx_mlp_forward),
+ * then each ray's step fit (bmx_step_fit): kb, n_lat x (n_ic / n_amp)
+ * predicted boundary indices.  n_knob + n_geo = dims[0]. */
+int32_t bmx_da_screen(bmx_ctx *ctx, int32_t n_dims, const int32_t *dims, const float *w, const void *w_bf16,
+                      int32_t n_lat, int32_t n_knob, const float *knobs, float knob_range, int32_t n_ic,
+                      int32_t n_geo, const float *geo, int32_t n_amp, float alive_level, int32_t *kb);
+
+/* A surrogate network owned by the library, for callers without the Python
+ * wrapper (a Fortran DA driver, a C optimiser).  Layer widths dims[0 ..
+ * n_dims-1] (dims[n_dims-1] = 1), SiLU after every hidden layer; params
+ * (host, FP32, *n_params of them) per layer W [out][in] row-major then
+ * b [out].  Training is minibatch Adam (beta 0.9, 0.999, eps 1e-8) on the
+ * mean-squared error with a cosine learning-rate decay from lr to lr_final
+ * over all steps, each epoch one pass over a shuffle (seeded) of the n rows
+ * of X (device, FP32, n x dims[0]) and Y (device, FP32, n); the remainder of
+ * n / batch is left out.  Each step is captured once into a hipGraph and
+ * replayed (flags & 1: launch every step eagerly instead; same results).
+ * epoch_loss (host, may be NULL): per epoch the mean of the losses sampled
+ * every 16 steps.  bmx_surrogate_infer: y (device, n) for BF16 input rows
+ * (one kernel when the network is dims[0] <= 64 -> 256 ... -> 1, see
+ * bmx_mlp_forward); bmx_surrogate_screen: bmx_da_screen with this network.
+ * A network belongs to its context: calls on it must not overlap, and it
+ * is used only while the context lives (destroying it needs no context). */
+typedef struct bmx_net bmx_net;
+int32_t bmx_surrogate_create(bmx_ctx *ctx, int32_t n_dims, const int32_t *dims, const float *params,
+                             int64_t *n_params, bmx_net **out);
+void bmx_surrogate_destroy(bmx_net *net);
+int32_t bmx_surrogate_set_params(bmx_net *net, const float *params);
+int32_t bmx_surrogate_get_params(const bmx_net *net, float *params);
+int32_t bmx_surrogate_train(bmx_net *net, const float *X, const float *Y, int64_t n, int32_t epochs, int32_t batch,
+                            float lr, float lr_final, uint64_t seed, int32_t flags, float *epoch_loss);
+int32_t bmx_surrogate_infer(bmx_net *net, const void *X_bf16, int64_t n, float *y);
+int32_t bmx_surrogate_screen(bmx_net *net, int32_t n_lat, int32_t n_knob, const float *knobs, float knob_range,
+                             int32_t n_ic, int32_t n_geo, const float *geo, int32_t n_amp, float alive_level,
+                             int32_t *kb);
+
 /* Wait for the work queued on the context's stream. */
 int32_t bmx_ctx_sync(bmx_ctx *ctx);
 

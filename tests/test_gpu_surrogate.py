@@ -247,3 +247,119 @@ def test_step_fit_matches_host(ctx, K):
     print(f"K={K}: identical k on {same.mean():.4f} of rays", flush=True)
     assert same.mean() > 0.99
     assert np.all(got[:50] == K) and np.all(want[:50] == K)
+
+
+@pytest.mark.parametrize("dims,n", [([35, 256, 256, 256, 256, 1], 5000), ([35, 256, 256, 256, 256, 1], 64),
+                                    ([3, 256, 1], 1000), ([64, 256, 256, 1], 777)])
+def test_fused_forward_equals_layered(ctx, dims, n):
+    """bmx_mlp_forward (all layers in one kernel, activations in LDS) gives
+    the layer-by-layer pass's outputs bit for bit: same MFMA operands in the
+    same k order, same epilogues and BF16 roundings."""
+    net = surrogate.MLP(ctx, dims, seed=3)
+    rng = np.random.default_rng(4)
+    xb = torch.from_numpy(rng.uniform(-1, 1, (n, dims[0])).astype(np.float32)).to(torch.bfloat16).to(DEV)
+    torch.cuda.synchronize()
+    y = net.forward(xb)
+    net.sync()  # the library's stream has written y before torch copies it
+    want = y.clone()
+    torch.cuda.synchronize()
+    got = net.forward_fused(xb)
+    net.sync()
+    a, b = got.cpu().numpy(), want.cpu().numpy()
+    print(f"fused vs layered: max |dy| {np.max(np.abs(a - b)):.3e}", flush=True)
+    assert np.array_equal(a, b)
+
+
+def test_fused_forward_rejects_other_shapes(ctx):
+    net = surrogate.MLP(ctx, [3, 64, 64, 1], seed=1)
+    assert not net.fused_ok()
+    xb = torch.zeros(8, 3, dtype=torch.bfloat16, device=DEV)
+    torch.cuda.synchronize()
+    with pytest.raises(RuntimeError, match="bmx_mlp_forward"):
+        net.forward_fused(xb)
+
+
+def test_da_screen_equals_layered_screening(ctx):
+    """bmx_da_screen (inputs formed in the kernel, fused forward, step fit)
+    gives the boundaries of bmx_da_features + layer-by-layer forward +
+    bmx_step_fit, over several lattice chunks."""
+    spec = surrogate.ScanSpec(n_turns=64, n_angle=16, n_amp=128, n_pz=5)
+    net = surrogate.MLP(ctx, [35, 256, 256, 256, 256, 1], seed=7)
+    rng = np.random.default_rng(9)
+    knobs = rng.uniform(-0.2, 0.2, (300, surrogate.N_KNOBS))
+    a = surrogate.predict_boundaries(net, knobs, spec)
+    b = surrogate.predict_boundaries(net, knobs, spec, lat_chunk=97, layered=True)
+    assert a.shape == (300, 5, 16)
+    assert np.array_equal(a, b)
+
+
+def _toy_data(n, d0, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, d0)).astype(np.float32)
+    t = (np.sin(2 * x[:, 0]) * np.cos(x[:, 1 % d0]) + 0.5 * x[:, 2 % d0] ** 2).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("dims", [[35, 256, 256, 256, 256, 1], [3, 64, 64, 1]])
+def test_library_network_graph_training_equals_eager(ctx, dims):
+    """bmx_surrogate_train replaying each step as a hipGraph gives the
+    eager launches' parameters bit for bit, over several epochs (host reads
+    of the loss between replays), and trains: the loss falls."""
+    x, t = _toy_data(8192, dims[0], 11)
+    a = surrogate.Net(ctx, dims, seed=2)
+    b = surrogate.Net(ctx, dims, seed=2)
+    la = a.train(x, t, epochs=3, batch=512, lr=3e-3, lr_final=3e-4, seed=5)
+    lb = b.train(x, t, epochs=3, batch=512, lr=3e-3, lr_final=3e-4, seed=5, eager=True)
+    print("graph losses", la, "eager losses", lb, flush=True)
+    assert la == lb
+    for (Wa, ba), (Wb, bb) in zip(a.params(), b.params()):
+        assert np.array_equal(Wa, Wb) and np.array_equal(ba, bb)
+    assert la[-1] < 0.7 * la[0]
+    a.close()
+    b.close()
+
+
+def test_library_network_trains_like_the_python_loop(ctx):
+    """Same network, data and schedule through the C++ loop and the Python
+    loop (different shuffles): held-out errors within 25 % of each other."""
+    dims = [3, 64, 64, 1]
+    x, t = _toy_data(16384, 3, 12)
+    xv, tv = _toy_data(4096, 3, 13)
+    cnet = surrogate.Net(ctx, dims, seed=4)
+    pnet = surrogate.MLP(ctx, dims, seed=4)
+    surrogate.train(cnet, x, t, epochs=6, batch=256, lr=3e-3, lr_final=3e-4, seed=1)
+    surrogate.train(pnet, x, t, epochs=6, batch=256, lr=3e-3, lr_final=3e-4, seed=1)
+    errs = []
+    for p in (cnet.params(), pnet.params()):
+        h = xv.astype(np.float64)
+        for W, b in p[:-1]:
+            z = h @ W.T + b
+            h = z / (1 + np.exp(-z))
+        y = (h @ p[-1][0].T + p[-1][1])[:, 0]
+        errs.append(float(np.mean((y - tv) ** 2)))
+    print("held-out mse C++ loop %.3e, Python loop %.3e" % tuple(errs), flush=True)
+    assert errs[0] < 1.25 * errs[1] and errs[1] < 1.25 * errs[0]
+    assert errs[0] < 0.05 * np.var(tv)
+    cnet.close()
+
+
+@pytest.mark.parametrize("dims", [[35, 256, 256, 256, 256, 1], [3, 64, 64, 1]])
+def test_library_network_infer_and_screen(ctx, dims):
+    """bmx_surrogate_infer and bmx_surrogate_screen equal the Python
+    network's passes with the same parameters bit for bit."""
+    pnet = surrogate.MLP(ctx, dims, seed=6)
+    cnet = surrogate.Net(ctx, dims, params=pnet.params())
+    rng = np.random.default_rng(14)
+    xb = torch.from_numpy(rng.uniform(-1, 1, (3000, dims[0])).astype(np.float32)).to(torch.bfloat16).to(DEV)
+    torch.cuda.synchronize()
+    want = pnet.forward(xb)
+    pnet.sync()
+    want = want.cpu().numpy()
+    got = cnet.infer(xb).cpu().numpy()
+    assert np.array_equal(got, want)
+    if dims[0] == 35:
+        spec = surrogate.ScanSpec(n_turns=64, n_angle=8, n_amp=64, n_pz=3)
+        knobs = rng.uniform(-0.2, 0.2, (40, surrogate.N_KNOBS))
+        assert np.array_equal(surrogate.predict_boundaries(cnet, knobs, spec),
+                              surrogate.predict_boundaries(pnet, knobs, spec, layered=True))
+    cnet.close()

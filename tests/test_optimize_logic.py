@@ -44,7 +44,7 @@ def _stubs(monkeypatch, noise=0.05):
         return _true_area(k), surv, 0.001
 
     monkeypatch.setattr(surrogate, "training_data", fake_training_data)
-    monkeypatch.setattr(surrogate, "MLP", FakeNet)
+    monkeypatch.setattr(surrogate, "Net", FakeNet)
     monkeypatch.setattr(surrogate, "train", fake_train)
     monkeypatch.setattr(optimize, "predicted_areas", fake_predicted)
     monkeypatch.setattr(optimize, "fp64_evaluate", fake_eval)
