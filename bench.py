@@ -31,6 +31,42 @@ N_PARTICLES = 1_000_000
 FP64_PEAK_TFLOPS = 78.6
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bmx_track_bunch_kernel.ring4000_e6GeV_1e6particles.md")
+
+
+def measured_traffic(particles, turns_per_step):
+    """HBM bytes per launch of the bunch kernel on the default workload as
+    the committed rocprofv3 counter runs measured them (FETCH_SIZE and
+    WRITE_SIZE in separate runs, each scaled by what the counter reads for a
+    copy of known size): the `traffic_bytes_per_launch:` line of the
+    kernel's profile summary.  None for another workload or without it."""
+    if particles != N_PARTICLES or turns_per_step != TURNS_PER_STEP or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
+def physical_cores(cpus):
+    """Physical cores (distinct package, core id) among the logical CPUs
+    this process may run on, from the kernel's CPU topology (what lscpu
+    reports as cores); the logical count when the topology is unreadable."""
+    ids = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+        except OSError:
+            return len(cpus)
+        ids.add((pkg, core))
+    return len(ids) or len(cpus)
+
+
 def cpu_baseline(eles, tw, budget_s=15.0):
     """Oracle (reference loop structure: element-outer, OpenMP over particles,
     bmad/multiparticle/beam_utils.f90:69-74) on a bounded sample of the same
@@ -39,8 +75,9 @@ def cpu_baseline(eles, tw, budget_s=15.0):
     import oracle
     from bmx import abi, beam
 
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 64))
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
+    threads = max(1, min(len(cpus), 64))
+    cores = physical_cores(cpus)
     n = 4096
     coords = beam.matched_gaussian(n, tw, 10e-9, 0.1e-9, 1e-3, 1e-3, seed=20261203)
     arr = oracle.abi_array(eles)
@@ -52,7 +89,7 @@ def cpu_baseline(eles, tw, budget_s=15.0):
         if time.perf_counter() - t0 > budget_s or turns >= 50:
             break
     dt = time.perf_counter() - t0
-    return {"value": n * turns / dt, "unit": "particle-turns/s", "cores": threads, "kind": "port",
+    return {"value": n * turns / dt, "unit": "particle-turns/s", "cores": cores, "threads": threads, "kind": "port",
             "sample": f"{n} particles x {len(eles)} elements x {turns} turns (1 turn per call), "
                       f"element-outer OpenMP loop, {dt:.1f} s",
             "pet_per_s": n * turns * len(eles) / dt}
@@ -281,7 +318,8 @@ def main():
                    "particles": args.particles, "elements": n_ele, "turns_per_step": args.turns_per_step,
                    "particle_element_turns_per_s": value * n_ele, "particles_alive_end": int(n_live)},
         "roofline": {"bound": "fp64-vector", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic(args.particles, args.turns_per_step),
                      "flops_per_pet": flops["flops_per_pet"], "kernel_ms_per_launch": kernel_ms / args.steps},
     }
     lat.close()
