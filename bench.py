@@ -139,12 +139,19 @@ def da_scan(ctx, n_train=4096, n_test=1024, train_ics=8192, epochs=6):
     out["speedup"] = band["speedup"]
     out["da_area_err_mean"] = band["da_area_err_mean"]
     out["da_area_err_max"] = band["da_area_err_max"]
+    out["frac_within_2pct"] = band["frac_within_2pct"]
+    out["da_area_err_all_slices_frac_within_2pct"] = band["da_area_err_all_slices_frac_within_2pct"]
+    out["da_area_err_summed_slices_mean"] = band["da_area_err_summed_slices_mean"]
+    out["error_definition"] = ("DA area = polygon through the surviving prefix of every ray on the on-momentum "
+                               "slice; relative error against the plain scan of the same lattice, over the "
+                               "lattices whose plain area is nonzero; all-slices and summed-slices figures "
+                               "beside it")
     out["method"] = ("band: every IC within +-10 % of the predicted boundary amplitude re-tracked in FP64 in one "
                      "round, the network standing for the untracked amplitudes; no neighbour estimate")
     out["workload"] = (f"error-seeded {8 * DA_CELLS}-element ring with 8 rf cavities, 32 sextupole-family knobs "
                        f"U(+-20%), grid 64 x 128 x 5 (0..{spec.amp_max_x * 1e3:.2f} mm = 30 sigma_x, pz -2..2 %), "
                        f"{DA_TURNS} turns, FP64; training set {n_train} lattices x {train_ics} ICs "
-                       f"(C3 asks 20,000 x 8,192)")
+                       f"(the configured set is 20,000 x 8,192)")
     return out
 
 

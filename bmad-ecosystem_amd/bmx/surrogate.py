@@ -754,7 +754,7 @@ def _resolve_rays(lat, n_lat, spec, lo, hi, refine, turns_per_block, max_rounds,
         per = np.split(c, np.cumsum(counts)[:-1], axis=1)
         tr0 = time.perf_counter()
         surv, st = tracker.track_ics(lat, per, N, turns_per_block, return_stats=True)
-        rounds.append((len(ray), time.perf_counter() - tr0, st["pet"]))
+        rounds.append((len(ray), time.perf_counter() - tr0, st["pet"], st.get("kernel_ms", 0.0) / 1e3))
         surv = np.concatenate(surv)
         lost = surv < N
         np.minimum.at(bnd, ray[lost], amp[lost])
@@ -889,17 +889,24 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
         return dict(lo=np.clip(kb - window[0], 0, spec.n_amp - 1), hi=np.clip(kb + window[1], 0, spec.n_amp - 1),
                     up=max(1, probe_up), down=max(1, probe_down))
 
+    timeline = [dict() for _ in range(workers)]
+
     def work(w):
         try:
             a, b = bounds[w], bounds[w + 1]
+            tw0 = time.perf_counter()
             lat = upload_variants(ctxs[w], knob_sets[a:b], spec)
+            tw1 = time.perf_counter()
             ready[w].wait()
+            tw2 = time.perf_counter()
             bx = boxes[w]
             if "lo" not in bx:  # the prediction failed
                 lat.close()
                 return
             results[w] = _resolve_rays(lat, b - a, spec, bx["lo"], bx["hi"], refine, turns_per_block, max_rounds,
                                        bx["up"], bx["down"])
+            tw3 = time.perf_counter()
+            timeline[w].update(upload_end=tw1 - t0, ready=tw2 - t0, resolved=tw3 - t0)
         except Exception as e:  # re-raised in the caller
             errors.append(e)
 
@@ -914,6 +921,7 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
             kb_w = predict_boundaries(net, knob_sets[bounds[w]:bounds[w + 1]], spec)
             kbs.append(kb_w)
             boxes[w].update(window_of(kb_w))
+            timeline[w]["predicted"] = time.perf_counter() - t0
             ready[w].set()
         t_pred = time.perf_counter()
     finally:
@@ -942,7 +950,8 @@ def surrogate_scan(ctx, net, knob_sets, spec, band=0.10, turns_per_block=0, max_
         runs = neighbour_estimate(bnd, a_top, kb)
     area = np.array([area_from_runs(runs[v, spec.n_pz // 2], spec) for v in range(n_lat)])
     t1 = time.perf_counter()
-    timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb, n_open=n_open)
+    timing = dict(total=t1 - t0, predict=t_pred - t0, track=t1 - t_pred, rounds=rounds, kb=kb, n_open=n_open,
+                  timeline=timeline)
     return runs, area, timing, n_tracked
 
 
@@ -1085,8 +1094,9 @@ def compare_scans(ctx, spec, n_train=64, n_test=512, epochs=4, batch=16384, seed
             "pred_boundary_err_q90": float(np.quantile(np.abs(t_sur["kb"] - plain_runs), 0.9)),
             "pred_boundary_within_band": float(np.mean(np.abs(t_sur["kb"] - plain_runs)
                                                        <= np.maximum(1, band * plain_runs))),
-            "rounds": [[int(n), round(t, 4), float(p)] for n, t, p in t_sur["rounds"]],
-            "predict_s": t_sur["predict"], "pet_surrogate": float(sum(p for _, _, p in t_sur["rounds"])),
+            "rounds": [[int(r[0]), round(r[1], 4), float(r[2]), round(r[3], 4)] for r in t_sur["rounds"]],
+            "predict_s": t_sur["predict"], "pet_surrogate": float(sum(r[2] for r in t_sur["rounds"])),
+            "timeline": t_sur["timeline"],
             "break_even_lattices": (t_gen["total"] + t_train) / per_lat_gain if per_lat_gain > 0 else None,
         }
         rec.update(area_errors(runs, plain_runs, spec))
