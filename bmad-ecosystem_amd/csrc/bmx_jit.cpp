@@ -83,10 +83,11 @@ bool same_window(const std::vector<DevOp>& prog, size_t a, size_t b, int len) {
   return true;
 }
 
-// Greedy parse: cell starts of non-overlapping matches of the window at rep.
-std::vector<size_t> parse_cells(const std::vector<DevOp>& prog, size_t rep, int len) {
+// Greedy parse: cell starts of non-overlapping matches of the window at rep,
+// over the first `limit` ops (all when 0).
+std::vector<size_t> parse_cells(const std::vector<DevOp>& prog, size_t rep, int len, size_t limit = 0) {
   std::vector<size_t> starts;
-  size_t n = prog.size();
+  size_t n = (limit && limit < prog.size()) ? limit : prog.size();
   for (size_t k = 0; k + len <= n;) {
     if (ele_start(prog, k) && same_window(prog, k, rep, len)) {
       starts.push_back(k);
@@ -152,7 +153,9 @@ bool plan_cells(std::vector<DevOp>& prog, CellPlan* plan) {
     int count = 0;
     int64_t rep = frequent_window(prog, n_probe, len, &count);
     if (rep < 0 || count < 4) continue;
-    size_t cov = parse_cells(prog, (size_t)rep, len).size() * (size_t)len;
+    // coverage over the probe prefix: a multi-variant program repeats its
+    // variants, so the prefix ranks the window lengths as the whole does
+    size_t cov = parse_cells(prog, (size_t)rep, len, n_probe).size() * (size_t)len;
     // a longer cell must cover clearly more ops than a shorter one
     if (cov > best_cov + best_cov / 50) {
       best_cov = cov;
@@ -160,8 +163,9 @@ bool plan_cells(std::vector<DevOp>& prog, CellPlan* plan) {
       best_rep = (size_t)rep;
     }
   }
-  if (best_len == 0 || best_cov * 2 < n) return false;
+  if (best_len == 0 || best_cov * 2 < n_probe) return false;
   std::vector<size_t> starts = parse_cells(prog, best_rep, best_len);
+  if (starts.size() * (size_t)best_len * 2 < n) return false;
   for (auto& op : prog) op.flags &= ~F_CELL;
   for (size_t k : starts) prog[k].flags |= F_CELL;
   plan->cell_prv.assign(n, -1);
