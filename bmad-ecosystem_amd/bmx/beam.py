@@ -137,6 +137,108 @@ def twiss_from_tracking(lat, orbit=None, d_orb=None, variant=0):
     return (bx, ax, by, ay, px / (2 * math.pi), py / (2 * math.pi)), M
 
 
+def _symp_conj2(a):
+    """Symplectic conjugate of a 2x2 block: [[d, -b], [-c, a]]."""
+    return np.array([[a[1, 1], -a[0, 1]], [-a[1, 0], a[0, 0]]])
+
+
+def twiss_coupled(m):
+    """Normal-mode (coupled) periodic Twiss from a one-turn matrix m (6x6 or
+    4x4 transverse block; 6x6 also gives the dispersion), as Bmad's
+    twiss_from_mat6 (bmad/code/twiss_from_mat6.f90:86-139) forms it with
+    mat_symp_decouple (bmad/low_level/mat_symp_decouple.f90:63-113), the
+    Edwards-Teng / Sagan-Rubin decomposition T = V U V^-1:
+      with T = [[M, m], [n, N]] and H = m + conj(n) (conj = symplectic
+      conjugate), t = tr M - tr N, D = t^2 + 4 det H (D <= 0: stop band),
+      gamma = sqrt(1/2 + 1/2 sqrt(t^2 / D)), C = -sign(t) H / (gamma sqrt D),
+      V = [[gamma I, C], [-conj(C), gamma I]], U = conj(V) T V,
+    the a and b modes' Twiss from U's diagonal blocks (twiss_from_mat2), the
+    lab dispersion (I - T4)^-1 m[0:4, 5] and the normal-mode one conj(V)
+    times it.  Returns a dict (beta_a, alpha_a, phi_a, beta_b, alpha_b,
+    phi_b, gamma_c, cmat [2x2], eta_x, etap_x, eta_y, etap_y, eta_a, etap_a,
+    eta_b, etap_b when m is 6x6).  Raises ValueError in a stop band or for
+    an unstable mode."""
+    m = np.asarray(m, dtype=np.float64)
+    t4 = m[:4, :4]
+    M, mm, n, N = t4[0:2, 0:2], t4[0:2, 2:4], t4[2:4, 0:2], t4[2:4, 2:4]
+    H = mm + _symp_conj2(n)
+    t = np.trace(M) - np.trace(N)
+    D = t * t + 4.0 * np.linalg.det(H)
+    if np.max(np.abs(H)) == 0.0:
+        gamma, C = 1.0, np.zeros((2, 2))
+    else:
+        if D <= 0.0:
+            raise ValueError("stop band: the modes cannot be decoupled")
+        gamma = math.sqrt(0.5 + 0.5 * math.sqrt(t * t / D))
+        C = -math.copysign(1.0, t) * H / (gamma * math.sqrt(D))
+    V = np.zeros((4, 4))
+    V[0:2, 0:2] = V[2:4, 2:4] = gamma * np.eye(2)
+    V[0:2, 2:4] = C
+    V[2:4, 0:2] = -_symp_conj2(C)
+    Vc = np.zeros((4, 4))  # symplectic conjugate of V
+    Vc[0:2, 0:2] = Vc[2:4, 2:4] = gamma * np.eye(2)
+    Vc[0:2, 2:4] = -C
+    Vc[2:4, 0:2] = _symp_conj2(C)
+    U = Vc @ t4 @ V
+    ba, aa, pa = twiss_from_mat2(U[0:2, 0:2])
+    bb, ab, pb = twiss_from_mat2(U[2:4, 2:4])
+    out = dict(beta_a=ba, alpha_a=aa, phi_a=pa, beta_b=bb, alpha_b=ab, phi_b=pb, gamma_c=gamma, cmat=C)
+    if m.shape == (6, 6):
+        eta = np.linalg.solve(np.eye(4) - t4, m[0:4, 5])
+        ev = Vc @ eta
+        out.update(eta_x=eta[0], etap_x=eta[1], eta_y=eta[2], etap_y=eta[3], eta_a=ev[0], etap_a=ev[1],
+                   eta_b=ev[2], etap_b=ev[3])
+    return out
+
+
+def closed_orbit_4d(one_turn, pz, guess=None, tol=1e-16, max_iter=40):
+    """Transverse closed orbit at fixed pz by Newton steps on the exact
+    one-turn map: one_turn(v) -> (out [6], J [6, 6]).  Returns v [6]."""
+    v = np.zeros(6) if guess is None else np.array(guess, dtype=np.float64)
+    v[5] = pz
+    for _ in range(max_iter):
+        out, J = one_turn(v)
+        dv = np.linalg.solve(J[:4, :4] - np.eye(4), -(out[:4] - v[:4]))
+        v[:4] += dv
+        if np.max(np.abs(dv)) <= tol * max(1.0, np.max(np.abs(v[:4]))):
+            break
+    return v
+
+
+def chromatic_twiss(one_turn, pz=0.0, delta=1e-4):
+    """d(beta, alpha)/d pz of both normal modes as chrom_calc forms them
+    (bmad/code/chrom_calc.f90:72-137): the rings' Twiss (rf off) at the
+    transverse closed orbits of pz +- delta (delta_e, default 1e-4), and
+    central differences over 2 delta.  Also the chromaticities (tune
+    change / (2 delta)).  one_turn as closed_orbit_4d."""
+    tw = {}
+    for s in (1, -1):
+        v = closed_orbit_4d(one_turn, pz + s * delta)
+        tw[s] = twiss_coupled(one_turn(v)[1])
+    out = {}
+    for k in ("beta", "alpha"):
+        for mode in ("a", "b"):
+            out[f"d{k}_dpz_{mode}"] = (tw[1][f"{k}_{mode}"] - tw[-1][f"{k}_{mode}"]) / (2 * delta)
+    for mode in ("a", "b"):
+        dphi = tw[1][f"phi_{mode}"] - tw[-1][f"phi_{mode}"]
+        dphi = (dphi + math.pi) % (2 * math.pi) - math.pi
+        out[f"chrom_{mode}"] = dphi / (2 * math.pi) / (2 * delta)
+    return out
+
+
+def twiss_coupled_from_tracking(lat, orbit=None):
+    """Normal-mode periodic Twiss at the lattice start from the device's
+    exact one-turn matrix about orbit (tracker.transfer_matrix; the closed
+    orbit, zero when None).  Upload the lattice without rf for the ring's
+    optics, as Bmad's Twiss calculation does."""
+    from . import tracker
+    v = np.zeros(6) if orbit is None else np.asarray(orbit, dtype=np.float64)
+    _, mat, st = tracker.transfer_matrix(lat, v)
+    if st[0] != 1:
+        raise ValueError("the orbit was lost within one turn")
+    return twiss_coupled(mat[0]), mat[0]
+
+
 def matched_gaussian(n, twiss, eps_x, eps_y, sigma_z, sigma_pz, seed):
     """(6, n) float64 array of a Twiss-matched 6-D Gaussian bunch."""
     bx, ax, by, ay = twiss[:4]

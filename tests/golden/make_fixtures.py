@@ -263,10 +263,41 @@ def mat6():
                 printed_digits=14, elements=MAT6_ELEMENTS, rows=rows)
 
 
+# nonlin_test/small_ring.bmad (lines 28-40): the ring whose file records the
+# coupled periodic Twiss Bmad computed for it (beginning[...], lines 7-22).
+# rf is off for the Twiss (voltage 0 here), as the recorded values need.
+SMALL_RING = [
+    dict(key="sextupole", l=0.01, num_steps=1),
+    dict(key="drift", l=0.3),
+    dict(key="quadrupole", l=0.5, tilt=0.05, k1=0.6, num_steps=1),
+    dict(key="drift", l=0.3),
+    dict(key="sbend", l=3.0, g=1.30899693899574704e-1, e1=0.07, e2=0.07, num_steps=1),
+    dict(key="drift", l=0.3),
+    dict(key="quadrupole", l=0.5, k1=-0.5, num_steps=1),
+    dict(key="drift", l=0.3),
+    dict(key="sextupole", l=0.01, num_steps=1),
+    dict(key="rfcavity", l=1.2, voltage=0.0, rf_frequency=2.5e8, num_steps=10),
+    dict(key="sbend", l=3.0, g=1.30899693899574704e-1, e1=0.07, e2=0.07, num_steps=1),
+]
+
+
+def nonlin():
+    """The beginning[...] Twiss of nonlin_test/small_ring.bmad (positron, p0c 1 GeV)."""
+    path = os.path.join(REF, "regression_tests/nonlin_test/small_ring.bmad")
+    twiss = {}
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = re.match(r"^beginning\[(\w+)\]\s*=\s*(\S+)", line.strip())
+            if m:
+                twiss[m.group(1)] = dict(value=float(m.group(2)), line=lineno)
+    return dict(source="regression_tests/nonlin_test/small_ring.bmad", particle="positron", p0c=1e9,
+                elements=SMALL_RING, twiss=twiss, rows=twiss)
+
+
 if __name__ == "__main__":
     for name, fn in (("tracking_method_bmad_standard.json", tracking_method), ("fringe_test.json", fringe),
                      ("aperture_test.json", aperture), ("multipole_test.json", multipole),
-                     ("mat6_calc_method_test.json", mat6)):
+                     ("mat6_calc_method_test.json", mat6), ("nonlin_small_ring.json", nonlin)):
         d = fn()
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(d, f, indent=1, sort_keys=True)

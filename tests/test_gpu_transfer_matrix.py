@@ -141,3 +141,26 @@ def test_mat6_golden_rows_on_the_device(ctx):
             return col
 
         tog.m6_check(name, M[0], out[0], fd)
+
+
+def test_coupled_and_chromatic_twiss_of_the_nonlin_test_ring(ctx):
+    """The Twiss nonlin_test's small ring (rf off): the device's one-turn
+    matrices give the coupled Twiss Bmad recorded at its start to 1e-12 and
+    chrom_calc's dbeta/dalpha_dpz of both modes to 1e-9 relative."""
+    import test_oracle_golden as tog
+    eles = tog._small_ring()
+    lat = ctx.lattice(eles, abi.POSITRON)
+    tw, T = beam.twiss_coupled_from_tracking(lat)
+    tog.check_coupled_twiss(T)
+
+    def one_turn(v):
+        out, M, st = tracker.transfer_matrix(lat, v)
+        assert st[0] == abi.ALIVE
+        return out[0], M[0]
+
+    res = beam.chromatic_twiss(one_turn)
+    lat.close()
+    g = {k: v["value"] for k, v in tog.NL["twiss"].items()}
+    for k in ("dbeta_dpz_a", "dbeta_dpz_b", "dalpha_dpz_a", "dalpha_dpz_b"):
+        print(f"{k}: device {res[k]:.12g} recorded {g[k]:.12g}", flush=True)
+        assert abs(res[k] - g[k]) <= 1e-9 * abs(g[k]), (k, res[k], g[k])

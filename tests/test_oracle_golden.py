@@ -262,3 +262,68 @@ def test_mat6_bmad_standard_rows(name):
         return col
 
     m6_check(name, J, out, fd)
+
+
+NL = _load("nonlin_small_ring.json")
+
+
+def _small_ring():
+    return [ele_from_spec(sp, abi.POSITRON, p0c=NL["p0c"]) for sp in NL["elements"]]
+
+
+def check_coupled_twiss(T):
+    """twiss_coupled of the small ring's one-turn matrix T against the
+    coupled Twiss Bmad recorded for it (beginning[...] of small_ring.bmad)."""
+    from bmx import beam
+    tw = beam.twiss_coupled(T)
+    g = {k: v["value"] for k, v in NL["twiss"].items()}
+    got = dict(beta_a=tw["beta_a"], alpha_a=tw["alpha_a"], beta_b=tw["beta_b"], alpha_b=tw["alpha_b"],
+               cmat_11=tw["cmat"][0, 0], cmat_12=tw["cmat"][0, 1], cmat_21=tw["cmat"][1, 0], cmat_22=tw["cmat"][1, 1],
+               eta_x=tw["eta_x"], etap_x=tw["etap_x"], eta_y=tw["eta_y"], etap_y=tw["etap_y"])
+    for k, v in got.items():
+        assert abs(v - g[k]) <= 1e-12 * max(1.0, abs(g[k])), (k, v, g[k])
+    return tw
+
+
+def test_coupled_twiss_of_the_nonlin_test_ring():
+    """nonlin_test/small_ring.bmad records the ring's coupled periodic Twiss
+    (tilted quadrupole): the oracle's exact one-turn matrix, decoupled by
+    twiss_coupled, gives beta, alpha of both modes, the coupling matrix and
+    the lab dispersion to 1e-12 (the file prints 18 digits)."""
+    _, T = oracle.jacobian(_small_ring(), abi.POSITRON, 1, np.zeros(6))
+    tw = check_coupled_twiss(T)
+    assert 0.9 < tw["gamma_c"] < 1.0
+
+
+def test_coupled_twiss_reduces_to_uncoupled_and_rebuilds_the_map():
+    """Invariants of the decomposition: without coupling it is twiss_from_mat2
+    of each plane; for a coupled map V U conj(V) = T (V symplectic)."""
+    from bmx import beam, lattice
+    eles = lattice.fodo_ring_c0(n_cells=10)
+    _, T = oracle.jacobian(eles, abi.ELECTRON, 1, np.zeros(6))
+    tw = beam.twiss_coupled(T)
+    bx, ax, _ = beam.twiss_from_mat2(T[0:2, 0:2])
+    by, ay, _ = beam.twiss_from_mat2(T[2:4, 2:4])
+    assert tw["gamma_c"] == 1.0 and np.all(tw["cmat"] == 0)
+    assert np.allclose([tw["beta_a"], tw["alpha_a"], tw["beta_b"], tw["alpha_b"]], [bx, ax, by, ay], rtol=1e-14)
+    _, T = oracle.jacobian(_small_ring(), abi.POSITRON, 1, np.zeros(6))
+    tw = beam.twiss_coupled(T)
+    g, Cm = tw["gamma_c"], tw["cmat"]
+    V = np.block([[g * np.eye(2), Cm], [-beam._symp_conj2(Cm), g * np.eye(2)]])
+    Vc = np.block([[g * np.eye(2), -Cm], [beam._symp_conj2(Cm), g * np.eye(2)]])
+    assert np.allclose(V @ Vc, np.eye(4), atol=1e-14)
+    U = Vc @ T[:4, :4] @ V
+    assert np.max(np.abs(U[0:2, 2:4])) < 1e-14 and np.max(np.abs(U[2:4, 0:2])) < 1e-14
+
+
+def test_chromatic_twiss_of_the_nonlin_test_ring():
+    """dbeta/dpz and dalpha/dpz of both modes recorded in small_ring.bmad,
+    by chrom_calc's central differences (delta 1e-4) of the coupled Twiss at
+    the off-momentum closed orbits: to 1e-9 relative (the differences'
+    rounding at delta 1e-4)."""
+    from bmx import beam
+    eles = _small_ring()
+    res = beam.chromatic_twiss(lambda v: oracle.jacobian(eles, abi.POSITRON, 1, v))
+    g = {k: v["value"] for k, v in NL["twiss"].items()}
+    for k in ("dbeta_dpz_a", "dbeta_dpz_b", "dalpha_dpz_a", "dalpha_dpz_b"):
+        assert abs(res[k] - g[k]) <= 1e-9 * abs(g[k]), (k, res[k], g[k])
