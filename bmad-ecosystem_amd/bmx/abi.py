@@ -91,13 +91,25 @@ class Schedule(C.Structure):
     _fields_ = [("turns_per_block", C.c_int32), ("graph_depth", C.c_int32), ("probe_pet_per_s", C.c_double)]
 
 
+class RadMap(C.Structure):
+    """bmx_rad_map: one half-element radiation map (rad_map_struct), row-major matrices."""
+    _fields_ = [("ref_orb", C.c_double * 6), ("damp_dmat", C.c_double * 36), ("xfer_damp_vec", C.c_double * 6),
+                ("stoc_mat", C.c_double * 36)]
+
+
+class Radiation(C.Structure):
+    _fields_ = [("damping_on", C.c_int32), ("fluctuations_on", C.c_int32), ("zero_average", C.c_int32),
+                ("reserved", C.c_int32), ("synch_rad_scale", C.c_double), ("seed", C.c_uint64)]
+
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libbmx.so")
 
 # every entry point declared in include/bmx.h
 EXPORTS = [
     "bmx_abi_version", "bmx_sizeof", "bmx_ctx_create", "bmx_ctx_destroy", "bmx_last_error",
-    "bmx_lattice_upload", "bmx_lattice_upload_scaled", "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch",
+    "bmx_lattice_upload", "bmx_lattice_upload_rad", "bmx_lattice_set_rad_stream", "bmx_lattice_upload_scaled",
+    "bmx_lattice_destroy", "bmx_lattice_check", "bmx_lattice_cells", "bmx_track_bunch",
     "bmx_track_bunch_noalloc", "bmx_track_multi", "bmx_track_grid",
     "bmx_track_ics", "bmx_bunch_moments", "bmx_closed_orbit", "bmx_one_turn_matrix", "bmx_transfer_matrix", "bmx_ctx_set_schedule", "bmx_ctx_get_schedule",
     "bmx_tune_grid",
@@ -130,6 +142,12 @@ def load_library(path=None):
     lib.bmx_lattice_upload.restype = C.c_int32
     lib.bmx_lattice_upload.argtypes = [C.c_void_p, C.POINTER(EleRec), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
                                        C.POINTER(Ref), C.POINTER(Config), C.POINTER(C.c_void_p)]
+    lib.bmx_lattice_upload_rad.restype = C.c_int32
+    lib.bmx_lattice_upload_rad.argtypes = [C.c_void_p, C.POINTER(EleRec), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                           C.POINTER(RadMap), C.POINTER(Radiation), C.POINTER(Ref), C.POINTER(Config),
+                                           C.POINTER(C.c_void_p)]
+    lib.bmx_lattice_set_rad_stream.restype = C.c_int32
+    lib.bmx_lattice_set_rad_stream.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
     lib.bmx_lattice_upload_scaled.restype = C.c_int32
     lib.bmx_lattice_upload_scaled.argtypes = [C.c_void_p, C.POINTER(EleRec), C.c_int32, C.c_int32, C.c_int32,
                                               C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double),
@@ -225,7 +243,7 @@ def load_library(path=None):
 
 def check_layout(lib):
     """Compare ctypes struct sizes with the compiled library's."""
-    sizes = {0: EleRec, 1: Particles, 2: Stats, 3: Grid, 4: Config, 5: Schedule}
+    sizes = {0: EleRec, 1: Particles, 2: Stats, 3: Grid, 4: Config, 5: Schedule, 6: RadMap, 7: Radiation}
     bad = {k: (C.sizeof(v), lib.bmx_sizeof(k)) for k, v in sizes.items() if C.sizeof(v) != lib.bmx_sizeof(k)}
     if bad:
         raise RuntimeError(f"ABI layout mismatch (ctypes, library): {bad}")

@@ -68,9 +68,9 @@ class Context:
         _check(self.h, self.lib.bmx_ctx_set_schedule(self.h, C.byref(sc)), "bmx_ctx_set_schedule")
 
     def lattice(self, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3,
-                tracked_species=None, absolute_time_tracking=False):
+                tracked_species=None, absolute_time_tracking=False, rad_maps=None, radiation=None):
         return Lattice(self, eles, species, variant_offsets, aperture_limit_on, max_aperture_limit, tracked_species,
-                       absolute_time_tracking)
+                       absolute_time_tracking, rad_maps, radiation)
 
 
 def check_lattice(eles, species, aperture_limit_on=True, max_aperture_limit=1e3, tracked_species=None, direction=1,
@@ -95,7 +95,13 @@ class Lattice:
     reference species (its kicks and voltages change sign)."""
 
     def __init__(self, ctx, eles, species, variant_offsets=None, aperture_limit_on=True, max_aperture_limit=1e3,
-                 tracked_species=None, absolute_time_tracking=False):
+                 tracked_species=None, absolute_time_tracking=False, rad_maps=None, radiation=None):
+        """rad_maps: None, or the radiation maps of every element as a
+        float64 array [n_ele, 2, 84] (per element rm0 then rm1, each
+        ref_orb[6], damp_dmat[36] row-major, xfer_damp_vec[6], stoc_mat[36]
+        row-major); radiation: dict of the bmx_radiation switches
+        (damping_on, fluctuations_on, zero_average, synch_rad_scale, seed)
+        (bmx_lattice_upload_rad)."""
         self.ctx = ctx
         self.species = species
         self.tracked_species = tracked_species or species
@@ -117,9 +123,30 @@ class Lattice:
             self.n_ele = [variant_offsets[i + 1] - variant_offsets[i] for i in range(n_var)]
         self.n_variants = n_var
         h = C.c_void_p()
-        rc = ctx.lib.bmx_lattice_upload(ctx.h, arr, len(arr), offs_p, n_var, C.byref(ref), C.byref(cfg), C.byref(h))
-        _check(ctx.h, rc, "bmx_lattice_upload")
+        if rad_maps is None and radiation is None:
+            rc = ctx.lib.bmx_lattice_upload(ctx.h, arr, len(arr), offs_p, n_var, C.byref(ref), C.byref(cfg),
+                                            C.byref(h))
+            _check(ctx.h, rc, "bmx_lattice_upload")
+        else:
+            r = dict(damping_on=True, fluctuations_on=False, zero_average=False, synch_rad_scale=1.0, seed=0)
+            r.update(radiation or {})
+            rad = abi.Radiation(int(bool(r["damping_on"])), int(bool(r["fluctuations_on"])),
+                                int(bool(r["zero_average"])), 0, float(r["synch_rad_scale"]), int(r["seed"]))
+            maps = np.ascontiguousarray(np.asarray(rad_maps, dtype=np.float64))
+            if maps.shape != (len(arr), 2, 84):
+                raise ValueError(f"rad_maps must be [n_ele, 2, 84], got {maps.shape}")
+            self._rad_maps = maps  # kept alive for the call
+            rc = ctx.lib.bmx_lattice_upload_rad(ctx.h, arr, len(arr), offs_p, n_var,
+                                                maps.ctypes.data_as(C.POINTER(abi.RadMap)), C.byref(rad),
+                                                C.byref(ref), C.byref(cfg), C.byref(h))
+            _check(ctx.h, rc, "bmx_lattice_upload_rad")
         self.h = h
+
+    def set_rad_stream(self, seed, next_call=0):
+        """Seed of the fluctuation stream and the number of the next call that
+        draws from it (bmx_lattice_set_rad_stream)."""
+        _check(self.ctx.h, self.ctx.lib.bmx_lattice_set_rad_stream(self.h, int(seed), int(next_call)),
+               "bmx_lattice_set_rad_stream")
 
     @classmethod
     def scaled(cls, ctx, eles, species, attr, sel, factors, aperture_limit_on=True, max_aperture_limit=1e3):

@@ -60,6 +60,7 @@ enum OpType : int32_t {
   OP_BEND_EKICK = 22,    // electric multipole kick inside a bend, scaled by (1 + g x) / (ps beta)
   OP_DRIFT_BACK = 23,    // exact drift tracked backward (direction = -1), c[0] = L, c[1] = beta_ref
   OP_ZBETA = 24,         // z -= c[0] beta (the quadrupole's backward 2 c beta delta_ref_time per step)
+  OP_RAD = 25,           // track1_radiation at one edge: c[0] = the map's index in LatticeConsts::rad
 };
 
 // op flags
@@ -97,7 +98,19 @@ enum : int32_t {
   F_END_INBOX = 1 << 21,
   F_END_CENTERED = 1 << 22,
   F_END_PS = 1 << 23,
+  // OP_RAD: damping (damp_dmat, xfer_damp_vec), fluctuations (stoc_mat), and
+  // the exit edge (orbit_too_large first, as track1 checks it before the
+  // exit half of the radiation)
+  F_RAD_DAMP = 1,
+  F_RAD_FLUCT = 2,
+  F_RAD_EXIT = 4,
 };
+
+// One radiation map in LatticeConsts::rad, pre-scaled by synch_rad_scale:
+// ref_orb[6], damp_dmat[36] (row-major), xfer_damp_vec[6] (zero with
+// radiation_zero_average), the lower triangle of stoc_mat row by row (21),
+// padded to 72 doubles.
+constexpr int RAD_REF = 0, RAD_DAMP = 6, RAD_VEC = 42, RAD_STOC = 48, RAD_STRIDE = 72;
 
 // aperture constants of an op carrying F_END (6 slots, see aperture())
 constexpr int END_C = 6;
@@ -151,6 +164,20 @@ struct LatticeConsts {
   int32_t tt_sign, ix_first;
   int32_t abs_time;  // the program has absolute-time rf cavities
   int32_t n_ele;
+  // radiation maps of the program's OP_RAD ops (RAD_STRIDE doubles each),
+  // and the fluctuation stream: Philox key (the seed) and this call's number
+  const double* rad;
+  uint32_t rad_key0, rad_key1, rad_call;
+  int32_t rad_fluct;  // some OP_RAD of the program carries F_RAD_FLUCT
+};
+
+// Radiation input of one variant for the lattice compiler: two maps per
+// element (rm0, rm1) and the switches.  The compiler appends each radiation
+// step's pre-scaled map to its table (RAD_STRIDE doubles) and points the
+// OP_RAD at it.
+struct RadInput {
+  const bmx_rad_map* maps;
+  bmx_radiation cfg;
 };
 
 // Reference time of the current pass relative to the lattice frame, as
@@ -159,6 +186,7 @@ struct LatticeConsts {
 struct TurnTime {
   int32_t k;
   double r;
+  int32_t turn;  // pass number within the call (the radiation stream's counter)
 };
 
 }  // namespace bmx

@@ -119,11 +119,19 @@ bool has_abs_time(const std::vector<DevOp>& prog) {
   return false;
 }
 
+// radiation steps anywhere in the program: the lanes carry their particle index
+bool has_rad(const std::vector<DevOp>& prog) {
+  for (const DevOp& op : prog)
+    if (op.type == OP_RAD) return true;
+  return false;
+}
+
 std::string cell_source(const std::vector<DevOp>& prog, size_t rep, int len) {
   std::string s;
   s += "#include \"bmx.h\"\n#include \"bmx_internal.h\"\nusing namespace bmx;\n";
   s += "#define BMX_CELL_LEN " + std::to_string(len) + "\n";
   s += std::string("#define BMX_ABS_TIME ") + (has_abs_time(prog) ? "1" : "0") + "\n";
+  s += std::string("#define BMX_RAD ") + (has_rad(prog) ? "1" : "0") + "\n";
   s += kSrcOpsH;
   s += "\ntemplate <class Loss>\n__device__ __forceinline__ void jit_cell(Lane& L, const DevOp* o, "
        "const LatticeConsts& lc, const TurnTime& tt, Loss loss) {\n";
@@ -191,7 +199,8 @@ bool plan_cells(std::vector<DevOp>& prog, CellPlan* plan) {
 const JitKernels* jit_kernels(int device, const std::vector<DevOp>& prog, const CellPlan& plan, std::string* err) {
   std::lock_guard<std::mutex> lock(g_mu);
   const char* fl = std::getenv("BMX_JIT_FLAGS");
-  auto key = std::make_pair(device, plan.key + "|" + (has_abs_time(prog) ? "abs|" : "") + (fl ? fl : ""));
+  auto key = std::make_pair(device, plan.key + "|" + (has_abs_time(prog) ? "abs|" : "") + (has_rad(prog) ? "rad|" : "") +
+                                        (fl ? fl : ""));
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return &it->second.k;
   hipDeviceProp_t prop;

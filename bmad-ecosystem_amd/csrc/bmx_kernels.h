@@ -46,6 +46,9 @@ __device__ __forceinline__ void park(Lane& L, const LatticeConsts& lc) {
 #if BMX_ABS_TIME
   L.dt = 0.0;
 #endif
+#if BMX_RAD
+  L.pid = 0;
+#endif
   L.state = BMX_ALIVE;
 }
 
@@ -86,6 +89,7 @@ __device__ __forceinline__ void walk(Pack<NP>& P, const DevOp* __restrict__ ops,
       TurnTime tt;
       tt.k = lc.tt_sign * (turn0 + t + seg);
       tt.r = (double)(lc.tt_sign * (turn0 + t)) * lc.tt_d;
+      tt.turn = turn0 + t;
       for (int c = lo; c < hi;) {
         int m = min(CHUNK, hi - c);
 #ifdef BMX_CELL_LEN
@@ -183,6 +187,9 @@ __device__ __forceinline__ void track_lanes(int64_t base, int64_t end, const Dev
     if (entered[j]) {
       Lane& L = P.L[j];
       L.x = p.x[i]; L.px = p.px[i]; L.y = p.y[i]; L.py = p.py[i]; L.z = p.z[i]; L.pz = p.pz[i]; L.beta = p.beta[i];
+#if BMX_RAD
+      L.pid = (uint32_t)i;
+#endif
       refresh_invariants(L, lc);
       P.live[j] = !too_large(L, lc);
       if (p.t && P.live[j]) {

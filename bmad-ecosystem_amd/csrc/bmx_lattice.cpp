@@ -478,7 +478,7 @@ int used_consts(const DevOp& op) {
   int nh = op.flags & 0xff;
   switch (op.type) {
     case OP_ELE_END: return 0;
-    case OP_DRIFT: case OP_ZBETA: return 1;
+    case OP_DRIFT: case OP_ZBETA: case OP_RAD: return 1;
     case OP_DRIFT_BACK: return 2;
     case OP_QUAD: return 3;
     case OP_TILT: case OP_HARD_MULT_EDGE: return 2;
@@ -497,7 +497,7 @@ int used_consts(const DevOp& op) {
 // Compile one variant.  Returns false and fills *err for unsupported input.
 bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, const bmx_config& cfg,
                      std::vector<DevOp>& prog, std::vector<int32_t>& ele_op_begin, LatticeConsts* lc,
-                     std::string* err) {
+                     std::string* err, const RadInput* rad, std::vector<double>* rad_table) {
   char buf[256];
   if (n_ele <= 0) { *err = "empty lattice"; return false; }
   double p0c = eles[0].p0c, e_tot = eles[0].e_tot;
@@ -639,6 +639,43 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
       }
       prog.push_back(op);
     }
+
+    // track1_radiation at the element's two edges (track1.f90:198-201,
+    // :316-319): the first half after the entrance aperture, the second
+    // after orbit_too_large (F_RAD_EXIT) and before the exit aperture.  No
+    // step for the elements track1_radiation skips (radiation_mod.f90:87-91)
+    // or for a map that changes nothing.
+    const bool rad_here = rad && (rad->cfg.damping_on || rad->cfg.fluctuations_on) && e.is_on && e.l != 0 &&
+                          e.key != BMX_KEY_DRIFT && e.key != BMX_KEY_PIPE && e.key != BMX_KEY_MARKER &&
+                          e.key != BMX_KEY_MULTIPOLE && e.key != BMX_KEY_AB_MULTIPOLE;
+    auto emit_rad = [&](bool exit_edge) -> bool {
+      if (!rad_here) return true;
+      const bmx_rad_map& rm = rad->maps[2 * ie + (exit_edge ? 1 : 0)];
+      const double sc = rad->cfg.synch_rad_scale;
+      double m[RAD_STRIDE] = {0.0};
+      bool damp = false, fluct = false;
+      for (int i = 0; i < 6; i++) m[RAD_REF + i] = rm.ref_orb[i];
+      if (rad->cfg.damping_on) {
+        for (int i = 0; i < 36; i++) { m[RAD_DAMP + i] = sc * rm.damp_dmat[i]; damp |= m[RAD_DAMP + i] != 0.0; }
+        if (!rad->cfg.zero_average)
+          for (int i = 0; i < 6; i++) { m[RAD_VEC + i] = sc * rm.xfer_damp_vec[i]; damp |= m[RAD_VEC + i] != 0.0; }
+      }
+      if (rad->cfg.fluctuations_on) {
+        int k = RAD_STOC;
+        for (int r = 0; r < 6; r++)
+          for (int c = 0; c <= r; c++) { m[k] = sc * rm.stoc_mat[r * 6 + c]; fluct |= m[k] != 0.0; k++; }
+      }
+      if (!damp && !fluct) return true;
+      if (dir == -1) return fail("radiation is tracked forward only");
+      DevOp op = make_op(OP_RAD, ie);
+      op.flags = (damp ? F_RAD_DAMP : 0) | (fluct ? F_RAD_FLUCT : 0) | (exit_edge ? F_RAD_EXIT : 0);
+      op.c[0] = (double)(rad_table->size() / RAD_STRIDE);
+      rad_table->insert(rad_table->end(), m, m + RAD_STRIDE);
+      if (fluct) lc->rad_fluct = 1;
+      prog.push_back(op);
+      return true;
+    };
+    if (!emit_rad(false)) return false;
 
     int key = e.key;
     if (!e.is_on) {
@@ -1006,6 +1043,7 @@ bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, con
         prog.pop_back();
       }
     }
+    if (!emit_rad(true)) return false;
     if (exit_ap && ps_set) {  // an element end of its own: phase-space limits at c[0..7], aperture at c[8..13]
       DevOp op = make_op(OP_ELE_END, ie);
       op.flags = end_flags | F_END_PS | ps_flags;

@@ -18,6 +18,11 @@
 #ifndef BMX_ABS_TIME
 #define BMX_ABS_TIME 1
 #endif
+// BMX_RAD 0 (runtime-compiled cell kernels of a lattice without radiation)
+// leaves out the radiation op and the particle index it draws with.
+#ifndef BMX_RAD
+#define BMX_RAD 1
+#endif
 struct Lane {
   double x, px, y, py, z, pz, beta;
   double irp;  // 1 / (1 + pz)
@@ -28,6 +33,9 @@ struct Lane {
   // path conserves (apply_energy_kick scales z with beta); zero for a
   // particle set up by init_coord (init_coord.f90:222)
   double dt;
+#endif
+#if BMX_RAD
+  uint32_t pid;  // the particle's index in its store: the radiation stream's counter
 #endif
   int state;
 };
@@ -1117,6 +1125,100 @@ __device__ __forceinline__ void ele_end(Lane& L, const bmx_f64* c, int flags, co
   }
 }
 
+#if BMX_RAD
+// Philox4x32-10 (J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11): ten rounds of two
+// 32x32 -> 64-bit products on the counter, the key bumped by the Weyl
+// constants between rounds.  Counter-based, so a particle's deviates depend
+// only on (key, counter), not on which lane or launch tracks it.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Two standard normal deviates (Box-Muller) from one Philox block: u1 in
+// (0, 1] and u2 in [0, 1) from 53 bits each.
+__device__ __forceinline__ void gauss_pair(const uint32_t r[4], bmx_f64* n0, bmx_f64* n1) {
+  bmx_f64 u1 = ((bmx_f64)(((uint64_t)r[0] << 21) | (r[1] >> 11)) + 1.0) * 0x1p-53;
+  bmx_f64 u2 = (bmx_f64)(((uint64_t)r[2] << 21) | (r[3] >> 11)) * 0x1p-53;
+  bmx_f64 rad = ::sqrt(-2.0 * ::log(u1));
+  bmx_f64 sn, cs;
+  ::sincos(D_TWOPI * u2, &sn, &cs);
+  *n0 = rad * cs;
+  *n1 = rad * sn;
+}
+
+// track1_radiation (bmad/modules/radiation_mod.f90:62-141) at one edge, with
+// the map radiation_map_setup made for that half of the element
+// (ele%rad_map%rm0 at the entrance, %rm1 at the exit), pre-scaled by
+// synch_rad_scale: vec += D (vec - ref_orb) + xfer_damp_vec, then
+// vec += S N(0, 1)^6 with the Cholesky factor S, then pz < -1 is lost_pz.
+// At the exit edge track1 first checks orbit_too_large (track1.f90:281-284).
+// beta keeps its value, as in the reference; 1/(1+pz) and the drift's
+// beta/beta0 - 1, which this code forms from pz, follow pz.  The particle
+// time does not change, so its offset from the reference relation takes
+// the change of z.
+__device__ void op_rad(Lane& L, const DevOp& op, int flags, const LatticeConsts& lc, const TurnTime& tt) {
+  if ((flags & F_RAD_EXIT) && too_large(L, lc)) return;
+  // the map's address, wave-uniform: its entries come in through the scalar cache
+  uint64_t a = (uint64_t)(uintptr_t)lc.rad + (uint64_t)(int64_t)(int)op.c[0] * (RAD_STRIDE * sizeof(bmx_f64));
+  uint32_t a_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  uint32_t a_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+  const cdouble* m = (const cdouble*)(uintptr_t)(((uint64_t)a_hi << 32) | a_lo);
+  double v[6] = {L.x, L.px, L.y, L.py, L.z, L.pz};
+  if (flags & F_RAD_DAMP) {
+    double d[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) d[c] = v[c] - sg(m[RAD_REF + c]);
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      double a = sg(m[RAD_DAMP + r * 6]) * d[0];
+#pragma unroll
+      for (int c = 1; c < 6; c++) a = a + sg(m[RAD_DAMP + r * 6 + c]) * d[c];
+      v[r] = v[r] + a;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) v[r] = v[r] + sg(m[RAD_VEC + r]);
+  }
+#ifndef BMX_TANGENT
+  if (flags & F_RAD_FLUCT) {
+    bmx_f64 g[6];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      uint32_t ctr[4] = {L.pid, (uint32_t)(op.ie * 2 + ((flags & F_RAD_EXIT) ? 1 : 0)), (uint32_t)tt.turn,
+                         lc.rad_call * 4u + (uint32_t)q};
+      philox4x32_10(ctr, lc.rad_key0, lc.rad_key1);
+      gauss_pair(ctr, &g[2 * q], &g[2 * q + 1]);
+    }
+    int k = RAD_STOC;
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      double a = 0.0;
+#pragma unroll
+      for (int c = 0; c <= r; c++) a = a + sg(m[k++]) * g[c];
+      v[r] = v[r] + a;
+    }
+  }
+#endif
+#if BMX_ABS_TIME
+  L.dt = L.dt + (v[4] - L.z) / (L.beta * D_C_LIGHT);
+#endif
+  L.x = v[0]; L.px = v[1]; L.y = v[2]; L.py = v[3]; L.z = v[4]; L.pz = v[5];
+  if (L.pz < -1.0) L.state = BMX_LOST_PZ;
+  refresh_invariants(L, lc);
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // one op on one lane
 // ---------------------------------------------------------------------------
@@ -1147,7 +1249,7 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
   constexpr unsigned RARE = TB(OP_KICK_REF0) | TB(OP_BEND_K1) | TB(OP_BEND_KICK) | TB(OP_LINEAR_EDGE) | TB(OP_TILT) |
                             TB(OP_OFFSET_SET) | TB(OP_OFFSET_UNSET) | TB(OP_TOO_LARGE) | TB(OP_APERTURE_IN) |
                             TB(OP_HARD_MULT_EDGE) | TB(OP_EKICK) | TB(OP_EFRINGE) | TB(OP_BEND_EKICK) |
-                            TB(OP_DRIFT_BACK) | TB(OP_ZBETA);
+                            TB(OP_DRIFT_BACK) | TB(OP_ZBETA) | TB(OP_RAD);
   if (tb & RARE) {
     if (tb & TB(OP_KICK_REF0)) op_kick_ref0(L, op);
     if (tb & TB(OP_BEND_K1)) op_bend_k1(L, op, lc);
@@ -1168,6 +1270,9 @@ __device__ __forceinline__ void run_op(Lane& L, const DevOp& op, int type, int f
     if (tb & TB(OP_BEND_EKICK)) op_bend_ekick(L, op, flags & 0xff, n, lc);
     if (tb & TB(OP_DRIFT_BACK)) op_drift_back(L, op);
     if (tb & TB(OP_ZBETA)) L.z -= op.c[0] * L.beta;
+#if BMX_RAD
+    if (tb & TB(OP_RAD)) op_rad(L, op, flags, lc, tt);
+#endif
   }
   if (flags & F_SHIFT_OUT) {  // offset_particle unset, pure translation (op_offset without pitch)
     uniform_block();

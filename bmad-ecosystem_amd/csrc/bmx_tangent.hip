@@ -144,8 +144,10 @@ __global__ __launch_bounds__(64) void bmx_transfer_kernel(const DevOp* __restric
   TurnTime tt;
   tt.k = 0;
   tt.r = 0.0;
+  tt.turn = 0;
   bool alive = !too_large(L, lc);
   for (int t = 0; t < n_turns && alive; t++) {
+    tt.turn = t;
     for (int seg = 0; seg < 2 && alive; seg++) {
       int lo = seg ? lo2 : lo1, hi = seg ? hi2 : hi1;
       for (int k = lo; k < hi; k++) {

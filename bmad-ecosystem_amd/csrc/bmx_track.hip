@@ -36,7 +36,7 @@
 namespace bmx {
 bool compile_variant(const bmx_ele_rec* eles, int n_ele, const bmx_ref& ref, const bmx_config& cfg,
                      std::vector<DevOp>& prog, std::vector<int32_t>& ele_op_begin, LatticeConsts* lc,
-                     std::string* err);
+                     std::string* err, const RadInput* rad = nullptr, std::vector<double>* rad_table = nullptr);
 }
 
 using namespace bmx;
@@ -121,7 +121,17 @@ struct bmx_lattice {
   std::vector<double> rt;
   double* d_rt = nullptr;
   bool abs_time = false;  // uploaded with absolute-time rf
+  // radiation maps of every variant's OP_RAD ops (LatticeConsts::rad) and the
+  // fluctuation stream: the number of the next call that draws deviates
+  double* d_rad = nullptr;
+  bool rad_fluct = false;
+  mutable std::atomic<uint64_t> rad_next_call{0};
 };
+
+// The fluctuation stream of one tracking call: the lattice's next call number.
+static void rad_call_setup(const bmx_lattice* lat, LatticeConsts& lc) {
+  if (lat->rad_fluct) lc.rad_call = (uint32_t)lat->rad_next_call.fetch_add(1);
+}
 
 // Launch a tracking kernel: the lattice's runtime-compiled cell kernel when
 // it has one, else the built-in one (same parameters).
@@ -173,6 +183,8 @@ int32_t bmx_sizeof(int32_t which) {
     case 3: return (int32_t)sizeof(bmx_grid);
     case 4: return (int32_t)sizeof(bmx_config);
     case 5: return (int32_t)sizeof(bmx_schedule);
+    case 6: return (int32_t)sizeof(bmx_rad_map);
+    case 7: return (int32_t)sizeof(bmx_radiation);
   }
   return -1;
 }
@@ -222,7 +234,9 @@ using VariantSource = std::function<void(int v, std::vector<bmx_ele_rec>& buf, c
 // Compile, upload and set up every variant of a lattice (bmx_lattice_upload,
 // bmx_lattice_upload_scaled).
 static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource& src, const bmx_ref* ref,
-                           const bmx_config* cfg, bmx_lattice** out) {
+                           const bmx_config* cfg, bmx_lattice** out,
+                           const std::function<const bmx_rad_map*(int v)>& rad_maps = nullptr,
+                           const bmx_radiation* rad = nullptr) {
   bmx_config c = cfg ? *cfg : bmx_config{1, 0, 1e3, 0, 0};
   c.direction = 1;  // the forward program; the backward one of variant 0 is compiled below
   bmx_lattice* lat = new bmx_lattice();
@@ -234,11 +248,13 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
   struct VarOut {
     std::vector<DevOp> prog;
     std::vector<int32_t> eb;
+    std::vector<double> rad_table;
     LatticeConsts lc;
     std::string err;
     int n_ele = 0;
     bool ok = false;
   };
+  const bool with_rad = rad && rad_maps && (rad->damping_on || rad->fluctuations_on);
   std::vector<VarOut> outs(n_variants);
   {
     std::atomic<int> next{0};
@@ -251,7 +267,12 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
         int n;
         src(v, buf, &e, &n);
         o.n_ele = n;
-        o.ok = compile_variant(e, n, *ref, c, o.prog, o.eb, &o.lc, &o.err);
+        RadInput ri;
+        if (with_rad) {
+          ri.maps = rad_maps(v);
+          ri.cfg = *rad;
+        }
+        o.ok = compile_variant(e, n, *ref, c, o.prog, o.eb, &o.lc, &o.err, with_rad ? &ri : nullptr, &o.rad_table);
       }
     };
     unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -262,6 +283,7 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
     for (auto& t : th) t.join();
   }
   std::vector<DevOp> prog;
+  std::vector<double> rad_table;
   size_t total_ops = 0;
   for (const VarOut& o : outs) total_ops += o.prog.size();
   prog.reserve(total_ops);
@@ -273,6 +295,13 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
     }
     int32_t base = (int32_t)prog.size();
     lat->var_op_begin.push_back(base);
+    // each variant numbered its radiation maps from 0: shift them past the
+    // maps of the variants before it
+    const double rad_base = (double)(rad_table.size() / RAD_STRIDE);
+    for (DevOp& op : o.prog)
+      if (op.type == OP_RAD) op.c[0] += rad_base;
+    rad_table.insert(rad_table.end(), o.rad_table.begin(), o.rad_table.end());
+    lat->rad_fluct = lat->rad_fluct || o.lc.rad_fluct;
     prog.insert(prog.end(), o.prog.begin(), o.prog.end());
     for (int32_t& x : o.eb) x += base;
     LatticeConsts& lc = o.lc;
@@ -304,6 +333,17 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
                          hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(lat->d_var_n_ele, lat->var_n_ele.data(), lat->var_n_ele.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
+  if (!rad_table.empty()) {
+    if (hipMalloc(&lat->d_rad, rad_table.size() * sizeof(double)) != hipSuccess) {
+      bmx_lattice_destroy(lat);
+      return set_err(ctx, "hipMalloc of the radiation maps failed");
+    }
+    HIP_TRY(ctx, hipMemcpy(lat->d_rad, rad_table.data(), rad_table.size() * sizeof(double), hipMemcpyHostToDevice));
+    lat->lc.rad = lat->d_rad;
+    lat->lc.rad_fluct = lat->rad_fluct ? 1 : 0;
+    lat->lc.rad_key0 = (uint32_t)(rad->seed & 0xffffffffu);
+    lat->lc.rad_key1 = (uint32_t)(rad->seed >> 32);
+  }
   if (cells) {
     std::string jerr;
     const JitKernels* jk = jit_kernels(ctx->device, prog, plan, &jerr);
@@ -341,7 +381,13 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
     LatticeConsts lb;
     std::memset(&lb, 0, sizeof(lb));
     std::string berr;
-    if (compile_variant(e0, n0, *ref, cb, bprog, lat->eb_bwd, &lb, &berr)) {
+    RadInput ri0;
+    std::vector<double> brad;
+    if (with_rad) {
+      ri0.maps = rad_maps(0);
+      ri0.cfg = *rad;
+    }
+    if (compile_variant(e0, n0, *ref, cb, bprog, lat->eb_bwd, &lb, &berr, with_rad ? &ri0 : nullptr, &brad)) {
       lb.max_aperture_limit = cb.max_aperture_limit;
       lb.two_max_ap = 2.0 * cb.max_aperture_limit;
       lb.aperture_limit_on = cb.aperture_limit_on;
@@ -381,6 +427,37 @@ int32_t bmx_lattice_upload(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_
     *p = eles + offs[v];
     *n = offs[v + 1] - offs[v];
   }, ref, cfg, out);
+}
+
+int32_t bmx_lattice_upload_rad(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele_total,
+                               const int32_t* variant_offsets, int32_t n_variants, const bmx_rad_map* maps,
+                               const bmx_radiation* rad, const bmx_ref* ref, const bmx_config* cfg,
+                               bmx_lattice** out) {
+  if (!ctx || !eles || !ref || !out || n_ele_total <= 0 || n_variants <= 0) return set_err(ctx, "invalid arguments");
+  *out = nullptr;
+  if (rad && (rad->damping_on || rad->fluctuations_on) && !maps)
+    return set_err(ctx, "bmx_lattice_upload_rad: radiation switched on without maps");
+  std::vector<int32_t> offs;
+  if (variant_offsets) offs.assign(variant_offsets, variant_offsets + n_variants + 1);
+  else if (n_variants == 1) offs = {0, n_ele_total};
+  else return set_err(ctx, "variant_offsets is required for n_variants > 1");
+  for (int v = 0; v < n_variants; v++)
+    if (offs[v] < 0 || offs[v + 1] <= offs[v] || offs[v + 1] > n_ele_total) return set_err(ctx, "bad variant_offsets");
+  return upload_impl(
+      ctx, n_variants,
+      [&](int v, std::vector<bmx_ele_rec>&, const bmx_ele_rec** p, int* n) {
+        *p = eles + offs[v];
+        *n = offs[v + 1] - offs[v];
+      },
+      ref, cfg, out, [&](int v) { return maps + 2 * (size_t)offs[v]; }, rad);
+}
+
+int32_t bmx_lattice_set_rad_stream(bmx_lattice* lat, uint64_t seed, uint64_t next_call) {
+  if (!lat) return -1;
+  lat->lc.rad_key0 = (uint32_t)(seed & 0xffffffffu);
+  lat->lc.rad_key1 = (uint32_t)(seed >> 32);
+  lat->rad_next_call.store(next_call);
+  return 0;
 }
 
 int32_t bmx_lattice_upload_scaled(bmx_ctx* ctx, const bmx_ele_rec* eles, int32_t n_ele, int32_t n_variants,
@@ -442,6 +519,7 @@ void bmx_lattice_destroy(bmx_lattice* lat) {
   if (lat->d_cell_prv) hipFree(lat->d_cell_prv);
   if (lat->d_ops_bwd) hipFree(lat->d_ops_bwd);
   if (lat->d_rt) hipFree(lat->d_rt);
+  if (lat->d_rad) hipFree(lat->d_rad);
   delete lat;
 }
 
@@ -595,6 +673,7 @@ static int32_t track_bunch_common(bmx_ctx* ctx, const bmx_lattice* lat, int32_t 
     lc.t_start = rt[ix_start];
     lc.t_end = rt[ix_start] + (double)lc.tt_sign * (double)n_turns * range;
   }
+  rad_call_setup(lat, lc);
   int64_t n = p->n;
   if (out) std::memset(out, 0, sizeof(*out));
   if (n <= 0 || n_turns == 0) return 0;
@@ -652,10 +731,12 @@ int32_t bmx_track_multi(bmx_ctx* ctx, const bmx_lattice* lat, const int64_t* par
   }
   (void)hipMemcpyAsync(d_off, part_offsets, (nv + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream);
   const int block = 256;
+  LatticeConsts lc = lat->lc;
+  rad_call_setup(lat, lc);
   (void)hipEventRecord(ctx->ev0, ctx->stream);
   launch_track(lat->jit ? lat->jit->multi : nullptr, bmx_track_multi_kernel,
                dim3((unsigned)((max_count + block * BMX_NP - 1) / (block * BMX_NP)), (unsigned)nv), dim3(block),
-               ctx->stream, lat->d_ops, lat->d_var_op_begin, d_off, lat->lc, n_turns, st.s, st.d_lost);
+               ctx->stream, lat->d_ops, lat->d_var_op_begin, d_off, lc, n_turns, st.s, st.d_lost);
   hipError_t le = hipGetLastError();
   (void)hipEventRecord(ctx->ev1, ctx->stream);
   int rc = finish_particles(ctx, p, st, le, out);
@@ -790,6 +871,7 @@ extern "C" {
 
 int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* grid, const double* ref_orbit,
                        int32_t n_turns, int32_t* survival_turn, double* da_area, bmx_stats* out) {
+  if (lat && lat->rad_fluct) return set_err(ctx, "bmx_track_grid: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !grid || !survival_turn) return set_err(ctx, "invalid arguments");
   if (grid->n_angle <= 0 || grid->n_amp <= 0 || grid->n_pz <= 0 || n_turns < 0) return set_err(ctx, "bad grid");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -855,6 +937,7 @@ int32_t bmx_track_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* gri
 
 int32_t bmx_track_ics(bmx_ctx* ctx, const bmx_lattice* lat, const double* ics, const int64_t* ic_offsets,
                       int32_t n_turns, int32_t turns_per_block, int32_t* survival_turn, bmx_stats* out) {
+  if (lat && lat->rad_fluct) return set_err(ctx, "bmx_track_ics: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !ics || !ic_offsets || !survival_turn || n_turns < 0) return set_err(ctx, "invalid arguments");
   int nv = lat->n_variants;
   if (ic_offsets[0] != 0) return set_err(ctx, "ic_offsets[0] must be 0");
@@ -922,6 +1005,7 @@ int32_t bmx_ctx_get_schedule(const bmx_ctx* ctx, bmx_schedule* s) {
 
 int32_t bmx_tune_grid(bmx_ctx* ctx, const bmx_lattice* lat, const bmx_grid* grid, const double* ref_orbit,
                       int32_t n_turns, bmx_schedule* best) {
+  if (lat && lat->rad_fluct) return set_err(ctx, "bmx_tune_grid: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !grid || !best || n_turns <= 0 || grid->n_angle <= 0 || grid->n_amp <= 0 || grid->n_pz <= 0)
     return set_err(ctx, "invalid arguments");
   // sparse sample of the grid: every 4th angle and amplitude, every pz slice, every variant
@@ -1022,6 +1106,7 @@ int32_t bmx_bunch_moments(bmx_ctx* ctx, const bmx_particles* p, double* moments)
 // keeps pz (and z) of the guess fixed; n_dim = 6 closes all six (rf on).
 int32_t bmx_closed_orbit(bmx_ctx* ctx, const bmx_lattice* lat, int32_t n_dim, double* orbit, int32_t* status,
                          bmx_stats* out) {
+  if (lat && lat->rad_fluct) return set_err(ctx, "bmx_closed_orbit: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !orbit || !status || (n_dim != 4 && n_dim != 6)) return set_err(ctx, "invalid arguments");
   int nv = lat->n_variants;
   int np = 1 + n_dim;
@@ -1110,6 +1195,7 @@ int32_t bmx_closed_orbit(bmx_ctx* ctx, const bmx_lattice* lat, int32_t n_dim, do
 
 int32_t bmx_one_turn_matrix(bmx_ctx* ctx, const bmx_lattice* lat, const double* orbit, const double* d_orb,
                             double* mat6, int32_t* status, bmx_stats* out) {
+  if (lat && lat->rad_fluct) return set_err(ctx, "bmx_one_turn_matrix: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !orbit || !mat6 || !status) return set_err(ctx, "invalid arguments");
   double d[6];
   for (int k = 0; k < 6; k++) {
@@ -1162,6 +1248,7 @@ int32_t bmx_one_turn_matrix(bmx_ctx* ctx, const bmx_lattice* lat, const double* 
 
 int32_t bmx_transfer_matrix(bmx_ctx* ctx, const bmx_lattice* lat, int32_t ix_start, int32_t ix_end, int32_t n_turns,
                             int64_t n, const double* vec_in, double* vec_out, double* mat6, int32_t* state) {
+  if (lat && lat->rad_fluct) return set_err(ctx, "bmx_transfer_matrix: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !vec_in || !vec_out || !mat6 || !state || n < 0 || n_turns < 0)
     return set_err(ctx, "invalid arguments");
   if (lat->abs_time) return set_err(ctx, "bmx_transfer_matrix: lattices with absolute-time rf are not supported");

@@ -18,11 +18,16 @@
 ! first element (the one after the bunch's start, bunch%ix_ele) it packs
 ! bunch%particle(:)%vec/state/beta/t, tracks the whole turn on the GPU,
 ! unpacks, and returns finished = .true.; for the turn's other elements it
-! returns finished = .true. with no work.  Refuses (finished = .false., the
-! CPU path runs) what the device does not model: a lattice libbmx refused at
-! bmx_hooks_init, centroid or bunch_track output, and ltt's ramping and
-! per-element stochastic radiation (lt_tracking_mod.f90:3199-3214,
-! :3383-3446), which drive track1_bunch element by element.
+! returns finished = .true. with no work.  With bmad_com's radiation
+! switches on, bmx_hooks_init hands libbmx the maps radiation_map_setup made
+! for every element (ele%rad_map%rm0 / %rm1), which libbmx applies where
+! track1 does (track1.f90:198-201, :316-319); its fluctuations come from its
+! own counter-based stream (seed: bmx_hooks_init's rad_seed).  Refuses
+! (finished = .false., the CPU path runs) what the device does not model: a
+! lattice libbmx refused at bmx_hooks_init, centroid or bunch_track output,
+! and ltt's ramping and split-bend radiation points
+! (lt_tracking_mod.f90:3199-3214, :3383-3446), which drive track1_bunch
+! element by element.
 !
 ! gpu_track_many_hook (track_many_hook_def, :3949-3959): the contract of the
 ! DA program's own hook (bsim/dynamic_aperture/track_many_hook.f90:27-106):
@@ -50,18 +55,21 @@ contains
 ! Flatten lat%branch(ix_branch)%ele(1:n_ele_track) (after bmad_parser and
 ! lattice_bookkeeper) and upload it.  Sets bmx_hook_ok; err for an element
 ! or species the device path refuses (the message is printed).
-subroutine bmx_hooks_init(lat, ix_branch, err)
+subroutine bmx_hooks_init(lat, ix_branch, err, rad_seed)
   type(lat_struct), target :: lat
   integer, optional :: ix_branch
+  integer(c_int64_t), optional :: rad_seed
   logical err
   type(branch_struct), pointer :: branch
   type(ele_struct), pointer :: ele
   type(bmx_ele_rec), allocatable :: recs(:)
+  type(bmx_rad_map), allocatable :: maps(:)
   type(bmx_ref) :: ref
   type(bmx_config) :: cfg
+  type(bmx_radiation) :: rad
   real(rp) :: a(0:n_pole_maxx), b(0:n_pole_maxx), ae(0:n_pole_maxx), be(0:n_pole_maxx)
   integer i, rc
-  logical ok
+  logical ok, rad_on, rerr
 
   err = .true.
   bmx_hook_ok = .false.
@@ -97,7 +105,40 @@ subroutine bmx_hooks_init(lat, ix_branch, err)
     if (rc /= 0) return
   endif
   if (c_associated(bmx_hook_lat)) call bmx_lattice_destroy(bmx_hook_lat)
-  rc = bmx_lattice_upload(bmx_hook_ctx, recs, int(bmx_hook_n_ele, c_int32_t), c_null_ptr, 1, ref, cfg, bmx_hook_lat)
+  rad_on = bmad_com%radiation_damping_on .or. bmad_com%radiation_fluctuations_on
+  if (rad_on) then
+    ! the maps track1_radiation would use (radiation_mod.f90:95-109), set up
+    ! where they are stale; C reads the matrices row-major, hence transpose
+    allocate(maps(2 * bmx_hook_n_ele))
+    do i = 1, bmx_hook_n_ele
+      ele => branch%ele(i)
+      if (ele%value(l$) == 0) cycle
+      rerr = .false.
+      if (.not. associated(ele%rad_map)) then
+        call radiation_map_setup(ele, rerr)
+      elseif (ele%rad_map%stale) then
+        call radiation_map_setup(ele, rerr)
+      endif
+      if (rerr .or. .not. associated(ele%rad_map)) cycle
+      maps(2*i-1)%ref_orb = ele%rad_map%rm0%ref_orb
+      maps(2*i-1)%damp_dmat = transpose(ele%rad_map%rm0%damp_dmat)
+      maps(2*i-1)%xfer_damp_vec = ele%rad_map%rm0%xfer_damp_vec
+      maps(2*i-1)%stoc_mat = transpose(ele%rad_map%rm0%stoc_mat)
+      maps(2*i)%ref_orb = ele%rad_map%rm1%ref_orb
+      maps(2*i)%damp_dmat = transpose(ele%rad_map%rm1%damp_dmat)
+      maps(2*i)%xfer_damp_vec = ele%rad_map%rm1%xfer_damp_vec
+      maps(2*i)%stoc_mat = transpose(ele%rad_map%rm1%stoc_mat)
+    enddo
+    rad%damping_on = merge(1, 0, bmad_com%radiation_damping_on)
+    rad%fluctuations_on = merge(1, 0, bmad_com%radiation_fluctuations_on)
+    rad%zero_average = merge(1, 0, bmad_com%radiation_zero_average)
+    rad%synch_rad_scale = bmad_com%synch_rad_scale
+    if (present(rad_seed)) rad%seed = rad_seed
+    rc = bmx_lattice_upload_rad(bmx_hook_ctx, recs, int(bmx_hook_n_ele, c_int32_t), c_null_ptr, 1, maps, rad, &
+                                ref, cfg, bmx_hook_lat)
+  else
+    rc = bmx_lattice_upload(bmx_hook_ctx, recs, int(bmx_hook_n_ele, c_int32_t), c_null_ptr, 1, ref, cfg, bmx_hook_lat)
+  endif
   if (rc /= 0) then
     print '(2a)', 'bmx_hooks_init: ', bmx_error_string(bmx_hook_ctx)
     return

@@ -87,6 +87,22 @@ type, bind(C) :: bmx_schedule
   real(c_double) :: probe_pet_per_s = 0
 end type
 
+! rad_map_struct (bmad/modules/bmad_struct.f90:974-980) without xfer_damp_mat;
+! C row-major matrices: damp_dmat(c, r) here holds the reference's damp_dmat(r, c),
+! so pass transpose(ele%rad_map%rm0%damp_dmat) (see bmx_bmad_hooks.f90)
+type, bind(C) :: bmx_rad_map
+  real(c_double) :: ref_orb(6) = 0
+  real(c_double) :: damp_dmat(6, 6) = 0
+  real(c_double) :: xfer_damp_vec(6) = 0
+  real(c_double) :: stoc_mat(6, 6) = 0
+end type
+
+type, bind(C) :: bmx_radiation
+  integer(c_int32_t) :: damping_on = 0, fluctuations_on = 0, zero_average = 0, reserved = 0
+  real(c_double) :: synch_rad_scale = 1
+  integer(c_int64_t) :: seed = 0
+end type
+
 interface
   function bmx_abi_version() bind(C, name = 'bmx_abi_version') result(v)
     import
@@ -116,6 +132,29 @@ interface
     import
     type(c_ptr), value :: ctx
     type(c_ptr) :: msg
+  end function
+
+  function bmx_lattice_upload_rad(ctx, eles, n_ele_total, variant_offsets, n_variants, maps, rad, ref, cfg, lat) &
+                                  bind(C, name = 'bmx_lattice_upload_rad') result(rc)
+    import
+    type(c_ptr), value :: ctx
+    type(bmx_ele_rec) :: eles(*)
+    integer(c_int32_t), value :: n_ele_total
+    type(c_ptr), value :: variant_offsets
+    integer(c_int32_t), value :: n_variants
+    type(bmx_rad_map) :: maps(*)
+    type(bmx_radiation) :: rad
+    type(bmx_ref) :: ref
+    type(bmx_config) :: cfg
+    type(c_ptr) :: lat
+    integer(c_int32_t) :: rc
+  end function
+
+  function bmx_lattice_set_rad_stream(lat, seed, next_call) bind(C, name = 'bmx_lattice_set_rad_stream') result(rc)
+    import
+    type(c_ptr), value :: lat
+    integer(c_int64_t), value :: seed, next_call
+    integer(c_int32_t) :: rc
   end function
 
   function bmx_lattice_upload(ctx, eles, n_ele_total, variant_offsets, n_variants, ref, cfg, lat) &

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define BMX_ABI_VERSION 3
+#define BMX_ABI_VERSION 4
 #define BMX_N_POLE 22 /* n_pole_maxx + 1, bmad/modules/bmad_struct.f90 */
 
 /* ---- particle state codes (bmad/modules/bmad_struct.f90:520-527) ---- */
@@ -183,6 +183,41 @@ typedef struct bmx_particles {
   int32_t reserved;
 } bmx_particles;
 
+/* Radiation map of one half of an element (rad_map_struct,
+ * bmad/modules/bmad_struct.f90:974-980), as radiation_map_setup leaves it in
+ * ele%rad_map%rm0 (entrance half) and %rm1 (exit half)
+ * (bmad/modules/radiation_mod.f90:160-232): the damping correction damp_dmat
+ * acting on (vec - ref_orb), the 0th-order damping vector xfer_damp_vec and
+ * the Cholesky factor stoc_mat of the stochastic kick variance (lower
+ * triangular).  Matrices are row-major [row][col] over (x, px, y, py, z, pz):
+ * damp_dmat[r * 6 + c] = damp_dmat(r+1, c+1) of the reference.  xfer_damp_mat
+ * is not read by tracking and is not carried.  An element with no radiation
+ * has all-zero maps. */
+typedef struct bmx_rad_map {
+  double ref_orb[6];
+  double damp_dmat[36];
+  double xfer_damp_vec[6];
+  double stoc_mat[36];
+} bmx_rad_map;
+
+/* Radiation switches (bmad_com%radiation_damping_on, %radiation_fluctuations_on,
+ * %radiation_zero_average, %synch_rad_scale; bmad/modules/bmad_struct.f90:2312-2330)
+ * and the seed of the fluctuation stream.  The Gaussian deviates come from
+ * Philox4x32-10 keyed by the seed, with the counter (particle index, element
+ * index and edge, turn of the call, call number and pair): the same particle
+ * sees the same kicks whatever the launch shape, and every tracking call on a
+ * lattice handle draws fresh ones (the call number advances per call).  The
+ * reference draws them from its own generator (ran_gauss), so fluctuating
+ * tracks agree with it in distribution, not particle by particle. */
+typedef struct bmx_radiation {
+  int32_t damping_on;
+  int32_t fluctuations_on;
+  int32_t zero_average;   /* 1: leave out xfer_damp_vec (no closed-orbit sawtooth) */
+  int32_t reserved;
+  double synch_rad_scale; /* default 1 */
+  uint64_t seed;
+} bmx_radiation;
+
 /* Per-call statistics returned by the tracking entry points. */
 typedef struct bmx_stats {
   int64_t n_live;           /* particles still alive after the call */
@@ -226,7 +261,8 @@ typedef struct bmx_lattice bmx_lattice;
 int32_t bmx_abi_version(void);
 
 /* sizeof of the ABI structs, for binding-side layout checks:
- * which = 0 bmx_ele_rec, 1 bmx_particles, 2 bmx_stats, 3 bmx_grid, 4 bmx_config, 5 bmx_schedule. */
+ * which = 0 bmx_ele_rec, 1 bmx_particles, 2 bmx_stats, 3 bmx_grid, 4 bmx_config, 5 bmx_schedule,
+ * 6 bmx_rad_map, 7 bmx_radiation. */
 int32_t bmx_sizeof(int32_t which);
 
 /* Create a context on HIP device `device`.  flags: reserved, pass 0. */
@@ -242,6 +278,35 @@ int32_t bmx_lattice_upload(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele_
                            const int32_t *variant_offsets, int32_t n_variants,
                            const bmx_ref *ref, const bmx_config *cfg, bmx_lattice **out);
 void bmx_lattice_destroy(bmx_lattice *lat);
+
+/* bmx_lattice_upload with radiation damping and/or fluctuations: maps holds
+ * two bmx_rad_map per element of eles (maps[2 i] = rm0, applied at the
+ * element's entrance; maps[2 i + 1] = rm1, at its exit), the maps
+ * radiation_map_setup computed for them (a Bmad caller passes
+ * ele%rad_map%rm0 / %rm1).  Each element gets track1's radiation steps
+ * (bmad/code/track1.f90:198-201, :316-319; track1_radiation,
+ * bmad/modules/radiation_mod.f90:62-141): after the entrance aperture,
+ * vec += synch_rad_scale (damp_dmat (vec - ref_orb) + xfer_damp_vec) and,
+ * with fluctuations, vec += synch_rad_scale stoc_mat N(0, 1)^6, then a
+ * particle with pz < -1 is lost (lost_pz); the same at the exit after
+ * orbit_too_large and before the exit aperture.  beta is left as it was
+ * (the reference does not update it there).  Elements that track1_radiation
+ * skips (zero length, drift, pipe, marker, multipole, ab_multipole, off)
+ * and all-zero maps add no step.  Radiation is tracked forward (direction
+ * -1 refuses the lattice) in relative or absolute time.  Fluctuations are
+ * accepted by bmx_track_bunch(_noalloc) and bmx_track_multi only; damping by
+ * every tracking entry point.  rad NULL or both switches off: the same as
+ * bmx_lattice_upload. */
+int32_t bmx_lattice_upload_rad(bmx_ctx *ctx, const bmx_ele_rec *eles, int32_t n_ele_total,
+                               const int32_t *variant_offsets, int32_t n_variants, const bmx_rad_map *maps,
+                               const bmx_radiation *rad, const bmx_ref *ref, const bmx_config *cfg,
+                               bmx_lattice **out);
+
+/* The fluctuation stream of a lattice handle: seed, and the call number the
+ * next tracking call draws with (it advances by one per call; 0 after
+ * upload).  No reference counterpart (ran_seed_put plays this role for the
+ * reference's generator). */
+int32_t bmx_lattice_set_rad_stream(bmx_lattice *lat, uint64_t seed, uint64_t next_call);
 
 /* Strength attributes bmx_lattice_upload_scaled can scale. */
 enum { BMX_ATTR_K1 = 1, BMX_ATTR_K2 = 2, BMX_ATTR_K3 = 3 };

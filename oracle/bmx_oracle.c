@@ -20,6 +20,7 @@
  * the coupler angle, reversed bends and every branch switch by closed forms,
  * symmetries and high-precision evaluation of the other branch.
  */
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include "oracle_real.h"
@@ -2080,11 +2081,128 @@ static int track1_bmad(o_coord *orbit, const bmx_ele_rec *ele, int ref_species) 
 }
 
 /* ------------------------------------------------------------------------ */
-/* track1: bmad/code/track1.f90:43-342 (bmad_standard, no radiation, no spin) */
+/* radiation kicks: track1_radiation, bmad/modules/radiation_mod.f90:62-141  */
 /* ------------------------------------------------------------------------ */
-static int track1(o_coord *start_orb, const bmx_ele_rec *ele, int ref_species, o_coord *end_orb) {
+/* bmad_com%radiation_damping_on, %radiation_fluctuations_on,
+ * %radiation_zero_average, %synch_rad_scale (bmad_struct.f90:2312-2330), the
+ * maps of every element of the list oracle_track_bunch tracks (ele%rad_map
+ * rm0 / rm1 in the bmx_rad_map layout, 84 doubles each, two per element) and
+ * the fluctuation stream (Philox key and call number, as libbmx draws it). */
+typedef struct o_rad_com_t {
+  int damping_on, fluctuations_on, zero_average;
+  double synch_rad_scale;
+  uint32_t key0, key1, call;
+  const double *maps;
+} o_rad_com_t;
+static o_rad_com_t o_rad = {0, 0, 0, 1.0, 0, 0, 0, NULL};
+
+enum { O_RAD_MAP = 84, O_RAD_REF = 0, O_RAD_DAMP = 6, O_RAD_VEC = 42, O_RAD_STOC = 48 };
+
+ORACLE_API void oracle_set_radiation(const double *maps, int damping_on, int fluctuations_on, int zero_average,
+                                     double synch_rad_scale, uint64_t seed, uint64_t call) {
+  o_rad.maps = maps;
+  o_rad.damping_on = damping_on;
+  o_rad.fluctuations_on = fluctuations_on;
+  o_rad.zero_average = zero_average;
+  o_rad.synch_rad_scale = synch_rad_scale;
+  o_rad.key0 = (uint32_t)(seed & 0xffffffffu);
+  o_rad.key1 = (uint32_t)(seed >> 32);
+  o_rad.call = (uint32_t)call;
+}
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): the counter-based
+ * generator libbmx draws its Gaussian deviates from.  The reference draws
+ * from ran_gauss (sim_utils/math/random_mod.f90), a different stream, so
+ * fluctuating tracks are compared with the reference in distribution and
+ * with libbmx deviate by deviate. */
+ORACLE_API void oracle_philox4x32_10(uint32_t *c, uint32_t k0, uint32_t k1) {
+  int r;
+  for (r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+/* Six standard normal deviates for particle pid at element ie, edge (0
+ * entrance, 1 exit), turn: Box-Muller on three Philox blocks, u1 in (0, 1]
+ * and u2 in [0, 1) from 53 bits each. */
+ORACLE_API void oracle_rad_gauss6(uint32_t pid, int ie, int edge, uint32_t turn, uint32_t call, uint32_t k0,
+                                  uint32_t k1, double *g) {
+  int q;
+  for (q = 0; q < 3; q++) {
+    uint32_t c[4];
+    double u1, u2, rad;
+    c[0] = pid;
+    c[1] = (uint32_t)(ie * 2 + edge);
+    c[2] = turn;
+    c[3] = call * 4u + (uint32_t)q;
+    oracle_philox4x32_10(c, k0, k1);
+    u1 = ((double)((((uint64_t)c[0]) << 21) | (c[1] >> 11)) + 1.0) * 0x1p-53;
+    u2 = (double)((((uint64_t)c[2]) << 21) | (c[3] >> 11)) * 0x1p-53;
+    rad = sqrt(-2.0 * log(u1));
+    g[2 * q] = rad * cos(O_TWOPI * u2);
+    g[2 * q + 1] = rad * sin(O_TWOPI * u2);
+  }
+}
+
+/* Where a radiation step draws its deviates: element index, particle index, turn. */
+typedef struct o_rad_at {
+  int ie;
+  uint32_t pid, turn;
+} o_rad_at;
+
+static void track1_radiation(o_coord *orbit, const bmx_ele_rec *ele, int edge, const o_rad_at *at) {
+  const double *rm;
+  real d[6], a, rr;
+  double g[6];
+  int i, j;
+  if (!o_rad.damping_on && !o_rad.fluctuations_on) return;
+  if (ele->l == 0) return;
+  switch (ele->key) {
+    case BMX_KEY_DRIFT: case BMX_KEY_PIPE: case BMX_KEY_MULTIPOLE: case BMX_KEY_AB_MULTIPOLE: case BMX_KEY_MARKER:
+      return;
+    default: break;
+  }
+  rm = o_rad.maps + ((size_t)at->ie * 2 + (edge == O_FIRST_EDGE ? 0 : 1)) * O_RAD_MAP;
+  if (o_rad.damping_on) {
+    rr = orbit->time_dir * o_rad.synch_rad_scale;
+    for (j = 0; j < 6; j++) d[j] = orbit->vec[j] - rm[O_RAD_REF + j];
+    for (i = 0; i < 6; i++) {
+      a = rm[O_RAD_DAMP + i * 6] * d[0];
+      for (j = 1; j < 6; j++) a = a + rm[O_RAD_DAMP + i * 6 + j] * d[j];
+      orbit->vec[i] = orbit->vec[i] + rr * a;
+    }
+    if (!o_rad.zero_average)
+      for (i = 0; i < 6; i++) orbit->vec[i] = orbit->vec[i] + rr * rm[O_RAD_VEC + i];
+  }
+  if (o_rad.fluctuations_on) {
+    oracle_rad_gauss6(at->pid, at->ie, edge == O_FIRST_EDGE ? 0 : 1, at->turn, o_rad.call, o_rad.key0, o_rad.key1,
+                      g);
+    for (i = 0; i < 6; i++) {
+      a = rm[O_RAD_STOC + i * 6] * g[0];
+      for (j = 1; j < 6; j++) a = a + rm[O_RAD_STOC + i * 6 + j] * g[j];
+      orbit->vec[i] = orbit->vec[i] + o_rad.synch_rad_scale * a;
+    }
+  }
+  if (orbit->vec[5] < -1) orbit->state = BMX_LOST_PZ;
+}
+
+/* ------------------------------------------------------------------------ */
+/* track1: bmad/code/track1.f90:43-342 (bmad_standard, no spin); radiation   */
+/* (:198-201, :316-319) when `rad` is given                                  */
+/* ------------------------------------------------------------------------ */
+static int track1(o_coord *start_orb, const bmx_ele_rec *ele, int ref_species, o_coord *end_orb,
+                  const o_rad_at *rad) {
   o_coord s2 = *start_orb;
   int err;
+  int rad_on = rad && (o_rad.damping_on || o_rad.fluctuations_on) && ele->is_on;
   if (s2.direction * s2.time_dir == 1) s2.location = O_UPSTREAM;
   else s2.location = O_DOWNSTREAM;
   if (s2.state != BMX_ALIVE) {
@@ -2092,6 +2210,7 @@ static int track1(o_coord *start_orb, const bmx_ele_rec *ele, int ref_species, o
     return 0;
   }
   check_aperture_limit(&s2, ele, O_FIRST_EDGE, NULL);
+  if (rad_on) track1_radiation(&s2, ele, O_FIRST_EDGE, rad);
   *end_orb = s2;
   if (end_orb->state != BMX_ALIVE) return 0;
   err = track1_bmad(end_orb, ele, ref_species);
@@ -2101,6 +2220,7 @@ static int track1(o_coord *start_orb, const bmx_ele_rec *ele, int ref_species, o
   else if (end_orb->direction * end_orb->time_dir == 1) end_orb->location = O_DOWNSTREAM;
   else end_orb->location = O_UPSTREAM;
   if (end_orb->state != BMX_ALIVE) return 0;
+  if (rad_on) track1_radiation(end_orb, ele, O_SECOND_EDGE, rad);
   check_aperture_limit(end_orb, ele, O_SECOND_EDGE, NULL);
   return 0;
 }
@@ -2164,7 +2284,7 @@ ORACLE_API int oracle_track1(const bmx_ele_rec *ele, int ref_species, int specie
   if (unmodelled(ele)) return -2;
   init_coord(&a, vec_in, ele, species, direction, direction == 1 ? 0 : 1);
   if (t0_override == t0_override && t0_override != 0) a.t = t0_override; /* NaN => keep */
-  err = track1(&a, ele, ref_species, &b);
+  err = track1(&a, ele, ref_species, &b, NULL);
   for (i = 0; i < 6; i++) vec_out[i] = RV(b.vec[i]);
   if (t_out) { t_out[0] = RV(a.t); t_out[1] = RV(b.t); }
   if (beta_out) { beta_out[0] = RV(a.beta); beta_out[1] = RV(b.beta); }
@@ -2250,7 +2370,13 @@ ORACLE_API int oracle_track_bunch(const bmx_ele_rec *eles, int n_ele, int ref_sp
         a.s = 0;
         a.phase1 = 0;
         a.location = O_UPSTREAM;
-        track1(&a, ele, ref_species, &b);
+        {
+          o_rad_at ra;
+          ra.ie = ie;
+          ra.pid = (uint32_t)i;
+          ra.turn = (uint32_t)turn;
+          track1(&a, ele, ref_species, &b, o_rad.maps ? &ra : NULL);
+        }
         if (o_times) o_times[i] = RV(b.t);
         x[i] = RV(b.vec[0]); px[i] = RV(b.vec[1]); y[i] = RV(b.vec[2]); py[i] = RV(b.vec[3]);
         z[i] = RV(b.vec[4]); pz[i] = RV(b.vec[5]);
@@ -2296,7 +2422,7 @@ ORACLE_API int oracle_track_particle_outer(const bmx_ele_rec *eles, int n_ele, i
     for (turn = 0; turn < n_turns && a.state == BMX_ALIVE; turn++) {
       for (ie = 0; ie < n_ele; ie++) {
         a.p0c = eles[ie].p0c_start;
-        track1(&a, &eles[ie], ref_species, &b);
+        track1(&a, &eles[ie], ref_species, &b, NULL);
         a = b;
         if (a.state != BMX_ALIVE) {
           if (ix_lost) ix_lost[i] = ie;
@@ -2345,6 +2471,654 @@ ORACLE_API double oracle_low_energy_z(int species, double p0c, double e_tot, dou
 }
 
 #ifdef ORACLE_DUAL
+/* ======================================================================== */
+/* Radiation maps (dual-number build only): radiation_map_setup              */
+/* (bmad/modules/radiation_mod.f90:160-232), tracking_rad_map_setup          */
+/* (bmad/low_level/tracking_rad_map_setup.f90:19-84) and                     */
+/* rad1_damp_and_stoc_mats with calc_rad_at_pt and add_bend_edge_to_damp_mat */
+/* (bmad/modules/rad_6d_mod.f90:406-718).  Transfer matrices (make_mat6) are */
+/* the Jacobians of this oracle's track1 by forward-mode differentiation;    */
+/* element slices follow create_element_slice / makeup_super_slave1          */
+/* (bmad/code/create_element_slice.f90, bmad/modules/bookkeeper_mod.f90);    */
+/* fields are em_field_calc's bmad_standard fields (bmad/code/              */
+/* em_field_calc.f90:323-720) through g_bending_strength_from_em_field      */
+/* (bmad/code/g_bending_strength_from_em_field.f90:24-76).                  */
+/* ======================================================================== */
+static const double O_R_E = 2.8179403227e-15;                                   /* physical_constants.f90:35 */
+static const double O_H_BAR = 4.135667696e-15 / (2 * 3.141592653589793238462643383279); /* :38-39 */
+static const double O_SQRT3 = 1.732050807568877293527446341505872;             /* :16 */
+
+/* classical_radius (bmad/code/classical_radius.f90:14-24) */
+static double o_classical_radius(int species) {
+  double q = charge_of(species);
+  return O_R_E * O_M_ELECTRON * q * q / mass_of(species);
+}
+
+static int o_nint(double x) { return (int)(x < 0 ? -floor(-x + 0.5) : floor(x + 0.5)); }
+
+/* A slice of `lord` of length l_slice starting `offset` into it, with its
+ * upstream / downstream ends as asked (create_element_slice ->
+ * makeup_super_slave1, bookkeeper_mod.f90): fringe_at and the bend edges cut
+ * to the ends kept, kicks, voltage and integrated multipoles scaled by the
+ * length ratio, offsets moved along the pitch, num_steps from ds_step
+ * (attribute_bookkeeper.f90:390-401).  No aperture (the slices carry the
+ * reference orbit).  no_misalignment: also drop offsets, pitches and tilts
+ * (the slices of calc_rad_at_pt, dispatch = no_misalignment$, which
+ * offset_particle.f90:176 honours).  Returns -1 for a pitched or misaligned
+ * bend, which this restatement does not slice. */
+static int o_slice(const bmx_ele_rec *lord, double l_slice, double offset, int inc_up, int inc_down,
+                   int no_misalignment, bmx_ele_rec *sl) {
+  double len_lord = lord->l, coef, ds_step, s_del;
+  int inc_ent, inc_exit, ifr, n;
+  *sl = *lord;
+  coef = (fabs(l_slice) >= fabs(len_lord)) ? 1.0 : l_slice / len_lord;
+  ds_step = fabs(len_lord) / (lord->num_steps > 0 ? lord->num_steps : 1);
+  sl->l = l_slice;
+  n = (ds_step > 0) ? o_nint(fabs(l_slice / ds_step)) : 1;
+  sl->num_steps = n > 1 ? n : 1;
+  inc_ent = lord->orientation == 1 ? inc_up : inc_down;
+  inc_exit = lord->orientation == 1 ? inc_down : inc_up;
+  if (lord->key != BMX_KEY_MULTIPOLE && lord->key != BMX_KEY_AB_MULTIPOLE) {
+    ifr = lord->fringe_at;
+    if (inc_ent && inc_exit) {
+    } else if (inc_ent) {
+      sl->fringe_at = (ifr == BMX_ENTRANCE_END || ifr == BMX_BOTH_ENDS) ? BMX_ENTRANCE_END : BMX_NO_END;
+    } else if (inc_exit) {
+      sl->fringe_at = (ifr == BMX_EXIT_END || ifr == BMX_BOTH_ENDS) ? BMX_EXIT_END : BMX_NO_END;
+    } else {
+      sl->fringe_at = BMX_NO_END;
+    }
+  }
+  if (lord->key == BMX_KEY_HKICKER || lord->key == BMX_KEY_VKICKER) {
+    sl->kick = lord->kick * coef;
+  } else if (has_hkick_attributes(lord->key)) {
+    sl->hkick = lord->hkick * coef;
+    sl->vkick = lord->vkick * coef;
+  }
+  if (lord->key == BMX_KEY_RFCAVITY) sl->voltage = lord->voltage * coef;
+  if (lord->key != BMX_KEY_MULTIPOLE && lord->key != BMX_KEY_AB_MULTIPOLE)
+    for (n = 0; n < BMX_N_POLE; n++) {
+      sl->a[n] = lord->a[n] * coef;
+      sl->b[n] = lord->b[n] * coef;
+    }
+  if (has_orientation_attributes(lord->key)) {
+    if (lord->key == BMX_KEY_SBEND && lord->g != 0) {
+      if (lord->x_offset != 0 || lord->y_offset != 0 || lord->z_offset != 0 || lord->x_pitch != 0 ||
+          lord->y_pitch != 0)
+        return -1;
+    } else {
+      s_del = offset + l_slice / 2 - len_lord / 2;
+      sl->x_offset = lord->x_offset + s_del * sin(lord->x_pitch) * cos(lord->y_pitch);
+      sl->y_offset = lord->y_offset + s_del * sin(lord->y_pitch);
+      sl->z_offset = lord->z_offset + s_del * (cos(lord->x_pitch) * cos(lord->y_pitch) - 1);
+    }
+  }
+  sl->x1_limit = sl->x2_limit = sl->y1_limit = sl->y2_limit = 0;
+  sl->px_aperture_width2 = sl->py_aperture_width2 = sl->z_aperture_width2 = sl->pz_aperture_width2 = 0;
+  sl->delta_ref_time = lord->delta_ref_time * coef;
+  if (lord->key == BMX_KEY_SBEND) {
+    ifr = sl->fringe_at;
+    if (ifr == BMX_NO_END || ifr == BMX_EXIT_END) sl->e1 = sl->fint = sl->hgap = 0;
+    if (ifr == BMX_NO_END || ifr == BMX_ENTRANCE_END) sl->e2 = sl->fintx = sl->hgapx = 0;
+  }
+  if (no_misalignment) {
+    sl->x_offset = sl->y_offset = sl->z_offset = sl->x_pitch = sl->y_pitch = 0;
+    sl->tilt = 0;
+    sl->ref_tilt = 0;
+  }
+  return 0;
+}
+
+/* A particle at vec (values) as the reference orbit: init_coord's beta from
+ * pz (carried as a function of pz), coordinates seeded with unit derivatives. */
+static void o_seed(o_coord *a, const double *vec, const bmx_ele_rec *ele, int species) {
+  int i;
+  real pc, mc2 = mass_of(species);
+  for (i = 0; i < 6; i++) {
+    a->vec[i] = real(vec[i]);
+    a->vec[i].d[i] = 1.0;
+  }
+  a->species = species;
+  a->direction = 1;
+  a->time_dir = 1;
+  a->state = BMX_ALIVE;
+  a->p0c = ele->p0c_start;
+  a->s = 0;
+  a->phase1 = 0;
+  a->location = O_UPSTREAM;
+  a->t = 0;
+  pc = real(ele->p0c_start) * (1 + a->vec[5]);
+  a->beta = pc / sqrt(pc * pc + mc2 * mc2);
+}
+
+static void o_take(const o_coord *b, double *vout, double *M) {
+  int i, j;
+  for (i = 0; i < 6; i++) {
+    if (vout) vout[i] = b->vec[i].v;
+    if (M)
+      for (j = 0; j < 6; j++) M[i * 6 + j] = b->vec[i].d[j];
+  }
+}
+
+/* make_mat6 (bmad/code/make_mat6.f90:23): the element's map and Jacobian at vin. */
+static int o_make_mat6(const bmx_ele_rec *ele, int ref_species, const double *vin, double *vout, double *M) {
+  o_coord a, b;
+  int i;
+  if (ele->l == 0 && ele->key != BMX_KEY_MULTIPOLE && ele->key != BMX_KEY_AB_MULTIPOLE) {
+    /* a zero-length slice: no body, no ends */
+    for (i = 0; i < 6; i++) vout[i] = vin[i];
+    for (i = 0; i < 36; i++) M[i] = (i % 7 == 0) ? 1.0 : 0.0;
+    return 0;
+  }
+  o_seed(&a, vin, ele, ref_species);
+  track1(&a, ele, ref_species, &b, NULL);
+  if (b.state != BMX_ALIVE) return -1;
+  o_take(&b, vout, M);
+  return 0;
+}
+
+static void m6_mul(const double *A, const double *B, double *C) {
+  double t[36];
+  int i, j, k;
+  for (i = 0; i < 6; i++)
+    for (j = 0; j < 6; j++) {
+      double a = 0;
+      for (k = 0; k < 6; k++) a += A[i * 6 + k] * B[k * 6 + j];
+      t[i * 6 + j] = a;
+    }
+  memcpy(C, t, sizeof(t));
+}
+
+static void m6_vec(const double *A, const double *v, double *out) {
+  double t[6];
+  int i, k;
+  for (i = 0; i < 6; i++) {
+    double a = 0;
+    for (k = 0; k < 6; k++) a += A[i * 6 + k] * v[k];
+    t[i] = a;
+  }
+  memcpy(out, t, sizeof(t));
+}
+
+static void m6_transpose(const double *A, double *T) {
+  double t[36];
+  int i, j;
+  for (i = 0; i < 6; i++)
+    for (j = 0; j < 6; j++) t[j * 6 + i] = A[i * 6 + j];
+  memcpy(T, t, sizeof(t));
+}
+
+/* mat_symp_conj (sim_utils/matrix/mat_symp_conj.f90): -S M^T S, S the
+ * block-diagonal [[0, 1], [-1, 0]]. */
+static void m6_symp_conj(const double *M, double *C) {
+  double S[36] = {0}, T[36];
+  int i;
+  for (i = 0; i < 6; i += 2) {
+    S[i * 6 + i + 1] = 1;
+    S[(i + 1) * 6 + i] = -1;
+  }
+  m6_transpose(M, T);
+  m6_mul(S, T, T);
+  m6_mul(T, S, C);
+  for (i = 0; i < 36; i++) C[i] = -C[i];
+}
+
+/* super_polint (sim_utils/math/super_recipes_mod.f90): Neville's polynomial
+ * extrapolation to x of the n points (xa, ya); y and its error estimate dy. */
+static void o_super_polint(const double *xa, const double *ya, int n, double x, double *y, double *dy) {
+  double c[8], d[8], den[8], ho[8];
+  int m, i, ns = 0;
+  double best = 1e300;
+  for (i = 0; i < n; i++) {
+    c[i] = ya[i];
+    d[i] = ya[i];
+    ho[i] = xa[i] - x;
+    if (fabs(x - xa[i]) < best) { best = fabs(x - xa[i]); ns = i; }
+  }
+  *y = ya[ns];
+  ns = ns - 1; /* 0-based: the reference's ns - 1 with ns 1-based */
+  *dy = 0;
+  for (m = 1; m <= n - 1; m++) {
+    for (i = 0; i < n - m; i++) den[i] = ho[i] - ho[i + m];
+    for (i = 0; i < n - m; i++) den[i] = (c[i + 1] - d[i]) / den[i];
+    for (i = 0; i < n - m; i++) {
+      d[i] = ho[i + m] * den[i];
+      c[i] = ho[i] * den[i];
+    }
+    /* reference: if (2*ns < n-m) dy = c(ns+1) else dy = d(ns), ns = ns-1 (ns 1-based after its -1) */
+    if (2 * (ns + 1) < n - m) {
+      *dy = c[ns + 1];
+    } else {
+      *dy = d[ns];
+      ns = ns - 1;
+    }
+    *y = *y + *dy;
+  }
+}
+
+/* em_field_calc's bmad_standard B field of the element at (x, y) of the
+ * local orbit (em_field_calc.f90:323-720 for this path's keys), with its x
+ * and y derivatives carried by dual numbers (dual slot 0: d/dx, slot 2:
+ * d/dy).  Returns -1 for what it does not model (rf cavity fields,
+ * electric multipoles). */
+static int o_field(const bmx_ele_rec *ele, int ref_species, const double *v, real B[3]) {
+  o_coord lo;
+  real x, y, kx, ky, a[BMX_N_POLE], b[BMX_N_POLE];
+  double f_p0c, ref_charge = charge_of(ref_species), tilt;
+  int i, ix, add_kicks;
+  memset(&lo, 0, sizeof(lo));
+  for (i = 0; i < 6; i++) lo.vec[i] = real(v[i]);
+  lo.vec[0].d[0] = 1.0;
+  lo.vec[2].d[2] = 1.0;
+  lo.species = ref_species;
+  lo.direction = 1;
+  lo.time_dir = 1;
+  lo.beta = 1;
+  lo.p0c = ele->p0c;
+  x = lo.vec[0];
+  y = lo.vec[2];
+  f_p0c = (ref_charge == 0) ? 0.0 : ele->p0c / (O_C_LIGHT * ref_charge);
+  B[0] = B[1] = B[2] = real(0.0);
+  if (ele->has_elec_poles) return -1;
+  switch (ele->key) {
+    case BMX_KEY_DRIFT: case BMX_KEY_ECOLLIMATOR: case BMX_KEY_RCOLLIMATOR: case BMX_KEY_INSTRUMENT:
+    case BMX_KEY_MONITOR: case BMX_KEY_PIPE: case BMX_KEY_MARKER: case BMX_KEY_MULTIPOLE: case BMX_KEY_AB_MULTIPOLE:
+    case BMX_KEY_THICK_MULTIPOLE:
+      break;
+    case BMX_KEY_HKICKER: B[1] = real(-ele->kick * f_p0c / ele->l); break;
+    case BMX_KEY_VKICKER: B[0] = real(ele->kick * f_p0c / ele->l); break;
+    case BMX_KEY_KICKER:
+      B[0] = real(ele->vkick * f_p0c / ele->l);
+      B[1] = real(-ele->hkick * f_p0c / ele->l);
+      break;
+    case BMX_KEY_OCTUPOLE:
+      B[0] = -(y * y * y - 3.0 * y * x * x) / 6.0 * ele->k3 * f_p0c;
+      B[1] = (x * x * x - 3.0 * x * y * y) / 6.0 * ele->k3 * f_p0c;
+      break;
+    case BMX_KEY_QUADRUPOLE:
+      B[0] = y * ele->k1 * f_p0c;
+      B[1] = x * ele->k1 * f_p0c;
+      break;
+    case BMX_KEY_SEXTUPOLE:
+      B[0] = x * y * ele->k2 * f_p0c;
+      B[1] = (x * x - y * y) / 2.0 * ele->k2 * f_p0c;
+      break;
+    case BMX_KEY_SBEND: B[1] = real(ele->g * f_p0c); break;
+    default: return -1; /* rf cavity fields are not restated */
+  }
+  /* multipoles (:661-707): a bend's include its kicks, dg, k1 and k2 */
+  if (ele->key == BMX_KEY_SBEND) {
+    multipole_ele_to_ab(ele, 0, O_MAGNETIC, 1, &ix, a, b, NULL);
+    add_kicks = 0;
+  } else {
+    multipole_ele_to_ab(ele, 0, O_MAGNETIC, 0, &ix, a, b, NULL);
+    add_kicks = 1;
+  }
+  if (ix > -1) {
+    if (ele->l == 0) return -1;
+    for (i = 0; i <= ix; i++) {
+      if (a[i] == 0 && b[i] == 0) continue;
+      ab_multipole_kick(a[i], b[i], i, ref_species, 0, &lo, &kx, &ky, O_MAGNETIC, 0, 1.0);
+      B[0] = B[0] + f_p0c * ky / ele->l;
+      B[1] = B[1] - f_p0c * kx / ele->l;
+    }
+  }
+  /* hkick / vkick of other elements (:1404-1416 of the field tail) */
+  if (add_kicks && has_hkick_attributes(ele->key) && (ele->hkick != 0 || ele->vkick != 0) &&
+      ele->key != BMX_KEY_KICKER && ele->key != BMX_KEY_HKICKER && ele->key != BMX_KEY_VKICKER) {
+    tilt = (ele->key == BMX_KEY_SBEND) ? ele->ref_tilt : ele->tilt;
+    if (tilt == 0) {
+      B[0] = B[0] + ele->vkick * f_p0c / ele->l;
+      B[1] = B[1] - ele->hkick * f_p0c / ele->l;
+    } else {
+      B[0] = B[0] + (ele->vkick * cos(tilt) - ele->hkick * sin(tilt)) * f_p0c / ele->l;
+      B[1] = B[1] - (ele->hkick * cos(tilt) + ele->vkick * sin(tilt)) * f_p0c / ele->l;
+    }
+  }
+  return 0;
+}
+
+/* g_from_field (g_bending_strength_from_em_field.f90:62-74) for a magnetic field */
+static void o_g_from_field(const double *B, const double *vel, double beta, double charge, double fact, double *g) {
+  double force[3], vb[3], dot;
+  int i;
+  vb[0] = vel[1] * B[2] - vel[2] * B[1];
+  vb[1] = vel[2] * B[0] - vel[0] * B[2];
+  vb[2] = vel[0] * B[1] - vel[1] * B[0];
+  for (i = 0; i < 3; i++) force[i] = (0.0 + vb[i] * beta * O_C_LIGHT) * charge;
+  dot = force[0] * vel[0] + force[1] * vel[1] + force[2] * vel[2];
+  for (i = 0; i < 3; i++) g[i] = -(force[i] - vel[i] * dot) * fact;
+}
+
+/* The integrand of rad1_damp_and_stoc_mats at z_pos (calc_rad_at_pt,
+ * rad_6d_mod.f90:590-686): ele is the half slice, orb0 / v0 its body-frame
+ * entrance orbit. */
+typedef struct o_rad_stash {
+  double gamma0, kd_coef, kf_coef;
+} o_rad_stash;
+
+static int o_calc_rad_at_pt(const bmx_ele_rec *ele, int ref_species, const o_rad_stash *st, int opening,
+                            const double *v0, double z_pos, double *D1, double *dv1, double *S1, int save,
+                            double *orb_save, double *mat_save) {
+  bmx_ele_rec runt;
+  double vz[6], M[36], mb[36], Bv[3], dBx[3], dBy[3], g[3], dgx[3], dgy[3], vel[3], fact, rel_p, g1, g2, g3,
+      dg2_dx, dg2_dy, v[6], kf, kv, gg;
+  real B[3];
+  int i, j;
+  if (o_slice(ele, z_pos, 0.0, 0, 0, 1, &runt) < 0) return -1;
+  if (o_make_mat6(&runt, ref_species, v0, vz, M) < 0) return -1;
+  if (save) {
+    memcpy(orb_save, vz, sizeof(vz));
+    memcpy(mat_save, M, sizeof(M));
+  }
+  m6_symp_conj(M, mb); /* matrix from z_pos back to 0 */
+  if (o_field(ele, ref_species, vz, B) < 0) return -1;
+  for (i = 0; i < 3; i++) {
+    Bv[i] = B[i].v;
+    dBx[i] = B[i].d[0];
+    dBy[i] = B[i].d[2];
+  }
+  memcpy(v, vz, sizeof(v));
+  rel_p = 1 + v[5];
+  vel[0] = v[1] / rel_p;
+  vel[1] = v[3] / rel_p;
+  vel[2] = sqrt(1 - vel[0] * vel[0] - vel[1] * vel[1]);
+  {
+    double pc = ele->p0c * rel_p, mc2 = mass_of(ref_species), beta = pc / sqrt(pc * pc + mc2 * mc2);
+    double charge = charge_of(O_TRACKED(ref_species));
+    fact = 1 / (beta * ele->p0c * rel_p);
+    o_g_from_field(Bv, vel, beta, charge, fact, g);
+    o_g_from_field(dBx, vel, beta, charge, fact, dgx);
+    o_g_from_field(dBy, vel, beta, charge, fact, dgy);
+  }
+  for (i = 0; i < 3; i++) {
+    g[i] = g[i] * rel_p;
+    dgx[i] = dgx[i] * rel_p;
+    dgy[i] = dgy[i] * rel_p;
+  }
+  g1 = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  g2 = g1 * g1;
+  g3 = g1 * g2;
+  dg2_dx = 2 * (g[0] * dgx[0] + g[1] * dgx[1] + g[2] * dgx[2]);
+  dg2_dy = 2 * (g[0] * dgy[0] + g[1] * dgy[1] + g[2] * dgy[2]);
+  if (ele->key == BMX_KEY_SBEND) {
+    gg = 1 + ele->g * v0[0]; /* the reference takes the entrance orbit's x here */
+    g1 = g1 * gg;
+    g2 = g2 * gg;
+    g3 = g3 * gg;
+    dg2_dx = dg2_dx * gg + g2 * ele->g;
+  }
+  /* damping matrix, columns (:645-650) */
+  for (i = 0; i < 6; i++) {
+    const double m2 = mb[i * 6 + 1], m4 = mb[i * 6 + 3], m6 = mb[i * 6 + 5], kd = st->kd_coef;
+    D1[i * 6 + 0] = -kd * (m2 * dg2_dx * v[1] * rel_p + m4 * dg2_dx * v[3] * rel_p + m6 * dg2_dx * rel_p * rel_p);
+    D1[i * 6 + 1] = -kd * (m2 * g2 * rel_p);
+    D1[i * 6 + 2] = -kd * (m2 * dg2_dy * v[1] * rel_p + m4 * dg2_dy * v[3] * rel_p + m6 * dg2_dy * rel_p * rel_p);
+    D1[i * 6 + 3] = -kd * (m4 * g2 * rel_p);
+    D1[i * 6 + 4] = 0;
+    D1[i * 6 + 5] = -kd * (m2 * g2 * v[1] + m4 * g2 * v[3] + m6 * 2 * g2 * rel_p);
+  }
+  /* damping vector (:654) */
+  {
+    double w[6] = {0.0, v[1] * g2 * rel_p, 0.0, v[3] * g2 * rel_p, 0.0, g2 * rel_p * rel_p};
+    m6_vec(mb, w, dv1);
+    for (i = 0; i < 6; i++) dv1[i] = -st->kd_coef * dv1[i];
+  }
+  /* stochastic variance (:658-674) */
+  kf = st->kf_coef * rel_p * rel_p * g3;
+  for (i = 0; i < 6; i++)
+    for (j = 0; j < 6; j++) {
+      double b2i = mb[i * 6 + 1], b4i = mb[i * 6 + 3], b6i = mb[i * 6 + 5];
+      double b2j = mb[j * 6 + 1], b4j = mb[j * 6 + 3], b6j = mb[j * 6 + 5];
+      S1[i * 6 + j] = kf * (rel_p * rel_p * b6i * b6j + v[1] * v[1] * b2i * b2j + v[3] * v[3] * b4i * b4j +
+                            rel_p * v[1] * (b2i * b6j + b6i * b2j) + rel_p * v[3] * (b4i * b6j + b6i * b4j) +
+                            v[1] * v[3] * (b2i * b4j + b4i * b2j));
+    }
+  if (opening) {
+    kv = st->kf_coef * 13.0 * g1 / (55.0 * st->gamma0 * st->gamma0);
+    for (i = 0; i < 6; i++)
+      for (j = 0; j < 6; j++) {
+        double b2i = mb[i * 6 + 1], b4i = mb[i * 6 + 3], b2j = mb[j * 6 + 1], b4j = mb[j * 6 + 3];
+        S1[i * 6 + j] = S1[i * 6 + j] + kv * (g[1] * g[1] * b2i * b2j + g[0] * g[0] * b4i * b4j -
+                                              g[0] * g[1] * (b2i * b4j + b4i * b2j));
+      }
+  }
+  return 0;
+}
+
+/* add_bend_edge_to_damp_mat (rad_6d_mod.f90:691-716) */
+static void o_add_bend_edge(double *D, const bmx_ele_rec *ele, double e_edge, const double *orb, const double *mat,
+                            double kd) {
+  double dm[36] = {0}, dg2_dx, rel_p, ci[36];
+  if (e_edge == 0) return;
+  dg2_dx = -tan(e_edge) * (ele->g + ele->dg) * (ele->g + ele->dg);
+  rel_p = 1 + orb[5];
+  dm[1 * 6 + 0] = -kd * dg2_dx * orb[1] * rel_p;
+  dm[1 * 6 + 2] = -kd * -dg2_dx * orb[1] * rel_p;
+  dm[3 * 6 + 0] = -kd * dg2_dx * orb[3] * rel_p;
+  dm[3 * 6 + 2] = -kd * -dg2_dx * orb[3] * rel_p;
+  dm[5 * 6 + 0] = -kd * dg2_dx * rel_p * rel_p;
+  dm[5 * 6 + 2] = -kd * -dg2_dx * rel_p * rel_p;
+  if (mat) {
+    m6_symp_conj(mat, ci);
+    m6_mul(ci, dm, dm);
+    m6_mul(dm, mat, dm);
+  }
+  for (int i = 0; i < 36; i++) D[i] += dm[i];
+}
+
+/* rad1_damp_and_stoc_mats (rad_6d_mod.f90:406-588) for the half slice `ele`
+ * whose map is ele_mat6 about orb_in -> orb_out; include_opening_angle. */
+static int o_rad1(const bmx_ele_rec *ele, int ref_species, const double *orb_in, const double *orb_out,
+                  const double *ele_mat6, double g2_tol, double g3_tol, double *ref, double *D, double *dv,
+                  double *S) {
+  double length = ele->l, mc2 = mass_of(ref_species), v0[6], mat0[36], mat0_inv[36], orb_end[6], mat_end[36],
+         eps_damp, eps_stoc, del_z, l_ref, z_pos, D1[36], dv1[6], S1[36], Dsum[36], dvsum[6], Ssum[36], ddamp[36],
+         ddvec[6], dstoc[36], d_max, T[36], xa[5];
+  struct { double h, dv[6], D[36], S[36]; } qi[5];
+  o_rad_stash st;
+  int j, j1, n, n_pts, i, k, j_min_test = 3, j_max = 10;
+  const double cd = 2.0 / 3.0, cf = 55.0 * O_H_BAR * O_C_LIGHT / (24.0 * O_SQRT3);
+  for (i = 0; i < 6; i++) ref[i] = orb_out[i];
+  for (i = 0; i < 36; i++) D[i] = S[i] = 0;
+  for (i = 0; i < 6; i++) dv[i] = 0;
+  if (length == 0) return 0;
+  if (orb_out[1] == orb_in[1] && orb_out[3] == orb_in[3] && ele->key != BMX_KEY_SBEND) return 0;
+  if (ele->p0c_start < 2.0 * mc2 || ele->p0c < 2.0 * mc2) return 0;
+  /* offsets and fringes at the upstream end */
+  {
+    o_coord a;
+    o_seed(&a, orb_in, ele, ref_species);
+    offset_particle(ele, ref_species, 1, &a, 1, 0, -1);
+    if (has_fringe(ele)) apply_element_edge_kick(&a, O_FIRST_EDGE, ele, ref_species);
+    o_take(&a, v0, mat0);
+  }
+  {
+    double pc = ele->p0c, e = sqrt(pc * pc + mc2 * mc2);
+    st.gamma0 = e / mc2; /* convert_pc_to (p0c, ref_species, gamma) */
+  }
+  {
+    double radi = o_classical_radius(ref_species);
+    st.kd_coef = cd * radi * st.gamma0 * st.gamma0 * st.gamma0;
+    st.kf_coef = cf * radi * pow(st.gamma0, 5) / mc2;
+  }
+  eps_damp = st.kd_coef * g2_tol / length;
+  eps_stoc = st.kf_coef * g3_tol / length;
+  memset(qi, 0, sizeof(qi));
+  qi[0].h = 4;
+  for (j = 1; j <= j_max; j++) {
+    if (j == 1) {
+      n_pts = 2;
+      del_z = length;
+      l_ref = 0;
+    } else {
+      n_pts = 1 << (j - 2);
+      del_z = ele->l / n_pts;
+      l_ref = del_z / 2;
+    }
+    memset(Dsum, 0, sizeof(Dsum));
+    memset(dvsum, 0, sizeof(dvsum));
+    memset(Ssum, 0, sizeof(Ssum));
+    for (n = 1; n <= n_pts; n++) {
+      z_pos = l_ref + (n - 1) * del_z;
+      int save = (j == 1 && n == 1);
+      if (o_calc_rad_at_pt(ele, ref_species, &st, 1, v0, z_pos, D1, dv1, S1, save, orb_end, mat_end) < 0)
+        return -1;
+      for (i = 0; i < 36; i++) { Dsum[i] += D1[i]; Ssum[i] += S1[i]; }
+      for (i = 0; i < 6; i++) dvsum[i] += dv1[i];
+    }
+    j1 = j < 4 ? j : 4;
+    if (j > 4)
+      for (k = 0; k < 4; k++) qi[k] = qi[k + 1];
+    qi[j1].h = 0.25 * qi[j1 - 1].h;
+    for (i = 0; i < 36; i++) {
+      qi[j1].D[i] = 0.5 * (qi[j1 - 1].D[i] + del_z * Dsum[i]);
+      qi[j1].S[i] = 0.5 * (qi[j1 - 1].S[i] + del_z * Ssum[i]);
+    }
+    for (i = 0; i < 6; i++) qi[j1].dv[i] = 0.5 * (qi[j1 - 1].dv[i] + del_z * dvsum[i]);
+    if (j < j_min_test) continue;
+    for (k = 0; k < j1; k++) xa[k] = qi[k + 1].h;
+    for (i = 0; i < 6; i++) {
+      double ya[5];
+      for (k = 0; k < j1; k++) ya[k] = qi[k + 1].dv[i];
+      o_super_polint(xa, ya, j1, 0.0, &dv1[i], &ddvec[i]);
+    }
+    for (i = 0; i < 36; i++) {
+      double ya[5];
+      for (k = 0; k < j1; k++) ya[k] = qi[k + 1].D[i];
+      o_super_polint(xa, ya, j1, 0.0, &D1[i], &ddamp[i]);
+      for (k = 0; k < j1; k++) ya[k] = qi[k + 1].S[i];
+      o_super_polint(xa, ya, j1, 0.0, &S1[i], &dstoc[i]);
+    }
+    d_max = 0;
+    for (i = 0; i < 36; i++) {
+      if (eps_damp * fabs(ddamp[i]) > d_max) d_max = eps_damp * fabs(ddamp[i]);
+      if (eps_stoc * fabs(dstoc[i]) > d_max) d_max = eps_stoc * fabs(dstoc[i]);
+    }
+    for (i = 0; i < 6; i++)
+      if (eps_damp * fabs(ddvec[i]) > d_max) d_max = eps_damp * fabs(ddvec[i]);
+    if (d_max < 1) break;
+  }
+  /* bend edges */
+  if (ele->key == BMX_KEY_SBEND) {
+    if (ele->orientation == 1) {
+      o_add_bend_edge(D1, ele, ele->e1, v0, NULL, st.kd_coef);
+      o_add_bend_edge(D1, ele, ele->e2, orb_end, mat_end, st.kd_coef);
+    } else {
+      o_add_bend_edge(D1, ele, ele->e2, v0, NULL, st.kd_coef);
+      o_add_bend_edge(D1, ele, ele->e1, orb_end, mat_end, st.kd_coef);
+    }
+  }
+  /* referred to the downstream end (:568-577) */
+  m6_symp_conj(mat0, mat0_inv);
+  m6_mul(ele_mat6, mat0_inv, T);
+  m6_vec(T, dv1, dv);
+  m6_mul(T, D1, D);
+  m6_mul(D, mat0, D);
+  m6_mul(mat0_inv, S1, S1);
+  m6_transpose(mat0_inv, T);
+  m6_mul(S1, T, S1);
+  m6_mul(ele_mat6, S1, S);
+  m6_transpose(ele_mat6, T);
+  m6_mul(S, T, S);
+  return 0;
+}
+
+/* dpotrf 'L': S = L L^T in place (lower triangle), upper set to zero; -1 when not positive definite */
+static int o_cholesky_lower(double *A) {
+  double L[36] = {0};
+  int i, j, k;
+  for (j = 0; j < 6; j++) {
+    double s = A[j * 6 + j];
+    for (k = 0; k < j; k++) s -= L[j * 6 + k] * L[j * 6 + k];
+    if (!(s > 0)) return -1;
+    L[j * 6 + j] = sqrt(s);
+    for (i = j + 1; i < 6; i++) {
+      double t = A[i * 6 + j];
+      for (k = 0; k < j; k++) t -= L[i * 6 + k] * L[j * 6 + k];
+      L[i * 6 + j] = t / L[j * 6 + j];
+    }
+  }
+  memcpy(A, L, sizeof(L));
+  return 0;
+}
+
+/* tracking_rad_map_setup for one half slice (tracking_rad_map_setup.f90:19-84):
+ * out = ref_orb[6], damp_dmat[36], xfer_damp_vec[6], stoc_mat[36]. */
+static int o_tracking_rad_map_setup(const bmx_ele_rec *runt, int ref_species, const double *orb0, const double *orb1,
+                                    const double *mat6, double tol, double g2_int, double g3_int, int upstream,
+                                    double *out) {
+  double *ref = out, *D = out + 6, *dv = out + 42, *S = out + 48, m_inv[36], T[36];
+  int i;
+  memset(out, 0, 84 * sizeof(double));
+  for (i = 0; i < 6; i++) ref[i] = -1; /* rad_map_struct default */
+  if (orb0[1] == orb1[1] && orb0[3] == orb1[3] && runt->key != BMX_KEY_SBEND) return 0;
+  if (o_rad1(runt, ref_species, orb0, orb1, mat6, tol * g2_int, tol * g3_int, ref, D, dv, S) < 0) return -1;
+  if (upstream) {
+    m6_symp_conj(mat6, m_inv);
+    m6_mul(m_inv, D, D);
+    m6_vec(m_inv, dv, dv);
+    m6_mul(m_inv, S, S);
+    m6_transpose(m_inv, T);
+    m6_mul(S, T, S);
+    for (i = 0; i < 6; i++) ref[i] = orb0[i];
+  } else {
+    for (i = 0; i < 6; i++) ref[i] = orb1[i];
+  }
+  if (o_cholesky_lower(S) < 0) memset(S, 0, 36 * sizeof(double));
+  return 0;
+}
+
+/* radiation_map_setup (radiation_mod.f90:160-232) about the reference orbit
+ * orb_in at the element's entrance: rm[0 .. 83] = rm0 (entrance half),
+ * rm[84 .. 167] = rm1 (exit half), in the bmx_rad_map layout.  total_length,
+ * g2_integral and g3_integral are the branch's (branch%param): they set the
+ * Romberg tolerance (1e-4 / total_length of each integral).  Returns 0, -1
+ * when an element or its reference orbit is outside what this restatement
+ * models, or -2 when the reference particle is lost in the element. */
+ORACLE_API int oracle_rad_map_setup(const bmx_ele_rec *ele, int ref_species, const double *orb_in,
+                                    double total_length, double g2_integral, double g3_integral, double *rm) {
+  bmx_ele_rec runt;
+  double orb_out[6], orb1[6], orb2[6], M[36], tol;
+  int i;
+  memset(rm, 0, 2 * 84 * sizeof(double));
+  for (i = 0; i < 6; i++) rm[i] = rm[84 + i] = -1;
+  if (ele->l == 0) return 0;
+  if (o_make_mat6(ele, ref_species, orb_in, orb_out, M) < 0) return -2;
+  if (orb_in[1] == orb_out[1] && orb_in[3] == orb_out[3] && ele->key != BMX_KEY_SBEND) {
+    for (i = 0; i < 6; i++) {
+      rm[i] = orb_in[i];
+      rm[84 + i] = orb_out[i];
+    }
+    return 0;
+  }
+  tol = 1e-4 / total_length;
+  /* first half: the upstream end */
+  if (o_slice(ele, 0.5 * ele->l, 0.0, 1, 0, 0, &runt) < 0) return -1;
+  if (o_make_mat6(&runt, ref_species, orb_in, orb1, M) < 0) return -2;
+  if (o_tracking_rad_map_setup(&runt, ref_species, orb_in, orb1, M, tol, g2_integral, g3_integral, 1, rm) < 0)
+    return -1;
+  /* second half: the downstream end, its map referred to the element's exit orbit */
+  if (o_slice(ele, 0.5 * ele->l, 0.5 * ele->l, 0, 1, 0, &runt) < 0) return -1;
+  if (o_make_mat6(&runt, ref_species, orb1, orb2, M) < 0) return -2;
+  if (o_tracking_rad_map_setup(&runt, ref_species, orb1, orb_out, M, tol, g2_integral, g3_integral, 0, rm + 84) < 0)
+    return -1;
+  return 0;
+}
+
+/* The element's map and Jacobian at vin (make_mat6), for tests. */
+ORACLE_API int oracle_make_mat6(const bmx_ele_rec *ele, int ref_species, const double *vin, double *vout, double *M) {
+  return o_make_mat6(ele, ref_species, vin, vout, M);
+}
+
+/* A slice of an element (create_element_slice), for tests. */
+ORACLE_API int oracle_ele_slice(const bmx_ele_rec *ele, double l_slice, double offset, int inc_up, int inc_down,
+                                bmx_ele_rec *out) {
+  return o_slice(ele, l_slice, offset, inc_up, inc_down, 0, out);
+}
+#endif
+
+#ifdef ORACLE_DUAL
 /* One particle through n_turns of the element list with dual numbers: returns
  * the final coordinates and the 6x6 Jacobian J[i*6+j] = d out_i / d in_j. */
 ORACLE_API int oracle_jacobian(const bmx_ele_rec *eles, int n_ele, int ref_species, int n_turns, const double *vec,
@@ -2373,7 +3147,7 @@ ORACLE_API int oracle_jacobian(const bmx_ele_rec *eles, int n_ele, int ref_speci
   for (turn = 0; turn < n_turns; turn++) {
     for (ie = 0; ie < n_ele; ie++) {
       a.p0c = eles[ie].p0c_start;
-      track1(&a, &eles[ie], ref_species, &b);
+      track1(&a, &eles[ie], ref_species, &b, NULL);
       a = b;
       if (a.state != BMX_ALIVE) return a.state;
     }

@@ -220,3 +220,103 @@ def symp_error(M):
         S[2 * k, 2 * k + 1] = 1
         S[2 * k + 1, 2 * k] = -1
     return float(np.max(np.abs(M.T @ S @ M - S)))
+
+
+# --------------------------------------------------------------------------
+# radiation (track1_radiation and radiation_map_setup restated in bmx_oracle.c)
+# --------------------------------------------------------------------------
+def _dual():
+    global _lib_dual
+    if _lib_dual is None:
+        if not os.path.exists(LIB_DUAL):
+            build()
+        _lib_dual = C.CDLL(LIB_DUAL)
+    L = _lib_dual
+    if not getattr(L, "_rad_ready", False):
+        L.oracle_rad_map_setup.restype = C.c_int
+        L.oracle_rad_map_setup.argtypes = [C.POINTER(abi.EleRec), C.c_int, _DP, C.c_double, C.c_double, C.c_double, _DP]
+        L.oracle_make_mat6.restype = C.c_int
+        L.oracle_make_mat6.argtypes = [C.POINTER(abi.EleRec), C.c_int, _DP, _DP, _DP]
+        L.oracle_ele_slice.restype = C.c_int
+        L.oracle_ele_slice.argtypes = [C.POINTER(abi.EleRec), C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.POINTER(abi.EleRec)]
+        L._rad_ready = True
+    return L
+
+
+def make_mat6(ele, species, vec):
+    """(orbit at the exit, 6x6 Jacobian) of one element at vec (make_mat6)."""
+    v = (C.c_double * 6)(*vec)
+    out = (C.c_double * 6)()
+    M = (C.c_double * 36)()
+    if _dual().oracle_make_mat6(C.byref(ele), species, v, out, M) != 0:
+        raise RuntimeError("particle lost in make_mat6")
+    return np.array(out[:]), np.array(M[:]).reshape(6, 6)
+
+
+def ele_slice(ele, l_slice, offset, inc_up, inc_down):
+    out = abi.EleRec()
+    if _dual().oracle_ele_slice(C.byref(ele), l_slice, offset, int(inc_up), int(inc_down), C.byref(out)) != 0:
+        raise RuntimeError("element not sliceable by the oracle")
+    return out
+
+
+def rad_map_setup(ele, species, orb_in, total_length, g2_integral=0.0, g3_integral=0.0):
+    """radiation_map_setup of one element about the reference orbit orb_in:
+    float64 [2, 84] (rm0, rm1 in the bmx_rad_map layout)."""
+    v = (C.c_double * 6)(*orb_in)
+    rm = np.zeros(168)
+    rc = _dual().oracle_rad_map_setup(C.byref(ele), species, v, total_length, g2_integral, g3_integral,
+                                      rm.ctypes.data_as(_DP))
+    if rc != 0:
+        raise RuntimeError(f"oracle_rad_map_setup returned {rc}")
+    return rm.reshape(2, 84)
+
+
+def rad_maps(eles, species, orbits, g2_integral=0.0, g3_integral=0.0):
+    """Radiation maps of every element about the reference orbit at each
+    element's entrance (orbits [n_ele, 6]): [n_ele, 2, 84]."""
+    arr = eles if isinstance(eles, C.Array) else abi_array(eles)
+    total = float(sum(e.l for e in arr))
+    return np.stack([rad_map_setup(arr[i], species, orbits[i], total, g2_integral, g3_integral)
+                     for i in range(len(arr))])
+
+
+def _rad_lib(L):
+    if not getattr(L, "_rad_set", False):
+        L.oracle_set_radiation.restype = None
+        L.oracle_set_radiation.argtypes = [_DP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64]
+        L.oracle_philox4x32_10.restype = None
+        L.oracle_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]
+        L.oracle_rad_gauss6.restype = None
+        L.oracle_rad_gauss6.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        _DP]
+        L._rad_set = True
+    return L
+
+
+def philox4x32_10(ctr, key):
+    L = _rad_lib(lib())
+    c = (C.c_uint32 * 4)(*ctr)
+    L.oracle_philox4x32_10(c, key[0], key[1])
+    return list(c)
+
+
+def rad_gauss6(pid, ie, edge, turn, call, seed):
+    L = _rad_lib(lib())
+    g = (C.c_double * 6)()
+    L.oracle_rad_gauss6(pid, ie, edge, turn, call, seed & 0xffffffff, seed >> 32, g)
+    return np.array(g[:])
+
+
+def track_bunch_rad(eles, species, n_turns, coords, maps, damping_on=True, fluctuations_on=False,
+                    zero_average=False, synch_rad_scale=1.0, seed=0, call=0, fma=False, **kw):
+    """track_bunch with track1_radiation at every element (maps [n_ele, 2, 84])."""
+    L = _rad_lib(lib_fma() if fma else lib())
+    m = np.ascontiguousarray(np.asarray(maps, dtype=np.float64))
+    L.oracle_set_radiation(m.ctypes.data_as(_DP), int(damping_on), int(fluctuations_on), int(zero_average),
+                           float(synch_rad_scale), int(seed), int(call))
+    try:
+        return track_bunch(eles, species, n_turns, coords, fma=fma, **kw)
+    finally:
+        L.oracle_set_radiation(None, 0, 0, 0, 1.0, 0, 0)

@@ -29,7 +29,7 @@ def test_library_exports_every_header_symbol():
 
 def test_struct_layout_matches_header():
     lib = abi.load_library()
-    assert lib.bmx_abi_version() == 3
+    assert lib.bmx_abi_version() == 4
     assert abi.check_layout(lib)
     assert ctypes.sizeof(abi.EleRec) == lib.bmx_sizeof(0)
 

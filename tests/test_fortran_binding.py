@@ -20,12 +20,16 @@ type(bmx_stats) :: s
 type(bmx_grid) :: g
 type(bmx_config) :: c
 type(bmx_schedule) :: sc
+type(bmx_rad_map) :: rm
+type(bmx_radiation) :: ra
 print '(i0,1x,i0)', c_sizeof(e), bmx_sizeof(0)
 print '(i0,1x,i0)', c_sizeof(p), bmx_sizeof(1)
 print '(i0,1x,i0)', c_sizeof(s), bmx_sizeof(2)
 print '(i0,1x,i0)', c_sizeof(g), bmx_sizeof(3)
 print '(i0,1x,i0)', c_sizeof(c), bmx_sizeof(4)
 print '(i0,1x,i0)', c_sizeof(sc), bmx_sizeof(5)
+print '(i0,1x,i0)', c_sizeof(rm), bmx_sizeof(6)
+print '(i0,1x,i0)', c_sizeof(ra), bmx_sizeof(7)
 print '(i0)', bmx_abi_version()
 end program
 """
@@ -39,7 +43,7 @@ def test_fortran_binding_layout(tmp_path):
     subprocess.check_call([FLANG, "t.f90", "bmx_c_binding.o", f"-L{LIBDIR}", "-lbmx", f"-Wl,-rpath,{LIBDIR}", "-o", "t"],
                           cwd=tmp_path)
     out = subprocess.check_output([str(tmp_path / "t")], cwd=tmp_path, text=True).split("\n")
-    for line in out[:6]:
+    for line in out[:8]:
         a, b = line.split()
         assert a == b, out
-    assert out[6].strip() == "3"
+    assert out[8].strip() == "4"
