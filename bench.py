@@ -218,6 +218,43 @@ def abi_species():
     return abi.ELECTRON
 
 
+def radiation_rates(ctx, eles, coords, steps=3, warmup=1, turns=10):
+    """The workload's tracking rate with radiation on (track1_radiation at
+    both edges of every element: bmx_lattice_upload_rad, with the maps of the
+    ring's 500 bends about the design orbit from bmx.radiation), damping
+    alone and damping with fluctuations, against the same bunch without."""
+    import torch
+    from bmx import abi, radiation, tracker
+
+    t0 = time.perf_counter()
+    maps = radiation.design_orbit_maps(ctx, eles, abi.ELECTRON)
+    t_maps = time.perf_counter() - t0
+    out = {"maps_s": t_maps, "radiating_elements": int(np.sum(np.any(maps[:, :, 6:] != 0, axis=(1, 2)))),
+           "energy_loss_per_turn_rel": float(-maps[:, :, 47].sum())}
+    for name, rad in (("plain", None), ("damping", dict(damping_on=True)),
+                      ("damping_fluctuations", dict(damping_on=True, fluctuations_on=True, seed=20261204))):
+        lat = ctx.lattice(eles, abi.ELECTRON) if rad is None else ctx.lattice(eles, abi.ELECTRON, rad_maps=maps,
+                                                                             radiation=rad)
+        b = tracker.Bunch(coords, lat.p0c, abi.M_ELECTRON, e_tot=lat.e_tot)
+        for _ in range(warmup):
+            tracker.track_bunch(lat, b, n_turns=turns, sync=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kms = 0.0
+        for _ in range(steps):
+            kms += tracker.track_bunch(lat, b, n_turns=turns, sync=False)["kernel_ms"]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        n_live = int(b.state.eq(abi.ALIVE).sum().item())
+        out[name] = {"particle_turns_per_s": b.n * turns * steps / wall, "kernel_ms_per_launch": kms / steps,
+                     "particles_alive_end": n_live}
+        lat.close()
+        del b
+    for name in ("damping", "damping_fluctuations"):
+        out[name]["rate_vs_plain"] = out[name]["particle_turns_per_s"] / out["plain"]["particle_turns_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +277,8 @@ def main():
                     help="generations of the surrogate-guided optimisation (20: the full loop; 0: skip it)")
     ap.add_argument("--opt-only", action="store_true",
                     help="only run the surrogate-guided optimisation and print its record")
+    ap.add_argument("--no-rad", action="store_true", help="skip the tracking rates with radiation")
+    ap.add_argument("--rad-only", action="store_true", help="only run the tracking rates with radiation")
     ap.add_argument("--error-study-sample", type=int, default=0, metavar="N_VAR",
                     help="only run the batched-DA error study on its first N_VAR variants and print its record")
     args = ap.parse_args()
@@ -276,6 +315,10 @@ def main():
     n_ele = len(eles)
     tw = beam.periodic_twiss(lattice.ring_c1(n_cells=500, n_rf=0)[:8])
     coords = beam.matched_gaussian(args.particles, tw, 10e-9, 0.1e-9, 1e-3, 1e-3, seed=20261203)
+
+    if args.rad_only:
+        print(json.dumps({"radiation": radiation_rates(tracker.Context(0), eles, coords)}), flush=True)
+        return
 
     with open(os.path.join(ROOT, "tests", "golden", "flops_per_pet.json")) as f:
         flops = json.load(f)["C1"]
@@ -330,6 +373,9 @@ def main():
                      "flops_per_pet": flops["flops_per_pet"], "kernel_ms_per_launch": kernel_ms / args.steps},
     }
     lat.close()
+    del bunch
+    if not args.no_rad:
+        result["radiation"] = radiation_rates(ctx, eles, coords)
     if not args.no_da:
         result["da_scan"] = da_scan(ctx, n_train=args.da_train, n_test=args.da_test, epochs=args.da_epochs)
         if args.opt_gens > 0:

@@ -244,3 +244,38 @@ def test_equilibrium_emittance_and_energy_spread(ctx):
     assert eps_y == pytest.approx(eps_y_eq, rel=0.06)
     assert sig_pz == pytest.approx(sig_pz_eq, rel=0.04)
     lat.close()
+
+
+def test_design_orbit_maps_equal_the_oracles(ctx):
+    """bmx.radiation.design_orbit_maps (the library's own transfer matrices)
+    against the oracle's radiation_map_setup on the design orbit: the FODO
+    ring's bends, the 6 GeV rf ring's, and gradient bends with Hwang and
+    linear edges at e1, e2 != 0."""
+    from bmx import radiation
+    extra = [
+        lattice.ele("drift", l=0.3),
+        lattice.ele("sbend", l=3.0, angle=np.pi / 8, e1=0.07, e2=0.07, fringe_type=abi.FRINGE_BASIC_BEND, num_steps=1),
+        lattice.ele("quadrupole", l=0.5, k1=0.6, num_steps=1),
+        lattice.ele("sbend", l=2.0, angle=0.1, k1=-0.2, e1=0.05, e2=-0.03, fringe_type=abi.FRINGE_LINEAR_EDGE,
+                    fint=0.5, hgap=0.02, num_steps=4),
+        lattice.ele("sbend", l=1.5, angle=0.08, e1=0.02, fringe_type=abi.FRINGE_HARD_EDGE_ONLY,
+                    fringe_at=abi.ENTRANCE_END, num_steps=1),
+    ]
+    lattice.set_reference(extra, abi.ELECTRON, e_tot=3e9)
+    for name, eles in (("c0", lattice.fodo_ring_c0(n_cells=20)), ("rf ring", lattice.ring_c1(n_cells=50, n_rf=8)),
+                       ("edges", extra)):
+        mp = radiation.design_orbit_maps(ctx, eles, abi.ELECTRON)
+        mo = oracle.rad_maps(eles, abi.ELECTRON, np.zeros((len(eles), 6)))
+        n_bends = 0
+        for i, e in enumerate(eles):
+            for h in range(2):
+                for lo, hi, what in ((6, 42, "damp_dmat"), (42, 48, "xfer_damp_vec"), (48, 84, "stoc_mat")):
+                    a, b = mp[i, h, lo:hi], mo[i, h, lo:hi]
+                    scale = np.max(np.abs(b))
+                    if e.key != abi.KEY_SBEND:
+                        assert np.all(a == 0.0) and scale == 0.0, (name, i, what)
+                        continue
+                    assert scale > 0, (name, i, h, what)
+                    assert np.max(np.abs(a - b)) <= 1e-11 * scale, (name, i, h, what, np.max(np.abs(a - b)) / scale)
+            n_bends += e.key == abi.KEY_SBEND
+        print(f"design-orbit maps {name}: {n_bends} bends equal to the oracle's", flush=True)
