@@ -32,9 +32,14 @@ constexpr int BM = 64, BN = 64, BK = 32;
 constexpr int LDA_S = BK + 8;  // LDS row pitch (bf16) for A[m][k]
 constexpr int LDB_S = BK + 8;  // LDS row pitch (bf16) for B^T[n][k]
 
-__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+// SiLU x sigma(x) with sigma = 1 / (1 + e^-x) from the hardware exponential and
+// reciprocal (v_exp_f32, v_rcp_f32: about 1 ulp each), the activation's cost
+// in the epilogues being what bounds the fused forward pass; every kernel
+// (GEMM epilogues, fused pass, backward) uses these same two functions.
+__device__ __forceinline__ float sigmoid(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+__device__ __forceinline__ float silu(float x) { return x * sigmoid(x); }
 __device__ __forceinline__ float silu_grad(float x) {
-  float s = 1.0f / (1.0f + __expf(-x));
+  float s = sigmoid(x);
   return s * (1.0f + x * (1.0f - s));
 }
 
@@ -349,8 +354,11 @@ __device__ __forceinline__ bf16x8_t load_w8(const __bf16* __restrict__ W, int64_
   return v;
 }
 
+// 128 VGPRs (the accumulators included) so that four waves share each SIMD:
+// one wave's SiLU epilogue and LDS stores overlap the others' MFMAs (at two
+// waves per SIMD the matrix cores idled through the epilogues).
 template <int RB>  // row blocks of 16 per workgroup: FR = 16 RB rows
-__global__ __launch_bounds__(256) void mlp_fused_forward_kernel(int64_t n, FusedNet net, FusedIn in,
+__global__ __launch_bounds__(256, 4) void mlp_fused_forward_kernel(int64_t n, FusedNet net, FusedIn in,
                                                                float* __restrict__ y) {
   constexpr int FR = 16 * RB;
   __shared__ __bf16 act[FR * FP];

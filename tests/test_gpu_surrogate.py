@@ -162,7 +162,12 @@ def test_network_forward_backward_vs_oracle(ctx, n):
     ey = np.max(np.abs(got_y - want_y)) / np.max(np.abs(want_y))
     el = abs(loss.item() - want_loss) / want_loss
     print(f"forward {ey:.2e} loss {el:.2e}", flush=True)
-    assert ey < 1e-3 and el < 1e-4
+    # the device's SiLU takes the hardware exponential and reciprocal (about
+    # 1 ulp each): where that moves a pre-activation's SiLU across a BF16
+    # rounding boundary, the oracle's BF16 emulation rounds the other way,
+    # one BF16 ulp (2^-8) on that activation, which reaches the output at
+    # about 1e-3 of its range
+    assert ey < 3e-3 and el < 1e-4
     for li, ((gW, gb), (wW, wb)) in enumerate(zip(net.grads(), want_g)):
         eW = np.linalg.norm(gW - wW) / np.linalg.norm(wW)
         eb = np.linalg.norm(gb - wb) / np.linalg.norm(wb)
