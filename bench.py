@@ -49,6 +49,20 @@ def measured_traffic(particles, turns_per_step):
     return None
 
 
+def measured_hw_flops_per_pet(particles, turns_per_step):
+    """FP64 flops per particle-element-turn the hardware counters counted on
+    the default workload (64 lanes x (ADD + MUL + 2 FMA + TRANS) of the
+    committed counter run): the `hardware_fp64_flops_per_pet:` line of the
+    kernel's profile summary.  None for another workload or without it."""
+    if particles != N_PARTICLES or turns_per_step != TURNS_PER_STEP or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.strip().startswith("hardware_fp64_flops_per_pet:"):
+                return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
 def physical_cores(cpus):
     """Physical cores (distinct package, core id) among the logical CPUs
     this process may run on, from the kernel's CPU topology (what lscpu
@@ -171,7 +185,7 @@ def opt_search(ctx, n_gen, n_init=2048):
     return r
 
 
-def error_study_grid(ctx, n_var):
+def error_study_grid(ctx, n_var, per_variant=True):
     """The batched-DA error study (n_var = 256 is the whole study; fewer is a
     sample): the first n_var of the 256 seeded error variants (504..4000-element C1-style rings with 8 rf
     cavities; quad k1 x (1 + 1e-3 g), sextupole k2 x (1 + 5e-3 g), 50 um
@@ -182,7 +196,8 @@ def error_study_grid(ctx, n_var):
     so the long-lived tails of the five pz slices share their blocks.
     lattices/hour = n_var / the wall time of the whole study (upload and
     compile of the variants, tuning, scan); the scan alone is reported
-    beside it."""
+    beside it.  per_variant=False reports the spread of the variants'
+    lengths and DA areas instead of one entry per variant."""
     from bmx import beam, lattice, tracker
     vs = lattice.da_variants(n_var=n_var)
     t_study = time.perf_counter()
@@ -206,11 +221,20 @@ def error_study_grid(ctx, n_var):
     print(f"error study: done in {wall:.1f} s, survivors per pz slice "
           f"{[round(float(np.mean(surv[:, i] == n_turns)), 4) for i in range(5)]}", file=sys.stderr, flush=True)
     lat.close()
-    return {"workload": f"{n_var} of the 256 seeded error variants, grid 64 x 128 x 5 (0..{amp * 1e3:.2f} mm, "
-                        f"pz -2..2 %), 1024 turns, FP64",
-            "n_elements": [len(v) for v in vs], "lattices_per_hour": n_var / wall_study * 3600.0,
-            "wall_s": wall_study, "scan_only_lattices_per_hour": n_var / wall * 3600.0, "scan_wall_s": wall,
-            "pet_per_s": pet / wall, "schedule": sched, "da_area_per_variant_m2": np.asarray(area).tolist()}
+    n_ele = [len(v) for v in vs]
+    area = np.asarray(area)
+    rec = {"workload": f"{n_var} of the 256 seeded error variants, grid 64 x 128 x 5 (0..{amp * 1e3:.2f} mm, "
+                       f"pz -2..2 %), 1024 turns, FP64",
+           "lattices_per_hour": n_var / wall_study * 3600.0,
+           "wall_s": wall_study, "scan_only_lattices_per_hour": n_var / wall * 3600.0, "scan_wall_s": wall,
+           "pet_per_s": pet / wall, "schedule": sched}
+    if per_variant:
+        rec["n_elements"] = n_ele
+        rec["da_area_per_variant_m2"] = area.tolist()
+    else:
+        rec["n_elements_min_max"] = [min(n_ele), max(n_ele)]
+        rec["da_area_m2_min_mean_max"] = [float(area.min()), float(area.mean()), float(area.max())]
+    return rec
 
 
 def abi_species():
@@ -278,6 +302,7 @@ def main():
     ap.add_argument("--opt-only", action="store_true",
                     help="only run the surrogate-guided optimisation and print its record")
     ap.add_argument("--no-rad", action="store_true", help="skip the tracking rates with radiation")
+    ap.add_argument("--no-error-study", action="store_true", help="skip the batched-DA error study")
     ap.add_argument("--rad-only", action="store_true", help="only run the tracking rates with radiation")
     ap.add_argument("--error-study-sample", type=int, default=0, metavar="N_VAR",
                     help="only run the batched-DA error study on its first N_VAR variants and print its record")
@@ -350,6 +375,7 @@ def main():
     value = particle_turns / wall
     kernel_s = kernel_ms / 1e3
     achieved = flops_per_particle_turn * particle_turns / kernel_s / 1e12
+    hw_per_pet = measured_hw_flops_per_pet(args.particles, args.turns_per_step)
     result = {
         "metric": "FP64 particle-turns/s",
         "value": value,
@@ -370,12 +396,19 @@ def main():
         "roofline": {"bound": "fp64-vector", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic(args.particles, args.turns_per_step),
-                     "flops_per_pet": flops["flops_per_pet"], "kernel_ms_per_launch": kernel_ms / args.steps},
+                     "flops_per_pet": flops["flops_per_pet"], "kernel_ms_per_launch": kernel_ms / args.steps,
+                     "hw_fp64_flops_per_pet": hw_per_pet,
+                     "hw_achieved": (None if hw_per_pet is None else
+                                     hw_per_pet * particle_turns * n_ele / kernel_s / 1e12),
+                     "hw_frac": (None if hw_per_pet is None else
+                                 hw_per_pet * particle_turns * n_ele / kernel_s / 1e12 / FP64_PEAK_TFLOPS)},
     }
     lat.close()
     del bunch
     if not args.no_rad:
         result["radiation"] = radiation_rates(ctx, eles, coords)
+    if not args.no_error_study:
+        result["error_study"] = error_study_grid(ctx, 256, per_variant=False)
     if not args.no_da:
         result["da_scan"] = da_scan(ctx, n_train=args.da_train, n_test=args.da_test, epochs=args.da_epochs)
         if args.opt_gens > 0:

@@ -364,6 +364,14 @@ __global__ __launch_bounds__(256, 4) void mlp_fused_forward_kernel(int64_t n, Fu
   __shared__ __bf16 act[FR * FP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t row0 = (int64_t)blockIdx.x * FR;
+  // DA features: the lattice l and grid point i of row0 by one division per
+  // workgroup; the rows after it step through i and wrap into the next lattice
+  int64_t l0 = 0;
+  int i0 = 0;
+  if (!in.X) {
+    l0 = row0 / in.n_ic;
+    i0 = (int)(row0 - l0 * in.n_ic);
+  }
   for (int e = tid; e < FR * FK0; e += 256) {
     int m = e / FK0, k = e % FK0;
     int64_t row = row0 + m;
@@ -372,8 +380,12 @@ __global__ __launch_bounds__(256, 4) void mlp_fused_forward_kernel(int64_t n, Fu
       if (in.X) {
         v = in.X[row * net.d0 + k];
       } else {
-        int64_t l = row / in.n_ic;
-        int i = (int)(row % in.n_ic);
+        int64_t l = l0;
+        int i = i0 + m;
+        if (i >= in.n_ic) {
+          l += i / in.n_ic;
+          i %= in.n_ic;
+        }
         v = (k < in.n_knob) ? (__bf16)(in.knobs[l * in.n_knob + k] * in.inv_range)
                             : (__bf16)in.geo[(int64_t)i * in.n_geo + (k - in.n_knob)];
       }
