@@ -1167,7 +1167,10 @@ __device__ __forceinline__ void gauss_pair(const uint32_t r[4], bmx_f64* n0, bmx
 // beta keeps its value, as in the reference; 1/(1+pz) and the drift's
 // beta/beta0 - 1, which this code forms from pz, follow pz.  The particle
 // time does not change, so its offset from the reference relation takes
-// the change of z.
+// the change of z.  In the transfer-matrix build (BMX_TANGENT) the kick
+// moves the orbit's values and leaves the derivatives as they are: track1
+// applies track1_radiation to the orbit only (track1.f90:198-201, :316-319),
+// so make_mat6's ele%mat6 is the body's matrix along the radiating orbit.
 __device__ void op_rad(Lane& L, const DevOp& op, int flags, const LatticeConsts& lc, const TurnTime& tt) {
   if ((flags & F_RAD_EXIT) && too_large(L, lc)) return;
   // the map's address, wave-uniform: its entries come in through the scalar cache
@@ -1176,6 +1179,22 @@ __device__ void op_rad(Lane& L, const DevOp& op, int flags, const LatticeConsts&
   uint32_t a_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
   const cdouble* m = (const cdouble*)(uintptr_t)(((uint64_t)a_hi << 32) | a_lo);
   double v[6] = {L.x, L.px, L.y, L.py, L.z, L.pz};
+#if BMX_TANGENT
+  if (flags & F_RAD_DAMP) {
+    bmx_f64 d[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) d[c] = v[c].v - sg(m[RAD_REF + c]);
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      bmx_f64 a = sg(m[RAD_DAMP + r * 6]) * d[0];
+#pragma unroll
+      for (int c = 1; c < 6; c++) a = a + sg(m[RAD_DAMP + r * 6 + c]) * d[c];
+      v[r].v = v[r].v + a;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) v[r].v = v[r].v + sg(m[RAD_VEC + r]);
+  }
+#else
   if (flags & F_RAD_DAMP) {
     double d[6];
 #pragma unroll
@@ -1190,7 +1209,6 @@ __device__ void op_rad(Lane& L, const DevOp& op, int flags, const LatticeConsts&
 #pragma unroll
     for (int r = 0; r < 6; r++) v[r] = v[r] + sg(m[RAD_VEC + r]);
   }
-#ifndef BMX_TANGENT
   if (flags & F_RAD_FLUCT) {
     bmx_f64 g[6];
 #pragma unroll

@@ -126,6 +126,9 @@ struct bmx_lattice {
   double* d_rad = nullptr;
   bool rad_fluct = false;
   mutable std::atomic<uint64_t> rad_next_call{0};
+  // per variant: the lowest rf frequency among the cavities that are on (0:
+  // none), which sets the rf wavelength of the closed-orbit search
+  std::vector<double> var_rf_freq;
 };
 
 // The fluctuation stream of one tracking call: the lattice's next call number.
@@ -252,6 +255,7 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
     LatticeConsts lc;
     std::string err;
     int n_ele = 0;
+    double rf_freq = 0;
     bool ok = false;
   };
   const bool with_rad = rad && rad_maps && (rad->damping_on || rad->fluctuations_on);
@@ -267,6 +271,9 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
         int n;
         src(v, buf, &e, &n);
         o.n_ele = n;
+        for (int i = 0; i < n; i++)
+          if (e[i].key == BMX_KEY_RFCAVITY && e[i].is_on && e[i].rf_frequency != 0)
+            o.rf_freq = o.rf_freq == 0 ? std::fabs(e[i].rf_frequency) : std::min(o.rf_freq, std::fabs(e[i].rf_frequency));
         RadInput ri;
         if (with_rad) {
           ri.maps = rad_maps(v);
@@ -315,6 +322,7 @@ static int32_t upload_impl(bmx_ctx* ctx, int32_t n_variants, const VariantSource
     }
     lat->ele_op_begin.push_back(std::move(o.eb));
     lat->var_n_ele.push_back(o.n_ele);
+    lat->var_rf_freq.push_back(o.rf_freq);
     std::vector<DevOp>().swap(o.prog);
   }
   lat->var_op_begin.push_back((int32_t)prog.size());
@@ -1099,36 +1107,94 @@ int32_t bmx_bunch_moments(bmx_ctx* ctx, const bmx_particles* p, double* moments)
   return rc;
 }
 
+// Spectral radius of a 6x6 matrix from the growth of its powers: M is
+// squared K times with the largest entry renormalised to 1 each time, and
+// log rho = log ||M^(2^K)|| / 2^K to within log(cond M) / 2^K.
+static double spectral_radius6(const double (&M)[6][6]) {
+  double A[6][6], T[6][6];
+  double nrm = 0;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) nrm = std::max(nrm, std::fabs(M[i][j]));
+  if (nrm == 0 || !std::isfinite(nrm)) return nrm;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) A[i][j] = M[i][j] / nrm;
+  double lg = std::log(nrm);  // M^(2^k) = exp(lg) A
+  const int K = 30;
+  for (int k = 0; k < K; k++) {
+    double n2 = 0;
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) {
+        double a = 0;
+        for (int q = 0; q < 6; q++) a += A[i][q] * A[q][j];
+        T[i][j] = a;
+        n2 = std::max(n2, std::fabs(a));
+      }
+    if (n2 == 0) return 0;
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) A[i][j] = T[i][j] / n2;
+    lg = 2 * lg + std::log(n2);
+  }
+  return std::exp(lg / (double)(1 << K));
+}
+
 // closed_orbit_calc (bmad/code/closed_orbit_calc.f90:81-...) in the form the
 // batched tracker serves: Newton iteration x <- x + (I - M)^-1 (T(x) - x) on
 // the one-turn map T of every variant at once, with M from one-sided finite
 // differences (one launch tracks 1 + n_dim probes per variant).  n_dim = 4
-// keeps pz (and z) of the guess fixed; n_dim = 6 closes all six (rf on).
+// keeps pz (and z) of the guess fixed; n_dim = 6 closes all six (rf on) and,
+// as the reference does (:241-256, :400-410, :491-520): keeps z within half
+// an rf wavelength of zero (the lowest rf frequency of the cavities that are
+// on), limits a z step to a tenth of the wavelength, and, when the orbit
+// found is longitudinally unstable (a one-turn eigenvalue beyond 1 + 1e-6),
+// moves z by multiples of an eighth of the wavelength to the start whose
+// one-turn matrix is the most stable and iterates again from there.  An
+// orbit has converged when a step is below 1e-15 (relative to 1 + |x|) or,
+// below 1e-9, no longer shrinks: the rounding floor of the one-turn map.
 int32_t bmx_closed_orbit(bmx_ctx* ctx, const bmx_lattice* lat, int32_t n_dim, double* orbit, int32_t* status,
                          bmx_stats* out) {
   if (lat && lat->rad_fluct) return set_err(ctx, "bmx_closed_orbit: radiation fluctuations are tracked by bmx_track_bunch and bmx_track_multi only");
   if (!ctx || !lat || !orbit || !status || (n_dim != 4 && n_dim != 6)) return set_err(ctx, "invalid arguments");
-  int nv = lat->n_variants;
-  int np = 1 + n_dim;
-  int64_t n = (int64_t)nv * np;
-  std::vector<int64_t> off(nv + 1);
-  for (int v = 0; v <= nv; v++) off[v] = (int64_t)v * np;
-  std::vector<double> c(6 * n), beta(n);
-  std::vector<int32_t> state(n), ixl(n), tl(n);
-  std::vector<int> done(nv, 0);
-  for (int v = 0; v < nv; v++) status[v] = 1;
+  const int nv = lat->n_variants;
+  const int np = 1 + n_dim;
   const double h = 1e-8;
-  double mc2 = lat->lc.mc2, p0c = lat->lc.p0c;
-  if (out) std::memset(out, 0, sizeof(*out));
-  for (int it = 0; it < 40; it++) {
+  const double mc2 = lat->lc.mc2, p0c = lat->lc.p0c;
+  std::vector<double> wavelen(nv, 0.0);
+  if (n_dim == 6)
     for (int v = 0; v < nv; v++)
-      for (int j = 0; j < np; j++) {
-        int64_t k = off[v] + j;
-        for (int d = 0; d < 6; d++) c[(size_t)d * n + k] = orbit[v * 6 + d] + ((j > 0 && d == j - 1) ? h : 0.0);
-        double pz = c[(size_t)5 * n + k], pc = p0c * (1.0 + pz);
-        beta[k] = pc / std::sqrt(pc * pc + mc2 * mc2);
-        state[k] = BMX_ALIVE;
-      }
+      if (lat->var_rf_freq[v] > 0) wavelen[v] = D_C_LIGHT * (p0c / std::sqrt(p0c * p0c + mc2 * mc2)) / lat->var_rf_freq[v];
+  // x folded into [-amp, amp) (modulo2, sim_utils/special_functions/modulo2_mod.f90:85-99)
+  auto modulo2 = [](double x, double amp) {
+    double m = x - 2 * amp * std::floor(x / (2 * amp));
+    return m >= amp ? m - 2 * amp : m;
+  };
+  if (out) std::memset(out, 0, sizeof(*out));
+
+  // One tracking launch over sets of probes: each base orbit of a variant
+  // with its n_dim displaced copies.  Per base: lost, T(base) and the
+  // finite-difference Jacobian (columns past n_dim are the identity's).
+  struct Probe {
+    bool lost;
+    double t[6];
+    double m[6][6];
+  };
+  auto run_probes = [&](const std::vector<std::vector<std::array<double, 6>>>& bases,
+                        std::vector<std::vector<Probe>>& res) -> int {
+    std::vector<int64_t> off(nv + 1, 0);
+    for (int v = 0; v < nv; v++) off[v + 1] = off[v] + (int64_t)bases[v].size() * np;
+    const int64_t n = off[nv];
+    res.assign(nv, {});
+    if (n == 0) return 0;
+    std::vector<double> c(6 * n), beta(n);
+    std::vector<int32_t> state(n), ixl(n), tl(n);
+    for (int v = 0; v < nv; v++)
+      for (size_t b = 0; b < bases[v].size(); b++)
+        for (int j = 0; j < np; j++) {
+          int64_t k = off[v] + (int64_t)b * np + j;
+          for (int d = 0; d < 6; d++) c[(size_t)d * n + k] = bases[v][b][d] + ((j > 0 && d == j - 1) ? h : 0.0);
+          double pz = c[(size_t)5 * n + k], pc = p0c * (1.0 + pz);
+          beta[k] = pc / std::sqrt(pc * pc + mc2 * mc2);
+          state[k] = BMX_ALIVE;
+        }
     bmx_particles p;
     std::memset(&p, 0, sizeof(p));
     p.x = &c[0]; p.px = &c[(size_t)n]; p.y = &c[(size_t)2 * n]; p.py = &c[(size_t)3 * n];
@@ -1139,56 +1205,131 @@ int32_t bmx_closed_orbit(bmx_ctx* ctx, const bmx_lattice* lat, int32_t n_dim, do
     bmx_stats st;
     if (bmx_track_multi(ctx, lat, off.data(), 1, &p, &st) != 0) return -1;
     if (out) out->kernel_ms += st.kernel_ms;
-    int n_open = 0;
+    for (int v = 0; v < nv; v++)
+      for (size_t b = 0; b < bases[v].size(); b++) {
+        int64_t k0 = off[v] + (int64_t)b * np;
+        Probe pr;
+        pr.lost = false;
+        for (int j = 0; j < np; j++) pr.lost |= state[k0 + j] != BMX_ALIVE;
+        for (int d = 0; d < 6; d++) {
+          pr.t[d] = c[(size_t)d * n + k0];
+          for (int j = 0; j < 6; j++)
+            pr.m[d][j] = j < n_dim ? (c[(size_t)d * n + k0 + 1 + j] - c[(size_t)d * n + k0]) / h : (d == j ? 1.0 : 0.0);
+        }
+        res[v].push_back(pr);
+      }
+    return 0;
+  };
+
+  std::vector<int> done(nv, 0), z_moves(nv, 0);
+  std::vector<double> last_step(nv, 0.0), rho_found(nv, 0.0);
+  for (int v = 0; v < nv; v++) {
+    status[v] = 1;
+    if (n_dim == 6 && wavelen[v] > 0) orbit[v * 6 + 4] = modulo2(orbit[v * 6 + 4], wavelen[v] / 2);
+  }
+  std::vector<std::vector<std::array<double, 6>>> bases(nv);
+  std::vector<std::vector<Probe>> res;
+  for (int it = 0; it < 60; it++) {
+    for (int v = 0; v < nv; v++) {
+      bases[v].clear();
+      if (!done[v]) bases[v].push_back({orbit[v * 6], orbit[v * 6 + 1], orbit[v * 6 + 2], orbit[v * 6 + 3],
+                                        orbit[v * 6 + 4], orbit[v * 6 + 5]});
+    }
+    if (run_probes(bases, res) != 0) return -1;
+    // variants whose converged orbit is longitudinally unstable: the z search
+    std::vector<int> search;
     for (int v = 0; v < nv; v++) {
       if (done[v]) continue;
-      bool lost = false;
-      for (int j = 0; j < np; j++) lost |= state[off[v] + j] != BMX_ALIVE;
-      if (lost) { status[v] = 2; done[v] = 1; continue; }
-      // residual r = T(x) - x and Jacobian columns over the first n_dim coordinates
+      const Probe& pr = res[v][0];
+      if (pr.lost) { status[v] = 2; done[v] = 1; continue; }
+      // residual r = T(x) - x and (I - M) over the first n_dim coordinates
       double r[6], A[6][7];
       double rmax = 0;
       for (int d = 0; d < n_dim; d++) {
-        r[d] = c[(size_t)d * n + off[v]] - orbit[v * 6 + d];
+        r[d] = pr.t[d] - orbit[v * 6 + d];
         rmax = std::max(rmax, std::fabs(r[d]));
       }
-      if (rmax < 1e-15) { status[v] = 0; done[v] = 1; continue; }
-      for (int d = 0; d < n_dim; d++) {
-        for (int j = 0; j < n_dim; j++) {
-          double mdj = (c[(size_t)d * n + off[v] + 1 + j] - c[(size_t)d * n + off[v]]) / h;
-          A[d][j] = (d == j ? 1.0 : 0.0) - mdj;  // (I - M)
+      bool conv = rmax < 1e-15;
+      if (!conv) {
+        for (int d = 0; d < n_dim; d++) {
+          for (int j = 0; j < n_dim; j++) A[d][j] = (d == j ? 1.0 : 0.0) - pr.m[d][j];
+          A[d][n_dim] = r[d];
         }
-        A[d][n_dim] = r[d];
-      }
-      // Gaussian elimination with partial pivoting: (I - M) dx = r
-      bool sing = false;
-      for (int col = 0; col < n_dim && !sing; col++) {
-        int piv = col;
-        for (int row = col + 1; row < n_dim; row++)
-          if (std::fabs(A[row][col]) > std::fabs(A[piv][col])) piv = row;
-        if (std::fabs(A[piv][col]) < 1e-14) { sing = true; break; }
-        for (int k = 0; k <= n_dim; k++) std::swap(A[col][k], A[piv][k]);
-        for (int row = col + 1; row < n_dim; row++) {
-          double f = A[row][col] / A[col][col];
-          for (int k = col; k <= n_dim; k++) A[row][k] -= f * A[col][k];
+        // Gaussian elimination with partial pivoting: (I - M) dx = r
+        bool sing = false;
+        for (int col = 0; col < n_dim && !sing; col++) {
+          int piv = col;
+          for (int row = col + 1; row < n_dim; row++)
+            if (std::fabs(A[row][col]) > std::fabs(A[piv][col])) piv = row;
+          if (std::fabs(A[piv][col]) < 1e-14) { sing = true; break; }
+          for (int k = 0; k <= n_dim; k++) std::swap(A[col][k], A[piv][k]);
+          for (int row = col + 1; row < n_dim; row++) {
+            double f = A[row][col] / A[col][col];
+            for (int k = col; k <= n_dim; k++) A[row][k] -= f * A[col][k];
+          }
         }
+        if (sing) { status[v] = 3; done[v] = 1; continue; }
+        double dx[6];
+        for (int row = n_dim - 1; row >= 0; row--) {
+          double acc = A[row][n_dim];
+          for (int k = row + 1; k < n_dim; k++) acc -= A[row][k] * dx[k];
+          dx[row] = acc / A[row][row];
+        }
+        if (n_dim == 6 && wavelen[v] > 0) {  // a z step of at most a tenth of the wavelength
+          double lim = std::fabs(dx[4]) / (wavelen[v] / 10);
+          if (lim > 1)
+            for (int d = 0; d < 6; d++) dx[d] /= lim;
+        }
+        double step = 0;
+        for (int d = 0; d < n_dim; d++) {
+          step = std::max(step, std::fabs(dx[d]) / (1.0 + std::fabs(orbit[v * 6 + d])));
+          orbit[v * 6 + d] += dx[d];
+        }
+        if (n_dim == 6 && wavelen[v] > 0) orbit[v * 6 + 4] = modulo2(orbit[v * 6 + 4], wavelen[v] / 2);
+        conv = step < 1e-15 || (it > 0 && step < 1e-9 && step > 0.5 * last_step[v]);
+        last_step[v] = step;
       }
-      if (sing) { status[v] = 3; done[v] = 1; continue; }
-      double dx[6];
-      for (int row = n_dim - 1; row >= 0; row--) {
-        double acc = A[row][n_dim];
-        for (int k = row + 1; k < n_dim; k++) acc -= A[row][k] * dx[k];
-        dx[row] = acc / A[row][row];
+      if (!conv) continue;
+      status[v] = 0;
+      done[v] = 1;
+      if (n_dim == 6 && wavelen[v] > 0 && z_moves[v] < 3) {
+        rho_found[v] = spectral_radius6(pr.m);
+        if (rho_found[v] > 1.000001) search.push_back(v);
       }
-      double dmax = 0;
-      for (int d = 0; d < n_dim; d++) {
-        orbit[v * 6 + d] += dx[d];
-        dmax = std::max(dmax, std::fabs(dx[d]));
-      }
-      if (dmax < 1e-15) { status[v] = 0; done[v] = 1; continue; }
-      n_open++;
     }
-    if (n_open == 0) break;
+    if (!search.empty()) {
+      // candidates z0 + j lambda / 8, j = -3..4 (j = 0 is the orbit found)
+      for (int v = 0; v < nv; v++) bases[v].clear();
+      for (int v : search)
+        for (int j = -3; j <= 4; j++) {
+          if (j == 0) continue;
+          std::array<double, 6> b = {orbit[v * 6], orbit[v * 6 + 1], orbit[v * 6 + 2],
+                                     orbit[v * 6 + 3], orbit[v * 6 + 4] + j * wavelen[v] / 8, orbit[v * 6 + 5]};
+          bases[v].push_back(b);
+        }
+      if (run_probes(bases, res) != 0) return -1;
+      for (int v : search) {
+        int jb = 0, q = 0;
+        double rho0 = rho_found[v];
+        for (int j = -3; j <= 4; j++) {
+          if (j == 0) continue;
+          const Probe& pr = res[v][q++];
+          if (pr.lost) continue;
+          double rho = spectral_radius6(pr.m);
+          if (rho < rho0) { rho0 = rho; jb = j; }
+        }
+        if (jb != 0) {
+          orbit[v * 6 + 4] = modulo2(orbit[v * 6 + 4] + jb * wavelen[v] / 8, wavelen[v] / 2);
+          z_moves[v]++;
+          status[v] = 1;
+          done[v] = 0;
+          last_step[v] = 0;
+        }
+      }
+    }
+    bool open = false;
+    for (int v = 0; v < nv; v++) open |= !done[v];
+    if (!open) break;
   }
   return 0;
 }

@@ -195,19 +195,13 @@ _lib_dual = None
 def jacobian(eles, species, n_turns, vec):
     """Exact (dual-number) Jacobian of the oracle's n_turns map at vec.
     Returns (out6, J 6x6); raises if the particle is lost."""
-    global _lib_dual
-    if _lib_dual is None:
-        if not os.path.exists(LIB_DUAL):
-            build()
-        _lib_dual = C.CDLL(LIB_DUAL)
-        _lib_dual.oracle_jacobian.restype = C.c_int
-        _lib_dual.oracle_jacobian.argtypes = [C.POINTER(abi.EleRec), C.c_int, C.c_int, C.c_int, _DP, C.c_double, _DP, _DP]
+    L = _dual()
     arr = eles if isinstance(eles, C.Array) else abi_array(eles)
     v = (C.c_double * 6)(*vec)
     out = (C.c_double * 6)()
     J = (C.c_double * 36)()
     beta0 = arr[0].p0c_start / arr[0].e_tot_start
-    rc = _lib_dual.oracle_jacobian(arr, len(arr), species, n_turns, v, beta0, out, J)
+    rc = L.oracle_jacobian(arr, len(arr), species, n_turns, v, beta0, out, J)
     if rc != 0:
         raise RuntimeError(f"particle lost (state {rc}) while forming the Jacobian")
     return np.array(out[:]), np.array(J[:]).reshape(6, 6)
@@ -233,6 +227,8 @@ def _dual():
         _lib_dual = C.CDLL(LIB_DUAL)
     L = _lib_dual
     if not getattr(L, "_rad_ready", False):
+        L.oracle_jacobian.restype = C.c_int
+        L.oracle_jacobian.argtypes = [C.POINTER(abi.EleRec), C.c_int, C.c_int, C.c_int, _DP, C.c_double, _DP, _DP]
         L.oracle_rad_map_setup.restype = C.c_int
         L.oracle_rad_map_setup.argtypes = [C.POINTER(abi.EleRec), C.c_int, _DP, C.c_double, C.c_double, C.c_double, _DP]
         L.oracle_make_mat6.restype = C.c_int

@@ -294,10 +294,115 @@ def nonlin():
                 elements=SMALL_RING, twiss=twiss, rows=twiss)
 
 
+# closed_orbit_test/ad05f_pos_RF.lat and ad05f_neg_RF.lat: an SLS-2 ring
+# (2.4 GeV e-, 12 cavities at harmonic 96) whose 6-d closed orbit with
+# radiation damping on (the test leaves bmad_com%radiation_damping_on set,
+# closed_orbit_test.f90:28-31) and synchrotron tune the test records.
+# Element attributes (lines 6-151) and lines (lines 155-213) as data; the
+# "_inv" bends swap e1 and e2; sbend[ds_step] = 0.005, sextupole and
+# quad::* num_steps = 16 (lines 210-212); a cavity's ds_step is
+# bmad_com%default_ds_step = 0.2 m, so 0.88 m is tracked in nint(4.4) = 4
+# slices.  The pos_RF file halves every sextupole's k2 (line 222) and the
+# neg_RF file negates the voltage (line 6).
+AD05F_DRIFTS = dict(d0=0.1, d1=0.2, d2=0.4, dq=0.3, dc=0.38, dc1=0.21, dc2=0.21, ds1=0.15, ds2=0.15, dm1=0.15,
+                    dm2=0.1, dl1=0.35, dl2=0.35, dsmgp=1.0, dm=3.06, dl=5.34, dqs=0.05, de=0.1, doc=0.05,
+                    no_sas=0.1, no_sam=0.1, no_sal=0.1, gj=0.0, gb=0.0, gm=0.0)
+AD05F_QUADS = dict(qf=(0.2, 7.420488), qf1=(0.15, 0), qcs=(0.15, 7.600167), qds=(0.1, 1.093151),
+                   qcs1=(0.15, 6.549262), qs1=(0.15, 0), qs2=(0.15, -7.385729), qs3=(0.2, 7.467984),
+                   qm1=(0.1, 1.537587), qm2=(0.15, -8.479469), qm3=(0.2, 7.204039), ql1=(0.1, -1.283269),
+                   ql2=(0.1, -5.033152), ql3=(0.2, 6.820985), ql4=(0.1, -5.664977), qd=(0.15, -7.129892),
+                   qes=(0.1, -2.892556))
+AD05F_SEXTUPOLES = dict(sf=650.94069, sfc=638.68885, sfc1=482.44199, sd1=-328.93798, sd2=-420.96168,
+                        sd3=-726.04434, sd4=-603.7521, sde=-305.55462, sd=-848.21693)
+# bs00..bs14: (l, e1, e2), angle 0.004420919 each
+AD05F_BS = [(0.0062, 0, 0.004420919), (0.00748, -0.004420919, 0.0088435833), (0.00897, -0.0088435833, 0.013264502),
+            (0.01072, -0.013264502, 0.017685421), (0.01275, -0.017685421, 0.022108086),
+            (0.0151, -0.022108086, 0.026529005), (0.01782, -0.026529005, 0.030949924),
+            (0.02094, -0.030949924, 0.035372588), (0.02452, -0.035372588, 0.039793507),
+            (0.02861, -0.039793507, 0.044214426), (0.03328, -0.044214426, 0.04863709),
+            (0.03859, -0.04863709, 0.053058009), (0.04461, -0.053058009, 0.057478928),
+            (0.05142, -0.057478928, 0.061901593), (0.0591, -0.061901593, 0.066322512)]
+AD05F_MULTIPOLES = dict(oyy0=3364.6247, oxy0=-2626.9988, ocyy0=1.0, ocxy0=-2002.6493, ocxx0=1082.4127)
+AD05F_MARKERS = ["no_oxxm0", "no_oxxs0", "no_oxyl0", "no_ocyy0", "osbend", "onbend", "obm", "obm2", "ol3"]
+AD05F_LINES = {
+    "mhv": "ch mon cv", "mhv_inv": "cv mon ch", "bn": "bn00 bn01", "bn_inv": "bn01_inv bn00_inv",
+    "ds": "dsend dsmgp", "oxy": "doc oxy0 doc", "oyy": "doc oyy0 doc", "ocxy": "doc ocxy0 doc",
+    "ocyy": "doc no_ocyy0 doc", "ocxx": "doc ocxx0 doc",
+    "bs": " ".join(f"bs{i:02d}" for i in range(15)),
+    "bs_inv": " ".join(f"bs{i:02d}_inv" for i in range(14, -1, -1)),
+    "hc": "bs gb d0 gj qd dqs mhv sd dqs qf dqs sf ocxx d1 bahq1 d2 ocxy dq sd1 mhv dqs ocyy gj gb bn_inv obm bn "
+          "gb gj dqs mhv sd2 dc",
+    "hc_inv": "dc sd2 mhv_inv dqs gj gb bn_inv obm bn gb gj ocyy dqs mhv_inv sd1 dq ocxy d2 bahq1_inv d1 ocxx sf "
+              "dqs qf dqs sd mhv_inv dqs qd gj d0 gb bs_inv",
+    "ts": "qcs dqs sfc dqs bahq2 dc sd3 mhv dqs gj gb bn_inv obm2 bn gb gj dqs mhv sd4 dqs qds dc1 qcs1 dqs sfc1 "
+          "dqs bahq3 dc2 qes dqs sde mhv dqs gj gb bn_inv gb gj oyy mhv",
+    "ts_inv": "mhv_inv oyy gj gb bn gb gj dqs mhv_inv sde dqs qes dc2 bahq3_inv dqs sfc1 dqs qcs1 dc1 qds dqs sd4 "
+              "mhv_inv dqs gj gb bn_inv obm2 bn gb gj dqs mhv_inv sd3 dc bahq2_inv dqs sfc dqs qcs",
+    "ms": "no_sas dqs qs2 ds2 oxy qs3 mhv gm ds", "ms_inv": "ds gm mhv_inv qs3 oxy ds2 qs2 dqs no_sas",
+    "mm": "qm1 dm1 no_sam dqs qm2 dm2 oxy qm3 mhv gm dm", "mm_inv": "dm gm mhv_inv qm3 oxy dm2 qm2 dqs no_sam dm1 qm1",
+    "ml": "ql1 dl1 ql2 dl2 no_sal oxy ol3 ql3 dl1 ql4 mhv gm dl",
+    "ml_inv": "dl gm mhv_inv ql4 dl1 ql3 ol3 oxy no_sal dl2 ql2 dl1 ql1",
+    "hs": "osbend hc ts ms", "hs_inv": "ms_inv ts_inv hc_inv osbend",
+    "hm": "osbend hc ts mm", "hm_inv": "mm_inv ts_inv hc_inv osbend",
+    "hl": "osbend hc ts ml", "hl_inv": "ml_inv ts_inv hc_inv osbend",
+    "six": "hl_inv hs hs_inv hm", "six_inv": "hm_inv hs hs_inv hl", "per": "six six_inv", "ring": "per per per",
+}
+
+
+def ad05f_elements():
+    """Element definitions of the ad05f files in the element-record vocabulary."""
+    el = {}
+    for n, l in AD05F_DRIFTS.items():
+        el[n] = dict(key="drift", l=l)
+    for n, (l, k1) in AD05F_QUADS.items():
+        el[n] = dict(key="quadrupole", l=l, k1=k1, num_steps=16)
+    for n, k2 in AD05F_SEXTUPOLES.items():
+        el[n] = dict(key="sextupole", l=0.1, k2=k2, num_steps=16)
+    bends = {f"bs{i:02d}": dict(l=l, angle=0.004420919, e1=e1, e2=e2, k1=0.0) for i, (l, e1, e2) in enumerate(AD05F_BS)}
+    bends["bn00"] = dict(l=0.1026, angle=0.021629865, e1=0.0, e2=0.021629865, k1=0.0)
+    bends["bn01"] = dict(l=0.2774, angle=0.02472608, e1=-0.021629865, e2=0.046355945, k1=-1.98)
+    bends["bahq1"] = dict(l=0.2, angle=-0.0113446401, k1=3.240843, e1=-0.00567232007, e2=-0.00567232007)
+    bends["bahq2"] = dict(l=0.2, angle=-0.0136135682, k1=4.757043, e1=-0.00680678408, e2=-0.00680678408)
+    bends["bahq3"] = dict(l=0.2, angle=-0.0113446401, k1=6.480930, e1=-0.00567232007, e2=-0.00567232007)
+    for n, b in list(bends.items()):
+        bends[n + "_inv"] = dict(b, e1=b["e2"], e2=b["e1"])
+    for n, b in bends.items():
+        # fringe_type = basic_bend, hgap = fint = 0; sbend[ds_step] = 0.005
+        el[n] = dict(key="sbend", num_steps=max(1, int(math.floor(b["l"] / 0.005 + 0.5))), **b)
+    for n, k3l in AD05F_MULTIPOLES.items():
+        el[n] = dict(key="multipole", knl={3: k3l})
+    for n in AD05F_MARKERS:
+        el[n] = dict(key="marker")
+    el["mon"] = dict(key="monitor")
+    el["ch"] = dict(key="hkicker")
+    el["cv"] = dict(key="vkicker")
+    # dsend: rfcav (voltage 0.7e7 / 12 V, harmon 96, l 0.88); its sign per file below
+    el["dsend"] = dict(key="rfcavity", l=0.88, voltage=0.7e7 / 12, num_steps=4)
+    return el
+
+
+def closed_orbit():
+    """Closed orbit z and synchrotron tune of the two ad05f rings (closed_orbit_test)."""
+    path = os.path.join(REF, "regression_tests/closed_orbit_test/output.correct")
+    rows = {}
+    with open(path) as f:
+        for lineno, line in enumerate(f, 1):
+            m = re.match(r'^"(ad05f_(pos|neg)_RF\.lat-\d-[AB] Z(_tune)?)"\s+ABS\s+(\S+)\s+(\S+)', line.strip())
+            if m:
+                rows[m.group(1)] = dict(value=float(m.group(5)), tol=float(m.group(4)), line=lineno)
+    lines = {k: v.split() for k, v in AD05F_LINES.items()}
+    return dict(source="regression_tests/closed_orbit_test (ad05f_pos_RF.lat, ad05f_neg_RF.lat, output.correct)",
+                particle="electron", e_tot=2.4e9, harmon=96, elements=ad05f_elements(), lines=lines, use="ring",
+                lattices={"ad05f_pos_RF.lat": dict(voltage_sign=1, sextupole_scale=0.5),
+                          "ad05f_neg_RF.lat": dict(voltage_sign=-1, sextupole_scale=1.0)},
+                rows=rows)
+
+
 if __name__ == "__main__":
     for name, fn in (("tracking_method_bmad_standard.json", tracking_method), ("fringe_test.json", fringe),
                      ("aperture_test.json", aperture), ("multipole_test.json", multipole),
-                     ("mat6_calc_method_test.json", mat6), ("nonlin_small_ring.json", nonlin)):
+                     ("mat6_calc_method_test.json", mat6), ("nonlin_small_ring.json", nonlin),
+                     ("closed_orbit_ad05f.json", closed_orbit)):
         d = fn()
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(d, f, indent=1, sort_keys=True)

@@ -26,6 +26,35 @@ def ele_from_spec(spec, species, e_tot=None, p0c=None):
     return e
 
 
+def ring_from_fixture(fx, name):
+    """Element list of one lattice of a ring fixture (elements, lines, use,
+    lattices[name]): the lines expanded in order, the file's voltage sign and
+    sextupole scale applied, the reference energy set, and every cavity at
+    harmon times the revolution frequency (Bmad's harmon bookkeeping)."""
+    opt = fx["lattices"][name]
+
+    def expand(n):
+        if n in fx["lines"]:
+            return [x for m in fx["lines"][n] for x in expand(m)]
+        return [n]
+
+    species = SPECIES[fx["particle"]]
+    eles = []
+    for n in expand(fx["use"]):
+        spec = dict(fx["elements"][n])
+        if spec["key"] == "sextupole":
+            spec["k2"] = spec["k2"] * opt["sextupole_scale"]
+        if spec["key"] == "rfcavity":
+            spec["voltage"] = spec["voltage"] * opt["voltage_sign"]
+        eles.append(lattice.ele(spec.pop("key"), **spec))
+    lattice.set_reference(eles, species, e_tot=fx["e_tot"])
+    t_rev = lattice.revolution_time(eles)
+    for e in eles:
+        if e.key == abi.KEY_RFCAVITY:
+            e.rf_frequency = fx["harmon"] / t_rev
+    return eles
+
+
 def printed_match(val, golden, digits=11):
     """True if val rounds to the golden value printed with `digits` significant digits."""
     if golden == 0:

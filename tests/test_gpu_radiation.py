@@ -16,9 +16,13 @@ test_oracle_radiation.py).
   emittance (photon opening angle) and the energy spread against the
   equilibrium of the one-turn map with damping and the maps' stochastic
   covariance (statistical, 16,384 particles);
+* closed_orbit_test's SLS-2 rings: bmx_closed_orbit in six dimensions from
+  the zero orbit with damping, maps from bmx.radiation, at the z the test
+  records, and the synchrotron tune of bmx_transfer_matrix about it;
 * what is refused: backward tracking, fluctuations in the survival scans.
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -27,7 +31,7 @@ import scipy.linalg as sla
 
 from bmx import abi, beam, lattice, tracker
 import oracle
-from helpers import NAMES, coord_errors
+from helpers import NAMES, coord_errors, ring_from_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -279,3 +283,48 @@ def test_design_orbit_maps_equal_the_oracles(ctx):
                     assert np.max(np.abs(a - b)) <= 1e-11 * scale, (name, i, h, what, np.max(np.abs(a - b)) / scale)
             n_bends += e.key == abi.KEY_SBEND
         print(f"design-orbit maps {name}: {n_bends} bends equal to the oracle's", flush=True)
+
+
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "closed_orbit_ad05f.json")) as _f:
+    CO = json.load(_f)
+
+
+@pytest.mark.parametrize("name", sorted(CO["lattices"]))
+def test_closed_orbit_test_rings(ctx, name):
+    """closed_orbit_test (closed_orbit_test.f90:45-80, output.correct:4-13)
+    on the device: the 3120-element SLS-2 ring with radiation damping, its
+    maps from bmx.radiation.design_orbit_maps (the library's own transfer
+    matrices), closed in six dimensions by bmx_closed_orbit from the zero
+    orbit as closed_orbit_calc starts (for ad05f_pos_RF the nearest fixed
+    point is the unstable one; the longitudinal stability search moves it to
+    the stable one).  z matches the recorded value to its ABS 5e-8.  The
+    synchrotron tune comes from bmx_transfer_matrix over one turn about that
+    orbit: its derivatives leave the radiation kicks out, as make_mat6's
+    element matrices do, so the one-turn matrix is transfer_matrix_calc's
+    t1 (calc_z_tune.f90:36-59)."""
+    from bmx import radiation
+    eles = ring_from_fixture(CO, name)
+    g2 = sum(e.g ** 2 * e.l for e in eles if e.key == abi.KEY_SBEND)
+    g3 = sum(e.g ** 3 * e.l for e in eles if e.key == abi.KEY_SBEND)
+    print(f"closed_orbit_test {name}: {len(eles)} elements", flush=True)
+    maps = radiation.design_orbit_maps(ctx, eles, abi.ELECTRON, g2, g3)
+    lat = ctx.lattice(eles, abi.ELECTRON, rad_maps=maps, radiation=dict(damping_on=1))
+    orb, status = tracker.closed_orbit(lat, n_dim=6)
+    assert status[0] == 0, status
+    rows = CO["rows"]
+    for tag in ("1-B", "2-A", "2-B"):
+        r = rows[f"{name}-{tag} Z"]
+        assert abs(orb[0, 4] - r["value"]) <= r["tol"], (tag, orb[0, 4], r["value"])
+    out, mat, state = tracker.transfer_matrix(lat, orb[0])
+    assert state[0] == abi.ALIVE
+    assert np.abs(out[0] - orb[0]).max() < 1e-12
+    a = mat[0]
+    denom = 2 * (a[4, 4] * a[5, 5] - a[4, 5] * a[5, 4])
+    assert (a[4, 4] + a[5, 5]) / denom - 1 < -1e-9
+    ang = np.abs(np.angle(np.linalg.eigvals(a)))
+    tune = np.sign(a[4, 5]) * ang.min() / (2 * np.pi)
+    for tag in ("1-B", "2-B"):
+        r = rows[f"{name}-{tag} Z_tune"]
+        assert abs(tune - r["value"]) <= r["tol"], (tag, tune, r["value"])
+    print(f"closed_orbit_test {name}: z {orb[0, 4]:.10f}, synchrotron tune {tune:.10f}", flush=True)
+    lat.close()

@@ -3,9 +3,14 @@
 rad1_damp_and_stoc_mats (radiation_mod.f90:160-232,
 bmad/modules/rad_6d_mod.f90:406-718), restated in oracle/bmx_oracle.c.
 
-No golden vector of the reference covers this path without spin and its own
-random stream (regression_tests/long_term_tracking_test runs ramping, spin
-and Bmad's ran_gauss), so the maps are pinned by what the physics fixes:
+The damping is pinned to the reference's own numbers by
+regression_tests/closed_orbit_test: the 6-d closed orbit's z of two SLS-2
+rings with radiation damping on sits where the cavities restore the energy
+the maps take, and the test records it with the synchrotron tune at
+ABS 5e-8.  No golden vector covers the fluctuations without spin and
+Bmad's own random stream (regression_tests/long_term_tracking_test runs
+ramping, spin and ran_gauss), so the maps are further pinned by what the
+physics fixes:
 the energy loss per turn U0/E = (2/3) r_e gamma^3 I2, Robinson's theorem
 (the damping decrements of the three modes add up to 2 U0/E, i.e.
 -ln det M = 4 U0/E for the one-turn matrix with damping), the partition
@@ -15,6 +20,7 @@ opening angle (13/55) C_q / (J_y I2) * integral of beta_y |g|^3, with the
 radiation integrals formed here from the ring's own optics.  The Gaussian
 stream is Philox4x32-10, pinned by its published known-answer vectors.
 """
+import json
 import os
 import sys
 
@@ -29,6 +35,7 @@ for p in (os.path.join(ROOT, "bmad-ecosystem_amd"), os.path.join(ROOT, "oracle")
 
 from bmx import abi, lattice  # noqa: E402
 import oracle  # noqa: E402
+from helpers import ring_from_fixture  # noqa: E402
 
 R_E = 2.8179403227e-15
 HBAR = 4.135667696e-15 / (2 * np.pi)
@@ -326,3 +333,111 @@ def test_fluctuating_tracking_draws_the_philox_stream():
         # to the bend's second-order terms (z: the path length at second order in the pz kick, ~1e-14 m)
         # (a deviate drawn with another counter differs by order one)
         assert np.all(np.abs(got - pred) < 1e-5 * np.abs(pred) + 5e-14)
+
+
+# ---------------------------------------------------------------------------
+# closed_orbit_test: the SLS-2 rings ad05f_pos_RF.lat and ad05f_neg_RF.lat
+# ---------------------------------------------------------------------------
+with open(os.path.join(ROOT, "tests", "golden", "closed_orbit_ad05f.json")) as _f:
+    CO = json.load(_f)
+VEC = ("x", "px", "y", "py", "z", "pz")
+
+
+def g_integrals(eles):
+    """g_integrals_calc (bmad/low_level/g_integrals_calc.f90:44-51): sums of
+    g^2 L and g^3 L over the bends, which set the Romberg tolerance."""
+    g2 = sum(e.g ** 2 * e.l for e in eles if e.key == abi.KEY_SBEND)
+    g3 = sum(e.g ** 3 * e.l for e in eles if e.key == abi.KEY_SBEND)
+    return g2, g3
+
+
+def oracle_closed_orbit_6d(arr, maps, wavelen):
+    """The oracle's one-turn map with damping closed in all six coordinates:
+    Newton steps with a finite-difference Jacobian from z = 0, z kept in
+    [-wavelen / 2, wavelen / 2), then, as closed_orbit_calc does when the
+    orbit found is longitudinally unstable, the most stable of the starts
+    z + j wavelen / 8 and Newton again."""
+    h = 1e-8
+
+    def fold(orb):
+        z = orb[4] % wavelen
+        orb[4] = z - wavelen if z >= wavelen / 2 else z
+        return orb
+
+    def turn(orb):
+        c = np.tile(np.asarray(orb, float)[:, None], (1, 7))
+        for j in range(6):
+            c[j, j + 1] += h
+        out = oracle.track_bunch_rad(arr, abi.ELECTRON, 1, c, maps, damping_on=True)
+        assert (out["state"] == abi.ALIVE).all()
+        v = np.stack([out[k] for k in VEC])
+        return v[:, 0], (v[:, 1:] - v[:, :1]) / h
+
+    def newton(orb):
+        for _ in range(30):
+            t, m = turn(orb)
+            dx = np.linalg.solve(np.eye(6) - m, t - orb)
+            orb = fold(orb + dx)
+            if np.abs(dx).max() < 1e-12:
+                break
+        return orb, m
+
+    orb, m = newton(np.zeros(6))
+    if np.abs(np.linalg.eigvals(m)).max() > 1.000001:
+        starts = [orb + np.array([0, 0, 0, 0, j * wavelen / 8, 0]) for j in range(-3, 5) if j != 0]
+        rho = [np.abs(np.linalg.eigvals(turn(s)[1])).max() for s in starts]
+        orb, m = newton(starts[int(np.argmin(rho))])
+    return orb, m
+
+
+def z_tune(arr, maps, orb):
+    """calc_z_tune (bmad/code/calc_z_tune.f90:36-59) of the matrix
+    lat_make_mat6 builds about the closed orbit: each element's make_mat6
+    (its body without radiation) at the orbit's entrance into it, the orbit
+    carried on with radiation (track1_radiation moves the orbit, not
+    ele%mat6).  Returns (tune / 2 pi, orbit after one turn)."""
+    t1 = np.eye(6)
+    v = np.array(orb, float).reshape(6, 1)
+    beta = None
+    for i in range(len(arr)):
+        t1 = oracle.make_mat6(arr[i], abi.ELECTRON, v[:, 0])[1] @ t1
+        out = oracle.track_bunch_rad(oracle.abi_array([arr[i]]), abi.ELECTRON, 1, v, maps[i:i + 1],
+                                     damping_on=True, beta=beta)
+        v = np.stack([out[k] for k in VEC])
+        beta = out["beta"]
+    a = t1
+    denom = 2 * (a[4, 4] * a[5, 5] - a[4, 5] * a[5, 4])
+    assert (a[4, 4] + a[5, 5]) / denom - 1 < -1e-9  # stable
+    ev = np.linalg.eigvals(a)
+    ang = np.abs(np.angle(ev))
+    # the synchrotron mode is the pair of the smallest phase advance
+    return np.sign(a[4, 5]) * ang.min() / (2 * np.pi), v[:, 0]
+
+
+@pytest.mark.parametrize("name", sorted(CO["lattices"]))
+def test_closed_orbit_z_and_synchrotron_tune_with_damping(name):
+    """closed_orbit_test (regression_tests/closed_orbit_test/closed_orbit_test.f90:
+    45-80, output.correct:4-13): the 6-d closed orbit's z of the 3120-element
+    SLS-2 ring with radiation damping on, and its synchrotron tune.  With the
+    cavities on, z is where the rf restores the energy the bends' maps take
+    per turn (sin(2 pi z / lambda) = U0 / V), so it pins the maps' xfer_damp_vec
+    and damping matrices (radiation_map_setup about the zero orbit, which the
+    parser's lat_make_mat6 leaves in map_ref_orb_in), bends with gradients
+    and edge angles, the thick sextupoles and the rf to 5e-8 m.  The tune
+    pins the matrices along that orbit."""
+    eles = ring_from_fixture(CO, name)
+    arr = oracle.abi_array(eles)
+    g2, g3 = g_integrals(eles)
+    maps = oracle.rad_maps(arr, abi.ELECTRON, np.zeros((len(arr), 6)), g2, g3)
+    f_rf = min(e.rf_frequency for e in eles if e.key == abi.KEY_RFCAVITY)
+    wavelen = C_LIGHT * eles[0].p0c / eles[0].e_tot / f_rf
+    orb, _ = oracle_closed_orbit_6d(arr, maps, wavelen)
+    rows = CO["rows"]
+    for tag in ("1-B", "2-A", "2-B"):
+        r = rows[f"{name}-{tag} Z"]
+        assert abs(orb[4] - r["value"]) <= r["tol"], (tag, orb[4], r["value"])
+    tune, closing = z_tune(arr, maps, orb)
+    assert np.abs(closing - orb).max() < 1e-12
+    for tag in ("1-B", "2-B"):
+        r = rows[f"{name}-{tag} Z_tune"]
+        assert abs(tune - r["value"]) <= r["tol"], (tag, tune, r["value"])
