@@ -415,7 +415,10 @@ int32_t bmx_bunch_moments(bmx_ctx *ctx, const bmx_particles *p, double *moments)
 /* Closed orbit of every variant (closed_orbit_calc, bmad/code/closed_orbit_calc.f90:81):
  * orbit is n_variants x 6, the guess on entry and the closed orbit on return.
  * n_dim = 4 closes x, px, y, py at the guess's pz; n_dim = 6 closes all six
- * (rf on).  status[v]: 0 converged, 1 not converged, 2 a probe was lost,
+ * (rf on), with z kept within half an rf wavelength (of the lowest frequency
+ * among the cavities that are on) of zero and, as closed_orbit_calc does
+ * (:491-520), moved off a longitudinally unstable fixed point to the stable
+ * one.  status[v]: 0 converged, 1 not converged, 2 a probe was lost,
  * 3 singular one-turn matrix. */
 int32_t bmx_closed_orbit(bmx_ctx *ctx, const bmx_lattice *lat, int32_t n_dim, double *orbit, int32_t *status,
                          bmx_stats *out);
@@ -442,6 +445,9 @@ int32_t bmx_one_turn_matrix(bmx_ctx *ctx, const bmx_lattice *lat, const double *
  * element or a thousand turns are exact to rounding.  beta at entry follows
  * init_coord (the reference beta at pz = 0).  state[i]: BMX_ALIVE, or the
  * loss state (its matrix is zeros, vec_out the coordinates at the loss).
+ * With radiation damping (bmx_lattice_upload_rad) the kicks move the orbit
+ * and leave the derivatives alone, as track1 applies track1_radiation to
+ * the orbit and not to ele%mat6 (bmad/code/track1.f90:198-201, :316-319).
  * Relative-time rf only. */
 int32_t bmx_transfer_matrix(bmx_ctx *ctx, const bmx_lattice *lat, int32_t ix_start, int32_t ix_end,
                             int32_t n_turns, int64_t n, const double *vec_in, double *vec_out, double *mat6,
