@@ -313,7 +313,7 @@ extern "C" __global__ __launch_bounds__(GRID_BLOCK, BMX_WAVES_PER_EU) void bmx_g
     const DevOp* __restrict__ ops, const int32_t* __restrict__ var_op_begin, const int32_t* __restrict__ var_n_ele,
     LatticeConsts lc, int n_ic, GridSoA src, GridSoA dst, const int32_t* __restrict__ count_in,
     int32_t* __restrict__ count_out, const int32_t* __restrict__ d_turn0, int turns_per_block, int n_turns,
-    int32_t* __restrict__ survival, unsigned long long* __restrict__ pet) {
+    int32_t* __restrict__ survival, unsigned long long* __restrict__ pet, const int32_t* __restrict__ var_order) {
   __shared__ DevOp sops[GRID_BLOCK];
   // the block's first turn lives in device memory so that captured graphs of
   // several blocks replay unchanged; every thread reads the same value, so a
@@ -322,7 +322,11 @@ extern "C" __global__ __launch_bounds__(GRID_BLOCK, BMX_WAVES_PER_EU) void bmx_g
   const int turns = min(turns_per_block, n_turns - turn0);
   if (turns <= 0) return;
   constexpr int NP = BMX_NP;
-  int v = blockIdx.y;
+  // the variants in dispatch order, longest program first: the workgroups
+  // are handed out in blockIdx order, so the long ones start first and the
+  // short ones fill the compute units they leave (longest-processing-time
+  // order)
+  int v = var_order[blockIdx.y];
   int64_t base = (int64_t)v * n_ic;
   int n_in = count_in[v];
   Pack<NP> P;
