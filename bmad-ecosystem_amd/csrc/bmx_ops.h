@@ -948,8 +948,17 @@ __device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, co
     coupler();
     if (L.state != BMX_ALIVE) return;
   }
+  // sin(phase) of every slice: one sincos at the entrance, then a rotation by
+  // each slice's phase advance as the running phase took it, (phase + d) -
+  // phase, so that (sp, cp) follow the rounded phase the reference passes to
+  // sin.  The advance is 2 pi f dz / (c beta), a few 1e-9 rad for relativistic
+  // particles; its sin and cos are series while it stays below 2^-8 (the
+  // next terms below 1e-21), and a wave holding a larger one takes sincos of
+  // the phase itself.
+  double sp, cp;
+  sincos(phase, &sp, &cp);
   for (int i = 0; i <= n_slice; i++) {
-    double dE = volt * sin(phase) / n_slice;
+    double dE = volt * sp / n_slice;
     if (i == 0 || i == n_slice) dE = dE / 2.0;
     energy_kick(L, dE, lc);
     if (L.state != BMX_ALIVE) return;
@@ -962,7 +971,19 @@ __device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, co
         op_drift(L, dl);
       }
       if (L.state != BMX_ALIVE) return;
-      phase = phase + D_TWOPI * freq * (L.z - z0) / (D_C_LIGHT * L.beta);
+      double p1 = phase + D_TWOPI * freq * (L.z - z0) / (D_C_LIGHT * L.beta);
+      double d = p1 - phase;
+      phase = p1;
+      if (__builtin_expect(wave_any(!(fabs(d) < 0x1p-8)), 0)) {
+        sincos(phase, &sp, &cp);
+      } else {
+        double d2 = d * d;
+        double sd = d + d * d2 * (-1.0 / 6.0 + d2 * (1.0 / 120.0 - d2 * (1.0 / 5040.0)));
+        double cd = 1.0 + d2 * (-0.5 + d2 * (1.0 / 24.0 - d2 * (1.0 / 720.0)));
+        double s1 = sp * cd + cp * sd;
+        cp = cp * cd - sp * sd;
+        sp = s1;
+      }
     }
   }
   if (flags & F_COUPLER_EXIT) coupler();
