@@ -989,20 +989,28 @@ __device__ void op_rfcavity(Lane& L, const DevOp& op, int n_slice, int flags, co
   if (flags & F_COUPLER_EXIT) coupler();
 }
 
+// Screen of the momentum conditions of orbit_too_large: |px| + |py| <
+// (1 + pz) / 2 implies 1 + pz > 0 and px^2 + py^2 < (1 + pz)^2 with room to
+// spare for rounding (halving is exact), in two VALU operations and a compare
+// where the exact test takes four and two compares.  A lane that fails it
+// (or holds a NaN) takes the exact tests.
+__device__ __forceinline__ bool p_screen(const Lane& L) { return fabs(L.px) + fabs(L.py) < 0.5 + 0.5 * L.pz; }
+
 // orbit_too_large (bmad/low_level/orbit_too_large.f90:26-120).  A straight-
 // line screen of all four conditions runs first; only lanes that fail it take
 // the reference's ordered tests, which pick the loss state.
 __device__ __forceinline__ bool too_large(Lane& L, const LatticeConsts& lc) {
   double mal = lc.max_aperture_limit;
+  bool ok = (fabs(L.x) <= mal) & (fabs(L.y) <= mal) & p_screen(L);
+  if (__builtin_expect(ok, 1)) return false;
   double rel_p = 1.0 + L.pz;
-  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
-  bool bad = (fabs(L.x) > mal) | (fabs(L.y) > mal) | (rel_p < 0.0) | (f > 0.0);
-  if (__builtin_expect(!bad, 1)) return false;
   if (L.x > mal) { L.state = BMX_LOST_POS_X; return true; }
   if (-L.x > mal) { L.state = BMX_LOST_NEG_X; return true; }
   if (L.y > mal) { L.state = BMX_LOST_POS_Y; return true; }
   if (-L.y > mal) { L.state = BMX_LOST_NEG_Y; return true; }
   if (rel_p < 0.0) { L.state = BMX_LOST_PZ; return true; }
+  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
+  if (!(f > 0.0)) return false;
   if (fabs(L.px) > fabs(L.py)) L.state = (L.px > 0.0) ? BMX_LOST_POS_X : BMX_LOST_NEG_X;
   else L.state = (L.py > 0.0) ? BMX_LOST_POS_Y : BMX_LOST_NEG_Y;
   return true;
@@ -1113,9 +1121,7 @@ __device__ __forceinline__ void op_sbend(Lane& L, const DevOp& op, int flags, bo
 __device__ __forceinline__ void ele_end(Lane& L, const bmx_f64* c, int flags, const LatticeConsts& lc,
                                         const bmx_f64* ps) {
   double mal = lc.max_aperture_limit;
-  double rel_p = 1.0 + L.pz;
-  double f = L.px * L.px + L.py * L.py - rel_p * rel_p;
-  bool inside = (rel_p >= 0.0) & (f <= 0.0);
+  bool inside = p_screen(L);
   // F_END_INBOX: the aperture lies inside the max_aperture_limit box, so its
   // screen implies |x|, |y| <= max_aperture_limit
   if (!(flags & F_END_INBOX)) inside &= (fabs(L.x) <= mal) & (fabs(L.y) <= mal);
